@@ -1,0 +1,247 @@
+"""Python front end of the C ABI (include/ear.h), used by tests and bench.py.
+
+Mirrors the reference's render unit: one call renders the (sound, keyframe,
+band) contexts that SceneContext::operator() runs through Scene::Render
+(src/SceneContext.h:46-48, src/Scene.cpp:111-318), for several bands at once.
+Errors come back as EarError carrying the ear_status, the way the reference's
+exceptions reach main (src/EAR.cpp:404-408).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _ffi as F
+
+FRAC_SCALE = 2.0 ** -F.EAR_FRAC_BITS
+
+
+class EarError(RuntimeError):
+    def __init__(self, status: int, message: str = ""):
+        self.status = status
+        self.status_name = F.STATUS_NAMES.get(status, str(status))
+        super().__init__(f"{self.status_name}: {message}")
+
+
+def _check(lib, status, ctx=None):
+    if status != F.EAR_OK:
+        msg = lib.ear_ctx_last_error(ctx).decode() if ctx else ""
+        raise EarError(status, msg or lib.ear_status_string(status).decode())
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """ear_ctx: one HIP device and one stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = F.load()
+        h = C.c_void_p()
+        _check(self.lib, self.lib.ear_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.ear_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_memory_cap(self, nbytes: int):
+        _check(self.lib, self.lib.ear_ctx_set_memory_cap(self.h, nbytes), self.h)
+
+    def set_tuning(self, block_extend=0, block_shade=0, block_shadow=0, ray_chunk=0, waves_per_cu=0):
+        t = F.ear_tuning(block_extend, block_shade, block_shadow, ray_chunk, waves_per_cu)
+        _check(self.lib, self.lib.ear_ctx_set_tuning(self.h, C.byref(t)), self.h)
+
+    def get_tuning(self) -> dict:
+        t = F.ear_tuning()
+        _check(self.lib, self.lib.ear_ctx_get_tuning(self.h, C.byref(t)), self.h)
+        return {k: getattr(t, k) for k, _ in F.ear_tuning._fields_ if k != "reserved"}
+
+
+def make_materials(specs):
+    """specs: [(refl[3], refr[3] | None, spec[3] | None)] as in .EAR MAT blocks."""
+    arr = (F.ear_material * max(1, len(specs)))()
+    for i, (rl, rr, sp) in enumerate(specs):
+        arr[i].refl[:] = [float(x) for x in rl]
+        flags = 0
+        if rr is not None:
+            arr[i].refr[:] = [float(x) for x in rr]
+            flags |= F.EAR_MAT_HAS_REFR
+        if sp is not None:
+            arr[i].spec[:] = [float(x) for x in sp]
+            flags |= F.EAR_MAT_HAS_SPEC
+        arr[i].flags = flags
+    return arr
+
+
+class Scene:
+    """ear_scene: the merged triangle soup on the device with its LBVH."""
+
+    def __init__(self, ctx: Context, tris, mat_ids, materials, leaf_size: int = 0):
+        self.ctx = ctx
+        lib = ctx.lib
+        t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 3, 3)
+        self.n_tris = len(t)
+        v0 = np.ascontiguousarray(t[:, 0, :])
+        v1 = np.ascontiguousarray(t[:, 1, :])
+        v2 = np.ascontiguousarray(t[:, 2, :])
+        m = np.ascontiguousarray(mat_ids, dtype=np.uint16).reshape(-1)
+        mats = make_materials(materials)
+        opts = F.ear_bvh_opts(leaf_size)
+        h = C.c_void_p()
+        _check(lib, lib.ear_scene_upload(ctx.h, _ptr(v0), _ptr(v1), _ptr(v2), _ptr(m), self.n_tris, mats,
+                                         len(materials), C.byref(opts), C.byref(h)), ctx.h)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ear_scene_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self) -> dict:
+        i = F.ear_scene_info()
+        _check(self.ctx.lib, self.ctx.lib.ear_scene_get_info(self.h, C.byref(i)), self.ctx.h)
+        return dict(n_tris=i.n_tris, n_nodes=i.n_nodes, max_depth=i.max_depth, leaf_size=i.leaf_size,
+                    bounds=list(i.bounds), build_ms=i.build_ms)
+
+    def query_closest(self, o, d):
+        o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+        n = len(o)
+        t = np.zeros(n, np.float32)
+        idx = np.zeros(n, np.int32)
+        _check(self.ctx.lib, self.ctx.lib.ear_query_closest(self.h, n, _ptr(o), _ptr(d), _ptr(t), _ptr(idx)),
+               self.ctx.h)
+        return t, idx
+
+    def query_occluded(self, p, q):
+        p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, 3)
+        n = len(p)
+        occ = np.zeros(n, np.uint8)
+        _check(self.ctx.lib, self.ctx.lib.ear_query_occluded(self.h, n, _ptr(p), _ptr(q), _ptr(occ)), self.ctx.h)
+        return occ.astype(bool)
+
+
+def make_recorders(recs):
+    arr = (F.ear_recorder * max(1, len(recs)))()
+    for i, r in enumerate(recs):
+        arr[i].kind = r.get("kind", F.EAR_REC_MONO)
+        arr[i].pos[:] = [float(x) for x in r["pos"]]
+        arr[i].right_ear[:] = [float(x) for x in r.get("right_ear", (1, 0, 0))]
+        arr[i].head_size = float(r.get("head_size", 0.0))
+        arr[i].head_abs[:] = [float(x) for x in r.get("head_abs", (1, 1, 1))]
+    return arr
+
+
+def n_channels(recs):
+    return sum(2 if r.get("kind", F.EAR_REC_MONO) == F.EAR_REC_STEREO else 1 for r in recs)
+
+
+def raw_to_float(raw: np.ndarray) -> np.ndarray:
+    """{lo, hi} int64 pairs (128-bit two's complement, 2^-88 units) -> float64."""
+    lo = raw[..., 0].view(np.uint64).astype(np.float64)
+    hi = raw[..., 1].astype(np.float64)
+    return (hi * 18446744073709551616.0 + lo) * FRAC_SCALE
+
+
+def raw_to_int(raw: np.ndarray):
+    """Exact Python integers (object array) of the 128-bit raw sums."""
+    lo = raw[..., 0].view(np.uint64)
+    hi = raw[..., 1]
+    out = np.empty(lo.shape, dtype=object)
+    for idx in np.ndindex(lo.shape):
+        out[idx] = (int(hi[idx]) << 64) + int(lo[idx])
+    return out
+
+
+@dataclass
+class RenderResult:
+    value: np.ndarray            # [3 band slots][channels][bins] float64
+    abs_value: np.ndarray | None
+    raw: np.ndarray | None       # [3][ch][bins][2] int64
+    abs_raw: np.ndarray | None
+    stats: dict
+    bands: tuple
+    trace_hits: np.ndarray | None = None  # [3][trace_rays][trace_segs]
+    trace_nseg: np.ndarray | None = None
+    wall_s: float = 0.0
+    extra: dict = field(default_factory=dict)
+
+
+def render(ctx: Context, scene: Scene, *, recorders, n_rays, bands=(0, 1, 2), air=(0.99, 0.98, 0.95), dry=1.0,
+           src_pos=(0, 0, 0), src_tris=None, gain=1.0, max_bounces=999, cutoff=1e-8, seed=0xEA2, instance=0,
+           keyframe=-1, sound=0, n_bins=88200, abs_hist=True, instrument=False, trace_rays=0, trace_segs=0,
+           want_raw=True, want_value=True) -> RenderResult:
+    import time
+
+    lib = ctx.lib
+    recs = make_recorders(recorders)
+    nch = n_channels(recorders)
+    d = F.ear_render_desc()
+    d.band_mask = sum(1 << b for b in bands)
+    d.air[:] = [float(x) for x in air]
+    d.dry = dry
+    src_arr = None
+    if src_tris is not None:
+        src_arr = np.ascontiguousarray(src_tris, np.float32).reshape(-1, 3, 3)
+        d.source.kind = F.EAR_SRC_MESH
+        d.source.mesh_v = src_arr.ctypes.data
+        d.source.mesh_n_tris = len(src_arr)
+    else:
+        d.source.kind = F.EAR_SRC_POINT
+    d.source.pos[:] = [float(x) for x in src_pos]
+    d.source.gain = gain
+    d.recorders = recs
+    d.n_recorders = len(recorders)
+    d.n_rays = n_rays
+    d.max_bounces = max_bounces
+    d.energy_cutoff = cutoff
+    d.seed = seed
+    d.instance = instance
+    d.keyframe = keyframe
+    d.sound = sound
+    d.hist_bins = n_bins
+    flags = 0
+    if abs_hist:
+        flags |= F.EAR_FLAG_ABS_HIST
+    if instrument:
+        flags |= F.EAR_FLAG_INSTRUMENT
+    if trace_rays and trace_segs:
+        flags |= F.EAR_FLAG_TRACE_LOG
+    d.flags = flags
+    d.trace_rays = trace_rays
+    d.trace_segs = trace_segs
+
+    value = np.zeros((3, nch, n_bins), np.float64) if want_value else None
+    abs_value = np.zeros((3, nch, n_bins), np.float64) if (abs_hist and want_value) else None
+    raw = np.zeros((3, nch, n_bins, 2), np.int64) if want_raw else None
+    abs_raw = np.zeros((3, nch, n_bins, 2), np.int64) if (abs_hist and want_raw) else None
+    th = tn = None
+    if flags & F.EAR_FLAG_TRACE_LOG:
+        th = np.full((3, trace_rays, trace_segs), -2, np.int32)
+        tn = np.zeros((3, trace_rays), np.uint32)
+    h = F.ear_hist(_ptr(raw), _ptr(value), _ptr(abs_raw), _ptr(abs_value), _ptr(th), _ptr(tn))
+    st = F.ear_stats()
+    t0 = time.perf_counter()
+    _check(lib, lib.ear_render(ctx.h, scene.h, C.byref(d), C.byref(h), C.byref(st)), ctx.h)
+    wall = time.perf_counter() - t0
+    stats = {k: getattr(st, k) for k, _ in F.ear_stats._fields_}
+    del src_arr
+    return RenderResult(value, abs_value, raw, abs_raw, stats, tuple(bands), th, tn, wall)

@@ -1,0 +1,188 @@
+// ear_api.hip -- extern "C" entry points of libear_hip.so (declared in include/ear.h).
+//
+// Status codes cross the boundary, never exceptions; the host C++ layer turns
+// them back into the reference's exceptions (DatatypeException /
+// std::runtime_error, src/EAR.cpp:404-408).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "ear_internal.hpp"
+
+namespace ear {
+
+ear_status hip_status(Ctx* c, hipError_t e, const char* what) {
+    if (c) c->last_error = std::string(what) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? EAR_ERR_OUT_OF_MEMORY : EAR_ERR_HIP;
+}
+
+ear_status dev_alloc(Ctx* c, void** p, size_t bytes) {
+    if (c->mem_cap && c->mem_used + bytes > c->mem_cap) {
+        c->last_error = "device memory cap exceeded";
+        *p = nullptr;
+        return EAR_ERR_OUT_OF_MEMORY;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return hip_status(c, e, "hipMalloc");
+    }
+    c->mem_used += bytes;
+    return EAR_OK;
+}
+
+void dev_free(Ctx* c, void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    c->mem_used = c->mem_used >= bytes ? c->mem_used - bytes : 0;
+}
+
+}  // namespace ear
+
+using namespace ear;
+
+struct ear_ctx : Ctx {};
+struct ear_scene : SceneHandle {};
+
+extern "C" {
+
+int ear_abi_version(void) { return EAR_ABI_VERSION; }
+
+const char* ear_status_string(ear_status s) {
+    switch (s) {
+        case EAR_OK: return "ok";
+        case EAR_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case EAR_ERR_INVALID_MATERIAL: return "Invalid material settings";
+        case EAR_ERR_INVALID_GEOMETRY: return "invalid geometry";
+        case EAR_ERR_NO_DEVICE: return "no HIP device";
+        case EAR_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case EAR_ERR_HIP: return "HIP runtime error";
+        case EAR_ERR_CAPACITY: return "capacity exceeded";
+        case EAR_ERR_INTERNAL: return "internal error";
+    }
+    return "unknown status";
+}
+
+ear_status ear_ctx_create(int device, ear_ctx** out) {
+    if (!out) return EAR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return EAR_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) return EAR_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    ear_ctx* c = new (std::nothrow) ear_ctx();
+    if (!c) return EAR_ERR_OUT_OF_MEMORY;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return EAR_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    *out = c;
+    return EAR_OK;
+}
+
+ear_status ear_ctx_destroy(ear_ctx* c) {
+    if (!c) return EAR_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return EAR_OK;
+}
+
+ear_status ear_ctx_set_memory_cap(ear_ctx* c, uint64_t bytes) {
+    if (!c) return EAR_ERR_INVALID_ARGUMENT;
+    c->mem_cap = bytes;
+    return EAR_OK;
+}
+
+const char* ear_ctx_last_error(const ear_ctx* c) { return c ? c->last_error.c_str() : ""; }
+
+ear_status ear_ctx_set_tuning(ear_ctx* c, const ear_tuning* t) {
+    if (!c || !t) return EAR_ERR_INVALID_ARGUMENT;
+    c->tuning = *t;
+    return EAR_OK;
+}
+
+ear_status ear_ctx_get_tuning(const ear_ctx* c, ear_tuning* t) {
+    if (!c || !t) return EAR_ERR_INVALID_ARGUMENT;
+    *t = c->tuning;
+    return EAR_OK;
+}
+
+ear_status ear_scene_upload(ear_ctx* c, const float* v0, const float* v1, const float* v2, const uint16_t* mat_id,
+                            uint32_t n_tris, const ear_material* mats, uint32_t n_mat, const ear_bvh_opts* opts,
+                            ear_scene** out) {
+    if (!c || !out) return EAR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    ear_scene* s = new (std::nothrow) ear_scene();
+    if (!s) return EAR_ERR_OUT_OF_MEMORY;
+    s->ctx = c;
+    ear_status st = build_scene(c, v0, v1, v2, mat_id, n_tris, mats, n_mat, opts ? opts->leaf_size : 0, &s->ds);
+    if (st != EAR_OK) {
+        delete s;
+        return st;
+    }
+    *out = s;
+    return EAR_OK;
+}
+
+ear_status ear_scene_destroy(ear_scene* s) {
+    if (!s) return EAR_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(s->ctx->device);
+    free_scene(s->ctx, &s->ds);
+    delete s;
+    return EAR_OK;
+}
+
+ear_status ear_scene_get_info(const ear_scene* s, ear_scene_info* info) {
+    if (!s || !info) return EAR_ERR_INVALID_ARGUMENT;
+    std::memset(info, 0, sizeof(*info));
+    info->n_tris = s->ds.n_tris;
+    info->n_nodes = s->ds.n_tris > 0 ? s->ds.n_nodes : 0;
+    info->max_depth = s->ds.max_depth;
+    info->leaf_size = s->ds.leaf_size;
+    for (int k = 0; k < 6; ++k) info->bounds[k] = s->ds.bounds[k];
+    info->build_ms = s->ds.build_ms;
+    return EAR_OK;
+}
+
+ear_status ear_query_closest(ear_scene* s, uint32_t n, const float* o, const float* d, float* t, int32_t* tri) {
+    if (!s) return EAR_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(s->ctx->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    return run_query_closest(s->ctx, &s->ds, n, o, d, t, tri);
+}
+
+ear_status ear_query_occluded(ear_scene* s, uint32_t n, const float* p, const float* q, uint8_t* occ) {
+    if (!s) return EAR_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(s->ctx->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    return run_query_occluded(s->ctx, &s->ds, n, p, q, occ);
+}
+
+ear_status ear_render(ear_ctx* c, ear_scene* s, const ear_render_desc* d, ear_hist* out, ear_stats* stats) {
+    if (!c || !s || !d) return EAR_ERR_INVALID_ARGUMENT;
+    if (s->ctx != c) {
+        c->last_error = "scene belongs to another context";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    try {
+        return run_render(c, &s->ds, d, out, stats);
+    } catch (const std::bad_alloc&) {
+        c->last_error = "host allocation failed";
+        return EAR_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        c->last_error = "internal exception";
+        return EAR_ERR_INTERNAL;
+    }
+}
+
+}  // extern "C"

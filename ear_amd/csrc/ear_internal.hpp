@@ -1,0 +1,123 @@
+// ear_internal.hpp -- structures shared by the translation units of libear_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ear.h"
+#include "ear_math.hpp"
+
+namespace ear {
+
+// BVH2 node, 64 B: both children's padded AABBs and child references.
+//   a = {lo0.x, lo0.y, lo0.z, hi0.x}  b = {hi0.y, hi0.z, lo1.x, lo1.y}
+//   c = {lo1.z, hi1.x, hi1.y, hi1.z}  d = {child0, child1, -, -}
+// child >= 0: internal node index; child < 0: leaf
+//   0x80000000 | (count - 1) << 24 | first   (first < 2^24, count <= 128)
+// kEmptyChild: a child slot whose box is empty (single-triangle scenes).
+struct alignas(64) BvhNode {
+    float4 a, b, c;
+    int4 d;
+};
+constexpr int32_t kEmptyChild = (int32_t)0x80000000u;
+constexpr uint32_t kMaxTris = 1u << 24;
+constexpr int kStackDepth = 64;   // > max tree depth (30 Morton bits + 24 index bits)
+constexpr int kMaxRecorders = 16;
+constexpr int kMaxChannels = 32;
+
+// Triangles in BVH (Morton) order, 48 B each:
+//   t0 = {v0.x, v0.y, v0.z, e1.x}  t1 = {e1.y, e1.z, e2.x, e2.y}  t2 = {e2.z, orig, -, -}
+// plus nm = {n.x, n.y, n.z, material} with n = normalize(cross(e1, e2)).
+
+struct DeviceScene {
+    uint32_t n_tris = 0;
+    uint32_t n_nodes = 0;
+    uint32_t n_mat = 0;
+    uint32_t leaf_size = 4;
+    uint32_t max_depth = 0;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};
+    double build_ms = 0;
+    BvhNode* nodes = nullptr;
+    float4* tris = nullptr;  // 3 * n_tris
+    float4* nm = nullptr;    // n_tris
+    // per (material, band): {fl2 (or -1: no draw), spec, retention, -}
+    float4* mat_band = nullptr;
+    std::vector<float4> mat_band_host;
+};
+
+struct Ctx;
+struct SceneHandle {
+    Ctx* ctx = nullptr;
+    DeviceScene ds;
+};
+
+// Launch parameters of one render (passed by value to kernels).
+struct RecParams {
+    float px, py, pz;
+    float rx, ry, rz;  // right ear
+    float head_size;
+    float head_abs[3];
+    int32_t kind;
+    int32_t channel;  // first channel
+};
+
+struct RenderParams {
+    uint64_t seed;
+    uint32_t instance;
+    uint32_t ctx_word[3];  // Philox counter word 2 per band
+    float air[3];
+    float dry;
+    double cutoff;
+    uint32_t max_bounces;
+    uint32_t n_bins;
+    uint32_t n_channels;
+    uint32_t n_rec;
+    uint32_t n_band_slots;  // always 3 (histograms laid out per band slot)
+    uint32_t hist_kinds;    // 1 signed, 2 signed + abs
+    int32_t src_kind;
+    float src_pos[3];
+    uint32_t src_n_tris;
+    float src_total_area;
+    uint32_t trace_rays, trace_segs;
+    uint32_t flags;
+    RecParams rec[kMaxRecorders];
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t mem_cap = 0;
+    uint64_t mem_used = 0;
+    int num_cus = 256;
+    std::string last_error;
+    ear_tuning tuning{};
+};
+
+// memory helpers honouring the context's cap
+ear_status dev_alloc(Ctx* c, void** p, size_t bytes);
+void dev_free(Ctx* c, void* p, size_t bytes);
+ear_status hip_status(Ctx* c, hipError_t e, const char* what);
+
+// lbvh.hip
+ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2, const uint16_t* mat_id,
+                       uint32_t n_tris, const ear_material* mats, uint32_t n_mat, uint32_t leaf_size,
+                       DeviceScene* out);
+void free_scene(Ctx* c, DeviceScene* s);
+
+// tracer.hip
+ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* d, ear_hist* out, ear_stats* st);
+ear_status run_query_closest(Ctx* c, DeviceScene* s, uint32_t n, const float* o, const float* d, float* t,
+                             int32_t* tri);
+ear_status run_query_occluded(Ctx* c, DeviceScene* s, uint32_t n, const float* p, const float* q,
+                              uint8_t* occ);
+
+#define EAR_HIP_TRY(ctx, expr)                                      \
+    do {                                                            \
+        hipError_t _e = (expr);                                     \
+        if (_e != hipSuccess) return ::ear::hip_status(ctx, _e, #expr); \
+    } while (0)
+
+}  // namespace ear

@@ -1,0 +1,593 @@
+// lbvh.hip -- device build of the linear BVH over the scene's triangle soup.
+//
+// Replaces the reference's absence of an acceleration structure: the
+// reference scans every triangle per query (src/Mesh.cpp:39-45, 61-69; the
+// missing grid is noted at src/Scene.h:37-41).  Build:
+//   1. per-triangle prep: e1, e2, unit normal, AABB, centroid
+//   2. centroid bounds (two-level reduction)
+//   3. 30-bit Morton codes of centroids
+//   4. stable LSD radix sort, 4-bit digits, 8 passes (own implementation)
+//   5. Karras (HPG 2012) binary radix tree, duplicate keys split by index
+//   6. bottom-up AABB refit with per-node arrival counters
+//   7. emit 64-byte BVH2 nodes (both child boxes in the parent, padded so the
+//      box test never rejects a hit the exact triangle test accepts); any
+//      subtree of at most leaf_size triangles becomes one leaf
+//   8. gather triangles into Morton order as (v0, e1, e2, original index)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ear_internal.hpp"
+
+namespace ear {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr uint32_t kLeafFlag = 0x80000000u;
+
+__global__ void k_prep(uint32_t n, const float* __restrict__ v0, const float* __restrict__ v1,
+                       const float* __restrict__ v2, float* __restrict__ box, float* __restrict__ cen) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a[3] = {v0[3 * i], v0[3 * i + 1], v0[3 * i + 2]};
+    float b[3] = {v1[3 * i], v1[3 * i + 1], v1[3 * i + 2]};
+    float c[3] = {v2[3 * i], v2[3 * i + 1], v2[3 * i + 2]};
+    for (int k = 0; k < 3; ++k) {
+        float lo = fminf(a[k], fminf(b[k], c[k]));
+        float hi = fmaxf(a[k], fmaxf(b[k], c[k]));
+        box[6 * i + k] = lo;
+        box[6 * i + 3 + k] = hi;
+        cen[3 * i + k] = 0.5f * lo + 0.5f * hi;
+    }
+}
+
+// two-level min/max reduction of centroid (6 floats) and vertex box (6 floats)
+__global__ void k_bounds_partial(uint32_t n, const float* __restrict__ box, const float* __restrict__ cen,
+                                 float* __restrict__ part) {
+    __shared__ float s[12][256];
+    float r[12];
+    for (int k = 0; k < 3; ++k) {
+        r[k] = INFINITY;
+        r[3 + k] = -INFINITY;
+        r[6 + k] = INFINITY;
+        r[9 + k] = -INFINITY;
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            r[k] = fminf(r[k], cen[3 * i + k]);
+            r[3 + k] = fmaxf(r[3 + k], cen[3 * i + k]);
+            r[6 + k] = fminf(r[6 + k], box[6 * i + k]);
+            r[9 + k] = fmaxf(r[9 + k], box[6 * i + 3 + k]);
+        }
+    }
+    for (int k = 0; k < 12; ++k) s[k][threadIdx.x] = r[k];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            for (int k = 0; k < 12; ++k) {
+                float o = s[k][threadIdx.x + st];
+                bool mx = (k >= 3 && k < 6) || k >= 9;
+                s[k][threadIdx.x] = mx ? fmaxf(s[k][threadIdx.x], o) : fminf(s[k][threadIdx.x], o);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 12) part[blockIdx.x * 12 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void k_morton(uint32_t n, const float* __restrict__ cen, const float* __restrict__ bnd,
+                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        float lo = bnd[k], hi = bnd[3 + k];
+        float ext = hi - lo;
+        float f = ext > 0.0f ? (cen[3 * i + k] - lo) / ext : 0.0f;
+        f = fminf(fmaxf(f * 1024.0f, 0.0f), 1023.0f);
+        q[k] = (uint32_t)f;
+    }
+    keys[i] = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+    vals[i] = i;
+}
+
+// ---- radix sort ---------------------------------------------------------
+__global__ void __launch_bounds__(kSortThreads) k_radix_count(uint32_t n, const uint32_t* __restrict__ keys,
+                                                             uint32_t shift, uint32_t* __restrict__ counts,
+                                                             uint32_t nblocks) {
+    __shared__ uint32_t h[16];
+    if (threadIdx.x < 16) h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kSortTile + threadIdx.x * kSortItems;
+    for (int j = 0; j < kSortItems; ++j) {
+        uint32_t i = base + j;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 15u], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) counts[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// single-block exclusive scan of m uint32 values (in place)
+__global__ void __launch_bounds__(1024) k_scan_u32(uint32_t m, uint32_t* __restrict__ a) {
+    __shared__ uint32_t s[1024];
+    uint32_t per = (m + 1023) / 1024;
+    uint32_t b = threadIdx.x * per, e = min(m, b + per);
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; ++i) sum += a[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (uint32_t i = b; i < e; ++i) {
+        uint32_t x = a[i];
+        a[i] = run;
+        run += x;
+    }
+}
+
+__global__ void __launch_bounds__(kSortThreads) k_radix_scatter(uint32_t n, const uint32_t* __restrict__ kin,
+                                                               const uint32_t* __restrict__ vin,
+                                                               uint32_t* __restrict__ kout,
+                                                               uint32_t* __restrict__ vout, uint32_t shift,
+                                                               const uint32_t* __restrict__ offs,
+                                                               uint32_t nblocks) {
+    __shared__ uint32_t pre[16][kSortThreads];
+    uint32_t base = blockIdx.x * kSortTile + threadIdx.x * kSortItems;
+    uint32_t k[kSortItems], v[kSortItems];
+    uint32_t cnt[16];
+    for (int d = 0; d < 16; ++d) cnt[d] = 0;
+    for (int j = 0; j < kSortItems; ++j) {
+        uint32_t i = base + j;
+        if (i < n) {
+            k[j] = kin[i];
+            v[j] = vin[i];
+            cnt[(k[j] >> shift) & 15u]++;
+        }
+    }
+    for (int d = 0; d < 16; ++d) pre[d][threadIdx.x] = cnt[d];
+    __syncthreads();
+    // inclusive Hillis-Steele scan over threads for each digit
+    for (int off = 1; off < kSortThreads; off <<= 1) {
+        uint32_t t[16];
+        for (int d = 0; d < 16; ++d) t[d] = threadIdx.x >= (uint32_t)off ? pre[d][threadIdx.x - off] : 0;
+        __syncthreads();
+        for (int d = 0; d < 16; ++d) pre[d][threadIdx.x] += t[d];
+        __syncthreads();
+    }
+    uint32_t run[16];
+    for (int d = 0; d < 16; ++d) run[d] = offs[d * nblocks + blockIdx.x] + pre[d][threadIdx.x] - cnt[d];
+    for (int j = 0; j < kSortItems; ++j) {
+        uint32_t i = base + j;
+        if (i < n) {
+            uint32_t d = (k[j] >> shift) & 15u;
+            uint32_t pos = run[d]++;
+            kout[pos] = k[j];
+            vout[pos] = v[j];
+        }
+    }
+}
+
+// ---- Karras 2012 hierarchy ---------------------------------------------
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = keys[i], b = keys[j];
+    if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+    return __clz(a ^ b);
+}
+
+__global__ void k_karras(int n, const uint32_t* __restrict__ keys, uint32_t* __restrict__ left,
+                         uint32_t* __restrict__ right, int* __restrict__ first, int* __restrict__ last,
+                         int* __restrict__ parent_int, int* __restrict__ parent_leaf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0;
+    int div = 2;
+    int t = (l + div - 1) / div;
+    while (true) {
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t == 1) break;
+        div <<= 1;
+        t = (l + div - 1) / div;
+    }
+    int g = i + s * d + (d < 0 ? -1 : 0);
+    int lo = min(i, j), hi = max(i, j);
+    uint32_t lc = (lo == g) ? (kLeafFlag | (uint32_t)g) : (uint32_t)g;
+    uint32_t rc = (hi == g + 1) ? (kLeafFlag | (uint32_t)(g + 1)) : (uint32_t)(g + 1);
+    left[i] = lc;
+    right[i] = rc;
+    first[i] = lo;
+    last[i] = hi;
+    if (lc & kLeafFlag)
+        parent_leaf[g] = i;
+    else
+        parent_int[g] = i;
+    if (rc & kLeafFlag)
+        parent_leaf[g + 1] = i;
+    else
+        parent_int[g + 1] = i;
+    if (i == 0) parent_int[0] = -1;
+}
+
+__device__ __forceinline__ float coherent_load(const float* p) {
+    uint32_t u = __hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __uint_as_float(u);
+}
+
+// bottom-up refit: box of internal node = union of children
+__global__ void k_refit(int n, const uint32_t* __restrict__ vals, const float* __restrict__ tri_box,
+                        const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                        const int* __restrict__ parent_int, const int* __restrict__ parent_leaf,
+                        float* node_box, uint32_t* __restrict__ arrive) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int node = parent_leaf[s];
+    while (node >= 0) {
+        __threadfence();
+        if (atomicAdd(&arrive[node], 1u) == 0) return;
+        __threadfence();
+        float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        uint32_t ch[2] = {left[node], right[node]};
+        for (int c = 0; c < 2; ++c) {
+            const float* src;
+            bool coherent;
+            if (ch[c] & kLeafFlag) {
+                src = tri_box + 6 * (size_t)vals[ch[c] & ~kLeafFlag];
+                coherent = false;
+            } else {
+                src = node_box + 6 * (size_t)ch[c];
+                coherent = true;
+            }
+            for (int k = 0; k < 3; ++k) {
+                float lo = coherent ? coherent_load(src + k) : src[k];
+                float hi = coherent ? coherent_load(src + 3 + k) : src[3 + k];
+                b[k] = fminf(b[k], lo);
+                b[3 + k] = fmaxf(b[3 + k], hi);
+            }
+        }
+        for (int k = 0; k < 6; ++k)
+            __hip_atomic_store((uint32_t*)(node_box + 6 * (size_t)node + k), __float_as_uint(b[k]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        node = parent_int[node];
+    }
+}
+
+__global__ void k_depth(int n, const int* __restrict__ parent_int, const int* __restrict__ parent_leaf,
+                        uint32_t* __restrict__ max_depth) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint32_t dpt = 1;
+    int node = parent_leaf[s];
+    while (node >= 0) {
+        ++dpt;
+        node = parent_int[node];
+    }
+    atomicMax(max_depth, dpt);
+}
+
+__device__ __forceinline__ void padded(const float* b, float pad, float out[6]) {
+    for (int k = 0; k < 3; ++k) {
+        out[k] = b[k] - (pad + fabsf(b[k]) * 1e-6f);
+        out[3 + k] = b[3 + k] + (pad + fabsf(b[3 + k]) * 1e-6f);
+    }
+}
+
+__global__ void k_emit(int n, uint32_t leaf_size, float pad, const uint32_t* __restrict__ vals,
+                       const float* __restrict__ tri_box, const uint32_t* __restrict__ left,
+                       const uint32_t* __restrict__ right, const int* __restrict__ first,
+                       const int* __restrict__ last, const float* __restrict__ node_box,
+                       BvhNode* __restrict__ nodes) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    uint32_t ch[2] = {left[i], right[i]};
+    float bx[2][6];
+    int32_t ref[2];
+    for (int c = 0; c < 2; ++c) {
+        if (ch[c] & kLeafFlag) {
+            uint32_t s = ch[c] & ~kLeafFlag;
+            padded(tri_box + 6 * (size_t)vals[s], pad, bx[c]);
+            ref[c] = (int32_t)(kLeafFlag | s);
+        } else {
+            uint32_t m = ch[c];
+            padded(node_box + 6 * (size_t)m, pad, bx[c]);
+            int cnt = last[m] - first[m] + 1;
+            if ((uint32_t)cnt <= leaf_size)
+                ref[c] = (int32_t)(kLeafFlag | ((uint32_t)(cnt - 1) << 24) | (uint32_t)first[m]);
+            else
+                ref[c] = (int32_t)m;
+        }
+    }
+    BvhNode nd;
+    nd.a = make_float4(bx[0][0], bx[0][1], bx[0][2], bx[0][3]);
+    nd.b = make_float4(bx[0][4], bx[0][5], bx[1][0], bx[1][1]);
+    nd.c = make_float4(bx[1][2], bx[1][3], bx[1][4], bx[1][5]);
+    nd.d = make_int4(ref[0], ref[1], 0, 0);
+    nodes[i] = nd;
+}
+
+__global__ void k_gather(uint32_t n, const uint32_t* __restrict__ vals, const float* __restrict__ v0,
+                         const float* __restrict__ v1, const float* __restrict__ v2,
+                         const uint16_t* __restrict__ mat, float4* __restrict__ tris, float4* __restrict__ nm) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint32_t o = vals[s];
+    V3 a{v0[3 * o], v0[3 * o + 1], v0[3 * o + 2]};
+    V3 b{v1[3 * o], v1[3 * o + 1], v1[3 * o + 2]};
+    V3 c{v2[3 * o], v2[3 * o + 1], v2[3 * o + 2]};
+    V3 e1 = sub(b, a), e2 = sub(c, a);
+    V3 nrm = normalized(cross(e1, e2));
+    tris[3 * s + 0] = make_float4(a.x, a.y, a.z, e1.x);
+    tris[3 * s + 1] = make_float4(e1.y, e1.z, e2.x, e2.y);
+    tris[3 * s + 2] = make_float4(e2.z, __uint_as_float(o), 0.0f, 0.0f);
+    nm[s] = make_float4(nrm.x, nrm.y, nrm.z, __uint_as_float((uint32_t)mat[o]));
+}
+
+struct Buf {
+    Ctx* c;
+    void* p = nullptr;
+    size_t n = 0;
+    explicit Buf(Ctx* cc) : c(cc) {}
+    ~Buf() {
+        if (p) dev_free(c, p, n);
+    }
+    ear_status alloc(size_t bytes) {
+        n = bytes ? bytes : 16;
+        return dev_alloc(c, &p, n);
+    }
+    template <class T>
+    T* as() {
+        return (T*)p;
+    }
+};
+
+}  // namespace
+
+void free_scene(Ctx* c, DeviceScene* s) {
+    if (s->nodes) dev_free(c, s->nodes, sizeof(BvhNode) * std::max<uint32_t>(1, s->n_nodes));
+    if (s->tris) dev_free(c, s->tris, sizeof(float4) * 3 * std::max<uint32_t>(1, s->n_tris));
+    if (s->nm) dev_free(c, s->nm, sizeof(float4) * std::max<uint32_t>(1, s->n_tris));
+    if (s->mat_band) dev_free(c, s->mat_band, sizeof(float4) * 3 * std::max<uint32_t>(1, s->n_mat));
+    s->nodes = nullptr;
+    s->tris = s->nm = s->mat_band = nullptr;
+}
+
+#define TRY(x)                               \
+    do {                                     \
+        ear_status _s = (x);                 \
+        if (_s != EAR_OK) return _s;         \
+    } while (0)
+
+ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2, const uint16_t* mat_id,
+                       uint32_t n, const ear_material* mats, uint32_t n_mat, uint32_t leaf_size,
+                       DeviceScene* out) {
+    if (n > 0 && (!v0 || !v1 || !v2 || !mat_id)) {
+        c->last_error = "null triangle array";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (n_mat > 0 && !mats) {
+        c->last_error = "null material array";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (n >= kMaxTris) {
+        c->last_error = "more than 2^24 - 1 triangles";
+        return EAR_ERR_CAPACITY;
+    }
+    if (leaf_size == 0) leaf_size = 4;
+    if (leaf_size > 16) {
+        c->last_error = "leaf_size must be 1..16";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    // materials: Material ctor (src/Material.cpp:26-72) and Material::Bounce
+    // (src/Material.cpp:76-83) folded into one row per band
+    std::vector<float4> mb(3 * (size_t)std::max<uint32_t>(1, n_mat));
+    for (uint32_t m = 0; m < n_mat; ++m) {
+        for (int b = 0; b < 3; ++b) {
+            float fl = mats[m].refl[b];
+            float fr = (mats[m].flags & EAR_MAT_HAS_REFR) ? mats[m].refr[b] : 0.0f;
+            float sp = (mats[m].flags & EAR_MAT_HAS_SPEC) ? mats[m].spec[b] : 0.0f;
+            float a = 1.0f;
+            a -= fl - 1e-9f;
+            if (mats[m].flags & EAR_MAT_HAS_REFR) a -= fr - 1e-9f;
+            if (!(a >= 0.0f)) {
+                c->last_error = "Invalid material settings (refl + refr > 1)";
+                return EAR_ERR_INVALID_MATERIAL;
+            }
+            float ret = 1.0f - a;
+            float fl2 = ((double)fl < 0.0001 && (double)fr < 0.0001) ? -1.0f : fl / (fl + fr);
+            mb[3 * m + b] = make_float4(fl2, sp, ret, fr);
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        if (mat_id[i] >= n_mat) {
+            c->last_error = "material id out of range";
+            return EAR_ERR_INVALID_GEOMETRY;
+        }
+    }
+    for (size_t i = 0; i < 3 * (size_t)n; ++i) {
+        if (!std::isfinite(v0[i]) || !std::isfinite(v1[i]) || !std::isfinite(v2[i])) {
+            c->last_error = "non-finite vertex coordinate";
+            return EAR_ERR_INVALID_GEOMETRY;
+        }
+    }
+
+    DeviceScene s;
+    s.n_tris = n;
+    s.n_mat = n_mat;
+    s.leaf_size = leaf_size;
+    s.mat_band_host = mb;
+    TRY(dev_alloc(c, (void**)&s.mat_band, sizeof(float4) * mb.size()));
+    hipStream_t st = c->stream;
+    EAR_HIP_TRY(c, hipMemcpyAsync(s.mat_band, mb.data(), sizeof(float4) * mb.size(), hipMemcpyHostToDevice, st));
+    if (n == 0) {
+        EAR_HIP_TRY(c, hipStreamSynchronize(st));
+        *out = s;
+        return EAR_OK;
+    }
+
+    hipEvent_t e0, e1;
+    EAR_HIP_TRY(c, hipEventCreate(&e0));
+    EAR_HIP_TRY(c, hipEventCreate(&e1));
+
+    Buf dv0(c), dv1(c), dv2(c), dmat(c), box(c), cen(c), part(c), bnd(c), keys(c), vals(c), keys2(c), vals2(c),
+        counts(c), left(c), right(c), first(c), last(c), pint(c), pleaf(c), nbox(c), arrive(c), depth(c);
+    size_t n3 = 3 * (size_t)n;
+    TRY(dv0.alloc(n3 * 4));
+    TRY(dv1.alloc(n3 * 4));
+    TRY(dv2.alloc(n3 * 4));
+    TRY(dmat.alloc((size_t)n * 2));
+    EAR_HIP_TRY(c, hipMemcpyAsync(dv0.p, v0, n3 * 4, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(dv1.p, v1, n3 * 4, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(dv2.p, v2, n3 * 4, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(dmat.p, mat_id, (size_t)n * 2, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipEventRecord(e0, st));
+
+    const int nb256 = (int)((n + 255) / 256);
+    TRY(box.alloc((size_t)n * 6 * 4));
+    TRY(cen.alloc(n3 * 4));
+    k_prep<<<nb256, 256, 0, st>>>(n, dv0.as<float>(), dv1.as<float>(), dv2.as<float>(), box.as<float>(),
+                                  cen.as<float>());
+    int npart = std::min(nb256, 1024);
+    TRY(part.alloc((size_t)npart * 12 * 4));
+    TRY(bnd.alloc(12 * 4));
+    k_bounds_partial<<<npart, 256, 0, st>>>(n, box.as<float>(), cen.as<float>(), part.as<float>());
+    k_bounds_partial<<<1, 256, 0, st>>>((uint32_t)npart, part.as<float>() /*unused as box*/, part.as<float>(),
+                                        bnd.as<float>());
+    // The second pass above reduced only centroid-style triples; redo the
+    // vertex box on the host from the partials for exactness of the info.
+    std::vector<float> hpart((size_t)npart * 12);
+    EAR_HIP_TRY(c, hipMemcpyAsync(hpart.data(), part.p, hpart.size() * 4, hipMemcpyDeviceToHost, st));
+    EAR_HIP_TRY(c, hipStreamSynchronize(st));
+    float hb[12];
+    for (int k = 0; k < 3; ++k) {
+        hb[k] = INFINITY;
+        hb[3 + k] = -INFINITY;
+        hb[6 + k] = INFINITY;
+        hb[9 + k] = -INFINITY;
+    }
+    for (int p = 0; p < npart; ++p)
+        for (int k = 0; k < 3; ++k) {
+            hb[k] = std::min(hb[k], hpart[12 * p + k]);
+            hb[3 + k] = std::max(hb[3 + k], hpart[12 * p + 3 + k]);
+            hb[6 + k] = std::min(hb[6 + k], hpart[12 * p + 6 + k]);
+            hb[9 + k] = std::max(hb[9 + k], hpart[12 * p + 9 + k]);
+        }
+    EAR_HIP_TRY(c, hipMemcpyAsync(bnd.p, hb, 12 * 4, hipMemcpyHostToDevice, st));
+    for (int k = 0; k < 3; ++k) {
+        s.bounds[k] = hb[6 + k];
+        s.bounds[3 + k] = hb[9 + k];
+    }
+    float ext = 0.0f;
+    for (int k = 0; k < 3; ++k) ext = std::max(ext, hb[9 + k] - hb[6 + k]);
+    float pad = 1e-5f * ext + 1e-6f;
+
+    TRY(keys.alloc((size_t)n * 4));
+    TRY(vals.alloc((size_t)n * 4));
+    TRY(keys2.alloc((size_t)n * 4));
+    TRY(vals2.alloc((size_t)n * 4));
+    k_morton<<<nb256, 256, 0, st>>>(n, cen.as<float>(), bnd.as<float>(), keys.as<uint32_t>(), vals.as<uint32_t>());
+    uint32_t nsb = (n + kSortTile - 1) / kSortTile;
+    TRY(counts.alloc((size_t)nsb * 16 * 4));
+    uint32_t *ka = keys.as<uint32_t>(), *va = vals.as<uint32_t>(), *kb = keys2.as<uint32_t>(),
+             *vb = vals2.as<uint32_t>();
+    for (uint32_t shift = 0; shift < 32; shift += 4) {
+        k_radix_count<<<nsb, kSortThreads, 0, st>>>(n, ka, shift, counts.as<uint32_t>(), nsb);
+        k_scan_u32<<<1, 1024, 0, st>>>(nsb * 16, counts.as<uint32_t>());
+        k_radix_scatter<<<nsb, kSortThreads, 0, st>>>(n, ka, va, kb, vb, shift, counts.as<uint32_t>(), nsb);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    // after 8 passes the sorted data is back in keys/vals (ka, va)
+
+    uint32_t n_int = n > 1 ? n - 1 : 1;
+    TRY(left.alloc((size_t)n_int * 4));
+    TRY(right.alloc((size_t)n_int * 4));
+    TRY(first.alloc((size_t)n_int * 4));
+    TRY(last.alloc((size_t)n_int * 4));
+    TRY(pint.alloc((size_t)n_int * 4));
+    TRY(pleaf.alloc((size_t)n * 4));
+    TRY(nbox.alloc((size_t)n_int * 6 * 4));
+    TRY(arrive.alloc((size_t)n_int * 4));
+    TRY(depth.alloc(4));
+    EAR_HIP_TRY(c, hipMemsetAsync(arrive.p, 0, (size_t)n_int * 4, st));
+    EAR_HIP_TRY(c, hipMemsetAsync(depth.p, 0, 4, st));
+
+    s.n_nodes = n_int;
+    TRY(dev_alloc(c, (void**)&s.nodes, sizeof(BvhNode) * n_int));
+    TRY(dev_alloc(c, (void**)&s.tris, sizeof(float4) * 3 * (size_t)n));
+    TRY(dev_alloc(c, (void**)&s.nm, sizeof(float4) * (size_t)n));
+
+    if (n == 1) {
+        // one triangle: a root whose second child is a far-away empty box
+        std::vector<float> tb(6);
+        EAR_HIP_TRY(c, hipMemcpyAsync(tb.data(), box.p, 24, hipMemcpyDeviceToHost, st));
+        EAR_HIP_TRY(c, hipStreamSynchronize(st));
+        BvhNode nd;
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = tb[k] - (pad + std::fabs(tb[k]) * 1e-6f);
+            hi[k] = tb[3 + k] + (pad + std::fabs(tb[3 + k]) * 1e-6f);
+        }
+        nd.a = make_float4(lo[0], lo[1], lo[2], hi[0]);
+        nd.b = make_float4(hi[1], hi[2], 1e30f, 1e30f);
+        nd.c = make_float4(1e30f, 1e30f, 1e30f, 1e30f);
+        nd.d = make_int4((int32_t)kLeafFlag, kEmptyChild, 0, 0);
+        EAR_HIP_TRY(c, hipMemcpyAsync(s.nodes, &nd, sizeof(nd), hipMemcpyHostToDevice, st));
+        s.max_depth = 1;
+    } else {
+        int nbi = (int)((n_int + 255) / 256);
+        k_karras<<<nbi, 256, 0, st>>>((int)n, ka, left.as<uint32_t>(), right.as<uint32_t>(), first.as<int>(),
+                                      last.as<int>(), pint.as<int>(), pleaf.as<int>());
+        k_refit<<<nb256, 256, 0, st>>>((int)n, va, box.as<float>(), left.as<uint32_t>(), right.as<uint32_t>(),
+                                       pint.as<int>(), pleaf.as<int>(), nbox.as<float>(), arrive.as<uint32_t>());
+        k_depth<<<nb256, 256, 0, st>>>((int)n, pint.as<int>(), pleaf.as<int>(), depth.as<uint32_t>());
+        k_emit<<<nbi, 256, 0, st>>>((int)n, leaf_size, pad, va, box.as<float>(), left.as<uint32_t>(),
+                                    right.as<uint32_t>(), first.as<int>(), last.as<int>(), nbox.as<float>(),
+                                    s.nodes);
+        EAR_HIP_TRY(c, hipMemcpyAsync(&s.max_depth, depth.p, 4, hipMemcpyDeviceToHost, st));
+    }
+    k_gather<<<nb256, 256, 0, st>>>(n, va, dv0.as<float>(), dv1.as<float>(), dv2.as<float>(), dmat.as<uint16_t>(),
+                                    s.tris, s.nm);
+    EAR_HIP_TRY(c, hipGetLastError());
+    EAR_HIP_TRY(c, hipEventRecord(e1, st));
+    EAR_HIP_TRY(c, hipStreamSynchronize(st));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    s.build_ms = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (s.max_depth >= (uint32_t)kStackDepth) {
+        free_scene(c, &s);
+        c->last_error = "BVH deeper than the traversal stack";
+        return EAR_ERR_CAPACITY;
+    }
+    *out = s;
+    return EAR_OK;
+}
+
+}  // namespace ear

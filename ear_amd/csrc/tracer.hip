@@ -1,0 +1,1132 @@
+// tracer.hip -- wavefront Monte-Carlo acoustic tracer (gfx950).
+//
+// One render replaces Scene::Render (src/Scene.cpp:111-318) for every band of
+// ear_render_desc.band_mask at once; each band is its own reference context
+// (SceneContext, src/SceneContext.h:26-54) with its own Philox streams.  The
+// reference's per-ray loop over num_bounces (src/Scene.cpp:143-281) becomes a
+// loop over wavefront iterations, each a chain of kernels over a compacted
+// queue of live rays:
+//   k_generate  emission, k = 0 (SoundRay, src/SoundFile.cpp:215-228; mesh
+//               sources also connect to the recorders at k = 0)
+//   k_extend    closest hit, BVH traversal (Mesh::RayIntersection)
+//   k_shade     Scene::Bounce + Material::Bounce + energy bookkeeping + the
+//               NEE weight of src/Scene.cpp:197-261; emits shadow requests and
+//               the next ray (Sample_Hemi) into the next queue
+//   k_shadow    any hit on the segment to the recorder (Mesh::LineIntersection)
+//               and, when visible, Recorder::Record as a fixed-point ramp splat
+//   k_finalize  exact prefix sums of the second-difference histograms,
+//               1/N, direct sound (src/Scene.cpp:286-316), gain^2
+// Histograms are 128-bit two's-complement fixed point (2^-88 units).  A ramp of
+// width w and slope S (MonoRecorder::Record, src/MonoRecorder.cpp:83-97) is
+// stored as three second-difference points (+wS at s, -(w+1)S at s+1, +S at
+// s+w+1); integer sums are order independent, so results are bitwise
+// deterministic whatever the launch shape.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "ear_internal.hpp"
+
+namespace ear {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kLdsDepth = 32;  // stack entries in LDS; deeper entries go to a global overflow area
+
+enum StatIdx {
+    ST_SEGMENTS = 0,
+    ST_SHADOW = 1,
+    ST_CONTRIB = 2,
+    ST_NV_CL = 3,
+    ST_NT_CL = 4,
+    ST_NV_SH = 5,
+    ST_NT_SH = 6,
+    ST_RAYS = 7,
+    ST_N = 8
+};
+
+struct Queue {
+    float4* o;  // {ox, oy, oz, I}
+    float4* d;  // {dx, dy, dz, L}
+    uint4* m;   // {ray id, draw counter, band | k << 8, -}
+};
+
+struct ShadowQ {
+    float4* p;  // {px, py, pz, c}
+    float4* m;  // {L, rec | band << 8, -, -}
+};
+
+struct Traversal {
+    const BvhNode* __restrict__ nodes;
+    const float4* __restrict__ tris;
+    uint32_t n_tris;
+    int* gstack;        // overflow stack [(kStackDepth - kLdsDepth)][gthreads]
+    uint32_t gthreads;  // persistent threads sharing gstack
+};
+
+// ---------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Appends the lanes with pred set; returns each such lane's slot.  All lanes
+// of the wave must call it.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    uint32_t base = 0;
+    if (mask) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        if ((int)lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    uint64_t below = mask & ((1ull << lane_id()) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint64_t v) {
+    // reduce within the wave, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (lane_id() == 0 && v) atomicAdd(counter, (unsigned long long)v);
+}
+
+// ---------------------------------------------------------------------------
+// BVH traversal
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float safe_inv(float x) {
+    float y = fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x);
+    return 1.0f / y;
+}
+
+struct SlabRay {
+    float ox, oy, oz, ix, iy, iz;
+};
+
+// entry/exit of the ray against box {lo, hi}; robust form (lo - o) * inv
+__device__ __forceinline__ void slab(const SlabRay& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                     float& tn, float& tf) {
+    float ax = (lx - r.ox) * r.ix, bx = (hx - r.ox) * r.ix;
+    float ay = (ly - r.oy) * r.iy, by = (hy - r.oy) * r.iy;
+    float az = (lz - r.oz) * r.iz, bz = (hz - r.oz) * r.iz;
+    tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+}
+
+// Closest hit (ANY = false): t in (0.001, best_t) with the lower original index
+// winning ties, as the reference's strict '<' scan does (src/Mesh.cpp:39-45).
+// Any hit (ANY = true): segment parameter t in (1e-5, 1) (src/Mesh.cpp:58-71).
+// Returns true on a hit; best_* hold the closest hit.
+template <bool ANY, bool INSTR>
+__device__ bool traverse(const Traversal& T, V3 o, V3 d, float& best_t, int& best_s, uint32_t& best_o, int* lds,
+                         uint32_t gtid, uint32_t& nvis, uint32_t& ntri) {
+    if (T.n_tris == 0) return false;
+    SlabRay sr{o.x, o.y, o.z, safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    int sp = 0;
+    int node = 0;
+    bool found = false;
+    for (;;) {
+        const BvhNode* N = T.nodes + node;
+        float4 a = N->a, b = N->b, c = N->c;
+        int4 ch = N->d;
+        if (INSTR) ++nvis;
+        float tn0, tf0, tn1, tf1;
+        slab(sr, a.x, a.y, a.z, a.w, b.x, b.y, tn0, tf0);
+        slab(sr, b.z, b.w, c.x, c.y, c.z, c.w, tn1, tf1);
+        float lim = ANY ? 1.0f : best_t;
+        bool h0 = tn0 <= tf0 && tf0 >= 0.0f && tn0 <= lim;
+        bool h1 = tn1 <= tf1 && tf1 >= 0.0f && tn1 <= lim;
+        // leaf children are tested right away
+        for (int k = 0; k < 2; ++k) {
+            int cref = k == 0 ? ch.x : ch.y;
+            bool hk = k == 0 ? h0 : h1;
+            if (hk && cref < 0) {
+                uint32_t u = (uint32_t)cref;
+                uint32_t first = u & 0x00FFFFFFu;
+                uint32_t cnt = ((u >> 24) & 0x7Fu) + 1;
+                for (uint32_t i = first; i < first + cnt; ++i) {
+                    float4 t0 = T.tris[3 * i], t1 = T.tris[3 * i + 1], t2 = T.tris[3 * i + 2];
+                    V3 v0{t0.x, t0.y, t0.z}, e1{t0.w, t1.x, t1.y}, e2{t1.z, t1.w, t2.x};
+                    float t;
+                    if (INSTR) ++ntri;
+                    if (mt_intersect(v0, e1, e2, o, d, t)) {
+                        if (ANY) {
+                            if (t > 1e-5f && t < 1.0f) return true;
+                        } else {
+                            uint32_t orig = __float_as_uint(t2.y);
+                            if (t > 0.001f && (t < best_t || (t == best_t && orig < best_o))) {
+                                best_t = t;
+                                best_s = (int)i;
+                                best_o = orig;
+                                found = true;
+                            }
+                        }
+                    }
+                }
+                if (k == 0)
+                    h0 = false;
+                else
+                    h1 = false;
+            }
+        }
+        if (h0 && h1) {
+            bool first0 = tn0 <= tn1;
+            int nearc = first0 ? ch.x : ch.y;
+            int farc = first0 ? ch.y : ch.x;
+            if (sp < kLdsDepth)
+                lds[sp * kBlock] = farc;
+            else
+                T.gstack[(size_t)(sp - kLdsDepth) * T.gthreads + gtid] = farc;
+            ++sp;
+            node = nearc;
+            continue;
+        }
+        if (h0) {
+            node = ch.x;
+            continue;
+        }
+        if (h1) {
+            node = ch.y;
+            continue;
+        }
+        if (sp == 0) break;
+        --sp;
+        node = sp < kLdsDepth ? lds[sp * kBlock] : T.gstack[(size_t)(sp - kLdsDepth) * T.gthreads + gtid];
+    }
+    return found;
+}
+
+// ---------------------------------------------------------------------------
+// histogram splat
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void acc128(unsigned long long* cell, I128 q) {
+    unsigned long long lo = q.lo;
+    unsigned long long old = atomicAdd(cell, lo);
+    unsigned long long carry = (old + lo) < old ? 1ull : 0ull;
+    unsigned long long hi = (unsigned long long)q.hi + carry;
+    if (hi) atomicAdd(cell + 1, hi);
+}
+
+// Ramp of values (w - i) * A / w at bins s + i, i in [0, w), bins < 0 and
+// >= nb dropped; h points at the channel's signed row, habs (or null) at its
+// |.| row.
+__device__ void splat_ramp(unsigned long long* h, unsigned long long* habs, uint32_t nb, int s, int w, float A) {
+    if (w <= 0) return;
+    double S = (double)A / (double)w;
+    int ww = w;
+    if (s < 0) {
+        ww = w + s;
+        s = 0;
+        if (ww <= 0) return;
+    }
+    if ((uint32_t)s >= nb) return;
+    I128 sq = fixed_from_double(S);
+    I128 aq = i128_mul_small(sq, (uint32_t)ww);
+    I128 p1 = i128_neg(i128_add(aq, sq));
+    uint32_t s0 = (uint32_t)s;
+    acc128(h + 2 * (size_t)s0, aq);
+    if (s0 + 1 < nb) acc128(h + 2 * (size_t)(s0 + 1), p1);
+    if (s0 + (uint32_t)ww + 1 < nb) acc128(h + 2 * (size_t)(s0 + ww + 1), sq);
+    if (habs) {
+        if (sq.hi < 0) {
+            sq = i128_neg(sq);
+            aq = i128_neg(aq);
+            p1 = i128_neg(p1);
+        }
+        acc128(habs + 2 * (size_t)s0, aq);
+        if (s0 + 1 < nb) acc128(habs + 2 * (size_t)(s0 + 1), p1);
+        if (s0 + (uint32_t)ww + 1 < nb) acc128(habs + 2 * (size_t)(s0 + ww + 1), sq);
+    }
+}
+
+struct Hist {
+    unsigned long long* cells;  // [band slot][channel][kind][bin] x {lo, hi}
+    uint32_t n_bins, n_ch, kinds;
+    __device__ unsigned long long* row(int band, int ch, int kind) const {
+        return cells + 2 * (size_t)n_bins * (((size_t)band * n_ch + ch) * kinds + kind);
+    }
+};
+
+// Recorder::Record: MonoRecorder (src/MonoRecorder.cpp:83-97) or
+// StereoRecorder (src/StereoRecorder.cpp:97-130).
+__device__ void record(const RenderParams& P, const Hist& H, int band, const RecParams& R, V3 dir, float a,
+                       float t, float dist) {
+    bool ab = H.kinds > 1;
+    if (R.kind == EAR_REC_MONO) {
+        int s = (int)((double)t * 44100.0);
+        float width = sqrtf(dist);
+        float ampl = 2.0f * a / width;
+        int w = (int)ceilf(width);
+        splat_ramp(H.row(band, R.channel, 0), ab ? H.row(band, R.channel, 1) : nullptr, H.n_bins, s, w, ampl);
+    } else {
+        float dt = dot(dir, V3{R.rx, R.ry, R.rz});
+        float td = R.head_size / 343.0f;
+        int s_right = (int)((double)(t - dt * td) * 44100.0);
+        int s_left = (int)((double)(t + dt * td) * 44100.0);
+        float width = sqrtf(dist);
+        float ampl = 2.0f * a / width;
+        float al = ampl, ar = ampl;
+        float factor = powf_ear(R.head_abs[band], fabsf(dt) * R.head_size);
+        if (dt < 0.0f)
+            ar *= factor * factor;
+        else
+            al *= factor * factor;
+        int w = (int)ceilf(width);
+        splat_ramp(H.row(band, R.channel, 0), ab ? H.row(band, R.channel, 1) : nullptr, H.n_bins, s_left, w, al);
+        splat_ramp(H.row(band, R.channel + 1, 0), ab ? H.row(band, R.channel + 1, 1) : nullptr, H.n_bins, s_right, w,
+                   ar);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+struct MeshSrc {
+    const float* v;     // [n][9]
+    const float* area;  // [n]
+    const float* nrm;   // [n][3]
+};
+
+__global__ void __launch_bounds__(kBlock) k_generate(RenderParams P, MeshSrc M, const int* __restrict__ bands,
+                                                      uint32_t n_bands, uint32_t ray_base, uint32_t n_chunk,
+                                                      Queue q, uint32_t* __restrict__ n_q, ShadowQ sq,
+                                                      uint32_t* __restrict__ n_sq,
+                                                      unsigned long long* __restrict__ stats) {
+    uint32_t total = n_bands * n_chunk;
+    for (uint32_t base = blockIdx.x * kBlock; base < total; base += gridDim.x * kBlock) {
+        uint32_t gid = base + threadIdx.x;
+        bool valid = gid < total;
+        bool live = false;
+        float4 ro{}, rd{};
+        uint4 rm{};
+        int band = 0;
+        uint32_t ray = 0;
+        Rng rng;
+        V3 o{0, 0, 0};
+        if (valid) {
+            band = bands[gid / n_chunk];
+            ray = ray_base + gid % n_chunk;
+            rng.init(P.seed, ray, P.ctx_word[band], P.instance, 0);
+            V3 d;
+            if (P.src_kind == EAR_SRC_MESH) {
+                float x = rng.next() * P.src_total_area;
+                uint32_t pick = P.src_n_tris - 1;
+                for (uint32_t i = 0; i < P.src_n_tris; ++i) {
+                    x -= M.area[i];
+                    if (x < 0.0f) {
+                        pick = i;
+                        break;
+                    }
+                }
+                const float* v = M.v + 9 * (size_t)pick;
+                float r1 = rng.next();
+                float r2 = rng.next();
+                float sr1 = sqrtf(r1);
+                float ca = 1.0f - sr1, cb = sr1 * (1.0f - r2), cc = sr1 * r2;
+                o = V3{ca * v[0] + cb * v[3] + cc * v[6], ca * v[1] + cb * v[4] + cc * v[7],
+                       ca * v[2] + cb * v[5] + cc * v[8]};
+                d = sample_hemi(rng, V3{M.nrm[3 * pick], M.nrm[3 * pick + 1], M.nrm[3 * pick + 2]});
+            } else {
+                o = V3{P.src_pos[0], P.src_pos[1], P.src_pos[2]};
+                d = sample_sphere(rng);
+            }
+            ro = make_float4(o.x, o.y, o.z, 1.0f);
+            rd = make_float4(d.x, d.y, d.z, 0.0f);
+            rm = make_uint4(ray, rng.draw, (uint32_t)band, 0u);
+            live = !((double)1.0f < P.cutoff) && P.max_bounces > 0;
+        }
+        // mesh sources connect to every recorder at k = 0 (src/Scene.cpp:185)
+        if (P.src_kind == EAR_SRC_MESH) {
+            for (uint32_t r = 0; r < P.n_rec; ++r) {
+                bool emit = false;
+                float c = 0.0f;
+                if (valid) {
+                    const RecParams& R = P.rec[r];
+                    V3 dv = sub(V3{R.px, R.py, R.pz}, o);
+                    float l = length(dv);
+                    c = 1.0f;
+                    c *= powf_ear(P.air[band], l);
+                    c *= 2.0f / (4.0f * kPi * l * l);
+                    emit = is_normal(c);
+                }
+                uint32_t slot = wave_append(n_sq, emit);
+                if (emit) {
+                    sq.p[slot] = make_float4(o.x, o.y, o.z, c);
+                    sq.m[slot] = make_float4(0.0f, __uint_as_float(r | ((uint32_t)band << 8)), 0.0f, 0.0f);
+                }
+            }
+        }
+        uint32_t slot = wave_append(n_q, live);
+        if (live) {
+            q.o[slot] = ro;
+            q.d[slot] = rd;
+            q.m[slot] = rm;
+        }
+        wave_add_u64(stats + ST_RAYS, valid ? 1 : 0);
+    }
+}
+
+template <bool INSTR>
+__global__ void __launch_bounds__(kBlock) k_extend(Traversal T, Queue q, const uint32_t* __restrict__ n_q,
+                                                    int2* __restrict__ hits, unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kLdsDepth * kBlock];
+    uint32_t n = *n_q;
+    uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t nvis = 0, ntri = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        uint32_t i = base + threadIdx.x;
+        if (i < n) {
+            float4 o = q.o[i], d = q.d[i];
+            float bt = 1000000.0f;
+            int bs = -1;
+            uint32_t bo = 0;
+            traverse<false, INSTR>(T, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, bt, bs, bo, lds + threadIdx.x, gtid,
+                                   nvis, ntri);
+            hits[i] = make_int2(__float_as_int(bt), bs);
+        }
+    }
+    if (INSTR) {
+        wave_add_u64(stats + ST_NV_CL, nvis);
+        wave_add_u64(stats + ST_NT_CL, ntri);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_shade(RenderParams P, const float4* __restrict__ tris,
+                                                   const float4* __restrict__ nm,
+                                                   const float4* __restrict__ mat_band, Queue qin,
+                                                   const uint32_t* __restrict__ n_in, const int2* __restrict__ hits,
+                                                   Queue qout, uint32_t* __restrict__ n_out, ShadowQ sq,
+                                                   uint32_t* __restrict__ n_sq, unsigned long long* __restrict__ stats,
+                                                   int32_t* __restrict__ log_hits, uint32_t* __restrict__ log_nseg) {
+    uint32_t n = *n_in;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool valid = i < n;
+        bool alive = false;  // hit and intensity normal: NEE runs
+        bool cont = false;   // continues to the next segment
+        float4 ro{}, rd{};
+        uint4 rm{};
+        V3 p{0, 0, 0}, nrm{0, 0, 0}, prev{0, 0, 0}, d{0, 0, 0};
+        float I = 0.0f, L = 0.0f, spec = 0.0f;
+        int bt = kReflect;
+        int band = 0;
+        uint32_t k = 0;
+        Rng rng;
+        if (valid) {
+            ro = qin.o[i];
+            rd = qin.d[i];
+            rm = qin.m[i];
+            band = (int)(rm.z & 0xFFu);
+            k = ((rm.z >> 8) & 0xFFFFu) + 1;  // this segment's bounce number
+            int2 h = hits[i];
+            if (log_hits && rm.x < P.trace_rays) {
+                if (k - 1 < P.trace_segs)
+                    log_hits[((size_t)band * P.trace_rays + rm.x) * P.trace_segs + (k - 1)] =
+                        h.y >= 0 ? (int32_t)__float_as_uint(tris[3 * h.y + 2].y) : -1;
+                log_nseg[(size_t)band * P.trace_rays + rm.x] = k;
+            }
+            if (h.y >= 0) {
+                float t = __int_as_float(h.x);
+                V3 o{ro.x, ro.y, ro.z};
+                d = V3{rd.x, rd.y, rd.z};
+                p = V3{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+                float4 nmv = nm[h.y];
+                V3 N{nmv.x, nmv.y, nmv.z};
+                nrm = dot(N, d) > 0.0f ? V3{N.x * -1.0f, N.y * -1.0f, N.z * -1.0f} : N;
+                uint32_t mat = __float_as_uint(nmv.w);
+                float4 mb = mat_band[3 * mat + band];  // {fl2 | -1, spec, retention, -}
+                rng.init(P.seed, rm.x, P.ctx_word[band], P.instance, rm.y);
+                bt = kReflect;
+                if (mb.x >= 0.0f) bt = (rng.next() <= mb.x) ? kReflect : kRefract;
+                spec = mb.y;
+                if (bt == kRefract) nrm = neg(nrm);
+                float l = length(sub(p, o));
+                I = ro.w;
+                I *= powf_ear(P.air[band], l);
+                L = rd.w + l;
+                I *= mb.z;
+                alive = is_normal(I);
+                prev = normalized(d);
+            }
+        }
+        // NEE to every recorder (src/Scene.cpp:185-268)
+        for (uint32_t r = 0; r < P.n_rec; ++r) {
+            bool emit = false;
+            float c = 0.0f;
+            if (alive) {
+                const RecParams& R = P.rec[r];
+                V3 dv = sub(V3{R.px, R.py, R.pz}, p);
+                V3 lsdir = normalized(dv);
+                float dt = dot(lsdir, nrm);
+                if (dt > 0.0f) {
+                    float factor;
+                    if (bt == kReflect) {
+                        V3 rv = reflect(prev, nrm);
+                        float diff = -dot(nrm, prev);
+                        float sf = fmaxf(0.0f, dot(rv, lsdir));
+                        factor = spec * kExpInt * powf_ear(sf, kExp) + (1.0f - spec) * diff;
+                    } else {
+                        float diff = dot(nrm, prev);
+                        float sf = fmaxf(0.0f, dot(prev, lsdir));
+                        factor = spec * kExpInt * powf_ear(sf, kExp) + (1.0f - spec) * diff;
+                    }
+                    c = I;
+                    c *= factor;
+                    float l = length(dv);
+                    c *= powf_ear(P.air[band], l);
+                    c *= 2.0f / (4.0f * kPi * l * l);
+                    if (is_normal(c)) {
+                        emit = true;
+                        if (k & 1u) c *= -1.0f;
+                    }
+                }
+            }
+            uint32_t slot = wave_append(n_sq, emit);
+            if (emit) {
+                sq.p[slot] = make_float4(p.x, p.y, p.z, c);
+                sq.m[slot] = make_float4(L, __uint_as_float(r | ((uint32_t)band << 8)), 0.0f, 0.0f);
+            }
+        }
+        if (alive) cont = !((double)I < P.cutoff) && k < P.max_bounces;
+        float4 no{}, nd{};
+        uint4 nmeta{};
+        if (cont) {
+            // REFRACT: the negated normal and the incoming direction; REFLECT:
+            // the mirror direction about the ray-facing normal (src/Scene.cpp:65-74)
+            V3 rdir = bt == kRefract ? d : reflect(d, nrm);
+            V3 v = sample_hemi_spec(rng, nrm, rdir, spec);
+            no = make_float4(p.x, p.y, p.z, I);
+            nd = make_float4(v.x, v.y, v.z, L);
+            nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), 0u);
+        }
+        uint32_t slot = wave_append(n_out, cont);
+        if (cont) {
+            qout.o[slot] = no;
+            qout.d[slot] = nd;
+            qout.m[slot] = nmeta;
+        }
+        wave_add_u64(stats + ST_SEGMENTS, valid ? 1 : 0);
+    }
+}
+
+template <bool INSTR>
+__global__ void __launch_bounds__(kBlock) k_shadow(RenderParams P, Traversal T, ShadowQ sq,
+                                                    const uint32_t* __restrict__ n_sq, Hist H,
+                                                    unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kLdsDepth * kBlock];
+    uint32_t n = *n_sq;
+    uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        uint32_t i = base + threadIdx.x;
+        if (i < n) {
+            float4 pc = sq.p[i], m = sq.m[i];
+            uint32_t packed = __float_as_uint(m.y);
+            int r = (int)(packed & 0xFFu), band = (int)((packed >> 8) & 0xFFu);
+            const RecParams& R = P.rec[r];
+            V3 p{pc.x, pc.y, pc.z};
+            V3 dv = sub(V3{R.px, R.py, R.pz}, p);
+            float bt = 0.0f;
+            int bs = -1;
+            uint32_t bo = 0;
+            ++nsh;
+            bool occ = traverse<true, INSTR>(T, p, dv, bt, bs, bo, lds + threadIdx.x, gtid, nvis, ntri);
+            if (!occ) {
+                float l = length(dv);
+                V3 lsdir = normalized(dv);
+                float L = m.x;
+                record(P, H, band, R, lsdir, pc.w, (L + l) / 343.0f, L + l);
+                ++ncon;
+            }
+        }
+    }
+    wave_add_u64(stats + ST_SHADOW, nsh);
+    wave_add_u64(stats + ST_CONTRIB, ncon);
+    if (INSTR) {
+        wave_add_u64(stats + ST_NV_SH, nvis);
+        wave_add_u64(stats + ST_NT_SH, ntri);
+    }
+}
+
+__global__ void k_reset2(uint32_t* a, uint32_t* b) {
+    if (threadIdx.x == 0) {
+        *a = 0;
+        *b = 0;
+    }
+}
+
+// direct sound per recorder (src/Scene.cpp:299-311): {visible, s, w, A} per channel
+struct DirectRamp {
+    int32_t visible, s, w;
+    float a;
+};
+
+__global__ void k_direct(RenderParams P, Traversal T, const int* __restrict__ bands, uint32_t n_bands,
+                         DirectRamp* __restrict__ out) {
+    __shared__ int lds[kLdsDepth * kBlock];
+    uint32_t t = threadIdx.x;
+    if (t >= n_bands * P.n_rec) return;
+    int band = bands[t / P.n_rec];
+    const RecParams& R = P.rec[t % P.n_rec];
+    DirectRamp* o = out + (size_t)band * P.n_channels + R.channel;
+    int nch = R.kind == EAR_REC_STEREO ? 2 : 1;
+    for (int c = 0; c < nch; ++c) o[c] = DirectRamp{0, 0, 0, 0.0f};
+    if (P.src_kind != EAR_SRC_POINT) return;
+    V3 rp{R.px, R.py, R.pz}, sp{P.src_pos[0], P.src_pos[1], P.src_pos[2]};
+    float bt = 0.0f;
+    int bs = -1;
+    uint32_t bo = 0, nv = 0, nt = 0;
+    if (traverse<true, false>(T, rp, sub(sp, rp), bt, bs, bo, lds + threadIdx.x, blockIdx.x * kBlock + t, nv, nt))
+        return;
+    V3 dist = sub(rp, sp);
+    float len = length(dist);
+    V3 dir = normalized(dist);
+    float a = 1.0f / (4.0f * kPi * len * len) * powf_ear(P.air[band], len) * P.dry;
+    float tt = len / 343.0f;
+    float width = sqrtf(len);
+    float ampl = 2.0f * a / width;
+    int w = (int)ceilf(width);
+    if (R.kind == EAR_REC_MONO) {
+        o[0] = DirectRamp{1, (int)((double)tt * 44100.0), w, ampl};
+    } else {
+        float dt = dot(dir, V3{R.rx, R.ry, R.rz});
+        float td = R.head_size / 343.0f;
+        float al = ampl, ar = ampl;
+        float factor = powf_ear(R.head_abs[band], fabsf(dt) * R.head_size);
+        if (dt < 0.0f)
+            ar *= factor * factor;
+        else
+            al *= factor * factor;
+        o[0] = DirectRamp{1, (int)((double)(tt + dt * td) * 44100.0), w, al};
+        o[1] = DirectRamp{1, (int)((double)(tt - dt * td) * 44100.0), w, ar};
+    }
+}
+
+// Two inclusive prefix sums over a second-difference row (128-bit, exact),
+// then value = (raw 2^-88 * inv_n + direct ramp) * gain2.  One block per row.
+constexpr int kFinBlock = 1024;
+__global__ void __launch_bounds__(kFinBlock) k_finalize(unsigned long long* __restrict__ cells, uint32_t nb,
+                                                       uint32_t n_ch, uint32_t kinds, const int* __restrict__ bands,
+                                                       const DirectRamp* __restrict__ direct, double inv_n,
+                                                       double gain2, double* __restrict__ value) {
+    __shared__ uint64_t slo[kFinBlock];
+    __shared__ int64_t shi[kFinBlock];
+    uint32_t row = blockIdx.x;  // [band list index][ch][kind]
+    uint32_t kind = row % kinds;
+    uint32_t ch = (row / kinds) % n_ch;
+    int band = bands[row / (kinds * n_ch)];
+    size_t roff = (((size_t)band * n_ch + ch) * kinds + kind) * nb;
+    unsigned long long* c = cells + 2 * roff;
+    uint32_t per = (nb + kFinBlock - 1) / kFinBlock;
+    uint32_t b = threadIdx.x * per, e = min(nb, b + per);
+    for (int pass = 0; pass < 2; ++pass) {
+        I128 sum{0, 0};
+        for (uint32_t j = b; j < e; ++j) sum = i128_add(sum, I128{c[2 * j], (int64_t)c[2 * j + 1]});
+        slo[threadIdx.x] = sum.lo;
+        shi[threadIdx.x] = sum.hi;
+        __syncthreads();
+        for (int off = 1; off < kFinBlock; off <<= 1) {
+            I128 v{0, 0};
+            if ((int)threadIdx.x >= off) v = I128{slo[threadIdx.x - off], shi[threadIdx.x - off]};
+            __syncthreads();
+            I128 cur = i128_add(I128{slo[threadIdx.x], shi[threadIdx.x]}, v);
+            slo[threadIdx.x] = cur.lo;
+            shi[threadIdx.x] = cur.hi;
+            __syncthreads();
+        }
+        I128 run = i128_add(I128{slo[threadIdx.x], shi[threadIdx.x]}, i128_neg(sum));
+        for (uint32_t j = b; j < e; ++j) {
+            run = i128_add(run, I128{c[2 * j], (int64_t)c[2 * j + 1]});
+            c[2 * j] = run.lo;
+            c[2 * j + 1] = (unsigned long long)run.hi;
+        }
+        __syncthreads();
+    }
+    if (value) {
+        const DirectRamp dr = direct[(size_t)band * n_ch + ch];
+        double S = dr.visible ? (double)dr.a / (double)dr.w : 0.0;
+        for (uint32_t j = b; j < e; ++j) {
+            double v = (double)(int64_t)c[2 * j + 1] * 18446744073709551616.0 + (double)c[2 * j];
+            v = v * 3.2311742677852644e-27;  // 2^-88
+            v = v * inv_n;
+            if (dr.visible && (int)j >= dr.s && (int)j < dr.s + dr.w) {
+                double dv = (double)(dr.w - ((int)j - dr.s)) * S;
+                v += kind == 1 ? fabs(dv) : dv;
+            }
+            value[roff + j] = v * gain2;
+        }
+    }
+}
+
+// batched queries for tests and host tools
+template <bool ANY>
+__global__ void __launch_bounds__(kBlock) k_query(Traversal T, uint32_t n, const float* __restrict__ a,
+                                                   const float* __restrict__ b, float* __restrict__ tout,
+                                                   int32_t* __restrict__ iout, uint8_t* __restrict__ occ) {
+    __shared__ int lds[kLdsDepth * kBlock];
+    uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t nv = 0, nt = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        uint32_t i = base + threadIdx.x;
+        if (i >= n) continue;
+        V3 o{a[3 * i], a[3 * i + 1], a[3 * i + 2]};
+        V3 x{b[3 * i], b[3 * i + 1], b[3 * i + 2]};
+        float bt = 1000000.0f;
+        int bs = -1;
+        uint32_t bo = 0;
+        if (ANY) {
+            occ[i] = traverse<true, false>(T, o, sub(x, o), bt, bs, bo, lds + threadIdx.x, gtid, nv, nt) ? 1 : 0;
+        } else {
+            bool h = traverse<false, false>(T, o, x, bt, bs, bo, lds + threadIdx.x, gtid, nv, nt);
+            tout[i] = h ? bt : 0.0f;
+            iout[i] = h ? (int32_t)bo : -1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    Ctx* c;
+    void* p = nullptr;
+    size_t n = 0;
+    explicit DevBuf(Ctx* cc) : c(cc) {}
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) dev_free(c, p, n);
+        p = nullptr;
+        n = 0;
+    }
+    ear_status alloc(size_t bytes) {
+        release();
+        n = bytes ? bytes : 16;
+        return dev_alloc(c, &p, n);
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+#define TRY(x)                       \
+    do {                             \
+        ear_status _s = (x);         \
+        if (_s != EAR_OK) return _s; \
+    } while (0)
+
+uint32_t persistent_grid(Ctx* c, uint32_t work, uint32_t waves_per_cu) {
+    uint32_t blocks_needed = (work + kBlock - 1) / kBlock;
+    uint32_t cap = (uint32_t)c->num_cus * std::max<uint32_t>(1, waves_per_cu) / (kBlock / 64);
+    return std::max<uint32_t>(1, std::min(blocks_needed, std::max<uint32_t>(1, cap)));
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+}  // namespace
+
+ear_status run_query_closest(Ctx* c, DeviceScene* s, uint32_t n, const float* o, const float* d, float* t,
+                             int32_t* tri) {
+    if (n == 0) return EAR_OK;
+    if (!o || !d || !t || !tri) return EAR_ERR_INVALID_ARGUMENT;
+    DevBuf da(c), db(c), dt(c), di(c), gs(c);
+    TRY(da.alloc((size_t)n * 12));
+    TRY(db.alloc((size_t)n * 12));
+    TRY(dt.alloc((size_t)n * 4));
+    TRY(di.alloc((size_t)n * 4));
+    uint32_t grid = persistent_grid(c, n, 16);
+    uint32_t gthreads = grid * kBlock;
+    TRY(gs.alloc((size_t)(kStackDepth - kLdsDepth) * gthreads * 4));
+    hipStream_t st = c->stream;
+    EAR_HIP_TRY(c, hipMemcpyAsync(da.p, o, (size_t)n * 12, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(db.p, d, (size_t)n * 12, hipMemcpyHostToDevice, st));
+    Traversal T{s->nodes, s->tris, s->n_tris, gs.as<int>(), gthreads};
+    k_query<false><<<grid, kBlock, 0, st>>>(T, n, da.as<float>(), db.as<float>(), dt.as<float>(), di.as<int32_t>(),
+                                            nullptr);
+    EAR_HIP_TRY(c, hipGetLastError());
+    EAR_HIP_TRY(c, hipMemcpyAsync(t, dt.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(tri, di.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    EAR_HIP_TRY(c, hipStreamSynchronize(st));
+    return EAR_OK;
+}
+
+ear_status run_query_occluded(Ctx* c, DeviceScene* s, uint32_t n, const float* p, const float* q, uint8_t* occ) {
+    if (n == 0) return EAR_OK;
+    if (!p || !q || !occ) return EAR_ERR_INVALID_ARGUMENT;
+    DevBuf da(c), db(c), dk(c), gs(c);
+    TRY(da.alloc((size_t)n * 12));
+    TRY(db.alloc((size_t)n * 12));
+    TRY(dk.alloc((size_t)n));
+    uint32_t grid = persistent_grid(c, n, 16);
+    uint32_t gthreads = grid * kBlock;
+    TRY(gs.alloc((size_t)(kStackDepth - kLdsDepth) * gthreads * 4));
+    hipStream_t st = c->stream;
+    EAR_HIP_TRY(c, hipMemcpyAsync(da.p, p, (size_t)n * 12, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(db.p, q, (size_t)n * 12, hipMemcpyHostToDevice, st));
+    Traversal T{s->nodes, s->tris, s->n_tris, gs.as<int>(), gthreads};
+    k_query<true><<<grid, kBlock, 0, st>>>(T, n, da.as<float>(), db.as<float>(), nullptr, nullptr, dk.as<uint8_t>());
+    EAR_HIP_TRY(c, hipGetLastError());
+    EAR_HIP_TRY(c, hipMemcpyAsync(occ, dk.p, n, hipMemcpyDeviceToHost, st));
+    EAR_HIP_TRY(c, hipStreamSynchronize(st));
+    return EAR_OK;
+}
+
+static float tri_area_host(const float* v) {
+    // Triangle::calcArea (src/Triangle.cpp:26-30): |edge(0) x edge(1)| / 2,
+    // edge(i) = v[(i+1)%3] - v[i]
+    V3 a = sub(V3{v[3], v[4], v[5]}, V3{v[0], v[1], v[2]});
+    V3 b = sub(V3{v[6], v[7], v[8]}, V3{v[3], v[4], v[5]});
+    return length(cross(a, b)) / 2.0f;
+}
+
+ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist* out, ear_stats* st_out) {
+    if (!D) return EAR_ERR_INVALID_ARGUMENT;
+    if ((D->band_mask & 7u) == 0 || (D->band_mask & ~7u)) {
+        c->last_error = "band_mask must select bands among bits 0..2";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (D->hist_bins == 0 || D->n_recorders == 0 || !D->recorders) {
+        c->last_error = "hist_bins and recorders are required";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (D->n_recorders > (uint32_t)kMaxRecorders) {
+        c->last_error = "too many recorders";
+        return EAR_ERR_CAPACITY;
+    }
+    if (D->source.kind != EAR_SRC_POINT && D->source.kind != EAR_SRC_MESH) return EAR_ERR_INVALID_ARGUMENT;
+    if (D->source.kind == EAR_SRC_MESH && (!D->source.mesh_v || D->source.mesh_n_tris == 0)) {
+        c->last_error = "mesh source without triangles";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    for (int b = 0; b < 3; ++b)
+        if (!std::isfinite(D->air[b])) return EAR_ERR_INVALID_ARGUMENT;
+
+    RenderParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.seed = D->seed;
+    P.instance = D->instance;
+    for (int b = 0; b < 3; ++b) {
+        P.ctx_word[b] = (uint32_t)b | ((D->sound & 0xFFu) << 8) | ((uint32_t)(D->keyframe + 1) << 16);
+        P.air[b] = D->air[b];
+    }
+    P.dry = D->dry;
+    P.cutoff = D->energy_cutoff;
+    P.max_bounces = D->max_bounces;
+    P.n_bins = D->hist_bins;
+    P.n_rec = D->n_recorders;
+    P.n_band_slots = 3;
+    P.hist_kinds = (D->flags & EAR_FLAG_ABS_HIST) ? 2 : 1;
+    P.src_kind = D->source.kind;
+    for (int k = 0; k < 3; ++k) P.src_pos[k] = D->source.pos[k];
+    P.flags = D->flags;
+    uint32_t nch = 0;
+    for (uint32_t r = 0; r < D->n_recorders; ++r) {
+        const ear_recorder& R = D->recorders[r];
+        if (R.kind != EAR_REC_MONO && R.kind != EAR_REC_STEREO) return EAR_ERR_INVALID_ARGUMENT;
+        RecParams& Q = P.rec[r];
+        Q.px = R.pos[0];
+        Q.py = R.pos[1];
+        Q.pz = R.pos[2];
+        Q.rx = R.right_ear[0];
+        Q.ry = R.right_ear[1];
+        Q.rz = R.right_ear[2];
+        Q.head_size = R.head_size;
+        for (int b = 0; b < 3; ++b) Q.head_abs[b] = R.head_abs[b];
+        Q.kind = R.kind;
+        Q.channel = (int32_t)nch;
+        nch += R.kind == EAR_REC_STEREO ? 2 : 1;
+    }
+    P.n_channels = nch;
+    bool want_log = (D->flags & EAR_FLAG_TRACE_LOG) && D->trace_rays && D->trace_segs;
+    P.trace_rays = want_log ? D->trace_rays : 0;
+    P.trace_segs = want_log ? D->trace_segs : 0;
+
+    std::vector<int> bands;
+    for (int b = 0; b < 3; ++b)
+        if (D->band_mask & (1u << b)) bands.push_back(b);
+    uint32_t n_bands = (uint32_t)bands.size();
+
+    // mesh source tables, computed once in host fp32 (same formulas as the reference)
+    std::vector<float> src_area, src_nrm;
+    if (D->source.kind == EAR_SRC_MESH) {
+        uint32_t m = D->source.mesh_n_tris;
+        src_area.resize(m);
+        src_nrm.resize(3 * (size_t)m);
+        float total = 0.0f;
+        for (uint32_t i = 0; i < m; ++i) {
+            const float* v = D->source.mesh_v + 9 * (size_t)i;
+            src_area[i] = tri_area_host(v);
+            total += src_area[i];
+            V3 n = normalized(cross(sub(V3{v[3], v[4], v[5]}, V3{v[0], v[1], v[2]}),
+                                    sub(V3{v[6], v[7], v[8]}, V3{v[0], v[1], v[2]})));
+            src_nrm[3 * i] = n.x;
+            src_nrm[3 * i + 1] = n.y;
+            src_nrm[3 * i + 2] = n.z;
+        }
+        P.src_n_tris = m;
+        P.src_total_area = total;
+    }
+
+    const ear_tuning& tu = c->tuning;
+    uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 22);
+    chunk = std::min<uint32_t>(chunk, std::max<uint32_t>(1, D->n_rays));
+    uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 20;
+    uint32_t cap_rays = chunk * n_bands;
+
+    hipStream_t st = c->stream;
+    DevBuf qo[2] = {DevBuf(c), DevBuf(c)}, qd[2] = {DevBuf(c), DevBuf(c)}, qm[2] = {DevBuf(c), DevBuf(c)};
+    DevBuf hits(c), so(c), sm(c), counters(c), stats(c), hist(c), gs(c), dbands(c), direct(c), value(c), logh(c),
+        logn(c), mv(c), ma(c), mn(c);
+    for (int k = 0; k < 2; ++k) {
+        TRY(qo[k].alloc((size_t)cap_rays * 16));
+        TRY(qd[k].alloc((size_t)cap_rays * 16));
+        TRY(qm[k].alloc((size_t)cap_rays * 16));
+    }
+    TRY(hits.alloc((size_t)cap_rays * 8));
+    size_t cap_sh = (size_t)cap_rays * D->n_recorders;
+    TRY(so.alloc(cap_sh * 16));
+    TRY(sm.alloc(cap_sh * 16));
+    TRY(counters.alloc(4 * 4));
+    TRY(stats.alloc(ST_N * 8));
+    size_t cells = (size_t)3 * nch * P.hist_kinds * D->hist_bins;
+    TRY(hist.alloc(cells * 16));
+    uint32_t max_grid = persistent_grid(c, 0xFFFFFFFFu, waves);
+    uint32_t gthreads = max_grid * kBlock;
+    TRY(gs.alloc((size_t)(kStackDepth - kLdsDepth) * gthreads * 4));
+    TRY(dbands.alloc(3 * 4));
+    TRY(direct.alloc((size_t)3 * nch * sizeof(DirectRamp)));
+    if (want_log) {
+        TRY(logh.alloc((size_t)3 * P.trace_rays * P.trace_segs * 4));
+        TRY(logn.alloc((size_t)3 * P.trace_rays * 4));
+        EAR_HIP_TRY(c, hipMemsetAsync(logh.p, 0xFE, logh.n, st));  // -2 pattern: 0xFEFEFEFE fixed below
+        EAR_HIP_TRY(c, hipMemsetAsync(logn.p, 0, logn.n, st));
+    }
+    MeshSrc M{nullptr, nullptr, nullptr};
+    if (D->source.kind == EAR_SRC_MESH) {
+        uint32_t m = D->source.mesh_n_tris;
+        TRY(mv.alloc((size_t)m * 36));
+        TRY(ma.alloc((size_t)m * 4));
+        TRY(mn.alloc((size_t)m * 12));
+        EAR_HIP_TRY(c, hipMemcpyAsync(mv.p, D->source.mesh_v, (size_t)m * 36, hipMemcpyHostToDevice, st));
+        EAR_HIP_TRY(c, hipMemcpyAsync(ma.p, src_area.data(), (size_t)m * 4, hipMemcpyHostToDevice, st));
+        EAR_HIP_TRY(c, hipMemcpyAsync(mn.p, src_nrm.data(), (size_t)m * 12, hipMemcpyHostToDevice, st));
+        M = MeshSrc{mv.as<float>(), ma.as<float>(), mn.as<float>()};
+    }
+    EAR_HIP_TRY(c, hipMemcpyAsync(dbands.p, bands.data(), n_bands * 4, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemsetAsync(hist.p, 0, hist.n, st));
+    EAR_HIP_TRY(c, hipMemsetAsync(stats.p, 0, stats.n, st));
+    EAR_HIP_TRY(c, hipMemsetAsync(counters.p, 0, counters.n, st));
+
+    uint32_t* cnt = counters.as<uint32_t>();  // [0],[1] ray queues, [2] shadow queue
+    unsigned long long* stp = stats.as<unsigned long long>();
+    Traversal T{s->nodes, s->tris, s->n_tris, gs.as<int>(), gthreads};
+    Hist H{hist.as<unsigned long long>(), D->hist_bins, nch, P.hist_kinds};
+    bool instr = (D->flags & EAR_FLAG_INSTRUMENT) != 0;
+
+    uint32_t* pinned = nullptr;
+    EAR_HIP_TRY(c, hipHostMalloc((void**)&pinned, 16, hipHostMallocDefault));
+    struct PinGuard {
+        uint32_t* p;
+        ~PinGuard() {
+            if (p) hipHostFree(p);
+        }
+    } pg{pinned};
+
+    std::vector<EventPair> ev_ext, ev_sha, ev_shw;
+    auto mk = [&](std::vector<EventPair>& v) -> EventPair& {
+        v.emplace_back();
+        hipEventCreate(&v.back().a);
+        hipEventCreate(&v.back().b);
+        return v.back();
+    };
+    hipEvent_t t0, t1;
+    EAR_HIP_TRY(c, hipEventCreate(&t0));
+    EAR_HIP_TRY(c, hipEventCreate(&t1));
+    EAR_HIP_TRY(c, hipEventRecord(t0, st));
+
+    uint32_t bounces = 0, launches = 0;
+    uint64_t extend_q = 0, shadow_q = 0;
+    ear_status status = EAR_OK;
+    for (uint32_t ray_base = 0; ray_base < D->n_rays && status == EAR_OK; ray_base += chunk) {
+        uint32_t n_chunk = std::min(chunk, D->n_rays - ray_base);
+        Queue q0{qo[0].as<float4>(), qd[0].as<float4>(), qm[0].as<uint4>()};
+        Queue q1{qo[1].as<float4>(), qd[1].as<float4>(), qm[1].as<uint4>()};
+        ShadowQ SQ{so.as<float4>(), sm.as<float4>()};
+        uint32_t total = n_chunk * n_bands;
+        uint32_t g = persistent_grid(c, total, waves);
+        k_generate<<<g, kBlock, 0, st>>>(P, M, dbands.as<int>(), n_bands, ray_base, n_chunk, q0, cnt + 0, SQ, cnt + 2,
+                                         stp);
+        ++launches;
+        if (P.src_kind == EAR_SRC_MESH) {
+            EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
+            EAR_HIP_TRY(c, hipStreamSynchronize(st));
+            uint32_t nsh = pinned[2];
+            if (nsh) {
+                uint32_t gsh = persistent_grid(c, nsh, waves);
+                EventPair& e = mk(ev_shw);
+                hipEventRecord(e.a, st);
+                if (instr)
+                    k_shadow<true><<<gsh, kBlock, 0, st>>>(P, T, SQ, cnt + 2, H, stp);
+                else
+                    k_shadow<false><<<gsh, kBlock, 0, st>>>(P, T, SQ, cnt + 2, H, stp);
+                hipEventRecord(e.b, st);
+                shadow_q += nsh;
+                ++launches;
+            }
+            k_reset2<<<1, 64, 0, st>>>(cnt + 2, cnt + 3);
+        }
+        int cur = 0;
+        EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 4, hipMemcpyDeviceToHost, st));
+        EAR_HIP_TRY(c, hipStreamSynchronize(st));
+        uint32_t n_live = pinned[0];
+        for (uint32_t k = 1; k <= D->max_bounces && n_live > 0; ++k) {
+            Queue qi = cur == 0 ? q0 : q1;
+            Queue qn = cur == 0 ? q1 : q0;
+            uint32_t g1 = persistent_grid(c, n_live, waves);
+            EventPair& e1 = mk(ev_ext);
+            hipEventRecord(e1.a, st);
+            if (instr)
+                k_extend<true><<<g1, kBlock, 0, st>>>(T, qi, cnt + cur, hits.as<int2>(), stp);
+            else
+                k_extend<false><<<g1, kBlock, 0, st>>>(T, qi, cnt + cur, hits.as<int2>(), stp);
+            hipEventRecord(e1.b, st);
+            EventPair& e2 = mk(ev_sha);
+            hipEventRecord(e2.a, st);
+            k_shade<<<g1, kBlock, 0, st>>>(P, s->tris, s->nm, s->mat_band, qi, cnt + cur, hits.as<int2>(), qn,
+                                           cnt + (1 - cur), SQ, cnt + 2, stp, logh.as<int32_t>(),
+                                           logn.as<uint32_t>());
+            hipEventRecord(e2.b, st);
+            EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
+            EAR_HIP_TRY(c, hipStreamSynchronize(st));
+            uint32_t nsh = pinned[2];
+            extend_q += n_live;
+            launches += 2;
+            if (nsh) {
+                uint32_t g2 = persistent_grid(c, nsh, waves);
+                EventPair& e3 = mk(ev_shw);
+                hipEventRecord(e3.a, st);
+                if (instr)
+                    k_shadow<true><<<g2, kBlock, 0, st>>>(P, T, SQ, cnt + 2, H, stp);
+                else
+                    k_shadow<false><<<g2, kBlock, 0, st>>>(P, T, SQ, cnt + 2, H, stp);
+                hipEventRecord(e3.b, st);
+                shadow_q += nsh;
+                ++launches;
+            }
+            n_live = pinned[1 - cur];
+            k_reset2<<<1, 64, 0, st>>>(cnt + cur, cnt + 2);
+            ++launches;
+            cur = 1 - cur;
+            ++bounces;
+        }
+        // make sure both ray counters are zero for the next chunk
+        k_reset2<<<1, 64, 0, st>>>(cnt + 0, cnt + 1);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) status = hip_status(c, le, "tracer launch");
+    }
+    if (status != EAR_OK) return status;
+    k_direct<<<1, kBlock, 0, st>>>(P, T, dbands.as<int>(), n_bands, direct.as<DirectRamp>());
+    bool want_value = out && (out->value || out->abs_value);
+    if (want_value) TRY(value.alloc(cells * 8));
+    float finv = 1.0f / (float)std::max<uint32_t>(1, D->n_rays);   // Recorder::Multiply(1.0f / amount)
+    float g2 = D->source.gain * D->source.gain;                      // Recorder::Multiply(gain * gain)
+    k_finalize<<<n_bands * nch * P.hist_kinds, kFinBlock, 0, st>>>(
+        hist.as<unsigned long long>(), D->hist_bins, nch, P.hist_kinds, dbands.as<int>(), direct.as<DirectRamp>(),
+        (double)finv, (double)g2, want_value ? value.as<double>() : nullptr);
+    EAR_HIP_TRY(c, hipGetLastError());
+    EAR_HIP_TRY(c, hipEventRecord(t1, st));
+    EAR_HIP_TRY(c, hipStreamSynchronize(st));
+
+    // copy out [band slot][channel][bin] rows for the rendered bands
+    if (out) {
+        size_t nb = D->hist_bins;
+        std::vector<unsigned long long> rowbuf;
+        std::vector<double> vbuf;
+        for (uint32_t bi = 0; bi < n_bands; ++bi) {
+            int b = bands[bi];
+            for (uint32_t ch = 0; ch < nch; ++ch) {
+                for (uint32_t kind = 0; kind < P.hist_kinds; ++kind) {
+                    size_t roff = (((size_t)b * nch + ch) * P.hist_kinds + kind) * nb;
+                    size_t ooff = ((size_t)b * nch + ch) * nb;
+                    int64_t* raw = kind == 0 ? out->raw : out->abs_raw;
+                    double* val = kind == 0 ? out->value : out->abs_value;
+                    if (raw)
+                        EAR_HIP_TRY(c, hipMemcpyAsync(raw + 2 * ooff, hist.as<unsigned long long>() + 2 * roff,
+                                                      nb * 16, hipMemcpyDeviceToHost, st));
+                    if (val && want_value)
+                        EAR_HIP_TRY(c, hipMemcpyAsync(val + ooff, value.as<double>() + roff, nb * 8,
+                                                      hipMemcpyDeviceToHost, st));
+                }
+            }
+        }
+        if (want_log) {
+            std::vector<int32_t> lh((size_t)3 * P.trace_rays * P.trace_segs);
+            std::vector<uint32_t> ln((size_t)3 * P.trace_rays);
+            EAR_HIP_TRY(c, hipMemcpyAsync(lh.data(), logh.p, lh.size() * 4, hipMemcpyDeviceToHost, st));
+            EAR_HIP_TRY(c, hipMemcpyAsync(ln.data(), logn.p, ln.size() * 4, hipMemcpyDeviceToHost, st));
+            EAR_HIP_TRY(c, hipStreamSynchronize(st));
+            for (uint32_t bi = 0; bi < n_bands; ++bi) {
+                int b = bands[bi];
+                size_t per = (size_t)P.trace_rays * P.trace_segs;
+                if (out->trace_hits) {
+                    for (size_t j = 0; j < per; ++j) {
+                        int32_t v = lh[(size_t)b * per + j];
+                        out->trace_hits[(size_t)b * per + j] = v == (int32_t)0xFEFEFEFE ? -2 : v;
+                    }
+                }
+                if (out->trace_nseg)
+                    std::memcpy(out->trace_nseg + (size_t)b * P.trace_rays, ln.data() + (size_t)b * P.trace_rays,
+                                (size_t)P.trace_rays * 4);
+            }
+        }
+        EAR_HIP_TRY(c, hipStreamSynchronize(st));
+    }
+
+    if (st_out) {
+        unsigned long long hs[ST_N];
+        EAR_HIP_TRY(c, hipMemcpy(hs, stats.p, sizeof(hs), hipMemcpyDeviceToHost));
+        std::memset(st_out, 0, sizeof(*st_out));
+        st_out->segments = hs[ST_SEGMENTS];
+        st_out->shadow_rays = hs[ST_SHADOW];
+        st_out->contributions = hs[ST_CONTRIB];
+        st_out->rays = hs[ST_RAYS];
+        st_out->node_visits_closest = hs[ST_NV_CL];
+        st_out->tri_tests_closest = hs[ST_NT_CL];
+        st_out->node_visits_shadow = hs[ST_NV_SH];
+        st_out->tri_tests_shadow = hs[ST_NT_SH];
+        st_out->bounces_launched = bounces;
+        st_out->launches = launches;
+        float ms = 0;
+        hipEventElapsedTime(&ms, t0, t1);
+        st_out->trace_ms = ms;
+        auto sum = [](std::vector<EventPair>& v) {
+            double a = 0;
+            for (auto& e : v) {
+                float m = 0;
+                hipEventElapsedTime(&m, e.a, e.b);
+                a += m;
+            }
+            return a;
+        };
+        st_out->extend_ms = sum(ev_ext);
+        st_out->shade_ms = sum(ev_sha);
+        st_out->shadow_ms = sum(ev_shw);
+        st_out->extend_launches = (uint32_t)ev_ext.size();
+        st_out->shadow_launches = (uint32_t)ev_shw.size();
+        st_out->extend_queries = extend_q;
+        st_out->shadow_queries = shadow_q;
+    }
+    for (auto* v : {&ev_ext, &ev_sha, &ev_shw})
+        for (auto& e : *v) {
+            hipEventDestroy(e.a);
+            hipEventDestroy(e.b);
+        }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return EAR_OK;
+}
+
+}  // namespace ear

@@ -1,0 +1,226 @@
+/*
+ * ear.h -- C ABI of the MI355X acoustic ray-tracing core (libear_hip.so).
+ *
+ * This is the drop-in boundary beneath the reference's
+ *     void Scene::Render(int band, int sound, float absorbtion_factor,
+ *                        int num_samples, float dry,
+ *                        const std::vector<Recorder*>& rec, int keyframeID)
+ * (src/Scene.h:77, defined at src/Scene.cpp:111-318).  The reference has no
+ * plugin API: its caller is SceneContext::operator() (src/SceneContext.h:46-48),
+ * one call per (sound, keyframe, band) context on its own thread
+ * (src/EAR.cpp:170-207).  The host C++ Scene::Render of this project keeps
+ * that signature and calls the functions below; every entry point takes plain
+ * pointers and sizes, returns an ear_status and never throws.
+ *
+ * Memory: the caller owns every host buffer; the library owns device memory
+ * behind the opaque handles.  Threading: one ear_ctx per host thread (each
+ * has its own HIP stream); handles are not shared between threads.
+ */
+#ifndef EAR_H
+#define EAR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EAR_ABI_VERSION 1
+
+typedef enum {
+    EAR_OK = 0,
+    EAR_ERR_INVALID_ARGUMENT = 1, /* null pointer, size 0 where >0 needed, bad enum */
+    EAR_ERR_INVALID_MATERIAL = 2, /* refl + refr > 1: the reference throws DatatypeException
+                                     ("Invalid material settings", src/Material.cpp:56) */
+    EAR_ERR_INVALID_GEOMETRY = 3, /* NaN / Inf vertex, material id out of range */
+    EAR_ERR_NO_DEVICE = 4,        /* no HIP device / bad ordinal */
+    EAR_ERR_OUT_OF_MEMORY = 5,    /* device allocation failed or exceeds ear_ctx memory cap */
+    EAR_ERR_HIP = 6,              /* any other HIP runtime error */
+    EAR_ERR_CAPACITY = 7,         /* a size exceeds a documented limit */
+    EAR_ERR_INTERNAL = 8
+} ear_status;
+
+typedef struct ear_ctx ear_ctx;
+typedef struct ear_scene ear_scene;
+
+/* ------------------------------------------------------------------ */
+/* context                                                              */
+/* ------------------------------------------------------------------ */
+
+/* Replaces the boost::thread per SceneContext (src/EAR.cpp:196-207): one
+ * device + one HIP stream. */
+ear_status ear_ctx_create(int device, ear_ctx** out);
+ear_status ear_ctx_destroy(ear_ctx* ctx);
+/* Cap on device bytes the context may hold (0 = unlimited); allocations past
+ * it fail with EAR_ERR_OUT_OF_MEMORY (lets host tests force that path). */
+ear_status ear_ctx_set_memory_cap(ear_ctx* ctx, uint64_t bytes);
+const char* ear_status_string(ear_status s);
+/* Last error message recorded on this context (never NULL). */
+const char* ear_ctx_last_error(const ear_ctx* ctx);
+int ear_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* scene: Mesh / Triangle / Material                                    */
+/* ------------------------------------------------------------------ */
+
+#define EAR_MAT_HAS_REFR 1u /* .EAR MAT block carried the optional refraction triple */
+#define EAR_MAT_HAS_SPEC 2u /* .EAR MAT block carried the optional specularity triple */
+
+/* One .EAR MAT block (src/Material.cpp:26-72): reflection, optional
+ * refraction, optional specularity per band.  Absent triples read as 0, as
+ * the reference's std::map<int,float>::operator[] default does. */
+typedef struct {
+    float refl[3];
+    float refr[3];
+    float spec[3];
+    uint32_t flags;
+} ear_material;
+
+typedef struct {
+    uint32_t leaf_size;  /* max triangles per BVH leaf, 1..16 (0 = library default) */
+    uint32_t reserved[3];
+} ear_bvh_opts;
+
+/* Uploads the merged triangle soup of every MESH block (the reference merges
+ * all meshes into meshes[0], src/Scene.cpp:103-106, src/Mesh.cpp:116-123)
+ * and builds the LBVH on the device.  Triangles are given as three vertex
+ * arrays of n_tris xyz triples each, in the reference's triangle order; the
+ * triangle index reported by queries is the position in that order.
+ * n_tris may be 0 (the reference's Mesh::Empty(), src/EAR.cpp:160-163). */
+ear_status ear_scene_upload(ear_ctx* ctx, const float* v0, const float* v1, const float* v2,
+                            const uint16_t* mat_id, uint32_t n_tris, const ear_material* mats,
+                            uint32_t n_mat, const ear_bvh_opts* opts, ear_scene** out);
+ear_status ear_scene_destroy(ear_scene* scene);
+
+typedef struct {
+    uint32_t n_tris;
+    uint32_t n_nodes;     /* internal BVH nodes */
+    uint32_t max_depth;   /* longest root-to-leaf path */
+    uint32_t leaf_size;
+    float bounds[6];      /* scene AABB lo xyz, hi xyz */
+    double build_ms;      /* device time of the LBVH build */
+} ear_scene_info;
+ear_status ear_scene_get_info(const ear_scene* scene, ear_scene_info* info);
+
+/* Batched geometric queries (Mesh::RayIntersection, src/Mesh.cpp:33-56, and
+ * Mesh::LineIntersection, src/Mesh.cpp:58-71), host arrays in and out.
+ * closest: o[n*3], d[n*3] -> t[n] (hit distance) and tri[n] (original index,
+ *          -1 on a miss); accepted t in (0.001, 1e6), ties to the lower index.
+ * occluded: segment p -> q, any triangle with t in (1e-5, 1) -> 1, else 0. */
+ear_status ear_query_closest(ear_scene* scene, uint32_t n, const float* o, const float* d, float* t,
+                             int32_t* tri);
+ear_status ear_query_occluded(ear_scene* scene, uint32_t n, const float* p, const float* q,
+                              uint8_t* occluded);
+
+/* ------------------------------------------------------------------ */
+/* render: Scene::Render                                                */
+/* ------------------------------------------------------------------ */
+
+#define EAR_REC_MONO 0   /* MonoRecorder (src/MonoRecorder.cpp), 1 channel */
+#define EAR_REC_STEREO 1 /* StereoRecorder (src/StereoRecorder.cpp), channels left, right */
+
+typedef struct {
+    int32_t kind;
+    float pos[3];        /* Recorder::getLocation(keyframe) */
+    float right_ear[3];  /* StereoRecorder::getRightEar(keyframe) */
+    float head_size;
+    float head_abs[3];   /* as stored by the StereoRecorder ctor: max(0,(1-h)^4) */
+} ear_recorder;
+
+#define EAR_SRC_POINT 0 /* AbstractSoundFile with a location */
+#define EAR_SRC_MESH 1  /* AbstractSoundFile with an emitting mesh */
+
+typedef struct {
+    int32_t kind;
+    float pos[3];          /* getLocation(keyframe) for point sources */
+    const float* mesh_v;   /* mesh sources: [n][3][3] vertices (host) */
+    uint32_t mesh_n_tris;
+    float gain;            /* AbstractSoundFile::getGain(), applied squared */
+} ear_source;
+
+#define EAR_FLAG_ABS_HIST 1u      /* also accumulate the |c| histogram (EDC / T60) */
+#define EAR_FLAG_INSTRUMENT 2u    /* count BVH node visits / triangle tests (slower) */
+#define EAR_FLAG_TRACE_LOG 4u     /* log per-ray hit triangle indices */
+
+typedef struct {
+    uint32_t band_mask;     /* bit b renders band b (0 low, 1 mid, 2 high) as its own context */
+    float air[3];           /* absorbtion_factor per band = 1 - SET absorption[b] */
+    float dry;              /* SET drylevel */
+    ear_source source;
+    const ear_recorder* recorders;
+    uint32_t n_recorders;
+    uint32_t n_rays;        /* num_samples */
+    uint32_t max_bounces;   /* closest-hit queries per ray; the reference loops to 999 */
+    double energy_cutoff;   /* path stops when intensity < cutoff; reference 1e-8 */
+    uint64_t seed;          /* Philox key */
+    uint32_t instance;      /* Philox counter word 3 (batch instance) */
+    int32_t keyframe;       /* -1 static */
+    uint32_t sound;         /* sound index (part of the Philox context word) */
+    uint32_t hist_bins;     /* histogram length per channel (44.1 kHz bins) */
+    uint32_t flags;         /* EAR_FLAG_* */
+    uint32_t trace_rays;    /* EAR_FLAG_TRACE_LOG: rays [0, trace_rays) of each band */
+    uint32_t trace_segs;    /* segments logged per ray */
+} ear_render_desc;
+
+/* Histogram output, all host memory provided by the caller, laid out
+ * [band slot b in 0..2][channel][bin] for the bands in band_mask (slots of
+ * bands not rendered are left untouched).  Channels are the recorders'
+ * channels in order (mono 1, stereo 2).
+ *   raw:       int64 pairs {lo, hi} = the exact two's-complement 128-bit sum of
+ *              the splatted ramps before 1/N, in units of 2^-EAR_FRAC_BITS.
+ *   value:     (raw * 2^-EAR_FRAC_BITS / n_rays + direct) * gain^2 as double,
+ *              i.e. what the reference's recorder track holds after
+ *              Scene::Render (src/Scene.cpp:286-316).
+ *   abs_raw / abs_value: the same for |ramp| (EAR_FLAG_ABS_HIST).
+ * Any pointer may be NULL to skip that output. */
+#define EAR_FRAC_BITS 88
+typedef struct {
+    int64_t* raw;
+    double* value;
+    int64_t* abs_raw;
+    double* abs_value;
+    int32_t* trace_hits;   /* [band slot][trace_rays][trace_segs]: tri index, -1 miss, -2 none */
+    uint32_t* trace_nseg;  /* [band slot][trace_rays] */
+} ear_hist;
+
+typedef struct {
+    uint64_t segments;        /* closest-hit queries (ray-segments) */
+    uint64_t shadow_rays;     /* recorder connections traced */
+    uint64_t contributions;   /* connections recorded */
+    uint64_t rays;            /* rays emitted */
+    uint64_t node_visits_closest, tri_tests_closest;   /* EAR_FLAG_INSTRUMENT */
+    uint64_t node_visits_shadow, tri_tests_shadow;     /* EAR_FLAG_INSTRUMENT */
+    uint32_t bounces_launched; /* wavefront iterations */
+    uint32_t launches;         /* kernel launches */
+    double trace_ms;           /* device time of the whole render (HIP events) */
+    double extend_ms;          /* device time in the closest-hit kernel */
+    double shade_ms;           /* device time in the shading kernel */
+    double shadow_ms;          /* device time in the shadow + splat kernel */
+    uint32_t extend_launches;
+    uint32_t shadow_launches;
+    uint64_t extend_queries;   /* rays handled by closest-hit launches */
+    uint64_t shadow_queries;
+} ear_stats;
+
+ear_status ear_render(ear_ctx* ctx, ear_scene* scene, const ear_render_desc* desc, ear_hist* out,
+                      ear_stats* stats);
+
+/* Launch configuration of the tracer (block sizes, ray chunk, stack split);
+ * values of 0 keep the library default.  Results do not depend on them
+ * (integer histogram sums are order-independent). */
+typedef struct {
+    uint32_t block_extend;
+    uint32_t block_shade;
+    uint32_t block_shadow;
+    uint32_t ray_chunk;      /* rays in flight per wavefront */
+    uint32_t waves_per_cu;   /* persistent grid size */
+    uint32_t reserved[3];
+} ear_tuning;
+ear_status ear_ctx_set_tuning(ear_ctx* ctx, const ear_tuning* t);
+ear_status ear_ctx_get_tuning(const ear_ctx* ctx, ear_tuning* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

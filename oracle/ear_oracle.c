@@ -1,0 +1,852 @@
+/*
+ * ear_oracle.c -- CPU oracle for the E.A.R acoustic ray tracer.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and the
+ * CPU-baseline leg of bench.py as the checker.  The product never links it.
+ *
+ * Plain C11, built with -O3 -ffp-contract=off (no fused multiply-add, no
+ * fast-math) so that every float operation below is one IEEE-754 binary32 or
+ * binary64 operation in the order written.  The arithmetic contract it pins
+ * (operation order, rounding points, random-stream discipline) is the one the
+ * HIP kernels implement independently.
+ *
+ * Reference functions restated here (file:line in aothms/ear):
+ *   Scene::Render             src/Scene.cpp:111-318   -> or_render / trace_ray
+ *   Scene::Bounce             src/Scene.cpp:49-82     -> trace_ray (k > 0 branch)
+ *   Scene::Connect            src/Scene.cpp:84-96     -> or_any_hit
+ *   Mesh::RayIntersection     src/Mesh.cpp:33-56      -> or_closest_hit
+ *   Mesh::LineIntersection    src/Mesh.cpp:58-71      -> or_any_hit
+ *   Mesh::SamplePoint         src/Mesh.cpp:143-154    -> emit_mesh
+ *   Triangle::SamplePoint     src/Triangle.cpp:44-53  -> emit_mesh
+ *   Triangle::calcArea        src/Triangle.cpp:26-30  -> tri_area
+ *   Material::Material        src/Material.cpp:26-72  -> or_material_retention
+ *   Material::Bounce          src/Material.cpp:76-83  -> material_bounce
+ *   Sample_Sphere/Sample_Hemi src/Distributions.h:48-75
+ *   MonoRecorder::Record      src/MonoRecorder.cpp:83-97
+ *   StereoRecorder::Record    src/StereoRecorder.cpp:97-130
+ *   gmtl 0.6.1 (not vendored): intersectDoubleSided = Moeller-Trumbore 1997
+ *   non-culling test with |det| < 1e-5 rejected; reflect(v,n) = v - 2(v.n)n;
+ *   normalize(v) = v / sqrt(v.v); unitRandom -> Philox4x32-10 (Salmon et al.,
+ *   SC'11) with u = (x >> 8) * 2^-24.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include "ear_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define OR_PI 3.14159265f /* src/Distributions.h:37 */
+#define OR_EXP 1000.0f    /* src/Scene.cpp:46 */
+#define OR_EXP_INT (OR_EXP + 1.0f)
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10                                                             */
+/* ------------------------------------------------------------------------ */
+void or_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+typedef struct {
+    uint32_t key[2];
+    uint32_t ray, ctx, instance;
+    uint32_t buf[4];
+    uint32_t draw;
+} or_rng;
+
+static void rng_init(or_rng* r, uint64_t seed, uint32_t ray, uint32_t ctx, uint32_t instance) {
+    r->key[0] = (uint32_t)seed;
+    r->key[1] = (uint32_t)(seed >> 32);
+    r->ray = ray;
+    r->ctx = ctx;
+    r->instance = instance;
+    r->draw = 0;
+}
+
+/* uniform number `draw` of the stream: word (draw & 3) of block (draw >> 2) */
+static float rng_next(or_rng* r) {
+    uint32_t w = r->draw & 3u;
+    if (w == 0) {
+        uint32_t ctr[4] = {r->draw >> 2, r->ray, r->ctx, r->instance};
+        or_philox4x32_10(ctr, r->key, r->buf);
+    }
+    r->draw++;
+    return (float)(r->buf[w] >> 8) * (1.0f / 16777216.0f);
+}
+
+float or_uniform(uint64_t seed, uint32_t ray, uint32_t ctx, uint32_t instance, uint32_t draw) {
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t ctr[4] = {draw >> 2, ray, ctx, instance}, out[4];
+    or_philox4x32_10(ctr, key, out);
+    return (float)(out[draw & 3u] >> 8) * (1.0f / 16777216.0f);
+}
+
+/* ------------------------------------------------------------------------ */
+/* pow: the reference calls pow(float,float) (float overload) at            */
+/* src/Scene.cpp:154,232,243,251,308 and powf at src/StereoRecorder.cpp:110. */
+/* Restated as exp2(y*log2 x) evaluated in binary64 with +,-,*,/ only:      */
+/* log via the atanh series ln m = 2 sum s^(2j+1)/(2j+1), s=(m-1)/(m+1),    */
+/* m in [sqrt(1/2), sqrt 2); exp2 via n = nearest(z), 2^f = exp(f ln2) by a */
+/* degree-17 Taylor polynomial; the binary32 result is the round-to-nearest */
+/* of the binary64 value.                                                   */
+/* ------------------------------------------------------------------------ */
+static uint64_t dbits(double x) {
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    return b;
+}
+static double dfrom(uint64_t b) {
+    double x;
+    memcpy(&x, &b, 8);
+    return x;
+}
+
+static double log2_pos(double x) { /* x > 0, finite, normal */
+    uint64_t b = dbits(x);
+    int e = (int)((b >> 52) & 0x7ffu) - 1023;
+    double m = dfrom((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    double s = (m - 1.0) / (m + 1.0);
+    double s2 = s * s;
+    double p = 1.0 / 23.0;
+    p = p * s2 + 1.0 / 21.0;
+    p = p * s2 + 1.0 / 19.0;
+    p = p * s2 + 1.0 / 17.0;
+    p = p * s2 + 1.0 / 15.0;
+    p = p * s2 + 1.0 / 13.0;
+    p = p * s2 + 1.0 / 11.0;
+    p = p * s2 + 1.0 / 9.0;
+    p = p * s2 + 1.0 / 7.0;
+    p = p * s2 + 1.0 / 5.0;
+    p = p * s2 + 1.0 / 3.0;
+    p = p * s2 + 1.0;
+    double ln_m = 2.0 * s * p;
+    return (double)e + ln_m * 1.4426950408889634; /* 1/ln 2 */
+}
+
+static double exp2_d(double z) {
+    if (z < -1060.0) return 0.0;
+    if (z > 1060.0) return INFINITY;
+    double zn = z + 0.5;
+    long long n = (long long)zn; /* trunc */
+    if ((double)n > zn) n -= 1;  /* floor(z + 0.5) */
+    double f = z - (double)n;    /* exact, |f| <= 0.5 */
+    double g = f * 0.6931471805599453;
+    double p = 1.0 / 355687428096000.0; /* 1/17! */
+    p = p * g + 1.0 / 20922789888000.0; /* 1/16! */
+    p = p * g + 1.0 / 1307674368000.0;
+    p = p * g + 1.0 / 87178291200.0;
+    p = p * g + 1.0 / 6227020800.0;
+    p = p * g + 1.0 / 479001600.0;
+    p = p * g + 1.0 / 39916800.0;
+    p = p * g + 1.0 / 3628800.0;
+    p = p * g + 1.0 / 362880.0;
+    p = p * g + 1.0 / 40320.0;
+    p = p * g + 1.0 / 5040.0;
+    p = p * g + 1.0 / 720.0;
+    p = p * g + 1.0 / 120.0;
+    p = p * g + 1.0 / 24.0;
+    p = p * g + 1.0 / 6.0;
+    p = p * g + 0.5;
+    p = p * g + 1.0;
+    p = p * g + 1.0;
+    /* scale by 2^n in two exact steps so that 2^n factors stay normal */
+    long long n1 = n / 2, n2 = n - n1;
+    double s1 = dfrom((uint64_t)(n1 + 1023) << 52);
+    double s2 = dfrom((uint64_t)(n2 + 1023) << 52);
+    return (p * s1) * s2;
+}
+
+double or_pow_d(double x, double y) {
+    if (isnan(x) || isnan(y)) return NAN;
+    if (y == 0.0 || x == 1.0) return 1.0;
+    if (x == 0.0) return y > 0.0 ? 0.0 : INFINITY;
+    if (x < 0.0) return NAN;
+    if (isinf(x)) return y > 0.0 ? INFINITY : 0.0;
+    if (isinf(y)) return (x < 1.0) == (y > 0.0) ? 0.0 : INFINITY;
+    return exp2_d(y * log2_pos(x));
+}
+
+float or_pow(float x, float y) { return (float)or_pow_d((double)x, (double)y); }
+
+/* ------------------------------------------------------------------------ */
+/* vector helpers: every component operation one rounding, left to right    */
+/* ------------------------------------------------------------------------ */
+static inline float dot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void cross3(const float a[3], const float b[3], float r[3]) {
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline void sub3(const float a[3], const float b[3], float r[3]) {
+    r[0] = a[0] - b[0];
+    r[1] = a[1] - b[1];
+    r[2] = a[2] - b[2];
+}
+static inline float len3(const float a[3]) { return sqrtf(dot3(a, a)); }
+/* gmtl::normalize: divide each component by the length unless it is 0 */
+static inline void normalize3(float a[3]) {
+    float l = len3(a);
+    if (l != 0.0f) {
+        a[0] = a[0] / l;
+        a[1] = a[1] / l;
+        a[2] = a[2] / l;
+    }
+}
+/* gmtl::reflect(result, v, n) = v - 2 (v.n) n */
+static inline void reflect3(const float v[3], const float n[3], float r[3]) {
+    float t = dot3(v, n);
+    r[0] = v[0] - 2.0f * (t * n[0]);
+    r[1] = v[1] - 2.0f * (t * n[1]);
+    r[2] = v[2] - 2.0f * (t * n[2]);
+}
+static inline int is_normal_f(float x) { return fpclassify(x) == FP_NORMAL; }
+
+/* gmtl::intersectDoubleSided(Tri, Ray, u, v, t): Moeller-Trumbore (JGT 1997)
+ * non-culling branch, EPSILON = 1e-5 on the determinant. */
+int or_mt_intersect(const float v0[3], const float e1[3], const float e2[3], const float o[3], const float d[3],
+                    float* t) {
+    float pvec[3], tvec[3], qvec[3];
+    cross3(d, e2, pvec);
+    float det = dot3(e1, pvec);
+    if (det > -0.00001f && det < 0.00001f) return 0;
+    float inv_det = 1.0f / det;
+    sub3(o, v0, tvec);
+    float u = dot3(tvec, pvec) * inv_det;
+    if (u < 0.0f || u > 1.0f) return 0;
+    cross3(tvec, e1, qvec);
+    float v = dot3(d, qvec) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return 0;
+    *t = dot3(e2, qvec) * inv_det;
+    return 1;
+}
+
+/* triangle normal: normalize(cross(v1 - v0, v2 - v0)) (gmtl::normal(Tri),
+ * src/Triangle.cpp:32) */
+void or_tri_normal(const float* v9, float n[3]) {
+    float e1[3], e2[3];
+    sub3(v9 + 3, v9, e1);
+    sub3(v9 + 6, v9, e2);
+    cross3(e1, e2, n);
+    normalize3(n);
+}
+
+/* Triangle::calcArea, src/Triangle.cpp:26-30: |edge(0) x edge(1)| / 2 with
+ * gmtl Tri::edge(i) = v[(i+1)%3] - v[i] */
+static float tri_area(const float* v9) {
+    float a[3], b[3], c[3];
+    sub3(v9 + 3, v9, a);
+    sub3(v9 + 6, v9 + 3, b);
+    cross3(a, b, c);
+    return len3(c) / 2.0f;
+}
+
+/* Material ctor, src/Material.cpp:26-72: the field the reference calls
+ * absorption_coefficient ends up holding retention = 1 - (1 - (fl-1e-9) - (fr-1e-9)).
+ * Returns 0 (the reference throws DatatypeException) when refl + refr > 1. */
+int or_material_retention(const or_material* m, float ret[3]) {
+    for (int i = 0; i < 3; i++) {
+        float a = 1.0f;
+        a -= m->refl[i] - 1e-9f;
+        if (m->flags & OR_MAT_HAS_REFR) a -= m->refr[i] - 1e-9f;
+        if (a < 0.0f) return 0;
+        ret[i] = 1.0f - a;
+    }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* scene queries (brute force, as the reference)                            */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint32_t n;
+    float* v0; /* [n][3] */
+    float* e1;
+    float* e2;
+    float* nrm;
+    uint16_t* mat;
+} prep_scene;
+
+static int prep(const or_scene* s, prep_scene* p) {
+    p->n = s->n_tris;
+    size_t n = s->n_tris ? s->n_tris : 1;
+    p->v0 = (float*)malloc(n * 3 * sizeof(float));
+    p->e1 = (float*)malloc(n * 3 * sizeof(float));
+    p->e2 = (float*)malloc(n * 3 * sizeof(float));
+    p->nrm = (float*)malloc(n * 3 * sizeof(float));
+    p->mat = (uint16_t*)malloc(n * sizeof(uint16_t));
+    if (!p->v0 || !p->e1 || !p->e2 || !p->nrm || !p->mat) return 0;
+    for (uint32_t i = 0; i < s->n_tris; i++) {
+        const float* v = s->v + 9 * (size_t)i;
+        memcpy(p->v0 + 3 * i, v, 3 * sizeof(float));
+        sub3(v + 3, v, p->e1 + 3 * i);
+        sub3(v + 6, v, p->e2 + 3 * i);
+        or_tri_normal(v, p->nrm + 3 * i);
+        p->mat[i] = s->mat ? s->mat[i] : 0;
+    }
+    return 1;
+}
+static void unprep(prep_scene* p) {
+    free(p->v0);
+    free(p->e1);
+    free(p->e2);
+    free(p->nrm);
+    free(p->mat);
+}
+
+/* Mesh::RayIntersection: accept t in (0.001, d), d starts at 1e6; strict '<'
+ * keeps the lowest index on ties. */
+static int closest_hit_p(const prep_scene* s, const float o[3], const float d[3], float* t_out, int32_t* idx) {
+    float best = 1000000.0f;
+    int32_t bi = -1;
+    for (uint32_t i = 0; i < s->n; i++) {
+        float t;
+        if (or_mt_intersect(s->v0 + 3 * i, s->e1 + 3 * i, s->e2 + 3 * i, o, d, &t) && t > 0.001f && t < best) {
+            best = t;
+            bi = (int32_t)i;
+        }
+    }
+    if (bi < 0) return 0;
+    *t_out = best;
+    *idx = bi;
+    return 1;
+}
+
+/* Mesh::LineIntersection on LineSeg(p, q): direction q - p, t in (1e-5, 1) */
+static int any_hit_p(const prep_scene* s, const float p[3], const float dv[3]) {
+    for (uint32_t i = 0; i < s->n; i++) {
+        float t;
+        if (or_mt_intersect(s->v0 + 3 * i, s->e1 + 3 * i, s->e2 + 3 * i, p, dv, &t) && t > 1e-5f && t < 1.0f) return 1;
+    }
+    return 0;
+}
+
+int or_closest_hit(const or_scene* s, const float o[3], const float d[3], float* t, int32_t* idx) {
+    prep_scene p;
+    if (!prep(s, &p)) return -1;
+    int r = closest_hit_p(&p, o, d, t, idx);
+    unprep(&p);
+    return r;
+}
+
+int or_any_hit(const or_scene* s, const float p[3], const float q[3]) {
+    prep_scene ps;
+    if (!prep(s, &ps)) return -1;
+    float dv[3];
+    sub3(q, p, dv);
+    int r = any_hit_p(&ps, p, dv);
+    unprep(&ps);
+    return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* sampling, src/Distributions.h:48-75                                      */
+/* ------------------------------------------------------------------------ */
+static void sample_sphere(or_rng* r, float v[3]) {
+    for (;;) {
+        float f1 = rng_next(r) * 2.0f - 1.0f;
+        float f2 = rng_next(r) * 2.0f - 1.0f;
+        float f3 = rng_next(r) * 2.0f - 1.0f;
+        float l = f1 * f1 + f2 * f2 + f3 * f3;
+        if (l < 0.001f || l > 1.0f) continue;
+        float s = sqrtf(l);
+        v[0] = f1 / s;
+        v[1] = f2 / s;
+        v[2] = f3 / s;
+        return;
+    }
+}
+static void sample_hemi(or_rng* r, const float n[3], float v[3]) {
+    for (;;) {
+        sample_sphere(r, v);
+        if (dot3(n, v) < 0.0f) continue;
+        return;
+    }
+}
+static void sample_hemi_spec(or_rng* r, const float n[3], const float refl[3], float f, float v[3]) {
+    sample_hemi(r, n, v);
+    float g = 1.0f - f;
+    v[0] = v[0] * g + refl[0] * f;
+    v[1] = v[1] * g + refl[1] * f;
+    v[2] = v[2] * g + refl[2] * f;
+    normalize3(v);
+}
+
+void or_sample_dirs(uint64_t seed, uint32_t ctx, uint32_t instance, uint32_t n, int mode, const float nrm[3],
+                    float* out, uint32_t* ndraws) {
+    for (uint32_t i = 0; i < n; i++) {
+        or_rng r;
+        rng_init(&r, seed, i, ctx, instance);
+        if (mode == 0)
+            sample_sphere(&r, out + 3 * i);
+        else
+            sample_hemi(&r, nrm, out + 3 * i);
+        if (ndraws) ndraws[i] = r.draw;
+    }
+}
+
+#define OR_REFLECT 0
+#define OR_REFRACT 1
+
+/* ------------------------------------------------------------------------ */
+/* render                                                                   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const or_desc* d;
+    const prep_scene* s;
+    const float (*ret)[3];   /* retention per material */
+    const or_material* mats;
+    /* mesh source */
+    const float* src_v;
+    const float* src_area;
+    float src_total_area;
+    const float* src_nrm;
+    int n_ch;
+    const int* ch_of_rec; /* first channel of recorder r */
+} rctx;
+
+typedef struct {
+    double* hs;
+    double* ha;
+    float* hf;
+    uint64_t segments, shadow_rays, contributions;
+} acc_t;
+
+static void add_ramp(const rctx* C, acc_t* acc, int ch, int s, int w, float A) {
+    uint32_t nb = C->d->n_bins;
+    double S = (double)A / (double)w;
+    for (int i = 0; i < w; i++) {
+        long long b = (long long)s + i;
+        if (b < 0 || b >= (long long)nb) continue;
+        double v = (double)(w - i) * S;
+        if (acc->hs) acc->hs[(size_t)ch * nb + (size_t)b] += v;
+        if (acc->ha) acc->ha[(size_t)ch * nb + (size_t)b] += fabs(v);
+    }
+}
+
+/* MonoRecorder::Record fp32 form into one track */
+static void ramp_f32(float* buf, uint32_t nb, int s, int w, float ampl) {
+    float step = ampl / (float)w;
+    for (int i = 0; i < w; i++) {
+        long long b = (long long)s + i;
+        if (b >= 0 && b < (long long)nb) buf[b] += ampl;
+        ampl -= step;
+    }
+}
+
+void or_record_f32(float* buf, uint32_t n, float a, float t, float dist) {
+    int s = (int)((double)t * 44100.0);
+    float width = sqrtf(dist);
+    float ampl = 2.0f * a / width;
+    int w = (int)ceilf(width);
+    ramp_f32(buf, n, s, w, ampl);
+}
+
+/* Recorder::Record for recorder r: mono (src/MonoRecorder.cpp:83-97) or
+ * stereo (src/StereoRecorder.cpp:97-130); fills the FP64 ramps and, when
+ * requested, the reference fp32 track. */
+static void record(const rctx* C, acc_t* acc, int r, const float dir[3], float a, float t, float dist) {
+    const or_recorder* rec = &C->d->rec[r];
+    int ch = C->ch_of_rec[r];
+    uint32_t nb = C->d->n_bins;
+    if (rec->kind == OR_REC_MONO) {
+        int s = (int)((double)t * 44100.0);
+        float width = sqrtf(dist);
+        float ampl = 2.0f * a / width;
+        int w = (int)ceilf(width);
+        add_ramp(C, acc, ch, s, w, ampl);
+        if (acc->hf) ramp_f32(acc->hf + (size_t)ch * nb, nb, s, w, ampl);
+    } else {
+        float dt = dot3(dir, rec->right_ear);
+        float td = rec->head_size / 343.0f;
+        int s_right = (int)((double)(t - dt * td) * 44100.0);
+        int s_left = (int)((double)(t + dt * td) * 44100.0);
+        float width = sqrtf(dist);
+        float ampl = 2.0f * a / width;
+        float al = ampl, ar = ampl;
+        float idiff = fabsf(dt);
+        float factor = or_pow(rec->head_abs[C->d->band], idiff * rec->head_size);
+        if (dt < 0.0f)
+            ar *= factor * factor;
+        else
+            al *= factor * factor;
+        int w = (int)ceilf(width);
+        add_ramp(C, acc, ch, s_left, w, al);
+        add_ramp(C, acc, ch + 1, s_right, w, ar);
+        if (acc->hf) {
+            ramp_f32(acc->hf + (size_t)ch * nb, nb, s_left, w, al);
+            ramp_f32(acc->hf + (size_t)(ch + 1) * nb, nb, s_right, w, ar);
+        }
+    }
+}
+
+/* emission from a mesh source: Mesh::SamplePoint + Triangle::SamplePoint +
+ * 2-argument Sample_Hemi about the (unflipped) triangle normal. */
+static void emit_mesh(const rctx* C, or_rng* r, float o[3], float d[3]) {
+    const or_desc* D = C->d;
+    float x = rng_next(r) * C->src_total_area; /* rangeRandom(0, total_area) */
+    uint32_t pick = D->src_n_tris ? D->src_n_tris - 1 : 0;
+    for (uint32_t i = 0; i < D->src_n_tris; i++) {
+        x -= C->src_area[i];
+        if (x < 0.0f) {
+            pick = i;
+            break;
+        }
+    }
+    const float* v = C->src_v + 9 * (size_t)pick;
+    float r1 = rng_next(r);
+    float r2 = rng_next(r);
+    float sr1 = sqrtf(r1);
+    float a = 1.0f - sr1, b = sr1 * (1.0f - r2), c = sr1 * r2;
+    for (int k = 0; k < 3; k++) o[k] = a * v[k] + b * v[3 + k] + c * v[6 + k];
+    sample_hemi(r, C->src_nrm + 3 * pick, d);
+}
+
+static uint32_t material_bounce(const rctx* C, or_rng* r, int mat, int band) {
+    float fl = C->mats[mat].refl[band];
+    float fr = (C->mats[mat].flags & OR_MAT_HAS_REFR) ? C->mats[mat].refr[band] : 0.0f;
+    if (fl < 0.0001 && fr < 0.0001) return OR_REFLECT;
+    float fl2 = fl / (fl + fr);
+    return (rng_next(r) <= fl2) ? OR_REFLECT : OR_REFRACT;
+}
+
+static void trace_ray(const rctx* C, acc_t* acc, uint32_t ray, int32_t* log_hits, uint32_t* log_nseg,
+                      uint32_t* log_ncon) {
+    const or_desc* D = C->d;
+    const int band = D->band;
+    const int mesh_src = D->src_kind == OR_SRC_MESH;
+    or_rng R;
+    rng_init(&R, D->seed, ray, D->ctx, D->instance);
+
+    float I = 1.0f, L = 0.0f;
+    float o[3], dd[3], prev[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+    int bt = OR_REFLECT, mat = -1;
+    uint32_t nseg = 0, ncon = 0;
+
+    for (uint32_t k = 0; k <= D->max_bounces; k++) {
+        float spec = 0.0f;
+        if (k == 0) {
+            if (mesh_src) {
+                emit_mesh(C, &R, o, dd);
+            } else {
+                o[0] = D->src_pos[0];
+                o[1] = D->src_pos[1];
+                o[2] = D->src_pos[2];
+                sample_sphere(&R, dd);
+            }
+        } else {
+            float t;
+            int32_t idx;
+            acc->segments++;
+            int hit = closest_hit_p(C->s, o, dd, &t, &idx);
+            if (log_hits && nseg < D->trace_segs) log_hits[nseg] = hit ? idx : -1;
+            nseg++;
+            if (!hit) break;
+            float p[3];
+            p[0] = o[0] + dd[0] * t;
+            p[1] = o[1] + dd[1] * t;
+            p[2] = o[2] + dd[2] * t;
+            const float* N = C->s->nrm + 3 * idx;
+            if (dot3(N, dd) > 0.0f) {
+                n[0] = N[0] * -1.0f;
+                n[1] = N[1] * -1.0f;
+                n[2] = N[2] * -1.0f;
+            } else {
+                n[0] = N[0];
+                n[1] = N[1];
+                n[2] = N[2];
+            }
+            mat = C->s->mat[idx];
+            bt = (int)material_bounce(C, &R, mat, band);
+            spec = (C->mats[mat].flags & OR_MAT_HAS_SPEC) ? C->mats[mat].spec[band] : 0.0f;
+            float v[3];
+            if (bt == OR_REFRACT) {
+                n[0] = -n[0];
+                n[1] = -n[1];
+                n[2] = -n[2];
+                sample_hemi_spec(&R, n, dd, spec, v);
+            } else {
+                float rf[3];
+                reflect3(dd, n, rf);
+                sample_hemi_spec(&R, n, rf, spec, v);
+            }
+            float dist[3];
+            sub3(p, o, dist);
+            float l = len3(dist);
+            I *= or_pow(D->air, l);
+            L += l;
+            o[0] = p[0];
+            o[1] = p[1];
+            o[2] = p[2];
+            dd[0] = v[0];
+            dd[1] = v[1];
+            dd[2] = v[2];
+            I *= C->ret[mat][band];
+        }
+        if (!is_normal_f(I)) break;
+        if (k > 0 || mesh_src) {
+            for (uint32_t r = 0; r < D->n_rec; r++) {
+                const float* RP = D->rec[r].pos;
+                float dv[3];
+                sub3(RP, o, dv);
+                acc->shadow_rays++;
+                if (any_hit_p(C->s, o, dv)) continue;
+                float lsdir[3] = {dv[0], dv[1], dv[2]};
+                normalize3(lsdir);
+                float dt = k ? dot3(lsdir, n) : 1.0f;
+                if (!(dt > 0.0f)) continue;
+                float c = I;
+                if (k) {
+                    float factor;
+                    if (bt == OR_REFLECT) {
+                        float rv[3];
+                        reflect3(prev, n, rv);
+                        float diff = -dot3(n, prev);
+                        float sf = fmaxf(0.0f, dot3(rv, lsdir));
+                        factor = spec * OR_EXP_INT * or_pow(sf, OR_EXP) + (1.0f - spec) * diff;
+                    } else {
+                        float diff = dot3(n, prev);
+                        float sf = fmaxf(0.0f, dot3(prev, lsdir));
+                        factor = spec * OR_EXP_INT * or_pow(sf, OR_EXP) + (1.0f - spec) * diff;
+                    }
+                    c *= factor;
+                }
+                float l = len3(dv);
+                c *= or_pow(D->air, l);
+                c *= 2.0f / (4.0f * OR_PI * l * l);
+                if (is_normal_f(c)) {
+                    if (k % 2) c *= -1.0f;
+                    record(C, acc, (int)r, lsdir, c, (L + l) / 343.0f, L + l);
+                    acc->contributions++;
+                    ncon++;
+                }
+            }
+        }
+        if ((double)I < D->cutoff) break;
+        prev[0] = dd[0];
+        prev[1] = dd[1];
+        prev[2] = dd[2];
+        normalize3(prev);
+    }
+    if (log_nseg) *log_nseg = nseg;
+    if (log_ncon) *log_ncon = ncon;
+}
+
+typedef struct {
+    const rctx* C;
+    or_out* out;
+    uint32_t begin, end;
+    acc_t acc;
+    int ok;
+} worker_t;
+
+static void* worker(void* arg) {
+    worker_t* W = (worker_t*)arg;
+    const rctx* C = W->C;
+    const or_desc* D = C->d;
+    for (uint32_t ray = W->begin; ray < W->end; ray++) {
+        uint32_t li = ray - D->ray_begin;
+        int logit = W->out->trace_hits && li < D->trace_rays;
+        int32_t* lh = logit ? W->out->trace_hits + (size_t)li * D->trace_segs : NULL;
+        if (lh)
+            for (uint32_t j = 0; j < D->trace_segs; j++) lh[j] = -2;
+        trace_ray(C, &W->acc, ray, lh, (logit && W->out->trace_nseg) ? W->out->trace_nseg + li : NULL,
+                  (logit && W->out->trace_ncon) ? W->out->trace_ncon + li : NULL);
+    }
+    W->ok = 1;
+    return NULL;
+}
+
+int or_num_channels(const or_desc* d) {
+    int n = 0;
+    for (uint32_t r = 0; r < d->n_rec; r++) n += d->rec[r].kind == OR_REC_STEREO ? 2 : 1;
+    return n;
+}
+
+double or_now(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+int or_render(const or_scene* s, const or_desc* D, or_out* out) {
+    if (D->band < 0 || D->band > 2 || D->n_bins == 0) return -1;
+    if (D->ray_end < D->ray_begin || D->ray_end > D->n_rays) return -1;
+    prep_scene ps;
+    if (!prep(s, &ps)) return -2;
+    int rc = 0;
+    float (*ret)[3] = (float (*)[3])calloc(s->n_mat ? s->n_mat : 1, sizeof(float[3]));
+    for (uint32_t m = 0; m < s->n_mat; m++)
+        if (!or_material_retention(&s->mats[m], ret[m])) rc = -3;
+    for (uint32_t i = 0; i < s->n_tris && rc == 0; i++)
+        if (ps.mat[i] >= s->n_mat) rc = -4;
+    int n_ch = or_num_channels(D);
+    int* ch_of = (int*)calloc(D->n_rec ? D->n_rec : 1, sizeof(int));
+    {
+        int c = 0;
+        for (uint32_t r = 0; r < D->n_rec; r++) {
+            ch_of[r] = c;
+            c += D->rec[r].kind == OR_REC_STEREO ? 2 : 1;
+        }
+    }
+    float* src_area = NULL;
+    float* src_nrm = NULL;
+    float src_total = 0.0f;
+    if (D->src_kind == OR_SRC_MESH) {
+        src_area = (float*)calloc(D->src_n_tris ? D->src_n_tris : 1, sizeof(float));
+        src_nrm = (float*)calloc(3 * (D->src_n_tris ? D->src_n_tris : 1), sizeof(float));
+        for (uint32_t i = 0; i < D->src_n_tris; i++) {
+            src_area[i] = tri_area(D->src_v + 9 * (size_t)i);
+            src_total += src_area[i];
+            or_tri_normal(D->src_v + 9 * (size_t)i, src_nrm + 3 * i);
+        }
+        if (D->src_n_tris == 0) rc = -5;
+    }
+    if (rc != 0) {
+        free(ret);
+        free(ch_of);
+        free(src_area);
+        free(src_nrm);
+        unprep(&ps);
+        return rc;
+    }
+
+    rctx C;
+    C.d = D;
+    C.s = &ps;
+    C.ret = (const float (*)[3])ret;
+    C.mats = s->mats;
+    C.src_v = D->src_v;
+    C.src_area = src_area;
+    C.src_total_area = src_total;
+    C.src_nrm = src_nrm;
+    C.n_ch = n_ch;
+    C.ch_of_rec = ch_of;
+
+    uint32_t nt = D->n_threads ? D->n_threads : 1;
+    uint32_t nr = D->ray_end - D->ray_begin;
+    if (nt > nr && nr > 0) nt = nr;
+    if (nt == 0) nt = 1;
+    size_t hsz = (size_t)n_ch * D->n_bins;
+    worker_t* W = (worker_t*)calloc(nt, sizeof(worker_t));
+    pthread_t* th = (pthread_t*)calloc(nt, sizeof(pthread_t));
+    for (uint32_t i = 0; i < nt; i++) {
+        W[i].C = &C;
+        W[i].out = out;
+        W[i].begin = D->ray_begin + (uint32_t)(((uint64_t)nr * i) / nt);
+        W[i].end = D->ray_begin + (uint32_t)(((uint64_t)nr * (i + 1)) / nt);
+        W[i].acc.hs = out->hist_signed ? (double*)calloc(hsz, sizeof(double)) : NULL;
+        W[i].acc.ha = out->hist_abs ? (double*)calloc(hsz, sizeof(double)) : NULL;
+        W[i].acc.hf = (out->hist_f32 && nt == 1) ? (float*)calloc(hsz, sizeof(float)) : NULL;
+    }
+    if (nt == 1) {
+        worker(&W[0]);
+    } else {
+        for (uint32_t i = 0; i < nt; i++) pthread_create(&th[i], NULL, worker, &W[i]);
+        for (uint32_t i = 0; i < nt; i++) pthread_join(th[i], NULL);
+    }
+    if (out->hist_signed) memset(out->hist_signed, 0, hsz * sizeof(double));
+    if (out->hist_abs) memset(out->hist_abs, 0, hsz * sizeof(double));
+    if (out->hist_f32) memset(out->hist_f32, 0, hsz * sizeof(float));
+    out->segments = out->shadow_rays = out->contributions = 0;
+    for (uint32_t i = 0; i < nt; i++) {
+        for (size_t j = 0; j < hsz; j++) {
+            if (W[i].acc.hs) out->hist_signed[j] += W[i].acc.hs[j];
+            if (W[i].acc.ha) out->hist_abs[j] += W[i].acc.ha[j];
+        }
+        if (W[i].acc.hf) memcpy(out->hist_f32, W[i].acc.hf, hsz * sizeof(float));
+        out->segments += W[i].acc.segments;
+        out->shadow_rays += W[i].acc.shadow_rays;
+        out->contributions += W[i].acc.contributions;
+        free(W[i].acc.hs);
+        free(W[i].acc.ha);
+        free(W[i].acc.hf);
+    }
+    free(W);
+    free(th);
+
+    /* direct sound, src/Scene.cpp:299-311: connect listener -> source */
+    if (D->src_kind == OR_SRC_POINT) {
+        for (uint32_t r = 0; r < D->n_rec; r++) {
+            const or_recorder* rec = &D->rec[r];
+            int ch = ch_of[r];
+            int nchr = rec->kind == OR_REC_STEREO ? 2 : 1;
+            for (int c = 0; c < nchr; c++) {
+                if (out->direct_visible) out->direct_visible[ch + c] = 0;
+            }
+            float dv[3];
+            sub3(D->src_pos, rec->pos, dv);
+            if (any_hit_p(&ps, rec->pos, dv)) continue;
+            float dist[3];
+            sub3(rec->pos, D->src_pos, dist);
+            float len = len3(dist);
+            float dir[3] = {dist[0], dist[1], dist[2]};
+            normalize3(dir);
+            float a = 1.0f / (4.0f * OR_PI * len * len) * or_pow(D->air, len) * D->dry;
+            float t = len / 343.0f;
+            float width = sqrtf(len);
+            float ampl = 2.0f * a / width;
+            int w = (int)ceilf(width);
+            if (rec->kind == OR_REC_MONO) {
+                if (out->direct_visible) {
+                    out->direct_visible[ch] = 1;
+                    out->direct_s[ch] = (int)((double)t * 44100.0);
+                    out->direct_w[ch] = w;
+                    out->direct_a[ch] = ampl;
+                }
+            } else {
+                float dt = dot3(dir, rec->right_ear);
+                float td = rec->head_size / 343.0f;
+                float al = ampl, ar = ampl;
+                float factor = or_pow(rec->head_abs[D->band], fabsf(dt) * rec->head_size);
+                if (dt < 0.0f)
+                    ar *= factor * factor;
+                else
+                    al *= factor * factor;
+                if (out->direct_visible) {
+                    out->direct_visible[ch] = 1;
+                    out->direct_visible[ch + 1] = 1;
+                    out->direct_s[ch] = (int)((double)(t + dt * td) * 44100.0);
+                    out->direct_s[ch + 1] = (int)((double)(t - dt * td) * 44100.0);
+                    out->direct_w[ch] = w;
+                    out->direct_w[ch + 1] = w;
+                    out->direct_a[ch] = al;
+                    out->direct_a[ch + 1] = ar;
+                }
+            }
+        }
+    } else if (out->direct_visible) {
+        for (int c = 0; c < n_ch; c++) out->direct_visible[c] = 0;
+    }
+
+    free(ret);
+    free(ch_of);
+    free(src_area);
+    free(src_nrm);
+    unprep(&ps);
+    return 0;
+}

@@ -1,0 +1,135 @@
+/*
+ * ear_oracle.h -- CPU oracle for the E.A.R acoustic ray tracer.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests, the smoke
+ * check and the CPU-baseline leg of the benchmark load it.  The product
+ * (libear_hip.so and the host C++ layer) never includes, links or calls it.
+ *
+ * It restates, in plain C11, what the reference's Scene::Render does for one
+ * (sound, keyframe, band) context (src/Scene.cpp:111-318) and everything that
+ * function calls:
+ *   - Mesh::RayIntersection / LineIntersection  (src/Mesh.cpp:33-71)
+ *   - Scene::Bounce / Scene::Connect             (src/Scene.cpp:49-96)
+ *   - Material ctor + Material::Bounce           (src/Material.cpp:26-83)
+ *   - Sample_Sphere / Sample_Hemi                (src/Distributions.h:48-75)
+ *   - AbstractSoundFile::SoundRay                (src/SoundFile.cpp:215-228)
+ *   - Mesh::SamplePoint, Triangle::SamplePoint   (src/Mesh.cpp:143-154,
+ *                                                 src/Triangle.cpp:44-53)
+ *   - MonoRecorder::Record / StereoRecorder::Record
+ *                                                (src/MonoRecorder.cpp:83-97,
+ *                                                 src/StereoRecorder.cpp:97-130)
+ * with the random stream replaced by counter-based Philox4x32-10 (the
+ * reference seeds libc rand() with time(NULL), src/Scene.cpp:116, so its
+ * output is not reproducible) and the un-vendored gmtl 0.6.1 arithmetic
+ * restated from the textbook routines (Moeller-Trumbore 1997 non-culling
+ * test, reflect, normalize).
+ *
+ * Histograms are accumulated two ways: in FP64 (exact ramp (w-i)*S per bin,
+ * signed and |c|) and, single-threaded only, in the reference's own fp32
+ * iterated-subtraction form.
+ */
+#ifndef EAR_ORACLE_H
+#define EAR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_REC_MONO 0
+#define OR_REC_STEREO 1
+#define OR_SRC_POINT 0
+#define OR_SRC_MESH 1
+
+#define OR_MAT_HAS_REFR 1u
+#define OR_MAT_HAS_SPEC 2u
+
+typedef struct {
+    float refl[3];
+    float refr[3];
+    float spec[3];
+    uint32_t flags; /* OR_MAT_HAS_REFR | OR_MAT_HAS_SPEC: optional .EAR fields present */
+} or_material;
+
+typedef struct {
+    int32_t kind; /* OR_REC_MONO / OR_REC_STEREO */
+    float pos[3];
+    float right_ear[3];
+    float head_size;
+    float head_abs[3]; /* already max(0,(1-h)^4) as in the StereoRecorder ctor */
+} or_recorder;
+
+typedef struct {
+    uint32_t n_tris;
+    const float* v;       /* [n_tris][3 vertices][xyz] */
+    const uint16_t* mat;  /* [n_tris] */
+    uint32_t n_mat;
+    const or_material* mats;
+} or_scene;
+
+typedef struct {
+    int32_t band;
+    float air; /* absorbtion_factor = 1 - absorption[band] */
+    float dry;
+    float gain;
+    int32_t src_kind;
+    float src_pos[3];
+    uint32_t src_n_tris;
+    const float* src_v; /* mesh source triangles [n][3][3] */
+    uint32_t n_rec;
+    const or_recorder* rec;
+    uint32_t ray_begin, ray_end; /* traced sub-range of [0, n_rays) */
+    uint32_t n_rays;             /* N used for 1/N */
+    uint32_t max_bounces;        /* reference: 999 (num_bounces < 1000) */
+    double cutoff;               /* reference: 1e-8 */
+    uint64_t seed;
+    uint32_t ctx;      /* context word of the Philox counter */
+    uint32_t instance; /* instance word of the Philox counter */
+    uint32_t n_bins;
+    uint32_t n_threads;
+    uint32_t trace_rays, trace_segs;
+} or_desc;
+
+typedef struct {
+    double* hist_signed;  /* [n_ch][n_bins] raw sums (before 1/N), may be NULL */
+    double* hist_abs;     /* [n_ch][n_bins] raw sums of |ramp|, may be NULL */
+    float* hist_f32;      /* [n_ch][n_bins] reference fp32 accumulation (n_threads==1) */
+    int32_t* trace_hits;  /* [trace_rays][trace_segs] original tri index, -1 miss, -2 unused */
+    uint32_t* trace_nseg; /* [trace_rays] closest-hit queries per ray */
+    uint32_t* trace_ncon; /* [trace_rays] recorded contributions per ray */
+    uint64_t segments, shadow_rays, contributions;
+    /* direct sound per channel (src/Scene.cpp:299-311): */
+    int32_t* direct_visible; /* [n_ch] */
+    int32_t* direct_s;       /* [n_ch] first bin */
+    int32_t* direct_w;       /* [n_ch] ramp width */
+    float* direct_a;         /* [n_ch] ramp amplitude (fp32, 2a/sqrt(d) and ILD applied) */
+} or_out;
+
+/* ---- building blocks (exported for the tests) ---- */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float or_uniform(uint64_t seed, uint32_t ray, uint32_t ctx, uint32_t instance, uint32_t draw);
+float or_pow(float x, float y);
+double or_pow_d(double x, double y);
+int or_mt_intersect(const float v0[3], const float e1[3], const float e2[3],
+                    const float o[3], const float d[3], float* t);
+int or_closest_hit(const or_scene* s, const float o[3], const float d[3], float* t, int32_t* idx);
+int or_any_hit(const or_scene* s, const float p[3], const float q[3]);
+void or_tri_normal(const float* v9, float n[3]);
+int or_material_retention(const or_material* m, float ret[3]);
+/* draws n samples from ray stream `ray`; mode 0 sphere, 1 hemisphere about n;
+ * writes xyz to out and the number of uniforms consumed per sample to ndraws */
+void or_sample_dirs(uint64_t seed, uint32_t ctx, uint32_t instance, uint32_t n, int mode,
+                    const float nrm[3], float* out, uint32_t* ndraws);
+/* reference ramp splat, fp32 iterated form, into buf[n] (src/MonoRecorder.cpp:83-97) */
+void or_record_f32(float* buf, uint32_t n, float a, float t, float dist);
+
+/* ---- the per-context render ---- */
+int or_render(const or_scene* s, const or_desc* d, or_out* out);
+int or_num_channels(const or_desc* d);
+double or_now(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
