@@ -433,6 +433,9 @@ typedef struct {
     double* hs;
     double* ha;
     float* hf;
+    double* tally;
+    double* tally_abs;
+    double path_length;
     uint64_t segments, shadow_rays, contributions;
 } acc_t;
 
@@ -473,6 +476,8 @@ static void record(const rctx* C, acc_t* acc, int r, const float dir[3], float a
     const or_recorder* rec = &C->d->rec[r];
     int ch = C->ch_of_rec[r];
     uint32_t nb = C->d->n_bins;
+    acc->tally[r] += (double)a;
+    acc->tally_abs[r] += fabs((double)a);
     if (rec->kind == OR_REC_MONO) {
         int s = (int)((double)t * 44100.0);
         float width = sqrtf(dist);
@@ -535,7 +540,7 @@ static uint32_t material_bounce(const rctx* C, or_rng* r, int mat, int band) {
 }
 
 static void trace_ray(const rctx* C, acc_t* acc, uint32_t ray, int32_t* log_hits, uint32_t* log_nseg,
-                      uint32_t* log_ncon) {
+                      uint32_t* log_ncon, float* log_od) {
     const or_desc* D = C->d;
     const int band = D->band;
     const int mesh_src = D->src_kind == OR_SRC_MESH;
@@ -564,6 +569,10 @@ static void trace_ray(const rctx* C, acc_t* acc, uint32_t ray, int32_t* log_hits
             acc->segments++;
             int hit = closest_hit_p(C->s, o, dd, &t, &idx);
             if (log_hits && nseg < D->trace_segs) log_hits[nseg] = hit ? idx : -1;
+            if (log_od && nseg < D->trace_segs) {
+                float* q = log_od + 6 * (size_t)nseg;
+                q[0] = o[0], q[1] = o[1], q[2] = o[2], q[3] = dd[0], q[4] = dd[1], q[5] = dd[2];
+            }
             nseg++;
             if (!hit) break;
             float p[3];
@@ -597,6 +606,7 @@ static void trace_ray(const rctx* C, acc_t* acc, uint32_t ray, int32_t* log_hits
             float dist[3];
             sub3(p, o, dist);
             float l = len3(dist);
+            acc->path_length += (double)l;
             I *= or_pow(D->air, l);
             L += l;
             o[0] = p[0];
@@ -675,7 +685,8 @@ static void* worker(void* arg) {
         if (lh)
             for (uint32_t j = 0; j < D->trace_segs; j++) lh[j] = -2;
         trace_ray(C, &W->acc, ray, lh, (logit && W->out->trace_nseg) ? W->out->trace_nseg + li : NULL,
-                  (logit && W->out->trace_ncon) ? W->out->trace_ncon + li : NULL);
+                  (logit && W->out->trace_ncon) ? W->out->trace_ncon + li : NULL,
+                  (logit && W->out->trace_rays_od) ? W->out->trace_rays_od + (size_t)li * D->trace_segs * 6 : NULL);
     }
     W->ok = 1;
     return NULL;
@@ -762,6 +773,8 @@ int or_render(const or_scene* s, const or_desc* D, or_out* out) {
         W[i].acc.hs = out->hist_signed ? (double*)calloc(hsz, sizeof(double)) : NULL;
         W[i].acc.ha = out->hist_abs ? (double*)calloc(hsz, sizeof(double)) : NULL;
         W[i].acc.hf = (out->hist_f32 && nt == 1) ? (float*)calloc(hsz, sizeof(float)) : NULL;
+        W[i].acc.tally = (double*)calloc(D->n_rec ? D->n_rec : 1, sizeof(double));
+        W[i].acc.tally_abs = (double*)calloc(D->n_rec ? D->n_rec : 1, sizeof(double));
     }
     if (nt == 1) {
         worker(&W[0]);
@@ -773,7 +786,19 @@ int or_render(const or_scene* s, const or_desc* D, or_out* out) {
     if (out->hist_abs) memset(out->hist_abs, 0, hsz * sizeof(double));
     if (out->hist_f32) memset(out->hist_f32, 0, hsz * sizeof(float));
     out->segments = out->shadow_rays = out->contributions = 0;
+    out->path_length = 0.0;
+    for (uint32_t r = 0; r < D->n_rec; r++) {
+        if (out->tally) out->tally[r] = 0.0;
+        if (out->tally_abs) out->tally_abs[r] = 0.0;
+    }
     for (uint32_t i = 0; i < nt; i++) {
+        for (uint32_t r = 0; r < D->n_rec; r++) {
+            if (out->tally) out->tally[r] += W[i].acc.tally[r];
+            if (out->tally_abs) out->tally_abs[r] += W[i].acc.tally_abs[r];
+        }
+        out->path_length += W[i].acc.path_length;
+        free(W[i].acc.tally);
+        free(W[i].acc.tally_abs);
         for (size_t j = 0; j < hsz; j++) {
             if (W[i].acc.hs) out->hist_signed[j] += W[i].acc.hs[j];
             if (W[i].acc.ha) out->hist_abs[j] += W[i].acc.ha[j];

@@ -104,6 +104,10 @@ typedef struct {
     int32_t* direct_s;       /* [n_ch] first bin */
     int32_t* direct_w;       /* [n_ch] ramp width */
     float* direct_a;         /* [n_ch] ramp amplitude (fp32, 2a/sqrt(d) and ILD applied) */
+    double* tally;           /* [n_rec] sum of recorded contributions c (before the splat) */
+    double* tally_abs;       /* [n_rec] sum of |c| */
+    double path_length;      /* sum of traced segment lengths (hits only) */
+    float* trace_rays_od;    /* [trace_rays][trace_segs][6] origin + direction of each query, may be NULL */
 } or_out;
 
 /* ---- building blocks (exported for the tests) ---- */

@@ -52,7 +52,8 @@ class Out(C.Structure):
                 ("trace_hits", C.c_void_p), ("trace_nseg", C.c_void_p), ("trace_ncon", C.c_void_p),
                 ("segments", C.c_uint64), ("shadow_rays", C.c_uint64), ("contributions", C.c_uint64),
                 ("direct_visible", C.c_void_p), ("direct_s", C.c_void_p), ("direct_w", C.c_void_p),
-                ("direct_a", C.c_void_p)]
+                ("direct_a", C.c_void_p), ("tally", C.c_void_p), ("tally_abs", C.c_void_p),
+                ("path_length", C.c_double), ("trace_rays_od", C.c_void_p)]
 
 
 _lib = None
@@ -247,19 +248,25 @@ def render(scene: OracleScene, *, band, air, recorders, n_rays, src_pos=(0, 0, 0
     o.hist_signed = hs.ctypes.data
     o.hist_abs = ha.ctypes.data if ha is not None else None
     o.hist_f32 = hf.ctypes.data if hf is not None else None
+    tod = np.zeros((max(trace_rays, 1), max(trace_segs, 1), 6), np.float32)
     if trace_rays:
         o.trace_hits, o.trace_nseg, o.trace_ncon = th.ctypes.data, tn.ctypes.data, tc.ctypes.data
+        o.trace_rays_od = tod.ctypes.data
     o.direct_visible, o.direct_s, o.direct_w, o.direct_a = dv.ctypes.data, ds.ctypes.data, dw.ctypes.data, da.ctypes.data
+    tl = np.zeros(max(len(recorders), 1), np.float64)
+    tla = np.zeros(max(len(recorders), 1), np.float64)
+    o.tally, o.tally_abs = tl.ctypes.data, tla.ctypes.data
     t0 = lib().or_now()
     rc = lib().or_render(C.byref(scene.s), C.byref(d), C.byref(o))
     t1 = lib().or_now()
     if rc != 0:
         raise RuntimeError(f"or_render failed: {rc}")
     res = dict(hist_signed=hs, hist_abs=ha, hist_f32=hf, segments=o.segments, shadow_rays=o.shadow_rays,
-               contributions=o.contributions, seconds=t1 - t0,
+               contributions=o.contributions, seconds=t1 - t0, path_length=o.path_length,
+               tally=tl[:len(recorders)].copy(), tally_abs=tla[:len(recorders)].copy(),
                direct=dict(visible=dv[:nch].copy(), s=ds[:nch].copy(), w=dw[:nch].copy(), a=da[:nch].copy()))
     if trace_rays:
-        res.update(trace_hits=th, trace_nseg=tn, trace_ncon=tc)
+        res.update(trace_hits=th, trace_nseg=tn, trace_ncon=tc, trace_od=tod)
     del src
     return res
 
