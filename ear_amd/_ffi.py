@@ -36,12 +36,13 @@ class ear_material(C.Structure):
 
 
 class ear_bvh_opts(C.Structure):
-    _fields_ = [("leaf_size", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+    _fields_ = [("leaf_size", C.c_uint32), ("width", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class ear_scene_info(C.Structure):
     _fields_ = [("n_tris", C.c_uint32), ("n_nodes", C.c_uint32), ("max_depth", C.c_uint32),
-                ("leaf_size", C.c_uint32), ("bounds", C.c_float * 6), ("build_ms", C.c_double)]
+                ("leaf_size", C.c_uint32), ("bounds", C.c_float * 6), ("build_ms", C.c_double),
+                ("width", C.c_uint32), ("node_bytes", C.c_uint32), ("collapse_ms", C.c_double)]
 
 
 class ear_recorder(C.Structure):
@@ -80,7 +81,8 @@ class ear_stats(C.Structure):
 
 class ear_tuning(C.Structure):
     _fields_ = [("block_extend", C.c_uint32), ("block_shade", C.c_uint32), ("block_shadow", C.c_uint32),
-                ("ray_chunk", C.c_uint32), ("waves_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("ray_chunk", C.c_uint32), ("waves_per_cu", C.c_uint32), ("stack_lds", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 _lib = None

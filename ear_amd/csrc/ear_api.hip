@@ -126,7 +126,8 @@ ear_status ear_scene_upload(ear_ctx* c, const float* v0, const float* v1, const 
     ear_scene* s = new (std::nothrow) ear_scene();
     if (!s) return EAR_ERR_OUT_OF_MEMORY;
     s->ctx = c;
-    ear_status st = build_scene(c, v0, v1, v2, mat_id, n_tris, mats, n_mat, opts ? opts->leaf_size : 0, &s->ds);
+    ear_status st = build_scene(c, v0, v1, v2, mat_id, n_tris, mats, n_mat, opts ? opts->leaf_size : 0,
+                                 opts ? opts->width : 0, &s->ds);
     if (st != EAR_OK) {
         delete s;
         return st;
@@ -152,6 +153,13 @@ ear_status ear_scene_get_info(const ear_scene* s, ear_scene_info* info) {
     info->leaf_size = s->ds.leaf_size;
     for (int k = 0; k < 6; ++k) info->bounds[k] = s->ds.bounds[k];
     info->build_ms = s->ds.build_ms;
+    info->width = s->ds.width;
+    info->node_bytes = s->ds.width == 4 ? 112 : 56;
+    info->collapse_ms = s->ds.collapse_ms;
+    if (s->ds.width == 4) {
+        info->n_nodes = s->ds.n_nodes4;
+        info->max_depth = s->ds.depth4 + 1;
+    }
     return EAR_OK;
 }
 

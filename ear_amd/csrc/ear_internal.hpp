@@ -24,7 +24,21 @@ struct alignas(64) BvhNode {
 };
 constexpr int32_t kEmptyChild = (int32_t)0x80000000u;
 constexpr uint32_t kMaxTris = 1u << 24;
-constexpr int kStackDepth = 64;   // > max tree depth (30 Morton bits + 24 index bits)
+// Traversal stack entries per thread (LDS part + global overflow part).  A
+// 2-wide tree pushes at most one entry per level (depth <= 30 Morton bits +
+// 24 index bits); a 4-wide tree at most three per level; the build checks
+// the bound for the tree it produced.
+constexpr int kStackDepth = 96;
+
+// BVH4 node, 128 B: child c's padded box in component c of each float4;
+// ref as in BvhNode; kEmpty4 marks an unused slot (its box lies at 1e30, so
+// the slab test never accepts it).
+struct alignas(128) Bvh4Node {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 ref;
+    int4 pad;
+};
+constexpr int32_t kEmpty4 = 0x7FFFFFFF;
 constexpr int kMaxRecorders = 16;
 constexpr int kMaxChannels = 32;
 
@@ -35,6 +49,11 @@ constexpr int kMaxChannels = 32;
 struct DeviceScene {
     uint32_t n_tris = 0;
     uint32_t n_nodes = 0;
+    uint32_t width = 2;
+    uint32_t n_nodes4 = 0;
+    uint32_t depth4 = 0;
+    double collapse_ms = 0;
+    Bvh4Node* nodes4 = nullptr;
     uint32_t n_mat = 0;
     uint32_t leaf_size = 4;
     uint32_t max_depth = 0;
@@ -104,7 +123,7 @@ ear_status hip_status(Ctx* c, hipError_t e, const char* what);
 // lbvh.hip
 ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2, const uint16_t* mat_id,
                        uint32_t n_tris, const ear_material* mats, uint32_t n_mat, uint32_t leaf_size,
-                       DeviceScene* out);
+                       uint32_t width, DeviceScene* out);
 void free_scene(Ctx* c, DeviceScene* s);
 
 // tracer.hip

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -370,6 +371,8 @@ struct Buf {
 
 void free_scene(Ctx* c, DeviceScene* s) {
     if (s->nodes) dev_free(c, s->nodes, sizeof(BvhNode) * std::max<uint32_t>(1, s->n_nodes));
+    if (s->nodes4) dev_free(c, s->nodes4, sizeof(Bvh4Node) * std::max<uint32_t>(1, s->n_nodes4));
+    s->nodes4 = nullptr;
     if (s->tris) dev_free(c, s->tris, sizeof(float4) * 3 * std::max<uint32_t>(1, s->n_tris));
     if (s->nm) dev_free(c, s->nm, sizeof(float4) * std::max<uint32_t>(1, s->n_tris));
     if (s->mat_band) dev_free(c, s->mat_band, sizeof(float4) * 3 * std::max<uint32_t>(1, s->n_mat));
@@ -383,9 +386,115 @@ void free_scene(Ctx* c, DeviceScene* s) {
         if (_s != EAR_OK) return _s;         \
     } while (0)
 
+// Collapse the 2-wide tree into a 4-wide one: each 4-wide node takes a 2-wide
+// node's two children and, while fewer than four, replaces the internal child
+// with the largest box surface area by its two children.  Boxes (already
+// padded) and leaf references carry over unchanged; nodes are laid out in
+// breadth-first order so siblings sit next to each other.
+static ear_status collapse_to_bvh4(Ctx* c, DeviceScene* s) {
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<BvhNode> n2(s->n_nodes);
+    EAR_HIP_TRY(c, hipMemcpy(n2.data(), s->nodes, sizeof(BvhNode) * s->n_nodes, hipMemcpyDeviceToHost));
+    struct Child {
+        float b[6];
+        int32_t ref;
+    };
+    auto kids = [&](int32_t node, Child out[2]) {
+        const BvhNode& N = n2[node];
+        float a[12] = {N.a.x, N.a.y, N.a.z, N.a.w, N.b.x, N.b.y, N.b.z, N.b.w, N.c.x, N.c.y, N.c.z, N.c.w};
+        for (int k = 0; k < 6; ++k) {
+            out[0].b[k] = a[k];
+            out[1].b[k] = a[6 + k];
+        }
+        out[0].ref = N.d.x;
+        out[1].ref = N.d.y;
+    };
+    auto area = [](const Child& ch) {
+        float dx = ch.b[3] - ch.b[0], dy = ch.b[4] - ch.b[1], dz = ch.b[5] - ch.b[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    std::vector<Bvh4Node> out;
+    out.reserve(s->n_nodes / 2 + 2);
+    struct Work {
+        int32_t n2, slot, depth;
+    };
+    std::vector<Work> queue;
+    queue.push_back({0, 0, 1});
+    out.emplace_back();
+    uint32_t depth = 1;
+    for (size_t qi = 0; qi < queue.size(); ++qi) {
+        Work w = queue[qi];
+        depth = std::max<uint32_t>(depth, (uint32_t)w.depth);
+        Child ch[4];
+        int nch = 2;
+        kids(w.n2, ch);
+        while (nch < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (int i = 0; i < nch; ++i)
+                if (ch[i].ref >= 0 && area(ch[i]) > ba) {
+                    ba = area(ch[i]);
+                    best = i;
+                }
+            if (best < 0) break;
+            Child sub[2];
+            kids(ch[best].ref, sub);
+            ch[best] = sub[0];
+            ch[nch++] = sub[1];
+        }
+        float lo[3][4], hi[3][4];
+        int32_t ref[4];
+        for (int i = 0; i < 4; ++i) {
+            if (i < nch) {
+                for (int k = 0; k < 3; ++k) {
+                    lo[k][i] = ch[i].b[k];
+                    hi[k][i] = ch[i].b[3 + k];
+                }
+                if (ch[i].ref >= 0) {
+                    int32_t slot = (int32_t)out.size();
+                    out.emplace_back();
+                    queue.push_back({ch[i].ref, slot, w.depth + 1});
+                    ref[i] = slot;
+                } else {
+                    ref[i] = ch[i].ref;
+                }
+            } else {
+                for (int k = 0; k < 3; ++k) lo[k][i] = hi[k][i] = 1e30f;
+                ref[i] = kEmpty4;
+            }
+        }
+        Bvh4Node& N = out[w.slot];
+        N.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+        N.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+        N.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+        N.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+        N.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+        N.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        N.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+        N.pad = make_int4(0, 0, 0, 0);
+    }
+    if (3 * depth + 1 >= (uint32_t)kStackDepth) {
+        c->last_error = "4-wide BVH deeper than the traversal stack";
+        return EAR_ERR_CAPACITY;
+    }
+    ear_status st = dev_alloc(c, (void**)&s->nodes4, sizeof(Bvh4Node) * out.size());
+    if (st != EAR_OK) return st;
+    s->n_nodes4 = (uint32_t)out.size();
+    EAR_HIP_TRY(c, hipMemcpy(s->nodes4, out.data(), sizeof(Bvh4Node) * out.size(), hipMemcpyHostToDevice));
+    s->depth4 = depth;
+    s->width = 4;
+    s->collapse_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return EAR_OK;
+}
+
 ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2, const uint16_t* mat_id,
-                       uint32_t n, const ear_material* mats, uint32_t n_mat, uint32_t leaf_size,
+                       uint32_t n, const ear_material* mats, uint32_t n_mat, uint32_t leaf_size, uint32_t width,
                        DeviceScene* out) {
+    if (width == 0) width = 4;
+    if (width != 2 && width != 4) {
+        c->last_error = "BVH width must be 2 or 4";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
     if (n > 0 && (!v0 || !v1 || !v2 || !mat_id)) {
         c->last_error = "null triangle array";
         return EAR_ERR_INVALID_ARGUMENT;
@@ -476,10 +585,7 @@ ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2
     TRY(part.alloc((size_t)npart * 12 * 4));
     TRY(bnd.alloc(12 * 4));
     k_bounds_partial<<<npart, 256, 0, st>>>(n, box.as<float>(), cen.as<float>(), part.as<float>());
-    k_bounds_partial<<<1, 256, 0, st>>>((uint32_t)npart, part.as<float>() /*unused as box*/, part.as<float>(),
-                                        bnd.as<float>());
-    // The second pass above reduced only centroid-style triples; redo the
-    // vertex box on the host from the partials for exactness of the info.
+    // final min/max over the (at most 1024) per-block partials on the host
     std::vector<float> hpart((size_t)npart * 12);
     EAR_HIP_TRY(c, hipMemcpyAsync(hpart.data(), part.p, hpart.size() * 4, hipMemcpyDeviceToHost, st));
     EAR_HIP_TRY(c, hipStreamSynchronize(st));
@@ -585,6 +691,13 @@ ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2
         free_scene(c, &s);
         c->last_error = "BVH deeper than the traversal stack";
         return EAR_ERR_CAPACITY;
+    }
+    if (width == 4) {
+        ear_status cs = collapse_to_bvh4(c, &s);
+        if (cs != EAR_OK) {
+            free_scene(c, &s);
+            return cs;
+        }
     }
     *out = s;
     return EAR_OK;

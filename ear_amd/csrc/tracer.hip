@@ -34,7 +34,7 @@ namespace ear {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kLdsDepth = 32;  // stack entries in LDS; deeper entries go to a global overflow area
+constexpr int kLdsMin = 16;  // smallest LDS part of the traversal stack; the overflow area covers the rest
 
 enum StatIdx {
     ST_SEGMENTS = 0,
@@ -60,10 +60,11 @@ struct ShadowQ {
 };
 
 struct Traversal {
-    const BvhNode* __restrict__ nodes;
+    const BvhNode* __restrict__ nodes;    // 2-wide tree
+    const Bvh4Node* __restrict__ nodes4;  // 4-wide tree (null when the scene is 2-wide)
     const float4* __restrict__ tris;
     uint32_t n_tris;
-    int* gstack;        // overflow stack [(kStackDepth - kLdsDepth)][gthreads]
+    int* gstack;        // overflow stack [(kStackDepth - LDS)][gthreads]
     uint32_t gthreads;  // persistent threads sharing gstack
 };
 
@@ -114,85 +115,175 @@ __device__ __forceinline__ void slab(const SlabRay& r, float lx, float ly, float
     tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
 }
 
-// Closest hit (ANY = false): t in (0.001, best_t) with the lower original index
-// winning ties, as the reference's strict '<' scan does (src/Mesh.cpp:39-45).
-// Any hit (ANY = true): segment parameter t in (1e-5, 1) (src/Mesh.cpp:58-71).
-// Returns true on a hit; best_* hold the closest hit.
+// Per-thread traversal stack: the first LDS entries in shared memory
+// (column-major over the block so a wave's accesses are conflict free), the
+// rest in a global overflow area indexed [entry][persistent thread].
+template <int LDS>
+struct Stack {
+    int* lds;
+    int* gstack;
+    uint32_t gtid, gthreads;
+    int sp = 0;
+    __device__ __forceinline__ void push(int v) {
+        if (sp < LDS)
+            lds[sp * kBlock] = v;
+        else
+            gstack[(size_t)(sp - LDS) * gthreads + gtid] = v;
+        ++sp;
+    }
+    __device__ __forceinline__ int pop() {
+        --sp;
+        return sp < LDS ? lds[sp * kBlock] : gstack[(size_t)(sp - LDS) * gthreads + gtid];
+    }
+};
+
+// Tests the triangles of leaf reference `cref`.  Closest hit (ANY = false):
+// t in (0.001, best_t) with the lower original index winning ties, as the
+// reference's strict '<' scan does (src/Mesh.cpp:39-45).  Any hit (ANY =
+// true): segment parameter t in (1e-5, 1) (src/Mesh.cpp:58-71); returns true
+// as soon as one is found.
 template <bool ANY, bool INSTR>
+__device__ __forceinline__ bool test_leaf(const Traversal& T, int32_t cref, V3 o, V3 d, float& best_t, int& best_s,
+                                          uint32_t& best_o, bool& found, uint32_t& ntri) {
+    uint32_t u = (uint32_t)cref;
+    uint32_t first = u & 0x00FFFFFFu;
+    uint32_t cnt = ((u >> 24) & 0x7Fu) + 1;
+    for (uint32_t i = first; i < first + cnt; ++i) {
+        float4 t0 = T.tris[3 * i], t1 = T.tris[3 * i + 1], t2 = T.tris[3 * i + 2];
+        V3 v0{t0.x, t0.y, t0.z}, e1{t0.w, t1.x, t1.y}, e2{t1.z, t1.w, t2.x};
+        float t;
+        if (INSTR) ++ntri;
+        if (mt_intersect(v0, e1, e2, o, d, t)) {
+            if (ANY) {
+                if (t > 1e-5f && t < 1.0f) return true;
+            } else {
+                uint32_t orig = __float_as_uint(t2.y);
+                if (t > 0.001f && (t < best_t || (t == best_t && orig < best_o))) {
+                    best_t = t;
+                    best_s = (int)i;
+                    best_o = orig;
+                    found = true;
+                }
+            }
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ void cswap(float& ka, int& va, float& kb, int& vb) {
+    if (kb < ka) {
+        float tk = ka;
+        ka = kb;
+        kb = tk;
+        int tv = va;
+        va = vb;
+        vb = tv;
+    }
+}
+
+// Returns true on a hit; best_* hold the closest hit (ANY = false).
+template <int LDS, int WIDTH, bool ANY, bool INSTR>
 __device__ bool traverse(const Traversal& T, V3 o, V3 d, float& best_t, int& best_s, uint32_t& best_o, int* lds,
                          uint32_t gtid, uint32_t& nvis, uint32_t& ntri) {
     if (T.n_tris == 0) return false;
     SlabRay sr{o.x, o.y, o.z, safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
-    int sp = 0;
+    Stack<LDS> stk{lds, T.gstack, gtid, T.gthreads};
     int node = 0;
     bool found = false;
     for (;;) {
-        const BvhNode* N = T.nodes + node;
-        float4 a = N->a, b = N->b, c = N->c;
-        int4 ch = N->d;
         if (INSTR) ++nvis;
-        float tn0, tf0, tn1, tf1;
-        slab(sr, a.x, a.y, a.z, a.w, b.x, b.y, tn0, tf0);
-        slab(sr, b.z, b.w, c.x, c.y, c.z, c.w, tn1, tf1);
-        float lim = ANY ? 1.0f : best_t;
-        bool h0 = tn0 <= tf0 && tf0 >= 0.0f && tn0 <= lim;
-        bool h1 = tn1 <= tf1 && tf1 >= 0.0f && tn1 <= lim;
-        // leaf children are tested right away
-        for (int k = 0; k < 2; ++k) {
-            int cref = k == 0 ? ch.x : ch.y;
-            bool hk = k == 0 ? h0 : h1;
-            if (hk && cref < 0) {
-                uint32_t u = (uint32_t)cref;
-                uint32_t first = u & 0x00FFFFFFu;
-                uint32_t cnt = ((u >> 24) & 0x7Fu) + 1;
-                for (uint32_t i = first; i < first + cnt; ++i) {
-                    float4 t0 = T.tris[3 * i], t1 = T.tris[3 * i + 1], t2 = T.tris[3 * i + 2];
-                    V3 v0{t0.x, t0.y, t0.z}, e1{t0.w, t1.x, t1.y}, e2{t1.z, t1.w, t2.x};
-                    float t;
-                    if (INSTR) ++ntri;
-                    if (mt_intersect(v0, e1, e2, o, d, t)) {
-                        if (ANY) {
-                            if (t > 1e-5f && t < 1.0f) return true;
-                        } else {
-                            uint32_t orig = __float_as_uint(t2.y);
-                            if (t > 0.001f && (t < best_t || (t == best_t && orig < best_o))) {
-                                best_t = t;
-                                best_s = (int)i;
-                                best_o = orig;
-                                found = true;
-                            }
+        if (WIDTH == 2) {
+            const BvhNode* N = T.nodes + node;
+            float4 a = N->a, b = N->b, c = N->c;
+            int4 ch = N->d;
+            float tn0, tf0, tn1, tf1;
+            slab(sr, a.x, a.y, a.z, a.w, b.x, b.y, tn0, tf0);
+            slab(sr, b.z, b.w, c.x, c.y, c.z, c.w, tn1, tf1);
+            float lim = ANY ? 1.0f : best_t;
+            bool h0 = tn0 <= tf0 && tf0 >= 0.0f && tn0 <= lim;
+            bool h1 = tn1 <= tf1 && tf1 >= 0.0f && tn1 <= lim;
+            if (h0 && ch.x < 0) {
+                if (test_leaf<ANY, INSTR>(T, ch.x, o, d, best_t, best_s, best_o, found, ntri)) return true;
+                h0 = false;
+            }
+            if (h1 && ch.y < 0) {
+                if (test_leaf<ANY, INSTR>(T, ch.y, o, d, best_t, best_s, best_o, found, ntri)) return true;
+                h1 = false;
+            }
+            if (h0 && h1) {
+                bool first0 = tn0 <= tn1;
+                stk.push(first0 ? ch.y : ch.x);
+                node = first0 ? ch.x : ch.y;
+                continue;
+            }
+            if (h0) {
+                node = ch.x;
+                continue;
+            }
+            if (h1) {
+                node = ch.y;
+                continue;
+            }
+        } else {
+            const Bvh4Node* N = T.nodes4 + node;
+            float4 lx = N->lox, hx = N->hix, ly = N->loy, hy = N->hiy, lz = N->loz, hz = N->hiz;
+            int4 rf = N->ref;
+            float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+            float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+            float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+            int r[4] = {rf.x, rf.y, rf.z, rf.w};
+            float tn[4];
+            bool h[4];
+            float lim = ANY ? 1.0f : best_t;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float tf;
+                slab(sr, lxa[k], lya[k], lza[k], hxa[k], hya[k], hza[k], tn[k], tf);
+                h[k] = tn[k] <= tf && tf >= 0.0f && tn[k] <= lim && r[k] != kEmpty4;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (h[k] && r[k] < 0) {
+                    if (test_leaf<ANY, INSTR>(T, r[k], o, d, best_t, best_s, best_o, found, ntri)) return true;
+                    h[k] = false;
+                }
+            }
+            // internal children still worth a visit, nearest first
+            float key[4];
+            int val[4];
+            int nh = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                bool keep = h[k] && (ANY || tn[k] <= best_t);
+                key[k] = keep ? tn[k] : INFINITY;
+                val[k] = r[k];
+                nh += keep ? 1 : 0;
+            }
+            if (nh > 0) {
+                if (!ANY) {
+                    cswap(key[0], val[0], key[1], val[1]);
+                    cswap(key[2], val[2], key[3], val[3]);
+                    cswap(key[0], val[0], key[2], val[2]);
+                    cswap(key[1], val[1], key[3], val[3]);
+                    cswap(key[1], val[1], key[2], val[2]);
+                } else {
+                    // compact the kept children to the front
+                    int j = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (key[k] != INFINITY) {
+                            val[j] = val[k];
+                            ++j;
                         }
                     }
                 }
-                if (k == 0)
-                    h0 = false;
-                else
-                    h1 = false;
+                for (int k = nh - 1; k >= 1; --k) stk.push(val[k]);
+                node = val[0];
+                continue;
             }
         }
-        if (h0 && h1) {
-            bool first0 = tn0 <= tn1;
-            int nearc = first0 ? ch.x : ch.y;
-            int farc = first0 ? ch.y : ch.x;
-            if (sp < kLdsDepth)
-                lds[sp * kBlock] = farc;
-            else
-                T.gstack[(size_t)(sp - kLdsDepth) * T.gthreads + gtid] = farc;
-            ++sp;
-            node = nearc;
-            continue;
-        }
-        if (h0) {
-            node = ch.x;
-            continue;
-        }
-        if (h1) {
-            node = ch.y;
-            continue;
-        }
-        if (sp == 0) break;
-        --sp;
-        node = sp < kLdsDepth ? lds[sp * kBlock] : T.gstack[(size_t)(sp - kLdsDepth) * T.gthreads + gtid];
+        if (stk.sp == 0) break;
+        node = stk.pop();
     }
     return found;
 }
@@ -367,10 +458,10 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderParams P, MeshSrc M, 
     }
 }
 
-template <bool INSTR>
+template <int LDS, int WIDTH, bool INSTR>
 __global__ void __launch_bounds__(kBlock) k_extend(Traversal T, Queue q, const uint32_t* __restrict__ n_q,
                                                     int2* __restrict__ hits, unsigned long long* __restrict__ stats) {
-    __shared__ int lds[kLdsDepth * kBlock];
+    __shared__ int lds[LDS * kBlock];
     uint32_t n = *n_q;
     uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     uint32_t nvis = 0, ntri = 0;
@@ -381,8 +472,8 @@ __global__ void __launch_bounds__(kBlock) k_extend(Traversal T, Queue q, const u
             float bt = 1000000.0f;
             int bs = -1;
             uint32_t bo = 0;
-            traverse<false, INSTR>(T, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, bt, bs, bo, lds + threadIdx.x, gtid,
-                                   nvis, ntri);
+            traverse<LDS, WIDTH, false, INSTR>(T, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, bt, bs, bo,
+                                               lds + threadIdx.x, gtid, nvis, ntri);
             hits[i] = make_int2(__float_as_int(bt), bs);
         }
     }
@@ -510,11 +601,11 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams P, const float4* 
     }
 }
 
-template <bool INSTR>
+template <int LDS, int WIDTH, bool INSTR>
 __global__ void __launch_bounds__(kBlock) k_shadow(RenderParams P, Traversal T, ShadowQ sq,
                                                     const uint32_t* __restrict__ n_sq, Hist H,
                                                     unsigned long long* __restrict__ stats) {
-    __shared__ int lds[kLdsDepth * kBlock];
+    __shared__ int lds[LDS * kBlock];
     uint32_t n = *n_sq;
     uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0;
@@ -531,7 +622,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(RenderParams P, Traversal T, 
             int bs = -1;
             uint32_t bo = 0;
             ++nsh;
-            bool occ = traverse<true, INSTR>(T, p, dv, bt, bs, bo, lds + threadIdx.x, gtid, nvis, ntri);
+            bool occ = traverse<LDS, WIDTH, true, INSTR>(T, p, dv, bt, bs, bo, lds + threadIdx.x, gtid, nvis, ntri);
             if (!occ) {
                 float l = length(dv);
                 V3 lsdir = normalized(dv);
@@ -562,9 +653,10 @@ struct DirectRamp {
     float a;
 };
 
+template <int WIDTH>
 __global__ void k_direct(RenderParams P, Traversal T, const int* __restrict__ bands, uint32_t n_bands,
                          DirectRamp* __restrict__ out) {
-    __shared__ int lds[kLdsDepth * kBlock];
+    __shared__ int lds[kLdsMin * kBlock];
     uint32_t t = threadIdx.x;
     if (t >= n_bands * P.n_rec) return;
     int band = bands[t / P.n_rec];
@@ -577,7 +669,8 @@ __global__ void k_direct(RenderParams P, Traversal T, const int* __restrict__ ba
     float bt = 0.0f;
     int bs = -1;
     uint32_t bo = 0, nv = 0, nt = 0;
-    if (traverse<true, false>(T, rp, sub(sp, rp), bt, bs, bo, lds + threadIdx.x, blockIdx.x * kBlock + t, nv, nt))
+    if (traverse<kLdsMin, WIDTH, true, false>(T, rp, sub(sp, rp), bt, bs, bo, lds + threadIdx.x,
+                                              blockIdx.x * kBlock + t, nv, nt))
         return;
     V3 dist = sub(rp, sp);
     float len = length(dist);
@@ -660,11 +753,11 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(unsigned long long* __re
 }
 
 // batched queries for tests and host tools
-template <bool ANY>
+template <int WIDTH, bool ANY>
 __global__ void __launch_bounds__(kBlock) k_query(Traversal T, uint32_t n, const float* __restrict__ a,
                                                    const float* __restrict__ b, float* __restrict__ tout,
                                                    int32_t* __restrict__ iout, uint8_t* __restrict__ occ) {
-    __shared__ int lds[kLdsDepth * kBlock];
+    __shared__ int lds[kLdsMin * kBlock];
     uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     uint32_t nv = 0, nt = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
@@ -676,12 +769,63 @@ __global__ void __launch_bounds__(kBlock) k_query(Traversal T, uint32_t n, const
         int bs = -1;
         uint32_t bo = 0;
         if (ANY) {
-            occ[i] = traverse<true, false>(T, o, sub(x, o), bt, bs, bo, lds + threadIdx.x, gtid, nv, nt) ? 1 : 0;
+            occ[i] = traverse<kLdsMin, WIDTH, true, false>(T, o, sub(x, o), bt, bs, bo, lds + threadIdx.x, gtid, nv,
+                                                          nt)
+                         ? 1
+                         : 0;
         } else {
-            bool h = traverse<false, false>(T, o, x, bt, bs, bo, lds + threadIdx.x, gtid, nv, nt);
+            bool h = traverse<kLdsMin, WIDTH, false, false>(T, o, x, bt, bs, bo, lds + threadIdx.x, gtid, nv, nt);
             tout[i] = h ? bt : 0.0f;
             iout[i] = h ? (int32_t)bo : -1;
         }
+    }
+}
+
+
+// Launch the traversal kernels with the stack split and tree width chosen at
+// run time (each combination is its own kernel instance).
+template <int LDS, int WIDTH>
+void launch_extend_t(bool instr, uint32_t g, hipStream_t st, Traversal T, Queue q, const uint32_t* n,
+                     int2* hits, unsigned long long* stp) {
+    if (instr)
+        k_extend<LDS, WIDTH, true><<<g, kBlock, 0, st>>>(T, q, n, hits, stp);
+    else
+        k_extend<LDS, WIDTH, false><<<g, kBlock, 0, st>>>(T, q, n, hits, stp);
+}
+void launch_extend(int lds, int width, bool instr, uint32_t g, hipStream_t st, Traversal T, Queue q,
+                   const uint32_t* n, int2* hits, unsigned long long* stp) {
+    if (width == 4) {
+        if (lds >= 32)
+            launch_extend_t<32, 4>(instr, g, st, T, q, n, hits, stp);
+        else
+            launch_extend_t<16, 4>(instr, g, st, T, q, n, hits, stp);
+    } else {
+        if (lds >= 32)
+            launch_extend_t<32, 2>(instr, g, st, T, q, n, hits, stp);
+        else
+            launch_extend_t<16, 2>(instr, g, st, T, q, n, hits, stp);
+    }
+}
+template <int LDS, int WIDTH>
+void launch_shadow_t(bool instr, uint32_t g, hipStream_t st, const RenderParams& P, Traversal T, ShadowQ sq,
+                     const uint32_t* n, Hist H, unsigned long long* stp) {
+    if (instr)
+        k_shadow<LDS, WIDTH, true><<<g, kBlock, 0, st>>>(P, T, sq, n, H, stp);
+    else
+        k_shadow<LDS, WIDTH, false><<<g, kBlock, 0, st>>>(P, T, sq, n, H, stp);
+}
+void launch_shadow(int lds, int width, bool instr, uint32_t g, hipStream_t st, const RenderParams& P, Traversal T,
+                   ShadowQ sq, const uint32_t* n, Hist H, unsigned long long* stp) {
+    if (width == 4) {
+        if (lds >= 32)
+            launch_shadow_t<32, 4>(instr, g, st, P, T, sq, n, H, stp);
+        else
+            launch_shadow_t<16, 4>(instr, g, st, P, T, sq, n, H, stp);
+    } else {
+        if (lds >= 32)
+            launch_shadow_t<32, 2>(instr, g, st, P, T, sq, n, H, stp);
+        else
+            launch_shadow_t<16, 2>(instr, g, st, P, T, sq, n, H, stp);
     }
 }
 
@@ -739,13 +883,17 @@ ear_status run_query_closest(Ctx* c, DeviceScene* s, uint32_t n, const float* o,
     TRY(di.alloc((size_t)n * 4));
     uint32_t grid = persistent_grid(c, n, 16);
     uint32_t gthreads = grid * kBlock;
-    TRY(gs.alloc((size_t)(kStackDepth - kLdsDepth) * gthreads * 4));
+    TRY(gs.alloc((size_t)(kStackDepth - kLdsMin) * gthreads * 4));
     hipStream_t st = c->stream;
     EAR_HIP_TRY(c, hipMemcpyAsync(da.p, o, (size_t)n * 12, hipMemcpyHostToDevice, st));
     EAR_HIP_TRY(c, hipMemcpyAsync(db.p, d, (size_t)n * 12, hipMemcpyHostToDevice, st));
-    Traversal T{s->nodes, s->tris, s->n_tris, gs.as<int>(), gthreads};
-    k_query<false><<<grid, kBlock, 0, st>>>(T, n, da.as<float>(), db.as<float>(), dt.as<float>(), di.as<int32_t>(),
-                                            nullptr);
+    Traversal T{s->nodes, s->nodes4, s->tris, s->n_tris, gs.as<int>(), gthreads};
+    if (s->width == 4)
+        k_query<4, false><<<grid, kBlock, 0, st>>>(T, n, da.as<float>(), db.as<float>(), dt.as<float>(),
+                                                   di.as<int32_t>(), nullptr);
+    else
+        k_query<2, false><<<grid, kBlock, 0, st>>>(T, n, da.as<float>(), db.as<float>(), dt.as<float>(),
+                                                   di.as<int32_t>(), nullptr);
     EAR_HIP_TRY(c, hipGetLastError());
     EAR_HIP_TRY(c, hipMemcpyAsync(t, dt.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     EAR_HIP_TRY(c, hipMemcpyAsync(tri, di.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
@@ -762,12 +910,17 @@ ear_status run_query_occluded(Ctx* c, DeviceScene* s, uint32_t n, const float* p
     TRY(dk.alloc((size_t)n));
     uint32_t grid = persistent_grid(c, n, 16);
     uint32_t gthreads = grid * kBlock;
-    TRY(gs.alloc((size_t)(kStackDepth - kLdsDepth) * gthreads * 4));
+    TRY(gs.alloc((size_t)(kStackDepth - kLdsMin) * gthreads * 4));
     hipStream_t st = c->stream;
     EAR_HIP_TRY(c, hipMemcpyAsync(da.p, p, (size_t)n * 12, hipMemcpyHostToDevice, st));
     EAR_HIP_TRY(c, hipMemcpyAsync(db.p, q, (size_t)n * 12, hipMemcpyHostToDevice, st));
-    Traversal T{s->nodes, s->tris, s->n_tris, gs.as<int>(), gthreads};
-    k_query<true><<<grid, kBlock, 0, st>>>(T, n, da.as<float>(), db.as<float>(), nullptr, nullptr, dk.as<uint8_t>());
+    Traversal T{s->nodes, s->nodes4, s->tris, s->n_tris, gs.as<int>(), gthreads};
+    if (s->width == 4)
+        k_query<4, true><<<grid, kBlock, 0, st>>>(T, n, da.as<float>(), db.as<float>(), nullptr, nullptr,
+                                                  dk.as<uint8_t>());
+    else
+        k_query<2, true><<<grid, kBlock, 0, st>>>(T, n, da.as<float>(), db.as<float>(), nullptr, nullptr,
+                                                  dk.as<uint8_t>());
     EAR_HIP_TRY(c, hipGetLastError());
     EAR_HIP_TRY(c, hipMemcpyAsync(occ, dk.p, n, hipMemcpyDeviceToHost, st));
     EAR_HIP_TRY(c, hipStreamSynchronize(st));
@@ -873,7 +1026,7 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
     const ear_tuning& tu = c->tuning;
     uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 22);
     chunk = std::min<uint32_t>(chunk, std::max<uint32_t>(1, D->n_rays));
-    uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 20;
+    uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 32;
     uint32_t cap_rays = chunk * n_bands;
 
     hipStream_t st = c->stream;
@@ -895,7 +1048,7 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
     TRY(hist.alloc(cells * 16));
     uint32_t max_grid = persistent_grid(c, 0xFFFFFFFFu, waves);
     uint32_t gthreads = max_grid * kBlock;
-    TRY(gs.alloc((size_t)(kStackDepth - kLdsDepth) * gthreads * 4));
+    TRY(gs.alloc((size_t)(kStackDepth - kLdsMin) * gthreads * 4));
     TRY(dbands.alloc(3 * 4));
     TRY(direct.alloc((size_t)3 * nch * sizeof(DirectRamp)));
     if (want_log) {
@@ -922,9 +1075,11 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
 
     uint32_t* cnt = counters.as<uint32_t>();  // [0],[1] ray queues, [2] shadow queue
     unsigned long long* stp = stats.as<unsigned long long>();
-    Traversal T{s->nodes, s->tris, s->n_tris, gs.as<int>(), gthreads};
+    Traversal T{s->nodes, s->nodes4, s->tris, s->n_tris, gs.as<int>(), gthreads};
     Hist H{hist.as<unsigned long long>(), D->hist_bins, nch, P.hist_kinds};
     bool instr = (D->flags & EAR_FLAG_INSTRUMENT) != 0;
+    const int width = (int)s->width;
+    const int lds_depth = tu.stack_lds >= 32 ? 32 : 16;
 
     uint32_t* pinned = nullptr;
     EAR_HIP_TRY(c, hipHostMalloc((void**)&pinned, 16, hipHostMallocDefault));
@@ -968,10 +1123,7 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
                 uint32_t gsh = persistent_grid(c, nsh, waves);
                 EventPair& e = mk(ev_shw);
                 hipEventRecord(e.a, st);
-                if (instr)
-                    k_shadow<true><<<gsh, kBlock, 0, st>>>(P, T, SQ, cnt + 2, H, stp);
-                else
-                    k_shadow<false><<<gsh, kBlock, 0, st>>>(P, T, SQ, cnt + 2, H, stp);
+                launch_shadow(lds_depth, width, instr, gsh, st, P, T, SQ, cnt + 2, H, stp);
                 hipEventRecord(e.b, st);
                 shadow_q += nsh;
                 ++launches;
@@ -988,10 +1140,7 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
             uint32_t g1 = persistent_grid(c, n_live, waves);
             EventPair& e1 = mk(ev_ext);
             hipEventRecord(e1.a, st);
-            if (instr)
-                k_extend<true><<<g1, kBlock, 0, st>>>(T, qi, cnt + cur, hits.as<int2>(), stp);
-            else
-                k_extend<false><<<g1, kBlock, 0, st>>>(T, qi, cnt + cur, hits.as<int2>(), stp);
+            launch_extend(lds_depth, width, instr, g1, st, T, qi, cnt + cur, hits.as<int2>(), stp);
             hipEventRecord(e1.b, st);
             EventPair& e2 = mk(ev_sha);
             hipEventRecord(e2.a, st);
@@ -1008,10 +1157,7 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
                 uint32_t g2 = persistent_grid(c, nsh, waves);
                 EventPair& e3 = mk(ev_shw);
                 hipEventRecord(e3.a, st);
-                if (instr)
-                    k_shadow<true><<<g2, kBlock, 0, st>>>(P, T, SQ, cnt + 2, H, stp);
-                else
-                    k_shadow<false><<<g2, kBlock, 0, st>>>(P, T, SQ, cnt + 2, H, stp);
+                launch_shadow(lds_depth, width, instr, g2, st, P, T, SQ, cnt + 2, H, stp);
                 hipEventRecord(e3.b, st);
                 shadow_q += nsh;
                 ++launches;
@@ -1028,7 +1174,10 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
         if (le != hipSuccess) status = hip_status(c, le, "tracer launch");
     }
     if (status != EAR_OK) return status;
-    k_direct<<<1, kBlock, 0, st>>>(P, T, dbands.as<int>(), n_bands, direct.as<DirectRamp>());
+    if (width == 4)
+        k_direct<4><<<1, kBlock, 0, st>>>(P, T, dbands.as<int>(), n_bands, direct.as<DirectRamp>());
+    else
+        k_direct<2><<<1, kBlock, 0, st>>>(P, T, dbands.as<int>(), n_bands, direct.as<DirectRamp>());
     bool want_value = out && (out->value || out->abs_value);
     if (want_value) TRY(value.alloc(cells * 8));
     float finv = 1.0f / (float)std::max<uint32_t>(1, D->n_rays);   // Recorder::Multiply(1.0f / amount)

@@ -79,7 +79,8 @@ typedef struct {
 
 typedef struct {
     uint32_t leaf_size;  /* max triangles per BVH leaf, 1..16 (0 = library default) */
-    uint32_t reserved[3];
+    uint32_t width;      /* children per node: 2 or 4 (0 = library default, 4) */
+    uint32_t reserved[2];
 } ear_bvh_opts;
 
 /* Uploads the merged triangle soup of every MESH block (the reference merges
@@ -100,6 +101,9 @@ typedef struct {
     uint32_t leaf_size;
     float bounds[6];      /* scene AABB lo xyz, hi xyz */
     double build_ms;      /* device time of the LBVH build */
+    uint32_t width;       /* children per node */
+    uint32_t node_bytes;  /* bytes fetched per node visit */
+    double collapse_ms;   /* host time of the 2-wide -> 4-wide collapse */
 } ear_scene_info;
 ear_status ear_scene_get_info(const ear_scene* scene, ear_scene_info* info);
 
@@ -215,7 +219,8 @@ typedef struct {
     uint32_t block_shadow;
     uint32_t ray_chunk;      /* rays in flight per wavefront */
     uint32_t waves_per_cu;   /* persistent grid size */
-    uint32_t reserved[3];
+    uint32_t stack_lds;      /* traversal stack entries kept in LDS (16 or 32) */
+    uint32_t reserved[2];
 } ear_tuning;
 ear_status ear_ctx_set_tuning(ear_ctx* ctx, const ear_tuning* t);
 ear_status ear_ctx_get_tuning(const ear_ctx* ctx, ear_tuning* t);

@@ -69,7 +69,8 @@ def test_shoebox_parity_bitexact_paths(gpu_ctx, oracle, shoebox):
         np.testing.assert_allclose(g.abs_value[b], expa, rtol=1e-9, atol=1e-12 * np.abs(expa).max())
 
 
-def test_hall_queries_match_bruteforce(gpu_ctx, oracle):
+@pytest.mark.parametrize("width", [2, 4])
+def test_hall_queries_match_bruteforce(gpu_ctx, oracle, width):
     s = scenes.hall(target_tris=20000)
     rng = np.random.default_rng(7)
     n = 4000
@@ -82,7 +83,8 @@ def test_hall_queries_match_bruteforce(gpu_ctx, oracle):
     o[: n // 8, 2] = 0.0
     d[n // 8: n // 4] = np.eye(3, dtype=np.float32)[rng.integers(0, 3, n // 8)] * rng.choice([-1, 1], (n // 8, 1))
     osc = oracle.OracleScene(s["tris"], s["mat"], s["materials"])
-    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], leaf_size=4) as sc:
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], leaf_size=4, width=width) as sc:
+        assert sc.info()["width"] == width
         t, idx = sc.query_closest(o, d)
         q = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
         occ = sc.query_occluded(o, q)
@@ -94,14 +96,14 @@ def test_hall_queries_match_bruteforce(gpu_ctx, oracle):
         assert occ[i] == osc.any_hit(o[i], q[i])
 
 
-@pytest.mark.parametrize("leaf", [1, 2, 8])
-def test_leaf_size_does_not_change_results(gpu_ctx, shoebox, leaf):
+@pytest.mark.parametrize("leaf,width", [(1, 4), (2, 2), (8, 4), (16, 2)])
+def test_bvh_shape_does_not_change_results(gpu_ctx, shoebox, leaf, width):
     s = scenes.shoebox(tess=(6, 6, 3))
     kw = dict(recorders=s["recorders"], n_rays=2000, src_pos=s["src"], air=s["air"], max_bounces=50, cutoff=1e-6,
               n_bins=44100)
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], leaf_size=4) as a:
         ra = api.render(gpu_ctx, a, **kw)
-    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], leaf_size=leaf) as b:
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], leaf_size=leaf, width=width) as b:
         rb = api.render(gpu_ctx, b, **kw)
     np.testing.assert_array_equal(ra.raw, rb.raw)
     np.testing.assert_array_equal(ra.abs_raw, rb.abs_raw)
@@ -114,12 +116,15 @@ def test_launch_shape_determinism(gpu_ctx, shoebox):
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
         gpu_ctx.set_tuning()
         ra = api.render(gpu_ctx, sc, **kw)
-        gpu_ctx.set_tuning(ray_chunk=777, waves_per_cu=3)
+        gpu_ctx.set_tuning(ray_chunk=777, waves_per_cu=3, stack_lds=32)
         rb = api.render(gpu_ctx, sc, **kw)
+        gpu_ctx.set_tuning(ray_chunk=4099, waves_per_cu=64, stack_lds=16)
+        rc = api.render(gpu_ctx, sc, **kw)
         gpu_ctx.set_tuning()
-    np.testing.assert_array_equal(ra.raw, rb.raw)
-    np.testing.assert_array_equal(ra.abs_raw, rb.abs_raw)
-    assert ra.stats["segments"] == rb.stats["segments"]
+    for r in (rb, rc):
+        np.testing.assert_array_equal(ra.raw, r.raw)
+        np.testing.assert_array_equal(ra.abs_raw, r.abs_raw)
+        assert ra.stats["segments"] == r.stats["segments"]
 
 
 def test_stereo_and_mesh_source_parity(gpu_ctx, oracle):
