@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ear.h"
+#include "ear.h"
 #include "ear_math.hpp"
 
 namespace ear {

@@ -1,0 +1,429 @@
+// ear_host.cpp -- host C++ API over the C ABI (see ear_host.h).
+#include "ear_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+namespace earhost {
+
+void throw_on_error(ear_status s, const ear_ctx* ctx, const char* what) {
+    if (s == EAR_OK) return;
+    std::string msg = std::string(what) + ": " + ear_status_string(s);
+    if (ctx) {
+        const char* e = ear_ctx_last_error(ctx);
+        if (e && *e) msg += std::string(" (") + e + ")";
+    }
+    if (s == EAR_ERR_INVALID_MATERIAL || s == EAR_ERR_INVALID_GEOMETRY) throw DatatypeException(msg);
+    throw std::runtime_error(msg);
+}
+
+// ---------------------------------------------------------------- Material
+Material::Material(std::string n, const float refl[3], const float* refr, const float* spec) : name(std::move(n)) {
+    // src/Material.cpp:32-59: absorption starts at 1, loses (f - 1e-9) for
+    // each coefficient given, must stay >= 0, and is stored as 1 - absorption
+    for (int i = 0; i < 3; ++i) {
+        reflection_coefficient[i] = refl[i];
+        float a = 1.0f;
+        a -= refl[i] - 1e-9f;
+        if (refr) {
+            refraction_coefficient[i] = refr[i];
+            a -= refr[i] - 1e-9f;
+        }
+        if (a < 0.0f) throw DatatypeException("Invalid material settings");
+        absorption_coefficient[i] = 1.0f - a;
+        if (spec) specularity_coefficient[i] = spec[i];
+    }
+    has_refraction = refr != nullptr;
+    has_specularity = spec != nullptr;
+}
+
+ear_material Material::to_abi() const {
+    ear_material m{};
+    for (int i = 0; i < 3; ++i) {
+        m.refl[i] = reflection_coefficient[i];
+        m.refr[i] = refraction_coefficient[i];
+        m.spec[i] = specularity_coefficient[i];
+    }
+    m.flags = (has_refraction ? EAR_MAT_HAS_REFR : 0u) | (has_specularity ? EAR_MAT_HAS_SPEC : 0u);
+    return m;
+}
+
+// ---------------------------------------------------------------- Triangle
+static Vec3 vsub(const Vec3& a, const Vec3& b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+static Vec3 vcross(const Vec3& a, const Vec3& b) {
+    return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+static float vlen(const Vec3& a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+Triangle::Triangle(const Vec3& a, const Vec3& b, const Vec3& c) {
+    mVerts[0] = a;
+    mVerts[1] = b;
+    mVerts[2] = c;
+    Vec3 n = vcross(vsub(b, a), vsub(c, a));
+    float l = vlen(n);
+    normal = l != 0.0f ? Vec3{n[0] / l, n[1] / l, n[2] / l} : n;
+    area = vlen(vcross(vsub(b, a), vsub(c, b))) / 2.0f;  // src/Triangle.cpp:26-30
+}
+
+float Triangle::SignedVolume() const {
+    const Vec3 &p1 = mVerts[0], &p2 = mVerts[1], &p3 = mVerts[2];
+    const float v321 = p3[0] * p2[1] * p1[2];
+    const float v231 = p2[0] * p3[1] * p1[2];
+    const float v312 = p3[0] * p1[1] * p2[2];
+    const float v132 = p1[0] * p3[1] * p2[2];
+    const float v213 = p2[0] * p1[1] * p3[2];
+    const float v123 = p1[0] * p2[1] * p3[2];
+    return (1.0f / 6.0f) * (-v321 + v231 + v312 - v132 - v213 + v123);
+}
+
+// ---------------------------------------------------------------- Mesh
+std::map<std::string, Material*> Mesh::materials;
+
+Mesh::Mesh(Material* m, const std::vector<std::array<Vec3, 3>>& triangles) : material(m) {
+    const float absorption = m ? 1.0f - m->absorption_coefficient[1] : 0.0f;  // src/Mesh.cpp:86
+    for (const auto& t : triangles) {
+        Triangle* tri = new Triangle(t[0], t[1], t[2]);
+        tri->m = m;
+        tris.push_back(tri);
+        total_area += tri->area;
+        total_weighted_area += tri->area * absorption;
+    }
+}
+
+Mesh::~Mesh() {
+    for (Triangle* t : tris) delete t;
+}
+
+void Mesh::Combine(Mesh* m) {
+    for (Triangle* t : m->tris) tris.push_back(t);
+    m->tris.clear();
+    total_area += m->total_area;  // weighted area is not added (src/Mesh.cpp:116-123)
+}
+
+float Mesh::Volume() const {
+    float v = 0.0f;
+    for (const Triangle* t : tris) v += t->SignedVolume();
+    return v;
+}
+
+// ---------------------------------------------------------------- animation
+static std::unique_ptr<Keyframes> g_keys;
+void Keyframes::Set(std::vector<float> k) {
+    g_keys.reset(new Keyframes());
+    g_keys->keys = std::move(k);
+}
+Keyframes* Keyframes::Get() { return g_keys.get(); }
+
+template <class T>
+Animated<T>::Animated(std::vector<T> f) : frames(std::move(f)) {
+    Keyframes* k = Keyframes::Get();
+    if (!k || k->keys.empty()) throw DatatypeException("Keyframe data not read");  // src/Animated.h:76-78
+    if (frames.size() != k->keys.size()) throw DatatypeException("Keyframe count does not match");
+}
+template class Animated<Vec3>;
+
+// ---------------------------------------------------------------- tracks
+float& RecorderTrack::operator[](unsigned i) {
+    if (i >= data.size()) data.resize((size_t)i + 44100, 0.0f);  // grows like FloatBuffer (src/Recorder.cpp:52-59)
+    if (i > real_length) real_length = i;
+    if (data.empty() || i < first_sample) first_sample = i;
+    return data[i];
+}
+float RecorderTrack::Maximum() const {
+    float x = 0.0f;
+    for (unsigned i = first_sample; i < real_length && i < data.size(); ++i) x = std::max(x, std::fabs(data[i]));
+    return x;
+}
+void RecorderTrack::Multiply(float f) {
+    for (unsigned i = first_sample; i < real_length && i < data.size(); ++i) data[i] *= f;
+}
+void RecorderTrack::Power(float a) {
+    for (unsigned i = first_sample; i < real_length && i < data.size(); ++i) {
+        const float f = std::pow(std::fabs(data[i]), a);
+        data[i] = data[i] < 0 ? f * -1.0f : f;
+    }
+}
+void RecorderTrack::Truncate(unsigned l) {
+    if (l == 0) l = 1;
+    if (l >= data.size()) data.resize(l, 0.0f);
+    real_length = l;
+}
+unsigned RecorderTrack::getLength(float tresh) const {
+    if (tresh < 0.0f) return real_length;
+    unsigned mx = 0;
+    for (unsigned i = first_sample; i < data.size(); ++i)
+        if (std::fabs(data[i]) >= tresh) mx = i;
+    return mx + 1;
+}
+float RecorderTrack::T60() const {
+    const float attenuation_gain = std::pow(10.0f, 60.0f / 20.0f);
+    float min_gain = 0.0f, previous = -1.0f;
+    bool inside = false;
+    int last_significant = -1, direct_offset = -1;
+    for (unsigned j = first_sample; j < real_length && j < data.size(); ++j) {
+        const float s = data[j];
+        if (inside) {
+            if (s > min_gain) last_significant = (int)j;
+        } else if (s < previous) {
+            inside = true;
+            min_gain = previous / attenuation_gain;
+            direct_offset = (int)j;
+        }
+        previous = s;
+    }
+    // the reference reads uninitialised offsets when no decay is found
+    if (last_significant < 0 || direct_offset < 0) return NAN;
+    return (float)(last_significant - direct_offset) / 44100.0f;
+}
+
+// ---------------------------------------------------------------- recorders
+Recorder::~Recorder() {
+    for (RecorderTrack* t : tracks) delete t;
+}
+void Recorder::Multiply(float f) {
+    for (RecorderTrack* t : tracks) t->Multiply(f);
+}
+void Recorder::Power(float a) {
+    for (RecorderTrack* t : tracks) t->Power(a);
+}
+unsigned Recorder::getLength(float tresh) const {
+    unsigned l = 0;
+    for (const RecorderTrack* t : tracks) l = std::max(l, t->getLength(tresh));
+    return l;
+}
+void Recorder::Truncate(int len) {
+    for (RecorderTrack* t : tracks) t->Truncate((unsigned)len);
+}
+void Recorder::AddHistogram(int channel, const double* values, size_t n) {
+    RecorderTrack* t = tracks.at((size_t)channel);
+    t->data.assign(n, 0.0f);
+    size_t first = n, last = 0;
+    for (size_t i = 0; i < n; ++i) {
+        t->data[i] = (float)values[i];
+        if (values[i] != 0.0) {
+            first = std::min(first, i);
+            last = i;
+        }
+    }
+    t->first_sample = first == n ? 0 : (unsigned)first;
+    t->real_length = first == n ? 0 : (unsigned)last + 1;
+    has_samples = has_samples || first != n;
+}
+
+static void splat(RecorderTrack* tr, int s, float ampl, int w, bool skip_negative) {
+    const float step = ampl / w;
+    for (int i = 0; i < w; ++i) {
+        if (!(skip_negative && i + s < 0)) (*tr)[(unsigned)(i + s)] += ampl;
+        ampl -= step;
+    }
+}
+
+MonoRecorder::MonoRecorder() { tracks.push_back(new RecorderTrack()); }
+MonoRecorder::MonoRecorder(const Vec3& loc) : MonoRecorder() { location = loc; }
+Vec3 MonoRecorder::getLocation(int i) const { return (i >= 0 && animation) ? animation->Evaluate(i) : location; }
+Recorder* MonoRecorder::getBlankCopy(int) const {
+    MonoRecorder* r = new MonoRecorder(location);
+    r->animation = animation;
+    return r;
+}
+void MonoRecorder::Record(const Vec3&, float a, float t, float dist, int, int) {
+    // src/MonoRecorder.cpp:83-97
+    const int s = (int)(t * 44100.0);
+    const float width = std::sqrt(dist);
+    splat(tracks[0], s, 2.0f * a / width, (int)std::ceil(width), false);
+    has_samples = true;
+}
+ear_recorder MonoRecorder::to_abi(int kf) const {
+    ear_recorder r{};
+    r.kind = EAR_REC_MONO;
+    Vec3 p = getLocation(kf);
+    std::memcpy(r.pos, p.data(), sizeof(r.pos));
+    r.right_ear[0] = 1.0f;
+    return r;
+}
+
+StereoRecorder::StereoRecorder() {
+    tracks.push_back(new RecorderTrack());
+    tracks.push_back(new RecorderTrack());
+}
+StereoRecorder::StereoRecorder(const Vec3& loc, const Vec3& right, float head, const Vec3& ha) : StereoRecorder() {
+    location = loc;
+    right_ear = right;
+    head_size = head;
+    for (int i = 0; i < 3; ++i) head_absorption[i] = std::max(0.0f, std::pow(1.0f - ha[i], 4.0f));
+}
+Vec3 StereoRecorder::getLocation(int i) const { return (i >= 0 && animation) ? animation->Evaluate(i) : location; }
+Vec3 StereoRecorder::getRightEar(int i) const {
+    return (i >= 0 && right_ear_animation) ? right_ear_animation->Evaluate(i) : right_ear;
+}
+Recorder* StereoRecorder::getBlankCopy(int) const {
+    StereoRecorder* r = new StereoRecorder();
+    r->location = location;
+    r->right_ear = right_ear;
+    r->head_size = head_size;
+    r->head_absorption = head_absorption;
+    r->animation = animation;
+    r->right_ear_animation = right_ear_animation;
+    return r;
+}
+void StereoRecorder::Record(const Vec3& dir, float a, float t, float dist, int band, int kf) {
+    // src/StereoRecorder.cpp:97-130
+    const Vec3 re = getRightEar(kf);
+    const float dot = dir[0] * re[0] + dir[1] * re[1] + dir[2] * re[2];
+    const float td = head_size / 343.0f;
+    const int s_right = (int)((t - dot * td) * 44100.0);
+    const int s_left = (int)((t + dot * td) * 44100.0);
+    const float width = std::sqrt(dist);
+    const float ampl = 2.0f * a / width;
+    float al = ampl, ar = ampl;
+    const float factor = std::pow(head_absorption[band], std::fabs(dot) * head_size);
+    if (dot < 0)
+        ar *= factor * factor;
+    else
+        al *= factor * factor;
+    const int w = (int)std::ceil(width);
+    splat(tracks[0], s_left, al, w, true);
+    splat(tracks[1], s_right, ar, w, true);
+    has_samples = true;
+}
+ear_recorder StereoRecorder::to_abi(int kf) const {
+    ear_recorder r{};
+    r.kind = EAR_REC_STEREO;
+    Vec3 p = getLocation(kf), e = getRightEar(kf);
+    std::memcpy(r.pos, p.data(), sizeof(r.pos));
+    std::memcpy(r.right_ear, e.data(), sizeof(r.right_ear));
+    r.head_size = head_size;
+    std::memcpy(r.head_abs, head_absorption.data(), sizeof(r.head_abs));
+    return r;
+}
+
+Vec3 AbstractSoundFile::getLocation(int i) const { return (i >= 0 && animation) ? animation->Evaluate(i) : location; }
+
+// ---------------------------------------------------------------- Scene
+Scene::Scene() = default;
+
+Scene::~Scene() {
+    if (dev_scene_) ear_scene_destroy(dev_scene_);
+    if (ctx_) ear_ctx_destroy(ctx_);
+    for (Recorder* r : listeners) delete r;
+    for (AbstractSoundFile* s : sources) delete s;
+    for (Mesh* m : meshes) delete m;
+}
+
+void Scene::addMesh(Mesh* m) {
+    if (meshes.empty())
+        meshes.push_back(m);
+    else {
+        meshes[0]->Combine(m);
+        delete m;
+    }
+}
+
+void Scene::addMaterial(Material* m) { Mesh::materials[m->name] = m; }
+
+void Scene::ensure_device() {
+    if (!ctx_) throw_on_error(ear_ctx_create(options.device, &ctx_), nullptr, "ear_ctx_create");
+    const size_t n = meshes.empty() ? 0 : meshes[0]->tris.size();
+    if (dev_scene_ && uploaded_tris_ == n) return;
+    if (dev_scene_) {
+        ear_scene_destroy(dev_scene_);
+        dev_scene_ = nullptr;
+    }
+    std::vector<float> v0(3 * n), v1(3 * n), v2(3 * n);
+    std::vector<uint16_t> mid(n);
+    std::vector<ear_material> mats;
+    std::map<const Material*, uint16_t> index;
+    for (size_t i = 0; i < n; ++i) {
+        const Triangle* t = meshes[0]->tris[i];
+        for (int k = 0; k < 3; ++k) {
+            v0[3 * i + k] = t->mVerts[0][k];
+            v1[3 * i + k] = t->mVerts[1][k];
+            v2[3 * i + k] = t->mVerts[2][k];
+        }
+        if (!t->m) throw DatatypeException("triangle without material");
+        auto it = index.find(t->m);
+        if (it == index.end()) {
+            it = index.emplace(t->m, (uint16_t)mats.size()).first;
+            mats.push_back(t->m->to_abi());
+        }
+        mid[i] = it->second;
+    }
+    ear_bvh_opts o{options.leaf_size, options.bvh_width, {0, 0}};
+    throw_on_error(ear_scene_upload(ctx_, v0.data(), v1.data(), v2.data(), mid.data(), (uint32_t)n,
+                                    mats.empty() ? nullptr : mats.data(), (uint32_t)mats.size(), &o, &dev_scene_),
+                   ctx_, "ear_scene_upload");
+    uploaded_tris_ = n;
+}
+
+void Scene::Render(int band, int sound, float absorbtion_factor, int num_samples, float dry,
+                   const std::vector<Recorder*>& rec, int keyframeID) {
+    if (band < 0 || band > 2) throw std::runtime_error("band out of range");
+    float air[3] = {1.0f, 1.0f, 1.0f};
+    air[band] = absorbtion_factor;
+    std::vector<Recorder*> per[3];
+    per[band] = rec;
+    RenderBands(sound, air, num_samples, dry, per, keyframeID, 1u << band);
+}
+
+void Scene::RenderBands(int sound, const float absorbtion_factor[3], int num_samples, float dry,
+                        const std::vector<Recorder*> rec[3], int keyframeID, unsigned band_mask) {
+    ensure_device();
+    AbstractSoundFile* src = sources.at((size_t)sound);
+    // the recorders of every band context describe the same listeners
+    const std::vector<Recorder*>* any = nullptr;
+    for (int b = 0; b < 3; ++b)
+        if ((band_mask >> b) & 1u) any = &rec[b];
+    if (!any || any->empty()) throw std::runtime_error("no recorders");
+    std::vector<ear_recorder> recs;
+    for (Recorder* r : *any) recs.push_back(r->to_abi(keyframeID));
+    std::vector<float> mesh_v;
+    ear_render_desc d{};
+    d.band_mask = band_mask;
+    for (int b = 0; b < 3; ++b) d.air[b] = absorbtion_factor[b];
+    d.dry = dry;
+    if (src->isMeshSource()) {
+        d.source.kind = EAR_SRC_MESH;
+        for (const Triangle* t : src->mesh->tris)
+            for (int v = 0; v < 3; ++v)
+                for (int k = 0; k < 3; ++k) mesh_v.push_back(t->mVerts[v][k]);
+        d.source.mesh_v = mesh_v.data();
+        d.source.mesh_n_tris = (uint32_t)src->mesh->tris.size();
+    } else {
+        d.source.kind = EAR_SRC_POINT;
+        Vec3 p = src->getLocation(keyframeID);
+        std::memcpy(d.source.pos, p.data(), sizeof(d.source.pos));
+    }
+    d.source.gain = src->getGain();
+    d.recorders = recs.data();
+    d.n_recorders = (uint32_t)recs.size();
+    d.n_rays = (uint32_t)std::max(0, num_samples);
+    d.max_bounces = options.max_bounces;
+    d.energy_cutoff = options.energy_cutoff;
+    d.seed = options.seed;
+    d.keyframe = keyframeID;
+    d.sound = (uint32_t)sound;
+    d.hist_bins = options.hist_bins;
+    int nch = 0;
+    for (const ear_recorder& r : recs) nch += r.kind == EAR_REC_STEREO ? 2 : 1;
+    std::vector<double> value((size_t)3 * nch * d.hist_bins);
+    ear_hist h{};
+    h.value = value.data();
+    throw_on_error(ear_render(ctx_, dev_scene_, &d, &h, &last_stats), ctx_, "ear_render");
+    for (int b = 0; b < 3; ++b) {
+        if (!((band_mask >> b) & 1u)) continue;
+        int ch = 0;
+        for (Recorder* r : rec[b]) {
+            for (int k = 0; k < r->trackCount(); ++k, ++ch)
+                r->AddHistogram(k, value.data() + ((size_t)b * nch + ch) * d.hist_bins, d.hist_bins);
+        }
+    }
+}
+
+SceneContext::SceneContext(Scene* scn, int b, int sf, int s, float ab, float dr, int kf)
+    : scene(scn), band(b), soundfile_id(sf), keyframe_id(kf), samples(s), absorption(ab), dry_level(dr) {
+    for (Recorder* r : scn->listeners) recorders.push_back(r->getBlankCopy(4));
+}
+
+}  // namespace earhost

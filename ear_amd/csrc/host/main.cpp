@@ -1,0 +1,227 @@
+// main.cpp -- the EAR command line over the GPU tracer.
+//
+//   EAR render <file>     src/EAR.cpp:55-291: parse, render every
+//                         (sound, keyframe, band) context, Power(0.335),
+//                         threshold max/256, truncate; impulse responses are
+//                         written as raw float tracks to SET debugdir (the
+//                         FloatBuffer::Write format, src/Recorder.cpp:120-123).
+//                         Convolution with the sound files is not part of
+//                         this build.
+//   EAR calc T60 <file>   src/EAR.cpp:246-285, :412-423: mid band, first sound
+//                         and keyframe; prints T60 of the rendered response
+//                         beside the Sabine and Eyring predictions.
+//   EAR test              src/EAR.cpp:424-425: returns 0.
+//   EAR info <file>       parses the file and prints a summary (no device).
+// Settings beyond the reference's: seed (int4), maxbounces (int4), cutoff
+// (flt4), irlength (flt4, seconds of impulse response), device (int4).
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+#include "ear_file.h"
+#include "ear_host.h"
+
+using namespace earhost;
+
+namespace {
+
+const char* kLoMiHi[] = {"low", "mid", "high"};
+
+Scene* build_scene(EarFile& F) {
+    Scene* scene = new Scene();
+    if (!F.keys.empty()) Keyframes::Set(F.keys);
+    for (auto& m : F.materials) scene->addMaterial(m.get());
+    for (auto& m : F.meshes) scene->addMesh(m.release());
+    for (auto& s : F.sources) {
+        SoundFile* sf = new SoundFile(s.location, s.gain);
+        sf->filename = s.wav.empty() ? "" : s.wav[0];
+        if (!s.anim.empty()) sf->animation = new Animated<Vec3>(s.anim);
+        scene->addSoundSource(sf);
+    }
+    for (auto& l : F.listeners) {
+        Recorder* r;
+        if (l.stereo) {
+            StereoRecorder* sr = new StereoRecorder(l.location, l.right_ear, l.head_size, l.head_abs);
+            if (!l.anim.empty()) sr->animation = new Animated<Vec3>(l.anim);
+            if (!l.right_anim.empty()) sr->right_ear_animation = new Animated<Vec3>(l.right_anim);
+            r = sr;
+        } else {
+            MonoRecorder* mr = new MonoRecorder(l.location);
+            if (!l.anim.empty()) mr->animation = new Animated<Vec3>(l.anim);
+            r = mr;
+        }
+        scene->addListener(r);
+    }
+    RenderOptions& o = scene->options;
+    if (F.has("seed")) o.seed = (uint64_t)(uint32_t)F.get_int("seed");
+    if (F.has("maxbounces")) o.max_bounces = (unsigned)F.get_int("maxbounces");
+    if (F.has("cutoff")) o.energy_cutoff = F.get_float("cutoff");
+    if (F.has("irlength")) o.hist_bins = (unsigned)std::lround(F.get_float("irlength") * 44100.0f);
+    if (F.has("device")) o.device = F.get_int("device");
+    return scene;
+}
+
+struct Ctx {
+    int sound, keyframe, band;
+    std::vector<Recorder*> recorders;
+};
+
+int Render(const std::string& filename, float* calc_T60 = nullptr, float* T60_Sabine = nullptr,
+           float* T60_Eyring = nullptr) {
+    EarFile F;
+    if (!read_ear_file(filename, &F)) {
+        std::cout << "Failed to read file" << std::endl;
+        return 1;
+    }
+    if (F.settings.empty()) {
+        std::cout << "No settings block found in file" << std::endl;
+        return 1;
+    }
+    const Vec3 absorption = F.get_vec("absorption");
+    const float dry_level = F.get_float("drylevel");
+    const int num_samples = F.get_int("samples") / 10;  // Release build (src/EAR.cpp:78-82)
+    std::unique_ptr<Scene> scene(build_scene(F));
+    if (scene->sources.empty()) {
+        std::cout << std::endl << "No sound sources defined" << std::endl << std::endl;
+        return 1;
+    }
+    if (scene->listeners.empty()) {
+        std::cout << std::endl << "No listeners defined" << std::endl << std::endl;
+        return 1;
+    }
+    if (scene->meshes.empty()) {
+        std::cout << std::endl << "Warning: no reflective geometry" << std::endl << std::endl;
+        scene->addMesh(new Mesh());
+    }
+    Keyframes* keys = Keyframes::Get();
+    std::cout << "Rendering..." << std::endl;
+
+    // contexts per (sound, keyframe, band) as src/EAR.cpp:170-191, rendered
+    // three bands per device call
+    std::vector<Ctx> ctxs;
+    const float air[3] = {1.0f - absorption[0], 1.0f - absorption[1], 1.0f - absorption[2]};
+    for (int sound = 0; sound < (int)scene->sources.size(); ++sound) {
+        for (int kf = keys ? 0 : -1; kf < (keys ? (int)keys->keys.size() : 0); ++kf) {
+            std::vector<Recorder*> per[3];
+            unsigned mask = 0;
+            for (int band = 0; band < 3; ++band) {
+                if (calc_T60 && band != 1) continue;
+                for (Recorder* l : scene->listeners) per[band].push_back(l->getBlankCopy(4));
+                mask |= 1u << band;
+            }
+            scene->RenderBands(sound, air, num_samples, dry_level, per, kf, mask);
+            for (int band = 0; band < 3; ++band)
+                if ((mask >> band) & 1u) ctxs.push_back(Ctx{sound, kf, band, per[band]});
+            if (calc_T60) break;
+        }
+        if (calc_T60) break;
+    }
+
+    // Power(0.335) and threshold truncation (src/EAR.cpp:210-244)
+    float mx = 0.0f;
+    for (Ctx& c : ctxs)
+        for (Recorder* r : c.recorders) {
+            r->Power(0.335f);
+            for (RecorderTrack* t : r->tracks) mx = std::max(mx, t->Maximum());
+        }
+    const float treshold = mx / 256.0f;
+    const bool has_debugdir = F.has("debugdir");
+    for (Ctx& c : ctxs) {
+        int rec_id = 0;
+        for (Recorder* r : c.recorders) {
+            r->Truncate((int)r->getLength(treshold));
+            if (has_debugdir) {
+                std::stringstream ss;
+                ss << F.get_string("debugdir") << "/response-" << rec_id << ".sound-" << c.sound;
+                if (c.keyframe != -1) ss << ".frame-" << (c.keyframe < 10 ? "0" : "") << c.keyframe;
+                ss << ".band-" << c.band << kLoMiHi[c.band] << ".bin";
+                const RecorderTrack* t = r->tracks[0];
+                std::ofstream f(ss.str().c_str(), std::ios_base::binary);
+                std::vector<float> d(t->data.begin(), t->data.end());
+                d.resize((size_t)t->real_length + 1, 0.0f);
+                f.write((const char*)d.data(), (std::streamsize)(sizeof(float) * d.size()));
+            }
+            ++rec_id;
+        }
+    }
+
+    std::cout << std::endl << "Not processing data" << std::endl;
+    if (calc_T60) {
+        const Recorder* rec = ctxs.front().recorders.front();
+        *calc_T60 = rec->tracks.front()->T60();
+        const Mesh* mesh = scene->meshes[0];
+        const float V = mesh->Volume(), A = mesh->TotalAbsorption(), S = mesh->Area(),
+                    a = mesh->AverageAbsorption(), m = absorption[1];
+        if (T60_Sabine) *T60_Sabine = 0.1611f * V / (A + 4.0f * m * V);
+        if (T60_Eyring) *T60_Eyring = 0.1611f * V / (-S * std::log(1.0f - a) + 4.0f * m * V);
+    }
+    for (Ctx& c : ctxs)
+        for (Recorder* r : c.recorders) delete r;
+    return 0;
+}
+
+int Info(const std::string& filename) {
+    EarFile F;
+    if (!read_ear_file(filename, &F)) {
+        std::cout << "Failed to read file" << std::endl;
+        return 1;
+    }
+    size_t ntri = 0;
+    for (auto& m : F.meshes) ntri += m->tris.size();
+    std::cout << "materials " << F.materials.size() << "\nmeshes " << F.meshes.size() << "\ntriangles " << ntri
+              << "\nsources " << F.sources.size() << "\nlisteners " << F.listeners.size() << "\nkeyframes "
+              << F.keys.size() << "\nsettings " << F.settings.size() << std::endl;
+    if (F.has("samples")) std::cout << "samples " << F.get_int("samples") << std::endl;
+    std::unique_ptr<Scene> scene(build_scene(F));
+    if (!scene->meshes.empty()) {
+        const Mesh* m = scene->meshes[0];
+        std::cout.precision(9);
+        std::cout << "volume " << m->Volume() << "\narea " << m->Area() << "\ntotal_absorption "
+                  << m->TotalAbsorption() << std::endl;
+    }
+    for (auto& l : F.listeners) std::cout << (l.stereo ? "stereo" : "mono") << " listener" << std::endl;
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::cout << "E.A.R acoustic ray tracer (MI355X)" << std::endl;
+    for (int i = 1; i < argc; ++i) {
+        const std::string cmd(argv[i]);
+        const std::string arg1 = (i + 1) < argc ? std::string(argv[i + 1]) : "";
+        const std::string arg2 = (i + 2) < argc ? std::string(argv[i + 2]) : "";
+        if (cmd == "render" && !arg1.empty()) {
+            int ret = 1;
+            try {
+                ret = Render(arg1);
+            } catch (std::exception& e) {
+                std::cout << std::endl << "Error: " << e.what() << std::endl << std::endl;
+            }
+            return ret;
+        } else if (cmd == "calc" && arg1 == "T60" && !arg2.empty()) {
+            float t60 = 0, sab = 0, ey = 0;
+            int ret = 1;
+            try {
+                ret = Render(arg2, &t60, &sab, &ey);
+                std::printf("T60_ear   : %.9fs\nT60_sabine: %.9fs\nT60_eyring: %.9fs\n", t60, sab, ey);
+            } catch (std::exception& e) {
+                std::cout << std::endl << "Error: " << e.what() << std::endl << std::endl;
+            }
+            return ret;
+        } else if (cmd == "info" && !arg1.empty()) {
+            try {
+                return Info(arg1);
+            } catch (std::exception& e) {
+                std::cout << "Error: " << e.what() << std::endl;
+                return 1;
+            }
+        } else if (cmd == "test") {
+            return 0;
+        }
+    }
+    std::cout << "Usage:" << std::endl << " EAR render <filename>" << std::endl << " EAR calc T60 <filename>" << std::endl;
+    return 1;
+}

@@ -118,7 +118,7 @@ __device__ __forceinline__ void slab(const SlabRay& r, float lx, float ly, float
 // Per-thread traversal stack: the first LDS entries in shared memory
 // (column-major over the block so a wave's accesses are conflict free), the
 // rest in a global overflow area indexed [entry][persistent thread].
-template <int LDS>
+template <int LDS, int BLOCK>
 struct Stack {
     int* lds;
     int* gstack;
@@ -126,14 +126,14 @@ struct Stack {
     int sp = 0;
     __device__ __forceinline__ void push(int v) {
         if (sp < LDS)
-            lds[sp * kBlock] = v;
+            lds[sp * BLOCK] = v;
         else
             gstack[(size_t)(sp - LDS) * gthreads + gtid] = v;
         ++sp;
     }
     __device__ __forceinline__ int pop() {
         --sp;
-        return sp < LDS ? lds[sp * kBlock] : gstack[(size_t)(sp - LDS) * gthreads + gtid];
+        return sp < LDS ? lds[sp * BLOCK] : gstack[(size_t)(sp - LDS) * gthreads + gtid];
     }
 };
 
@@ -182,12 +182,12 @@ __device__ __forceinline__ void cswap(float& ka, int& va, float& kb, int& vb) {
 }
 
 // Returns true on a hit; best_* hold the closest hit (ANY = false).
-template <int LDS, int WIDTH, bool ANY, bool INSTR>
+template <int LDS, int BLOCK, int WIDTH, bool ANY, bool INSTR>
 __device__ bool traverse(const Traversal& T, V3 o, V3 d, float& best_t, int& best_s, uint32_t& best_o, int* lds,
                          uint32_t gtid, uint32_t& nvis, uint32_t& ntri) {
     if (T.n_tris == 0) return false;
     SlabRay sr{o.x, o.y, o.z, safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
-    Stack<LDS> stk{lds, T.gstack, gtid, T.gthreads};
+    Stack<LDS, BLOCK> stk{lds, T.gstack, gtid, T.gthreads};
     int node = 0;
     bool found = false;
     for (;;) {
@@ -458,22 +458,22 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderParams P, MeshSrc M, 
     }
 }
 
-template <int LDS, int WIDTH, bool INSTR>
-__global__ void __launch_bounds__(kBlock) k_extend(Traversal T, Queue q, const uint32_t* __restrict__ n_q,
-                                                    int2* __restrict__ hits, unsigned long long* __restrict__ stats) {
-    __shared__ int lds[LDS * kBlock];
+template <int BLOCK, int LDS, int WIDTH, bool INSTR>
+__global__ void __launch_bounds__(BLOCK) k_extend(Traversal T, Queue q, const uint32_t* __restrict__ n_q,
+                                                   int2* __restrict__ hits, unsigned long long* __restrict__ stats) {
+    __shared__ int lds[LDS * BLOCK];
     uint32_t n = *n_q;
-    uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t nvis = 0, ntri = 0;
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         uint32_t i = base + threadIdx.x;
         if (i < n) {
             float4 o = q.o[i], d = q.d[i];
             float bt = 1000000.0f;
             int bs = -1;
             uint32_t bo = 0;
-            traverse<LDS, WIDTH, false, INSTR>(T, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, bt, bs, bo,
-                                               lds + threadIdx.x, gtid, nvis, ntri);
+            traverse<LDS, BLOCK, WIDTH, false, INSTR>(T, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, bt, bs, bo,
+                                                      lds + threadIdx.x, gtid, nvis, ntri);
             hits[i] = make_int2(__float_as_int(bt), bs);
         }
     }
@@ -601,15 +601,15 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams P, const float4* 
     }
 }
 
-template <int LDS, int WIDTH, bool INSTR>
-__global__ void __launch_bounds__(kBlock) k_shadow(RenderParams P, Traversal T, ShadowQ sq,
-                                                    const uint32_t* __restrict__ n_sq, Hist H,
-                                                    unsigned long long* __restrict__ stats) {
-    __shared__ int lds[LDS * kBlock];
+template <int BLOCK, int LDS, int WIDTH, bool INSTR>
+__global__ void __launch_bounds__(BLOCK) k_shadow(RenderParams P, Traversal T, ShadowQ sq,
+                                                   const uint32_t* __restrict__ n_sq, Hist H,
+                                                   unsigned long long* __restrict__ stats) {
+    __shared__ int lds[LDS * BLOCK];
     uint32_t n = *n_sq;
-    uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0;
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         uint32_t i = base + threadIdx.x;
         if (i < n) {
             float4 pc = sq.p[i], m = sq.m[i];
@@ -622,7 +622,8 @@ __global__ void __launch_bounds__(kBlock) k_shadow(RenderParams P, Traversal T, 
             int bs = -1;
             uint32_t bo = 0;
             ++nsh;
-            bool occ = traverse<LDS, WIDTH, true, INSTR>(T, p, dv, bt, bs, bo, lds + threadIdx.x, gtid, nvis, ntri);
+            bool occ = traverse<LDS, BLOCK, WIDTH, true, INSTR>(T, p, dv, bt, bs, bo, lds + threadIdx.x, gtid, nvis,
+                                                                ntri);
             if (!occ) {
                 float l = length(dv);
                 V3 lsdir = normalized(dv);
@@ -669,7 +670,7 @@ __global__ void k_direct(RenderParams P, Traversal T, const int* __restrict__ ba
     float bt = 0.0f;
     int bs = -1;
     uint32_t bo = 0, nv = 0, nt = 0;
-    if (traverse<kLdsMin, WIDTH, true, false>(T, rp, sub(sp, rp), bt, bs, bo, lds + threadIdx.x,
+    if (traverse<kLdsMin, kBlock, WIDTH, true, false>(T, rp, sub(sp, rp), bt, bs, bo, lds + threadIdx.x,
                                               blockIdx.x * kBlock + t, nv, nt))
         return;
     V3 dist = sub(rp, sp);
@@ -769,12 +770,13 @@ __global__ void __launch_bounds__(kBlock) k_query(Traversal T, uint32_t n, const
         int bs = -1;
         uint32_t bo = 0;
         if (ANY) {
-            occ[i] = traverse<kLdsMin, WIDTH, true, false>(T, o, sub(x, o), bt, bs, bo, lds + threadIdx.x, gtid, nv,
+            occ[i] = traverse<kLdsMin, kBlock, WIDTH, true, false>(T, o, sub(x, o), bt, bs, bo, lds + threadIdx.x, gtid, nv,
                                                           nt)
                          ? 1
                          : 0;
         } else {
-            bool h = traverse<kLdsMin, WIDTH, false, false>(T, o, x, bt, bs, bo, lds + threadIdx.x, gtid, nv, nt);
+            bool h = traverse<kLdsMin, kBlock, WIDTH, false, false>(T, o, x, bt, bs, bo, lds + threadIdx.x, gtid, nv,
+                                                                    nt);
             tout[i] = h ? bt : 0.0f;
             iout[i] = h ? (int32_t)bo : -1;
         }
@@ -782,51 +784,45 @@ __global__ void __launch_bounds__(kBlock) k_query(Traversal T, uint32_t n, const
 }
 
 
-// Launch the traversal kernels with the stack split and tree width chosen at
-// run time (each combination is its own kernel instance).
-template <int LDS, int WIDTH>
-void launch_extend_t(bool instr, uint32_t g, hipStream_t st, Traversal T, Queue q, const uint32_t* n,
-                     int2* hits, unsigned long long* stp) {
-    if (instr)
-        k_extend<LDS, WIDTH, true><<<g, kBlock, 0, st>>>(T, q, n, hits, stp);
-    else
-        k_extend<LDS, WIDTH, false><<<g, kBlock, 0, st>>>(T, q, n, hits, stp);
+// Traversal kernels are instantiated for every (block, LDS stack part, tree
+// width, instrumentation) combination; the render picks one at run time.
+// `TravCfg` names the combination and carries its occupancy-derived grid cap.
+struct TravCfg {
+    int block, lds, width;
+    bool instr;
+};
+
+using ExtendFn = void (*)(Traversal, Queue, const uint32_t*, int2*, unsigned long long*);
+using ShadowFn = void (*)(RenderParams, Traversal, ShadowQ, const uint32_t*, Hist, unsigned long long*);
+
+template <int B, int L, int W>
+void pick_t(bool instr, ExtendFn* e, ShadowFn* s) {
+    *e = instr ? (ExtendFn)k_extend<B, L, W, true> : (ExtendFn)k_extend<B, L, W, false>;
+    *s = instr ? (ShadowFn)k_shadow<B, L, W, true> : (ShadowFn)k_shadow<B, L, W, false>;
 }
-void launch_extend(int lds, int width, bool instr, uint32_t g, hipStream_t st, Traversal T, Queue q,
-                   const uint32_t* n, int2* hits, unsigned long long* stp) {
-    if (width == 4) {
-        if (lds >= 32)
-            launch_extend_t<32, 4>(instr, g, st, T, q, n, hits, stp);
-        else
-            launch_extend_t<16, 4>(instr, g, st, T, q, n, hits, stp);
-    } else {
-        if (lds >= 32)
-            launch_extend_t<32, 2>(instr, g, st, T, q, n, hits, stp);
-        else
-            launch_extend_t<16, 2>(instr, g, st, T, q, n, hits, stp);
+
+void pick_kernels(const TravCfg& c, ExtendFn* e, ShadowFn* s) {
+    int key = (c.block >= 256 ? 4 : 0) | (c.lds >= 32 ? 2 : 0) | (c.width == 4 ? 1 : 0);
+    switch (key) {
+        case 0: pick_t<128, 16, 2>(c.instr, e, s); break;
+        case 1: pick_t<128, 16, 4>(c.instr, e, s); break;
+        case 2: pick_t<128, 32, 2>(c.instr, e, s); break;
+        case 3: pick_t<128, 32, 4>(c.instr, e, s); break;
+        case 4: pick_t<256, 16, 2>(c.instr, e, s); break;
+        case 5: pick_t<256, 16, 4>(c.instr, e, s); break;
+        case 6: pick_t<256, 32, 2>(c.instr, e, s); break;
+        default: pick_t<256, 32, 4>(c.instr, e, s); break;
     }
 }
-template <int LDS, int WIDTH>
-void launch_shadow_t(bool instr, uint32_t g, hipStream_t st, const RenderParams& P, Traversal T, ShadowQ sq,
-                     const uint32_t* n, Hist H, unsigned long long* stp) {
-    if (instr)
-        k_shadow<LDS, WIDTH, true><<<g, kBlock, 0, st>>>(P, T, sq, n, H, stp);
-    else
-        k_shadow<LDS, WIDTH, false><<<g, kBlock, 0, st>>>(P, T, sq, n, H, stp);
-}
-void launch_shadow(int lds, int width, bool instr, uint32_t g, hipStream_t st, const RenderParams& P, Traversal T,
-                   ShadowQ sq, const uint32_t* n, Hist H, unsigned long long* stp) {
-    if (width == 4) {
-        if (lds >= 32)
-            launch_shadow_t<32, 4>(instr, g, st, P, T, sq, n, H, stp);
-        else
-            launch_shadow_t<16, 4>(instr, g, st, P, T, sq, n, H, stp);
-    } else {
-        if (lds >= 32)
-            launch_shadow_t<32, 2>(instr, g, st, P, T, sq, n, H, stp);
-        else
-            launch_shadow_t<16, 2>(instr, g, st, P, T, sq, n, H, stp);
+
+// Blocks per CU that can be resident for kernel `fn` (0 on error).
+int resident_blocks(const void* fn, int block) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
     }
+    return nb;
 }
 
 // ---------------------------------------------------------------------------
@@ -860,10 +856,15 @@ struct DevBuf {
         if (_s != EAR_OK) return _s; \
     } while (0)
 
-uint32_t persistent_grid(Ctx* c, uint32_t work, uint32_t waves_per_cu) {
-    uint32_t blocks_needed = (work + kBlock - 1) / kBlock;
-    uint32_t cap = (uint32_t)c->num_cus * std::max<uint32_t>(1, waves_per_cu) / (kBlock / 64);
+uint32_t persistent_grid(Ctx* c, uint32_t work, uint32_t waves_per_cu, uint32_t block = kBlock) {
+    uint32_t blocks_needed = (work + block - 1) / block;
+    uint32_t cap = (uint32_t)c->num_cus * std::max<uint32_t>(1, waves_per_cu) / std::max<uint32_t>(1, block / 64);
     return std::max<uint32_t>(1, std::min(blocks_needed, std::max<uint32_t>(1, cap)));
+}
+
+uint32_t grid_for(uint32_t work, uint32_t cap_blocks, uint32_t block) {
+    uint32_t blocks_needed = (work + block - 1) / block;
+    return std::max<uint32_t>(1, std::min(blocks_needed, std::max<uint32_t>(1, cap_blocks)));
 }
 
 struct EventPair {
@@ -1026,8 +1027,24 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
     const ear_tuning& tu = c->tuning;
     uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 22);
     chunk = std::min<uint32_t>(chunk, std::max<uint32_t>(1, D->n_rays));
-    uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 32;
     uint32_t cap_rays = chunk * n_bands;
+    // traversal kernels and their persistent grids: resident blocks per CU from
+    // the occupancy calculator unless the caller fixed waves per CU
+    const bool instr = (D->flags & EAR_FLAG_INSTRUMENT) != 0;
+    const TravCfg cfg_e{tu.block_extend == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, (int)s->width, instr};
+    const TravCfg cfg_s{tu.block_shadow == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, (int)s->width, instr};
+    ExtendFn fe, fe_unused;
+    ShadowFn fs, fs_unused;
+    pick_kernels(cfg_e, &fe, &fs_unused);
+    pick_kernels(cfg_s, &fe_unused, &fs);
+    auto cap_for = [&](const void* fn, int block) -> uint32_t {
+        if (tu.waves_per_cu) return (uint32_t)c->num_cus * tu.waves_per_cu / std::max(1, block / 64);
+        int nb = resident_blocks(fn, block);
+        return (uint32_t)c->num_cus * (uint32_t)std::max(1, nb);
+    };
+    const uint32_t cap_e = std::max<uint32_t>(1, cap_for((const void*)fe, cfg_e.block));
+    const uint32_t cap_s = std::max<uint32_t>(1, cap_for((const void*)fs, cfg_s.block));
+    const uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 32;  // non-traversal kernels
 
     hipStream_t st = c->stream;
     DevBuf qo[2] = {DevBuf(c), DevBuf(c)}, qd[2] = {DevBuf(c), DevBuf(c)}, qm[2] = {DevBuf(c), DevBuf(c)};
@@ -1046,8 +1063,7 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
     TRY(stats.alloc(ST_N * 8));
     size_t cells = (size_t)3 * nch * P.hist_kinds * D->hist_bins;
     TRY(hist.alloc(cells * 16));
-    uint32_t max_grid = persistent_grid(c, 0xFFFFFFFFu, waves);
-    uint32_t gthreads = max_grid * kBlock;
+    uint32_t gthreads = std::max(cap_e * (uint32_t)cfg_e.block, cap_s * (uint32_t)cfg_s.block);
     TRY(gs.alloc((size_t)(kStackDepth - kLdsMin) * gthreads * 4));
     TRY(dbands.alloc(3 * 4));
     TRY(direct.alloc((size_t)3 * nch * sizeof(DirectRamp)));
@@ -1077,9 +1093,7 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
     unsigned long long* stp = stats.as<unsigned long long>();
     Traversal T{s->nodes, s->nodes4, s->tris, s->n_tris, gs.as<int>(), gthreads};
     Hist H{hist.as<unsigned long long>(), D->hist_bins, nch, P.hist_kinds};
-    bool instr = (D->flags & EAR_FLAG_INSTRUMENT) != 0;
     const int width = (int)s->width;
-    const int lds_depth = tu.stack_lds >= 32 ? 32 : 16;
 
     uint32_t* pinned = nullptr;
     EAR_HIP_TRY(c, hipHostMalloc((void**)&pinned, 16, hipHostMallocDefault));
@@ -1120,10 +1134,11 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
             EAR_HIP_TRY(c, hipStreamSynchronize(st));
             uint32_t nsh = pinned[2];
             if (nsh) {
-                uint32_t gsh = persistent_grid(c, nsh, waves);
+                uint32_t gsh = grid_for(nsh, cap_s, cfg_s.block);
                 EventPair& e = mk(ev_shw);
                 hipEventRecord(e.a, st);
-                launch_shadow(lds_depth, width, instr, gsh, st, P, T, SQ, cnt + 2, H, stp);
+                hipLaunchKernelGGL(fs, dim3(gsh), dim3(cfg_s.block), 0, st, P, T, SQ, (const uint32_t*)(cnt + 2), H,
+                                   stp);
                 hipEventRecord(e.b, st);
                 shadow_q += nsh;
                 ++launches;
@@ -1137,14 +1152,16 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
         for (uint32_t k = 1; k <= D->max_bounces && n_live > 0; ++k) {
             Queue qi = cur == 0 ? q0 : q1;
             Queue qn = cur == 0 ? q1 : q0;
-            uint32_t g1 = persistent_grid(c, n_live, waves);
+            uint32_t g1 = grid_for(n_live, cap_e, cfg_e.block);
             EventPair& e1 = mk(ev_ext);
             hipEventRecord(e1.a, st);
-            launch_extend(lds_depth, width, instr, g1, st, T, qi, cnt + cur, hits.as<int2>(), stp);
+            hipLaunchKernelGGL(fe, dim3(g1), dim3(cfg_e.block), 0, st, T, qi, (const uint32_t*)(cnt + cur),
+                               hits.as<int2>(), stp);
+            uint32_t gsd = persistent_grid(c, n_live, waves);
             hipEventRecord(e1.b, st);
             EventPair& e2 = mk(ev_sha);
             hipEventRecord(e2.a, st);
-            k_shade<<<g1, kBlock, 0, st>>>(P, s->tris, s->nm, s->mat_band, qi, cnt + cur, hits.as<int2>(), qn,
+            k_shade<<<gsd, kBlock, 0, st>>>(P, s->tris, s->nm, s->mat_band, qi, cnt + cur, hits.as<int2>(), qn,
                                            cnt + (1 - cur), SQ, cnt + 2, stp, logh.as<int32_t>(),
                                            logn.as<uint32_t>());
             hipEventRecord(e2.b, st);
@@ -1154,10 +1171,11 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
             extend_q += n_live;
             launches += 2;
             if (nsh) {
-                uint32_t g2 = persistent_grid(c, nsh, waves);
+                uint32_t g2 = grid_for(nsh, cap_s, cfg_s.block);
                 EventPair& e3 = mk(ev_shw);
                 hipEventRecord(e3.a, st);
-                launch_shadow(lds_depth, width, instr, g2, st, P, T, SQ, cnt + 2, H, stp);
+                hipLaunchKernelGGL(fs, dim3(g2), dim3(cfg_s.block), 0, st, P, T, SQ, (const uint32_t*)(cnt + 2), H,
+                                   stp);
                 hipEventRecord(e3.b, st);
                 shadow_q += nsh;
                 ++launches;

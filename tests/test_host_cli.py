@@ -1,0 +1,142 @@
+"""The host C++ layer and the EAR command line (.EAR reader, error paths).
+
+CPU tests run `EAR info` / `EAR test`, which make no device call; the GPU test
+runs `EAR calc T60` and checks it against the same render through the Python
+front end."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from ear_amd import earfile, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EAR = os.path.join(ROOT, "ear_amd", "bin", "EAR")
+
+
+def run(*args):
+    r = subprocess.run([EAR, *args], capture_output=True, text=True, timeout=120)
+    return r.returncode, r.stdout
+
+
+def info(path):
+    rc, out = run("info", str(path))
+    assert rc == 0, out
+    d = {}
+    for line in out.splitlines():
+        parts = line.split()
+        if len(parts) == 2:
+            try:
+                d[parts[0]] = float(parts[1])
+            except ValueError:
+                pass
+    return d, out
+
+
+def test_ear_test_returns_zero():
+    assert run("test")[0] == 0
+
+
+def test_usage_without_arguments():
+    rc, out = run()
+    assert rc == 1 and "EAR calc T60" in out
+
+
+def test_reader_round_trip_shoebox(tmp_path):
+    s = scenes.shoebox(stereo=True)
+    p = tmp_path / "box.EAR"
+    earfile.scene_to_ear(s, p)
+    d, out = info(p)
+    assert d["materials"] == 6 and d["meshes"] == 6 and d["triangles"] == 12
+    assert d["sources"] == 1 and d["listeners"] == 1 and d["samples"] == 1e5
+    assert abs(d["volume"] - 320.0) < 1e-3 and abs(d["area"] - 304.0) < 1e-3
+    # Mesh::Combine drops the weighted area of merged meshes (src/Mesh.cpp:116-123):
+    # the total absorption is the first MESH block's only
+    alpha_mid_first = 1.0 - s["materials"][0][0][1]
+    first_area = 10.0 * 8.0
+    assert abs(d["total_absorption"] - alpha_mid_first * first_area) < 1e-3
+    assert "stereo listener" in out
+
+
+def test_string_padding_and_keyframes(tmp_path):
+    w = earfile.EarWriter()
+    w.material("abcd", (0.5, 0.5, 0.5))  # 4 chars -> four NUL bytes
+    w.settings()
+    w.keys([0.0, 0.5, 1.0])
+    w.mesh("abcd", [[(0, 0, 0), (1, 0, 0), (0, 1, 0)]])
+    w.source("x.wav", anim=[(0, 0, 1), (0.5, 0, 1), (1, 0, 1)], gain=0.5)
+    w.mono("o.wav", anim=[(2, 0, 1), (2, 0.5, 1), (2, 1, 1)])
+    p = tmp_path / "k.EAR"
+    w.save(p)
+    d, _ = info(p)
+    assert d["keyframes"] == 3 and d["triangles"] == 1
+
+
+@pytest.mark.parametrize("case,needle", [
+    ("bad_material", "Invalid material settings"),
+    ("mesh_before_mat", "before its MAT block"),
+    ("anim_count", "Keyframe count does not match"),
+    ("truncated", "unexpected end"),
+])
+def test_reader_errors(tmp_path, case, needle):
+    w = earfile.EarWriter()
+    if case == "bad_material":
+        w.material("m", (0.8, 0.8, 0.8), (0.5, 0.5, 0.5))
+    elif case == "mesh_before_mat":
+        w.mesh("m", [[(0, 0, 0), (1, 0, 0), (0, 1, 0)]])
+    elif case == "anim_count":
+        w.material("m", (0.5, 0.5, 0.5))
+        w.settings()
+        w.keys([0.0, 1.0])
+        w.source("x.wav", anim=[(0, 0, 1)])
+    elif case == "truncated":
+        w.material("m", (0.5, 0.5, 0.5))
+        w.raw(b"SET \x40\x00\x00\x00str ")
+    p = tmp_path / f"{case}.EAR"
+    w.save(p)
+    rc, out = run("info", str(p))
+    assert rc == 1 and needle in out, out
+
+
+def test_not_an_ear_file(tmp_path):
+    p = tmp_path / "x.EAR"
+    p.write_bytes(b"RIFF0000")
+    rc, out = run("calc", "T60", str(p))
+    assert rc == 1 and "Failed to read file" in out
+
+
+def test_missing_sources_and_listeners(tmp_path):
+    w = earfile.EarWriter()
+    w.material("m", (0.5, 0.5, 0.5))
+    w.settings()
+    p = tmp_path / "n.EAR"
+    w.save(p)
+    rc, out = run("calc", "T60", str(p))
+    assert rc == 1 and "No sound sources defined" in out
+
+
+@pytest.mark.gpu
+def test_calc_t60_matches_python_front_end(tmp_path):
+    from ear_amd import acoustics, api
+
+    s = scenes.shoebox(spec=0.2)
+    p = tmp_path / "box.EAR"
+    earfile.scene_to_ear(s, p, samples=200_000, seed=1234, maxbounces=200, irlength=2.0)
+    rc, out = run("calc", "T60", str(p))
+    assert rc == 0, out
+    vals = {l.split(":")[0].strip(): float(l.split(":")[1].strip().rstrip("s")) for l in out.splitlines()
+            if l.startswith("T60_")}
+    # same render through the C ABI directly: 20000 rays (samples / 10), mid band
+    with api.Context(0) as ctx, api.Scene(ctx, s["tris"], s["mat"], s["materials"]) as sc:
+        air = (0.99, 0.98, 0.95)
+        r = api.render(ctx, sc, recorders=s["recorders"], n_rays=20000, bands=(1,), air=air, src_pos=s["src"],
+                       seed=1234, max_bounces=200, cutoff=1e-8, n_bins=88200, abs_hist=False)
+    track = acoustics.power(r.value[1, 0].astype(np.float32))
+    thr = np.abs(track).max() / 256.0
+    nz = np.nonzero(np.abs(track) >= thr)[0]
+    t60 = acoustics.t60_reference(track, first=int(np.nonzero(track)[0][0]), last=int(nz[-1]) + 1)
+    assert abs(vals["T60_ear"] - t60) < 2.0 / 44100.0, (vals, t60)
+    V, S = 320.0, 304.0
+    A = (1.0 - s["materials"][0][0][1]) * 80.0
+    assert abs(vals["T60_sabine"] - 0.1611 * V / (A + 4 * 0.02 * V)) < 1e-4
