@@ -27,7 +27,7 @@ EAR_FRAC_BITS = 88
 EXPORTS = [
     "ear_ctx_create", "ear_ctx_destroy", "ear_ctx_set_memory_cap", "ear_status_string", "ear_ctx_last_error",
     "ear_abi_version", "ear_scene_upload", "ear_scene_destroy", "ear_scene_get_info", "ear_query_closest",
-    "ear_query_occluded", "ear_render", "ear_ctx_set_tuning", "ear_ctx_get_tuning",
+    "ear_query_occluded", "ear_render", "ear_render_batch", "ear_ctx_set_tuning", "ear_ctx_get_tuning",
 ]
 
 
@@ -79,6 +79,10 @@ class ear_stats(C.Structure):
                 ("shadow_queries", C.c_uint64)]
 
 
+class ear_batch_item(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("desc", C.POINTER(ear_render_desc)), ("out", C.POINTER(ear_hist))]
+
+
 class ear_tuning(C.Structure):
     _fields_ = [("block_extend", C.c_uint32), ("block_shade", C.c_uint32), ("block_shadow", C.c_uint32),
                 ("ray_chunk", C.c_uint32), ("waves_per_cu", C.c_uint32), ("stack_lds", C.c_uint32),
@@ -113,6 +117,7 @@ def load(path: str = LIB_PATH):
     L.ear_query_closest.argtypes = [vp, C.c_uint32, fp, fp, fp, fp]
     L.ear_query_occluded.argtypes = [vp, C.c_uint32, fp, fp, fp]
     L.ear_render.argtypes = [vp, vp, C.POINTER(ear_render_desc), C.POINTER(ear_hist), C.POINTER(ear_stats)]
+    L.ear_render_batch.argtypes = [vp, C.POINTER(ear_batch_item), C.c_uint32, C.POINTER(ear_stats)]
     L.ear_ctx_set_tuning.argtypes = [vp, C.POINTER(ear_tuning)]
     L.ear_ctx_get_tuning.argtypes = [vp, C.POINTER(ear_tuning)]
     for name in EXPORTS:

@@ -186,63 +186,93 @@ class RenderResult:
     extra: dict = field(default_factory=dict)
 
 
-def render(ctx: Context, scene: Scene, *, recorders, n_rays, bands=(0, 1, 2), air=(0.99, 0.98, 0.95), dry=1.0,
-           src_pos=(0, 0, 0), src_tris=None, gain=1.0, max_bounces=999, cutoff=1e-8, seed=0xEA2, instance=0,
-           keyframe=-1, sound=0, n_bins=88200, abs_hist=True, instrument=False, trace_rays=0, trace_segs=0,
-           want_raw=True, want_value=True) -> RenderResult:
+class _Prepared:
+    """ctypes descriptors of one render and the numpy arrays they point at."""
+
+    def __init__(self, *, recorders, n_rays, bands=(0, 1, 2), air=(0.99, 0.98, 0.95), dry=1.0, src_pos=(0, 0, 0),
+                 src_tris=None, gain=1.0, max_bounces=999, cutoff=1e-8, seed=0xEA2, instance=0, keyframe=-1,
+                 sound=0, n_bins=88200, abs_hist=True, instrument=False, trace_rays=0, trace_segs=0, want_raw=True,
+                 want_value=True):
+        self.recs = make_recorders(recorders)
+        nch = n_channels(recorders)
+        d = F.ear_render_desc()
+        d.band_mask = sum(1 << b for b in bands)
+        d.air[:] = [float(x) for x in air]
+        d.dry = dry
+        self.src_arr = None
+        if src_tris is not None:
+            self.src_arr = np.ascontiguousarray(src_tris, np.float32).reshape(-1, 3, 3)
+            d.source.kind = F.EAR_SRC_MESH
+            d.source.mesh_v = self.src_arr.ctypes.data
+            d.source.mesh_n_tris = len(self.src_arr)
+        else:
+            d.source.kind = F.EAR_SRC_POINT
+        d.source.pos[:] = [float(x) for x in src_pos]
+        d.source.gain = gain
+        d.recorders = self.recs
+        d.n_recorders = len(recorders)
+        d.n_rays = n_rays
+        d.max_bounces = max_bounces
+        d.energy_cutoff = cutoff
+        d.seed = seed
+        d.instance = instance
+        d.keyframe = keyframe
+        d.sound = sound
+        d.hist_bins = n_bins
+        flags = 0
+        if abs_hist:
+            flags |= F.EAR_FLAG_ABS_HIST
+        if instrument:
+            flags |= F.EAR_FLAG_INSTRUMENT
+        if trace_rays and trace_segs:
+            flags |= F.EAR_FLAG_TRACE_LOG
+        d.flags = flags
+        d.trace_rays = trace_rays
+        d.trace_segs = trace_segs
+        self.desc = d
+        self.bands = tuple(bands)
+        self.value = np.zeros((3, nch, n_bins), np.float64) if want_value else None
+        self.abs_value = np.zeros((3, nch, n_bins), np.float64) if (abs_hist and want_value) else None
+        self.raw = np.zeros((3, nch, n_bins, 2), np.int64) if want_raw else None
+        self.abs_raw = np.zeros((3, nch, n_bins, 2), np.int64) if (abs_hist and want_raw) else None
+        self.th = self.tn = None
+        if flags & F.EAR_FLAG_TRACE_LOG:
+            self.th = np.full((3, trace_rays, trace_segs), -2, np.int32)
+            self.tn = np.zeros((3, trace_rays), np.uint32)
+        self.hist = F.ear_hist(_ptr(self.raw), _ptr(self.value), _ptr(self.abs_raw), _ptr(self.abs_value),
+                               _ptr(self.th), _ptr(self.tn))
+
+    def result(self, stats: dict, wall: float) -> "RenderResult":
+        return RenderResult(self.value, self.abs_value, self.raw, self.abs_raw, stats, self.bands, self.th, self.tn,
+                            wall)
+
+
+def render(ctx: Context, scene: Scene, **kw) -> RenderResult:
+    """One render (ear_render): the band contexts of one (sound, keyframe)."""
     import time
 
-    lib = ctx.lib
-    recs = make_recorders(recorders)
-    nch = n_channels(recorders)
-    d = F.ear_render_desc()
-    d.band_mask = sum(1 << b for b in bands)
-    d.air[:] = [float(x) for x in air]
-    d.dry = dry
-    src_arr = None
-    if src_tris is not None:
-        src_arr = np.ascontiguousarray(src_tris, np.float32).reshape(-1, 3, 3)
-        d.source.kind = F.EAR_SRC_MESH
-        d.source.mesh_v = src_arr.ctypes.data
-        d.source.mesh_n_tris = len(src_arr)
-    else:
-        d.source.kind = F.EAR_SRC_POINT
-    d.source.pos[:] = [float(x) for x in src_pos]
-    d.source.gain = gain
-    d.recorders = recs
-    d.n_recorders = len(recorders)
-    d.n_rays = n_rays
-    d.max_bounces = max_bounces
-    d.energy_cutoff = cutoff
-    d.seed = seed
-    d.instance = instance
-    d.keyframe = keyframe
-    d.sound = sound
-    d.hist_bins = n_bins
-    flags = 0
-    if abs_hist:
-        flags |= F.EAR_FLAG_ABS_HIST
-    if instrument:
-        flags |= F.EAR_FLAG_INSTRUMENT
-    if trace_rays and trace_segs:
-        flags |= F.EAR_FLAG_TRACE_LOG
-    d.flags = flags
-    d.trace_rays = trace_rays
-    d.trace_segs = trace_segs
-
-    value = np.zeros((3, nch, n_bins), np.float64) if want_value else None
-    abs_value = np.zeros((3, nch, n_bins), np.float64) if (abs_hist and want_value) else None
-    raw = np.zeros((3, nch, n_bins, 2), np.int64) if want_raw else None
-    abs_raw = np.zeros((3, nch, n_bins, 2), np.int64) if (abs_hist and want_raw) else None
-    th = tn = None
-    if flags & F.EAR_FLAG_TRACE_LOG:
-        th = np.full((3, trace_rays, trace_segs), -2, np.int32)
-        tn = np.zeros((3, trace_rays), np.uint32)
-    h = F.ear_hist(_ptr(raw), _ptr(value), _ptr(abs_raw), _ptr(abs_value), _ptr(th), _ptr(tn))
+    p = _Prepared(**kw)
     st = F.ear_stats()
     t0 = time.perf_counter()
-    _check(lib, lib.ear_render(ctx.h, scene.h, C.byref(d), C.byref(h), C.byref(st)), ctx.h)
+    _check(ctx.lib, ctx.lib.ear_render(ctx.h, scene.h, C.byref(p.desc), C.byref(p.hist), C.byref(st)), ctx.h)
+    wall = time.perf_counter() - t0
+    return p.result({k: getattr(st, k) for k, _ in F.ear_stats._fields_}, wall)
+
+
+def render_batch(ctx: Context, items) -> tuple[list, dict, float]:
+    """Several renders in one wavefront (ear_render_batch).  items: [(scene,
+    kwargs of render)].  Returns (results, aggregated stats, wall seconds)."""
+    import time
+
+    preps = [_Prepared(**kw) for _, kw in items]
+    arr = (F.ear_batch_item * len(items))()
+    for i, ((scene, _), p) in enumerate(zip(items, preps)):
+        arr[i].scene = scene.h
+        arr[i].desc = C.pointer(p.desc)
+        arr[i].out = C.pointer(p.hist)
+    st = F.ear_stats()
+    t0 = time.perf_counter()
+    _check(ctx.lib, ctx.lib.ear_render_batch(ctx.h, arr, len(items), C.byref(st)), ctx.h)
     wall = time.perf_counter() - t0
     stats = {k: getattr(st, k) for k, _ in F.ear_stats._fields_}
-    del src_arr
-    return RenderResult(value, abs_value, raw, abs_raw, stats, tuple(bands), th, tn, wall)
+    return [p.result({}, wall) for p in preps], stats, wall

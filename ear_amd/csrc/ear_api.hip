@@ -8,6 +8,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "ear_internal.hpp"
 
@@ -184,6 +185,33 @@ ear_status ear_render(ear_ctx* c, ear_scene* s, const ear_render_desc* d, ear_hi
     if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
     try {
         return run_render(c, &s->ds, d, out, stats);
+    } catch (const std::bad_alloc&) {
+        c->last_error = "host allocation failed";
+        return EAR_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        c->last_error = "internal exception";
+        return EAR_ERR_INTERNAL;
+    }
+}
+
+ear_status ear_render_batch(ear_ctx* c, const ear_batch_item* items, uint32_t n, ear_stats* stats) {
+    if (!c || !items || n == 0) return EAR_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    try {
+        std::vector<DeviceScene*> sc(n);
+        std::vector<const ear_render_desc*> ds(n);
+        std::vector<ear_hist*> os(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!items[i].scene || !items[i].desc) return EAR_ERR_INVALID_ARGUMENT;
+            if (items[i].scene->ctx != c) {
+                c->last_error = "scene belongs to another context";
+                return EAR_ERR_INVALID_ARGUMENT;
+            }
+            sc[i] = &items[i].scene->ds;
+            ds[i] = items[i].desc;
+            os[i] = items[i].out;
+        }
+        return run_batch(c, n, sc.data(), ds.data(), os.data(), stats);
     } catch (const std::bad_alloc&) {
         c->last_error = "host allocation failed";
         return EAR_ERR_OUT_OF_MEMORY;

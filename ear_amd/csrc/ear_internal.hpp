@@ -95,6 +95,7 @@ struct RenderParams {
     uint32_t n_channels;
     uint32_t n_rec;
     uint32_t n_band_slots;  // always 3 (histograms laid out per band slot)
+    uint32_t band_mask;
     uint32_t hist_kinds;    // 1 signed, 2 signed + abs
     int32_t src_kind;
     float src_pos[3];
@@ -128,6 +129,8 @@ void free_scene(Ctx* c, DeviceScene* s);
 
 // tracer.hip
 ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* d, ear_hist* out, ear_stats* st);
+ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const ear_render_desc* const* descs,
+                     ear_hist* const* outs, ear_stats* st);
 ear_status run_query_closest(Ctx* c, DeviceScene* s, uint32_t n, const float* o, const float* d, float* t,
                              int32_t* tri);
 ear_status run_query_occluded(Ctx* c, DeviceScene* s, uint32_t n, const float* p, const float* q,

@@ -341,8 +341,7 @@ struct Hist {
 
 // Recorder::Record: MonoRecorder (src/MonoRecorder.cpp:83-97) or
 // StereoRecorder (src/StereoRecorder.cpp:97-130).
-__device__ void record(const RenderParams& P, const Hist& H, int band, const RecParams& R, V3 dir, float a,
-                       float t, float dist) {
+__device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, float a, float t, float dist) {
     bool ab = H.kinds > 1;
     if (R.kind == EAR_REC_MONO) {
         int s = (int)((double)t * 44100.0);
@@ -371,7 +370,7 @@ __device__ void record(const RenderParams& P, const Hist& H, int band, const Rec
 }
 
 // ---------------------------------------------------------------------------
-// kernels
+// per-instance device table
 // ---------------------------------------------------------------------------
 struct MeshSrc {
     const float* v;     // [n][9]
@@ -379,12 +378,56 @@ struct MeshSrc {
     const float* nrm;   // [n][3]
 };
 
-__global__ void __launch_bounds__(kBlock) k_generate(RenderParams P, MeshSrc M, const int* __restrict__ bands,
-                                                      uint32_t n_bands, uint32_t ray_base, uint32_t n_chunk,
-                                                      Queue q, uint32_t* __restrict__ n_q, ShadowQ sq,
-                                                      uint32_t* __restrict__ n_sq,
-                                                      unsigned long long* __restrict__ stats) {
-    uint32_t total = n_bands * n_chunk;
+// direct sound per (band, channel) (src/Scene.cpp:299-311): {visible, s, w, A}
+struct DirectRamp {
+    int32_t visible, s, w;
+    float a;
+};
+
+// One render instance: a (scene, render description) pair of a batch.  A
+// plain render is a batch of one.
+struct InstDev {
+    RenderParams P;
+    const BvhNode* nodes;
+    const Bvh4Node* nodes4;
+    const float4* tris;
+    const float4* nm;
+    const float4* mat_band;
+    uint32_t n_tris;
+    uint32_t pad0;
+    MeshSrc M;
+    unsigned long long* hist;  // [3][n_ch][kinds][bins] x 2
+    DirectRamp* direct;        // [3][n_ch]
+    double* value;             // [3][n_ch][kinds][bins] or null
+    int32_t* log_hits;         // [3][trace_rays][trace_segs] or null
+    uint32_t* log_nseg;        // [3][trace_rays]
+    double inv_n, gain2;
+};
+
+// a run of rays of one (instance, band) context inside a chunk
+struct GenSeg {
+    uint32_t inst, band, ray0, count;
+};
+
+__device__ __forceinline__ Traversal make_trav(const InstDev& I, int* gstack, uint32_t gthreads) {
+    return Traversal{I.nodes, I.nodes4, I.tris, I.n_tris, gstack, gthreads};
+}
+__device__ __forceinline__ Hist make_hist(const InstDev& I) {
+    return Hist{I.hist, I.P.n_bins, I.P.n_channels, I.P.hist_kinds};
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+// Emission, k = 0 (AbstractSoundFile::SoundRay, src/SoundFile.cpp:215-228).
+// Thread gid of the chunk maps to (instance, band, ray) through the segment
+// table (exclusive prefix `seg_off`, binary search).
+template <bool BATCH>
+__global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__ inst, const GenSeg* __restrict__ segs,
+                                                      const uint32_t* __restrict__ seg_off, uint32_t n_segs,
+                                                      uint32_t total, uint32_t max_rec, uint32_t any_mesh, Queue q,
+                                                      uint32_t* __restrict__ n_q, ShadowQ sq,
+                                                      uint32_t* __restrict__ n_sq, unsigned long long* __restrict__ stats) {
     for (uint32_t base = blockIdx.x * kBlock; base < total; base += gridDim.x * kBlock) {
         uint32_t gid = base + threadIdx.x;
         bool valid = gid < total;
@@ -392,59 +435,76 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderParams P, MeshSrc M, 
         float4 ro{}, rd{};
         uint4 rm{};
         int band = 0;
-        uint32_t ray = 0;
+        uint32_t id = 0;
         Rng rng;
         V3 o{0, 0, 0};
+        bool mesh_src = false;
+        uint32_t n_rec = 0;
         if (valid) {
-            band = bands[gid / n_chunk];
-            ray = ray_base + gid % n_chunk;
+            uint32_t lo = 0, hi = n_segs - 1;
+            while (lo < hi) {  // last segment with seg_off <= gid
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (seg_off[mid] <= gid)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const GenSeg g = segs[lo];
+            id = BATCH ? g.inst : 0;
+            const InstDev& I = inst[id];
+            const RenderParams& P = I.P;
+            band = (int)g.band;
+            uint32_t ray = g.ray0 + (gid - seg_off[lo]);
             rng.init(P.seed, ray, P.ctx_word[band], P.instance, 0);
             V3 d;
-            if (P.src_kind == EAR_SRC_MESH) {
+            mesh_src = P.src_kind == EAR_SRC_MESH;
+            n_rec = P.n_rec;
+            if (mesh_src) {
                 float x = rng.next() * P.src_total_area;
                 uint32_t pick = P.src_n_tris - 1;
                 for (uint32_t i = 0; i < P.src_n_tris; ++i) {
-                    x -= M.area[i];
+                    x -= I.M.area[i];
                     if (x < 0.0f) {
                         pick = i;
                         break;
                     }
                 }
-                const float* v = M.v + 9 * (size_t)pick;
+                const float* v = I.M.v + 9 * (size_t)pick;
                 float r1 = rng.next();
                 float r2 = rng.next();
                 float sr1 = sqrtf(r1);
                 float ca = 1.0f - sr1, cb = sr1 * (1.0f - r2), cc = sr1 * r2;
                 o = V3{ca * v[0] + cb * v[3] + cc * v[6], ca * v[1] + cb * v[4] + cc * v[7],
                        ca * v[2] + cb * v[5] + cc * v[8]};
-                d = sample_hemi(rng, V3{M.nrm[3 * pick], M.nrm[3 * pick + 1], M.nrm[3 * pick + 2]});
+                d = sample_hemi(rng, V3{I.M.nrm[3 * pick], I.M.nrm[3 * pick + 1], I.M.nrm[3 * pick + 2]});
             } else {
                 o = V3{P.src_pos[0], P.src_pos[1], P.src_pos[2]};
                 d = sample_sphere(rng);
             }
             ro = make_float4(o.x, o.y, o.z, 1.0f);
             rd = make_float4(d.x, d.y, d.z, 0.0f);
-            rm = make_uint4(ray, rng.draw, (uint32_t)band, 0u);
+            rm = make_uint4(ray, rng.draw, (uint32_t)band, id);
             live = !((double)1.0f < P.cutoff) && P.max_bounces > 0;
         }
         // mesh sources connect to every recorder at k = 0 (src/Scene.cpp:185)
-        if (P.src_kind == EAR_SRC_MESH) {
-            for (uint32_t r = 0; r < P.n_rec; ++r) {
+        if (any_mesh) {
+            for (uint32_t r = 0; r < max_rec; ++r) {
                 bool emit = false;
                 float c = 0.0f;
-                if (valid) {
-                    const RecParams& R = P.rec[r];
+                if (valid && mesh_src && r < n_rec) {
+                    const RecParams& R = inst[id].P.rec[r];
                     V3 dv = sub(V3{R.px, R.py, R.pz}, o);
                     float l = length(dv);
                     c = 1.0f;
-                    c *= powf_ear(P.air[band], l);
+                    c *= powf_ear(inst[id].P.air[band], l);
                     c *= 2.0f / (4.0f * kPi * l * l);
                     emit = is_normal(c);
                 }
                 uint32_t slot = wave_append(n_sq, emit);
                 if (emit) {
                     sq.p[slot] = make_float4(o.x, o.y, o.z, c);
-                    sq.m[slot] = make_float4(0.0f, __uint_as_float(r | ((uint32_t)band << 8)), 0.0f, 0.0f);
+                    sq.m[slot] = make_float4(0.0f, __uint_as_float(r | ((uint32_t)band << 8)), __uint_as_float(id),
+                                             0.0f);
                 }
             }
         }
@@ -458,9 +518,10 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderParams P, MeshSrc M, 
     }
 }
 
-template <int BLOCK, int LDS, int WIDTH, bool INSTR>
-__global__ void __launch_bounds__(BLOCK) k_extend(Traversal T, Queue q, const uint32_t* __restrict__ n_q,
-                                                   int2* __restrict__ hits, unsigned long long* __restrict__ stats) {
+template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
+__global__ void __launch_bounds__(BLOCK) k_extend(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
+                                                   Queue q, const uint32_t* __restrict__ n_q, int2* __restrict__ hits,
+                                                   unsigned long long* __restrict__ stats) {
     __shared__ int lds[LDS * BLOCK];
     uint32_t n = *n_q;
     uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -469,6 +530,8 @@ __global__ void __launch_bounds__(BLOCK) k_extend(Traversal T, Queue q, const ui
         uint32_t i = base + threadIdx.x;
         if (i < n) {
             float4 o = q.o[i], d = q.d[i];
+            uint32_t id = BATCH ? q.m[i].w : 0;
+            Traversal T = make_trav(inst[id], gstack, gthreads);
             float bt = 1000000.0f;
             int bs = -1;
             uint32_t bo = 0;
@@ -483,50 +546,54 @@ __global__ void __launch_bounds__(BLOCK) k_extend(Traversal T, Queue q, const ui
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_shade(RenderParams P, const float4* __restrict__ tris,
-                                                   const float4* __restrict__ nm,
-                                                   const float4* __restrict__ mat_band, Queue qin,
+// Scene::Bounce + Material::Bounce + the energy bookkeeping and NEE weight of
+// Scene::Render (src/Scene.cpp:49-82, 151-277, src/Material.cpp:76-83).
+template <bool BATCH>
+__global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ inst, Queue qin,
                                                    const uint32_t* __restrict__ n_in, const int2* __restrict__ hits,
                                                    Queue qout, uint32_t* __restrict__ n_out, ShadowQ sq,
-                                                   uint32_t* __restrict__ n_sq, unsigned long long* __restrict__ stats,
-                                                   int32_t* __restrict__ log_hits, uint32_t* __restrict__ log_nseg) {
+                                                   uint32_t* __restrict__ n_sq, uint32_t max_rec,
+                                                   unsigned long long* __restrict__ stats) {
     uint32_t n = *n_in;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < n;
         bool alive = false;  // hit and intensity normal: NEE runs
         bool cont = false;   // continues to the next segment
-        float4 ro{}, rd{};
         uint4 rm{};
         V3 p{0, 0, 0}, nrm{0, 0, 0}, prev{0, 0, 0}, d{0, 0, 0};
         float I = 0.0f, L = 0.0f, spec = 0.0f;
         int bt = kReflect;
         int band = 0;
-        uint32_t k = 0;
+        uint32_t k = 0, id = 0, n_rec = 0;
         Rng rng;
         if (valid) {
-            ro = qin.o[i];
-            rd = qin.d[i];
+            float4 ro = qin.o[i];
+            float4 rd = qin.d[i];
             rm = qin.m[i];
+            id = BATCH ? rm.w : 0;
+            const InstDev& In = inst[id];
+            const RenderParams& P = In.P;
+            n_rec = P.n_rec;
             band = (int)(rm.z & 0xFFu);
             k = ((rm.z >> 8) & 0xFFFFu) + 1;  // this segment's bounce number
             int2 h = hits[i];
-            if (log_hits && rm.x < P.trace_rays) {
+            if (In.log_hits && rm.x < P.trace_rays) {
                 if (k - 1 < P.trace_segs)
-                    log_hits[((size_t)band * P.trace_rays + rm.x) * P.trace_segs + (k - 1)] =
-                        h.y >= 0 ? (int32_t)__float_as_uint(tris[3 * h.y + 2].y) : -1;
-                log_nseg[(size_t)band * P.trace_rays + rm.x] = k;
+                    In.log_hits[((size_t)band * P.trace_rays + rm.x) * P.trace_segs + (k - 1)] =
+                        h.y >= 0 ? (int32_t)__float_as_uint(In.tris[3 * h.y + 2].y) : -1;
+                In.log_nseg[(size_t)band * P.trace_rays + rm.x] = k;
             }
             if (h.y >= 0) {
                 float t = __int_as_float(h.x);
                 V3 o{ro.x, ro.y, ro.z};
                 d = V3{rd.x, rd.y, rd.z};
                 p = V3{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
-                float4 nmv = nm[h.y];
+                float4 nmv = In.nm[h.y];
                 V3 N{nmv.x, nmv.y, nmv.z};
                 nrm = dot(N, d) > 0.0f ? V3{N.x * -1.0f, N.y * -1.0f, N.z * -1.0f} : N;
                 uint32_t mat = __float_as_uint(nmv.w);
-                float4 mb = mat_band[3 * mat + band];  // {fl2 | -1, spec, retention, -}
+                float4 mb = In.mat_band[3 * mat + band];  // {fl2 | -1, spec, retention, -}
                 rng.init(P.seed, rm.x, P.ctx_word[band], P.instance, rm.y);
                 bt = kReflect;
                 if (mb.x >= 0.0f) bt = (rng.next() <= mb.x) ? kReflect : kRefract;
@@ -542,10 +609,11 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams P, const float4* 
             }
         }
         // NEE to every recorder (src/Scene.cpp:185-268)
-        for (uint32_t r = 0; r < P.n_rec; ++r) {
+        for (uint32_t r = 0; r < max_rec; ++r) {
             bool emit = false;
             float c = 0.0f;
-            if (alive) {
+            if (alive && r < n_rec) {
+                const RenderParams& P = inst[id].P;
                 const RecParams& R = P.rec[r];
                 V3 dv = sub(V3{R.px, R.py, R.pz}, p);
                 V3 lsdir = normalized(dv);
@@ -576,10 +644,10 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams P, const float4* 
             uint32_t slot = wave_append(n_sq, emit);
             if (emit) {
                 sq.p[slot] = make_float4(p.x, p.y, p.z, c);
-                sq.m[slot] = make_float4(L, __uint_as_float(r | ((uint32_t)band << 8)), 0.0f, 0.0f);
+                sq.m[slot] = make_float4(L, __uint_as_float(r | ((uint32_t)band << 8)), __uint_as_float(id), 0.0f);
             }
         }
-        if (alive) cont = !((double)I < P.cutoff) && k < P.max_bounces;
+        if (alive) cont = !((double)I < inst[id].P.cutoff) && k < inst[id].P.max_bounces;
         float4 no{}, nd{};
         uint4 nmeta{};
         if (cont) {
@@ -589,7 +657,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams P, const float4* 
             V3 v = sample_hemi_spec(rng, nrm, rdir, spec);
             no = make_float4(p.x, p.y, p.z, I);
             nd = make_float4(v.x, v.y, v.z, L);
-            nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), 0u);
+            nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), id);
         }
         uint32_t slot = wave_append(n_out, cont);
         if (cont) {
@@ -601,9 +669,11 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams P, const float4* 
     }
 }
 
-template <int BLOCK, int LDS, int WIDTH, bool INSTR>
-__global__ void __launch_bounds__(BLOCK) k_shadow(RenderParams P, Traversal T, ShadowQ sq,
-                                                   const uint32_t* __restrict__ n_sq, Hist H,
+// Scene::Connect + Mesh::LineIntersection (src/Scene.cpp:84-96,
+// src/Mesh.cpp:58-71), then Recorder::Record for visible connections.
+template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
+__global__ void __launch_bounds__(BLOCK) k_shadow(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
+                                                   ShadowQ sq, const uint32_t* __restrict__ n_sq,
                                                    unsigned long long* __restrict__ stats) {
     __shared__ int lds[LDS * BLOCK];
     uint32_t n = *n_sq;
@@ -615,20 +685,23 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(RenderParams P, Traversal T, S
             float4 pc = sq.p[i], m = sq.m[i];
             uint32_t packed = __float_as_uint(m.y);
             int r = (int)(packed & 0xFFu), band = (int)((packed >> 8) & 0xFFu);
-            const RecParams& R = P.rec[r];
+            uint32_t id = BATCH ? __float_as_uint(m.z) : 0;
+            const InstDev& In = inst[id];
+            const RecParams& R = In.P.rec[r];
             V3 p{pc.x, pc.y, pc.z};
             V3 dv = sub(V3{R.px, R.py, R.pz}, p);
             float bt = 0.0f;
             int bs = -1;
             uint32_t bo = 0;
             ++nsh;
+            Traversal T = make_trav(In, gstack, gthreads);
             bool occ = traverse<LDS, BLOCK, WIDTH, true, INSTR>(T, p, dv, bt, bs, bo, lds + threadIdx.x, gtid, nvis,
                                                                 ntri);
             if (!occ) {
                 float l = length(dv);
                 V3 lsdir = normalized(dv);
                 float L = m.x;
-                record(P, H, band, R, lsdir, pc.w, (L + l) / 343.0f, L + l);
+                record(make_hist(In), band, R, lsdir, pc.w, (L + l) / 343.0f, L + l);
                 ++ncon;
             }
         }
@@ -648,21 +721,19 @@ __global__ void k_reset2(uint32_t* a, uint32_t* b) {
     }
 }
 
-// direct sound per recorder (src/Scene.cpp:299-311): {visible, s, w, A} per channel
-struct DirectRamp {
-    int32_t visible, s, w;
-    float a;
-};
-
+// Direct sound (src/Scene.cpp:299-311): one block per instance, one thread
+// per (band, recorder).
 template <int WIDTH>
-__global__ void k_direct(RenderParams P, Traversal T, const int* __restrict__ bands, uint32_t n_bands,
-                         DirectRamp* __restrict__ out) {
+__global__ void k_direct(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads) {
     __shared__ int lds[kLdsMin * kBlock];
+    const InstDev& In = inst[blockIdx.x];
+    const RenderParams& P = In.P;
     uint32_t t = threadIdx.x;
-    if (t >= n_bands * P.n_rec) return;
-    int band = bands[t / P.n_rec];
+    if (t >= 3 * P.n_rec) return;
+    int band = (int)(t / P.n_rec);
+    if (!((P.band_mask >> band) & 1u)) return;
     const RecParams& R = P.rec[t % P.n_rec];
-    DirectRamp* o = out + (size_t)band * P.n_channels + R.channel;
+    DirectRamp* o = In.direct + (size_t)band * P.n_channels + R.channel;
     int nch = R.kind == EAR_REC_STEREO ? 2 : 1;
     for (int c = 0; c < nch; ++c) o[c] = DirectRamp{0, 0, 0, 0.0f};
     if (P.src_kind != EAR_SRC_POINT) return;
@@ -670,8 +741,9 @@ __global__ void k_direct(RenderParams P, Traversal T, const int* __restrict__ ba
     float bt = 0.0f;
     int bs = -1;
     uint32_t bo = 0, nv = 0, nt = 0;
+    Traversal T = make_trav(In, gstack, gthreads);
     if (traverse<kLdsMin, kBlock, WIDTH, true, false>(T, rp, sub(sp, rp), bt, bs, bo, lds + threadIdx.x,
-                                              blockIdx.x * kBlock + t, nv, nt))
+                                                      blockIdx.x * kBlock + t, nv, nt))
         return;
     V3 dist = sub(rp, sp);
     float len = length(dist);
@@ -698,20 +770,19 @@ __global__ void k_direct(RenderParams P, Traversal T, const int* __restrict__ ba
 }
 
 // Two inclusive prefix sums over a second-difference row (128-bit, exact),
-// then value = (raw 2^-88 * inv_n + direct ramp) * gain2.  One block per row.
+// then value = (raw 2^-88 * inv_n + direct ramp) * gain2.  One block per row;
+// row r of the table is {instance, band, channel, kind}.
 constexpr int kFinBlock = 1024;
-__global__ void __launch_bounds__(kFinBlock) k_finalize(unsigned long long* __restrict__ cells, uint32_t nb,
-                                                       uint32_t n_ch, uint32_t kinds, const int* __restrict__ bands,
-                                                       const DirectRamp* __restrict__ direct, double inv_n,
-                                                       double gain2, double* __restrict__ value) {
+__global__ void __launch_bounds__(kFinBlock) k_finalize(const InstDev* __restrict__ inst, const uint4* __restrict__ rows) {
     __shared__ uint64_t slo[kFinBlock];
     __shared__ int64_t shi[kFinBlock];
-    uint32_t row = blockIdx.x;  // [band list index][ch][kind]
-    uint32_t kind = row % kinds;
-    uint32_t ch = (row / kinds) % n_ch;
-    int band = bands[row / (kinds * n_ch)];
+    const uint4 rw = rows[blockIdx.x];
+    const InstDev& In = inst[rw.x];
+    const RenderParams& P = In.P;
+    const int band = (int)rw.y;
+    const uint32_t ch = rw.z, kind = rw.w, nb = P.n_bins, n_ch = P.n_channels, kinds = P.hist_kinds;
     size_t roff = (((size_t)band * n_ch + ch) * kinds + kind) * nb;
-    unsigned long long* c = cells + 2 * roff;
+    unsigned long long* c = In.hist + 2 * roff;
     uint32_t per = (nb + kFinBlock - 1) / kFinBlock;
     uint32_t b = threadIdx.x * per, e = min(nb, b + per);
     for (int pass = 0; pass < 2; ++pass) {
@@ -737,18 +808,18 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(unsigned long long* __re
         }
         __syncthreads();
     }
-    if (value) {
-        const DirectRamp dr = direct[(size_t)band * n_ch + ch];
+    if (In.value) {
+        const DirectRamp dr = In.direct[(size_t)band * n_ch + ch];
         double S = dr.visible ? (double)dr.a / (double)dr.w : 0.0;
         for (uint32_t j = b; j < e; ++j) {
             double v = (double)(int64_t)c[2 * j + 1] * 18446744073709551616.0 + (double)c[2 * j];
             v = v * 3.2311742677852644e-27;  // 2^-88
-            v = v * inv_n;
+            v = v * In.inv_n;
             if (dr.visible && (int)j >= dr.s && (int)j < dr.s + dr.w) {
                 double dv = (double)(dr.w - ((int)j - dr.s)) * S;
                 v += kind == 1 ? fabs(dv) : dv;
             }
-            value[roff + j] = v * gain2;
+            In.value[roff + j] = v * In.gain2;
         }
     }
 }
@@ -770,8 +841,8 @@ __global__ void __launch_bounds__(kBlock) k_query(Traversal T, uint32_t n, const
         int bs = -1;
         uint32_t bo = 0;
         if (ANY) {
-            occ[i] = traverse<kLdsMin, kBlock, WIDTH, true, false>(T, o, sub(x, o), bt, bs, bo, lds + threadIdx.x, gtid, nv,
-                                                          nt)
+            occ[i] = traverse<kLdsMin, kBlock, WIDTH, true, false>(T, o, sub(x, o), bt, bs, bo, lds + threadIdx.x,
+                                                                   gtid, nv, nt)
                          ? 1
                          : 0;
         } else {
@@ -783,36 +854,42 @@ __global__ void __launch_bounds__(kBlock) k_query(Traversal T, uint32_t n, const
     }
 }
 
-
 // Traversal kernels are instantiated for every (block, LDS stack part, tree
-// width, instrumentation) combination; the render picks one at run time.
-// `TravCfg` names the combination and carries its occupancy-derived grid cap.
+// width, instrumentation, batch) combination; the render picks one at run time.
 struct TravCfg {
     int block, lds, width;
-    bool instr;
+    bool instr, batch;
 };
 
-using ExtendFn = void (*)(Traversal, Queue, const uint32_t*, int2*, unsigned long long*);
-using ShadowFn = void (*)(RenderParams, Traversal, ShadowQ, const uint32_t*, Hist, unsigned long long*);
+using ExtendFn = void (*)(const InstDev*, int*, uint32_t, Queue, const uint32_t*, int2*, unsigned long long*);
+using ShadowFn = void (*)(const InstDev*, int*, uint32_t, ShadowQ, const uint32_t*, unsigned long long*);
 
-template <int B, int L, int W>
+template <int B, int L, int W, bool BA>
 void pick_t(bool instr, ExtendFn* e, ShadowFn* s) {
-    *e = instr ? (ExtendFn)k_extend<B, L, W, true> : (ExtendFn)k_extend<B, L, W, false>;
-    *s = instr ? (ShadowFn)k_shadow<B, L, W, true> : (ShadowFn)k_shadow<B, L, W, false>;
+    *e = instr ? (ExtendFn)k_extend<B, L, W, true, BA> : (ExtendFn)k_extend<B, L, W, false, BA>;
+    *s = instr ? (ShadowFn)k_shadow<B, L, W, true, BA> : (ShadowFn)k_shadow<B, L, W, false, BA>;
+}
+
+template <bool BA>
+void pick_b(const TravCfg& c, ExtendFn* e, ShadowFn* s) {
+    int key = (c.block >= 256 ? 4 : 0) | (c.lds >= 32 ? 2 : 0) | (c.width == 4 ? 1 : 0);
+    switch (key) {
+        case 0: pick_t<128, 16, 2, BA>(c.instr, e, s); break;
+        case 1: pick_t<128, 16, 4, BA>(c.instr, e, s); break;
+        case 2: pick_t<128, 32, 2, BA>(c.instr, e, s); break;
+        case 3: pick_t<128, 32, 4, BA>(c.instr, e, s); break;
+        case 4: pick_t<256, 16, 2, BA>(c.instr, e, s); break;
+        case 5: pick_t<256, 16, 4, BA>(c.instr, e, s); break;
+        case 6: pick_t<256, 32, 2, BA>(c.instr, e, s); break;
+        default: pick_t<256, 32, 4, BA>(c.instr, e, s); break;
+    }
 }
 
 void pick_kernels(const TravCfg& c, ExtendFn* e, ShadowFn* s) {
-    int key = (c.block >= 256 ? 4 : 0) | (c.lds >= 32 ? 2 : 0) | (c.width == 4 ? 1 : 0);
-    switch (key) {
-        case 0: pick_t<128, 16, 2>(c.instr, e, s); break;
-        case 1: pick_t<128, 16, 4>(c.instr, e, s); break;
-        case 2: pick_t<128, 32, 2>(c.instr, e, s); break;
-        case 3: pick_t<128, 32, 4>(c.instr, e, s); break;
-        case 4: pick_t<256, 16, 2>(c.instr, e, s); break;
-        case 5: pick_t<256, 16, 4>(c.instr, e, s); break;
-        case 6: pick_t<256, 32, 2>(c.instr, e, s); break;
-        default: pick_t<256, 32, 4>(c.instr, e, s); break;
-    }
+    if (c.batch)
+        pick_b<true>(c, e, s);
+    else
+        pick_b<false>(c, e, s);
 }
 
 // Blocks per CU that can be resident for kernel `fn` (0 on error).
@@ -871,6 +948,82 @@ struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
 };
 
+float tri_area_host(const float* v) {
+    // Triangle::calcArea (src/Triangle.cpp:26-30): |edge(0) x edge(1)| / 2,
+    // edge(i) = v[(i+1)%3] - v[i]
+    V3 a = sub(V3{v[3], v[4], v[5]}, V3{v[0], v[1], v[2]});
+    V3 b = sub(V3{v[6], v[7], v[8]}, V3{v[3], v[4], v[5]});
+    return length(cross(a, b)) / 2.0f;
+}
+
+// Validates one render description and fills its RenderParams.
+ear_status make_params(Ctx* c, const ear_render_desc* D, RenderParams* P) {
+    if (!D) return EAR_ERR_INVALID_ARGUMENT;
+    if ((D->band_mask & 7u) == 0 || (D->band_mask & ~7u)) {
+        c->last_error = "band_mask must select bands among bits 0..2";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (D->hist_bins == 0 || D->n_recorders == 0 || !D->recorders) {
+        c->last_error = "hist_bins and recorders are required";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (D->n_recorders > (uint32_t)kMaxRecorders) {
+        c->last_error = "too many recorders";
+        return EAR_ERR_CAPACITY;
+    }
+    if (D->source.kind != EAR_SRC_POINT && D->source.kind != EAR_SRC_MESH) return EAR_ERR_INVALID_ARGUMENT;
+    if (D->source.kind == EAR_SRC_MESH && (!D->source.mesh_v || D->source.mesh_n_tris == 0)) {
+        c->last_error = "mesh source without triangles";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if ((uint64_t)D->hist_bins * 3ull * 2ull * kMaxRecorders > (1ull << 36)) {
+        c->last_error = "histogram too long";
+        return EAR_ERR_CAPACITY;
+    }
+    for (int b = 0; b < 3; ++b)
+        if (!std::isfinite(D->air[b])) return EAR_ERR_INVALID_ARGUMENT;
+    std::memset(P, 0, sizeof(*P));
+    P->seed = D->seed;
+    P->instance = D->instance;
+    for (int b = 0; b < 3; ++b) {
+        P->ctx_word[b] = (uint32_t)b | ((D->sound & 0xFFu) << 8) | ((uint32_t)(D->keyframe + 1) << 16);
+        P->air[b] = D->air[b];
+    }
+    P->dry = D->dry;
+    P->cutoff = D->energy_cutoff;
+    P->max_bounces = D->max_bounces;
+    P->n_bins = D->hist_bins;
+    P->n_rec = D->n_recorders;
+    P->n_band_slots = 3;
+    P->band_mask = D->band_mask;
+    P->hist_kinds = (D->flags & EAR_FLAG_ABS_HIST) ? 2 : 1;
+    P->src_kind = D->source.kind;
+    for (int k = 0; k < 3; ++k) P->src_pos[k] = D->source.pos[k];
+    P->flags = D->flags;
+    uint32_t nch = 0;
+    for (uint32_t r = 0; r < D->n_recorders; ++r) {
+        const ear_recorder& R = D->recorders[r];
+        if (R.kind != EAR_REC_MONO && R.kind != EAR_REC_STEREO) return EAR_ERR_INVALID_ARGUMENT;
+        RecParams& Q = P->rec[r];
+        Q.px = R.pos[0];
+        Q.py = R.pos[1];
+        Q.pz = R.pos[2];
+        Q.rx = R.right_ear[0];
+        Q.ry = R.right_ear[1];
+        Q.rz = R.right_ear[2];
+        Q.head_size = R.head_size;
+        for (int b = 0; b < 3; ++b) Q.head_abs[b] = R.head_abs[b];
+        Q.kind = R.kind;
+        Q.channel = (int32_t)nch;
+        nch += R.kind == EAR_REC_STEREO ? 2 : 1;
+    }
+    P->n_channels = nch;
+    bool want_log = (D->flags & EAR_FLAG_TRACE_LOG) && D->trace_rays && D->trace_segs;
+    P->trace_rays = want_log ? D->trace_rays : 0;
+    P->trace_segs = want_log ? D->trace_segs : 0;
+    return EAR_OK;
+}
+
 }  // namespace
 
 ear_status run_query_closest(Ctx* c, DeviceScene* s, uint32_t n, const float* o, const float* d, float* t,
@@ -928,111 +1081,94 @@ ear_status run_query_occluded(Ctx* c, DeviceScene* s, uint32_t n, const float* p
     return EAR_OK;
 }
 
-static float tri_area_host(const float* v) {
-    // Triangle::calcArea (src/Triangle.cpp:26-30): |edge(0) x edge(1)| / 2,
-    // edge(i) = v[(i+1)%3] - v[i]
-    V3 a = sub(V3{v[3], v[4], v[5]}, V3{v[0], v[1], v[2]});
-    V3 b = sub(V3{v[6], v[7], v[8]}, V3{v[3], v[4], v[5]});
-    return length(cross(a, b)) / 2.0f;
-}
-
-ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist* out, ear_stats* st_out) {
-    if (!D) return EAR_ERR_INVALID_ARGUMENT;
-    if ((D->band_mask & 7u) == 0 || (D->band_mask & ~7u)) {
-        c->last_error = "band_mask must select bands among bits 0..2";
-        return EAR_ERR_INVALID_ARGUMENT;
-    }
-    if (D->hist_bins == 0 || D->n_recorders == 0 || !D->recorders) {
-        c->last_error = "hist_bins and recorders are required";
-        return EAR_ERR_INVALID_ARGUMENT;
-    }
-    if (D->n_recorders > (uint32_t)kMaxRecorders) {
-        c->last_error = "too many recorders";
-        return EAR_ERR_CAPACITY;
-    }
-    if (D->source.kind != EAR_SRC_POINT && D->source.kind != EAR_SRC_MESH) return EAR_ERR_INVALID_ARGUMENT;
-    if (D->source.kind == EAR_SRC_MESH && (!D->source.mesh_v || D->source.mesh_n_tris == 0)) {
-        c->last_error = "mesh source without triangles";
-        return EAR_ERR_INVALID_ARGUMENT;
-    }
-    for (int b = 0; b < 3; ++b)
-        if (!std::isfinite(D->air[b])) return EAR_ERR_INVALID_ARGUMENT;
-
-    RenderParams P;
-    std::memset(&P, 0, sizeof(P));
-    P.seed = D->seed;
-    P.instance = D->instance;
-    for (int b = 0; b < 3; ++b) {
-        P.ctx_word[b] = (uint32_t)b | ((D->sound & 0xFFu) << 8) | ((uint32_t)(D->keyframe + 1) << 16);
-        P.air[b] = D->air[b];
-    }
-    P.dry = D->dry;
-    P.cutoff = D->energy_cutoff;
-    P.max_bounces = D->max_bounces;
-    P.n_bins = D->hist_bins;
-    P.n_rec = D->n_recorders;
-    P.n_band_slots = 3;
-    P.hist_kinds = (D->flags & EAR_FLAG_ABS_HIST) ? 2 : 1;
-    P.src_kind = D->source.kind;
-    for (int k = 0; k < 3; ++k) P.src_pos[k] = D->source.pos[k];
-    P.flags = D->flags;
-    uint32_t nch = 0;
-    for (uint32_t r = 0; r < D->n_recorders; ++r) {
-        const ear_recorder& R = D->recorders[r];
-        if (R.kind != EAR_REC_MONO && R.kind != EAR_REC_STEREO) return EAR_ERR_INVALID_ARGUMENT;
-        RecParams& Q = P.rec[r];
-        Q.px = R.pos[0];
-        Q.py = R.pos[1];
-        Q.pz = R.pos[2];
-        Q.rx = R.right_ear[0];
-        Q.ry = R.right_ear[1];
-        Q.rz = R.right_ear[2];
-        Q.head_size = R.head_size;
-        for (int b = 0; b < 3; ++b) Q.head_abs[b] = R.head_abs[b];
-        Q.kind = R.kind;
-        Q.channel = (int32_t)nch;
-        nch += R.kind == EAR_REC_STEREO ? 2 : 1;
-    }
-    P.n_channels = nch;
-    bool want_log = (D->flags & EAR_FLAG_TRACE_LOG) && D->trace_rays && D->trace_segs;
-    P.trace_rays = want_log ? D->trace_rays : 0;
-    P.trace_segs = want_log ? D->trace_segs : 0;
-
-    std::vector<int> bands;
-    for (int b = 0; b < 3; ++b)
-        if (D->band_mask & (1u << b)) bands.push_back(b);
-    uint32_t n_bands = (uint32_t)bands.size();
-
-    // mesh source tables, computed once in host fp32 (same formulas as the reference)
-    std::vector<float> src_area, src_nrm;
-    if (D->source.kind == EAR_SRC_MESH) {
-        uint32_t m = D->source.mesh_n_tris;
-        src_area.resize(m);
-        src_nrm.resize(3 * (size_t)m);
-        float total = 0.0f;
-        for (uint32_t i = 0; i < m; ++i) {
-            const float* v = D->source.mesh_v + 9 * (size_t)i;
-            src_area[i] = tri_area_host(v);
-            total += src_area[i];
-            V3 n = normalized(cross(sub(V3{v[3], v[4], v[5]}, V3{v[0], v[1], v[2]}),
-                                    sub(V3{v[6], v[7], v[8]}, V3{v[0], v[1], v[2]})));
-            src_nrm[3 * i] = n.x;
-            src_nrm[3 * i + 1] = n.y;
-            src_nrm[3 * i + 2] = n.z;
+// A batch of renders in one wavefront: every (instance, band) context's rays
+// share the queues; per-instance scene data, parameters and histograms come
+// from the device instance table.  A plain render is a batch of one.
+ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const ear_render_desc* const* descs,
+                     ear_hist* const* outs, ear_stats* st_out) {
+    if (n_items == 0) return EAR_ERR_INVALID_ARGUMENT;
+    const bool batch = n_items > 1;
+    const int width = (int)scenes[0]->width;
+    std::vector<InstDev> inst(n_items);
+    std::vector<std::vector<float>> src_area(n_items), src_nrm(n_items);
+    uint64_t total_rays = 0;
+    uint32_t max_bounces = 0;
+    bool instr = false, any_log = false;
+    size_t hist_cells = 0, value_cells = 0, direct_cells = 0, log_hits = 0, log_nseg = 0;
+    std::vector<size_t> hist_off(n_items), value_off(n_items), direct_off(n_items), logh_off(n_items),
+        logn_off(n_items);
+    bool any_value = false;
+    size_t max_recs = 0;
+    uint32_t any_mesh = 0;
+    for (uint32_t i = 0; i < n_items; ++i) {
+        if (!scenes[i] || !descs[i]) return EAR_ERR_INVALID_ARGUMENT;
+        if ((int)scenes[i]->width != width) {
+            c->last_error = "all scenes of a batch must have the same BVH width";
+            return EAR_ERR_INVALID_ARGUMENT;
         }
-        P.src_n_tris = m;
-        P.src_total_area = total;
+        InstDev& I = inst[i];
+        std::memset(&I, 0, sizeof(I));
+        TRY(make_params(c, descs[i], &I.P));
+        const ear_render_desc* D = descs[i];
+        const DeviceScene* s = scenes[i];
+        I.nodes = s->nodes;
+        I.nodes4 = s->nodes4;
+        I.tris = s->tris;
+        I.nm = s->nm;
+        I.mat_band = s->mat_band;
+        I.n_tris = s->n_tris;
+        I.inv_n = (double)(1.0f / (float)std::max<uint32_t>(1, D->n_rays));  // Recorder::Multiply(1.0f / amount)
+        I.gain2 = (double)(D->source.gain * D->source.gain);                  // Recorder::Multiply(gain * gain)
+        if (D->source.kind == EAR_SRC_MESH) {
+            any_mesh = 1;
+            uint32_t m = D->source.mesh_n_tris;
+            src_area[i].resize(m);
+            src_nrm[i].resize(3 * (size_t)m);
+            float total = 0.0f;
+            for (uint32_t k = 0; k < m; ++k) {
+                const float* v = D->source.mesh_v + 9 * (size_t)k;
+                src_area[i][k] = tri_area_host(v);
+                total += src_area[i][k];
+                V3 nn = normalized(cross(sub(V3{v[3], v[4], v[5]}, V3{v[0], v[1], v[2]}),
+                                         sub(V3{v[6], v[7], v[8]}, V3{v[0], v[1], v[2]})));
+                src_nrm[i][3 * k] = nn.x;
+                src_nrm[i][3 * k + 1] = nn.y;
+                src_nrm[i][3 * k + 2] = nn.z;
+            }
+            I.P.src_n_tris = m;
+            I.P.src_total_area = total;
+        }
+        int nb = 0;
+        for (int b = 0; b < 3; ++b) nb += (D->band_mask >> b) & 1u;
+        total_rays += (uint64_t)D->n_rays * nb;
+        max_bounces = std::max(max_bounces, D->max_bounces);
+        instr = instr || (D->flags & EAR_FLAG_INSTRUMENT);
+        max_recs = std::max<size_t>(max_recs, D->n_recorders);
+        size_t cells = (size_t)3 * I.P.n_channels * I.P.hist_kinds * I.P.n_bins;
+        hist_off[i] = hist_cells;
+        hist_cells += cells;
+        value_off[i] = value_cells;
+        if (outs[i] && (outs[i]->value || outs[i]->abs_value)) {
+            value_cells += cells;
+            any_value = true;
+        }
+        direct_off[i] = direct_cells;
+        direct_cells += (size_t)3 * I.P.n_channels;
+        logh_off[i] = log_hits;
+        logn_off[i] = log_nseg;
+        if (I.P.trace_rays) {
+            log_hits += (size_t)3 * I.P.trace_rays * I.P.trace_segs;
+            log_nseg += (size_t)3 * I.P.trace_rays;
+            any_log = true;
+        }
     }
 
     const ear_tuning& tu = c->tuning;
     uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 22);
-    chunk = std::min<uint32_t>(chunk, std::max<uint32_t>(1, D->n_rays));
-    uint32_t cap_rays = chunk * n_bands;
-    // traversal kernels and their persistent grids: resident blocks per CU from
-    // the occupancy calculator unless the caller fixed waves per CU
-    const bool instr = (D->flags & EAR_FLAG_INSTRUMENT) != 0;
-    const TravCfg cfg_e{tu.block_extend == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, (int)s->width, instr};
-    const TravCfg cfg_s{tu.block_shadow == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, (int)s->width, instr};
+    uint64_t cap64 = std::min<uint64_t>((uint64_t)chunk * 3, std::max<uint64_t>(1, total_rays));
+    const uint32_t cap_rays = (uint32_t)cap64;
+    const TravCfg cfg_e{tu.block_extend == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch};
+    const TravCfg cfg_s{tu.block_shadow == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch};
     ExtendFn fe, fe_unused;
     ShadowFn fs, fs_unused;
     pick_kernels(cfg_e, &fe, &fs_unused);
@@ -1048,52 +1184,73 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
 
     hipStream_t st = c->stream;
     DevBuf qo[2] = {DevBuf(c), DevBuf(c)}, qd[2] = {DevBuf(c), DevBuf(c)}, qm[2] = {DevBuf(c), DevBuf(c)};
-    DevBuf hits(c), so(c), sm(c), counters(c), stats(c), hist(c), gs(c), dbands(c), direct(c), value(c), logh(c),
-        logn(c), mv(c), ma(c), mn(c);
+    DevBuf hits(c), so(c), sm(c), counters(c), stats(c), hist(c), gs(c), direct(c), value(c), logh(c), logn(c),
+        dinst(c), dsegs(c), dsegoff(c), drows(c), msrc(c);
     for (int k = 0; k < 2; ++k) {
         TRY(qo[k].alloc((size_t)cap_rays * 16));
         TRY(qd[k].alloc((size_t)cap_rays * 16));
         TRY(qm[k].alloc((size_t)cap_rays * 16));
     }
     TRY(hits.alloc((size_t)cap_rays * 8));
-    size_t cap_sh = (size_t)cap_rays * D->n_recorders;
+    size_t cap_sh = (size_t)cap_rays * max_recs;
     TRY(so.alloc(cap_sh * 16));
     TRY(sm.alloc(cap_sh * 16));
     TRY(counters.alloc(4 * 4));
     TRY(stats.alloc(ST_N * 8));
-    size_t cells = (size_t)3 * nch * P.hist_kinds * D->hist_bins;
-    TRY(hist.alloc(cells * 16));
+    TRY(hist.alloc(hist_cells * 16));
     uint32_t gthreads = std::max(cap_e * (uint32_t)cfg_e.block, cap_s * (uint32_t)cfg_s.block);
+    gthreads = std::max<uint32_t>(gthreads, n_items * kBlock);
     TRY(gs.alloc((size_t)(kStackDepth - kLdsMin) * gthreads * 4));
-    TRY(dbands.alloc(3 * 4));
-    TRY(direct.alloc((size_t)3 * nch * sizeof(DirectRamp)));
-    if (want_log) {
-        TRY(logh.alloc((size_t)3 * P.trace_rays * P.trace_segs * 4));
-        TRY(logn.alloc((size_t)3 * P.trace_rays * 4));
-        EAR_HIP_TRY(c, hipMemsetAsync(logh.p, 0xFE, logh.n, st));  // -2 pattern: 0xFEFEFEFE fixed below
+    TRY(direct.alloc(direct_cells * sizeof(DirectRamp)));
+    if (any_value) TRY(value.alloc(value_cells * 8));
+    if (any_log) {
+        TRY(logh.alloc(log_hits * 4));
+        TRY(logn.alloc(log_nseg * 4));
+        EAR_HIP_TRY(c, hipMemsetAsync(logh.p, 0xFE, logh.n, st));  // 0xFEFEFEFE read back as -2
         EAR_HIP_TRY(c, hipMemsetAsync(logn.p, 0, logn.n, st));
     }
-    MeshSrc M{nullptr, nullptr, nullptr};
-    if (D->source.kind == EAR_SRC_MESH) {
-        uint32_t m = D->source.mesh_n_tris;
-        TRY(mv.alloc((size_t)m * 36));
-        TRY(ma.alloc((size_t)m * 4));
-        TRY(mn.alloc((size_t)m * 12));
-        EAR_HIP_TRY(c, hipMemcpyAsync(mv.p, D->source.mesh_v, (size_t)m * 36, hipMemcpyHostToDevice, st));
-        EAR_HIP_TRY(c, hipMemcpyAsync(ma.p, src_area.data(), (size_t)m * 4, hipMemcpyHostToDevice, st));
-        EAR_HIP_TRY(c, hipMemcpyAsync(mn.p, src_nrm.data(), (size_t)m * 12, hipMemcpyHostToDevice, st));
-        M = MeshSrc{mv.as<float>(), ma.as<float>(), mn.as<float>()};
+    // mesh-source tables, one allocation
+    size_t msrc_floats = 0;
+    std::vector<size_t> msrc_off(n_items);
+    for (uint32_t i = 0; i < n_items; ++i) {
+        msrc_off[i] = msrc_floats;
+        msrc_floats += 9 * (size_t)inst[i].P.src_n_tris + src_area[i].size() + src_nrm[i].size();
     }
-    EAR_HIP_TRY(c, hipMemcpyAsync(dbands.p, bands.data(), n_bands * 4, hipMemcpyHostToDevice, st));
+    if (msrc_floats) TRY(msrc.alloc(msrc_floats * 4));
+    for (uint32_t i = 0; i < n_items; ++i) {
+        InstDev& I = inst[i];
+        I.hist = hist.as<unsigned long long>() + 2 * hist_off[i];
+        I.direct = direct.as<DirectRamp>() + direct_off[i];
+        I.value = (outs[i] && (outs[i]->value || outs[i]->abs_value)) ? value.as<double>() + value_off[i] : nullptr;
+        I.log_hits = I.P.trace_rays ? logh.as<int32_t>() + logh_off[i] : nullptr;
+        I.log_nseg = I.P.trace_rays ? logn.as<uint32_t>() + logn_off[i] : nullptr;
+        if (I.P.src_n_tris) {
+            float* base = msrc.as<float>() + msrc_off[i];
+            size_t m = I.P.src_n_tris;
+            EAR_HIP_TRY(c, hipMemcpyAsync(base, descs[i]->source.mesh_v, m * 36, hipMemcpyHostToDevice, st));
+            EAR_HIP_TRY(c, hipMemcpyAsync(base + 9 * m, src_area[i].data(), m * 4, hipMemcpyHostToDevice, st));
+            EAR_HIP_TRY(c, hipMemcpyAsync(base + 10 * m, src_nrm[i].data(), m * 12, hipMemcpyHostToDevice, st));
+            I.M = MeshSrc{base, base + 9 * m, base + 10 * m};
+        }
+    }
+    TRY(dinst.alloc(sizeof(InstDev) * n_items));
+    EAR_HIP_TRY(c, hipMemcpyAsync(dinst.p, inst.data(), sizeof(InstDev) * n_items, hipMemcpyHostToDevice, st));
     EAR_HIP_TRY(c, hipMemsetAsync(hist.p, 0, hist.n, st));
     EAR_HIP_TRY(c, hipMemsetAsync(stats.p, 0, stats.n, st));
     EAR_HIP_TRY(c, hipMemsetAsync(counters.p, 0, counters.n, st));
 
+    // every (instance, band) context as a run of rays, cut into chunks
+    std::vector<GenSeg> all;
+    for (uint32_t i = 0; i < n_items; ++i)
+        for (int b = 0; b < 3; ++b)
+            if ((descs[i]->band_mask >> b) & 1u && descs[i]->n_rays) all.push_back(GenSeg{i, (uint32_t)b, 0, descs[i]->n_rays});
+    TRY(dsegs.alloc(sizeof(GenSeg) * std::max<size_t>(1, all.size())));
+    TRY(dsegoff.alloc(4 * std::max<size_t>(1, all.size())));
+
     uint32_t* cnt = counters.as<uint32_t>();  // [0],[1] ray queues, [2] shadow queue
     unsigned long long* stp = stats.as<unsigned long long>();
-    Traversal T{s->nodes, s->nodes4, s->tris, s->n_tris, gs.as<int>(), gthreads};
-    Hist H{hist.as<unsigned long long>(), D->hist_bins, nch, P.hist_kinds};
-    const int width = (int)s->width;
+    const InstDev* di = dinst.as<InstDev>();
+    int* gsp = gs.as<int>();
 
     uint32_t* pinned = nullptr;
     EAR_HIP_TRY(c, hipHostMalloc((void**)&pinned, 16, hipHostMallocDefault));
@@ -1119,51 +1276,73 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
     uint32_t bounces = 0, launches = 0;
     uint64_t extend_q = 0, shadow_q = 0;
     ear_status status = EAR_OK;
-    for (uint32_t ray_base = 0; ray_base < D->n_rays && status == EAR_OK; ray_base += chunk) {
-        uint32_t n_chunk = std::min(chunk, D->n_rays - ray_base);
-        Queue q0{qo[0].as<float4>(), qd[0].as<float4>(), qm[0].as<uint4>()};
-        Queue q1{qo[1].as<float4>(), qd[1].as<float4>(), qm[1].as<uint4>()};
-        ShadowQ SQ{so.as<float4>(), sm.as<float4>()};
-        uint32_t total = n_chunk * n_bands;
-        uint32_t g = persistent_grid(c, total, waves);
-        k_generate<<<g, kBlock, 0, st>>>(P, M, dbands.as<int>(), n_bands, ray_base, n_chunk, q0, cnt + 0, SQ, cnt + 2,
-                                         stp);
-        ++launches;
-        if (P.src_kind == EAR_SRC_MESH) {
-            EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
-            EAR_HIP_TRY(c, hipStreamSynchronize(st));
-            uint32_t nsh = pinned[2];
-            if (nsh) {
-                uint32_t gsh = grid_for(nsh, cap_s, cfg_s.block);
-                EventPair& e = mk(ev_shw);
-                hipEventRecord(e.a, st);
-                hipLaunchKernelGGL(fs, dim3(gsh), dim3(cfg_s.block), 0, st, P, T, SQ, (const uint32_t*)(cnt + 2), H,
-                                   stp);
-                hipEventRecord(e.b, st);
-                shadow_q += nsh;
-                ++launches;
+    Queue q0{qo[0].as<float4>(), qd[0].as<float4>(), qm[0].as<uint4>()};
+    Queue q1{qo[1].as<float4>(), qd[1].as<float4>(), qm[1].as<uint4>()};
+    ShadowQ SQ{so.as<float4>(), sm.as<float4>()};
+    size_t seg_i = 0;
+    uint32_t seg_pos = 0;
+    while (seg_i < all.size() && status == EAR_OK) {
+        // next chunk of at most cap_rays rays
+        std::vector<GenSeg> cs;
+        std::vector<uint32_t> off;
+        uint32_t total = 0;
+        while (seg_i < all.size() && total < cap_rays) {
+            GenSeg g = all[seg_i];
+            uint32_t take = std::min(g.count - seg_pos, cap_rays - total);
+            cs.push_back(GenSeg{g.inst, g.band, g.ray0 + seg_pos, take});
+            off.push_back(total);
+            total += take;
+            seg_pos += take;
+            if (seg_pos == g.count) {
+                ++seg_i;
+                seg_pos = 0;
             }
+        }
+        EAR_HIP_TRY(c, hipMemcpyAsync(dsegs.p, cs.data(), sizeof(GenSeg) * cs.size(), hipMemcpyHostToDevice, st));
+        EAR_HIP_TRY(c, hipMemcpyAsync(dsegoff.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, st));
+        uint32_t g = persistent_grid(c, total, waves);
+        if (batch)
+            k_generate<true><<<g, kBlock, 0, st>>>(di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), (uint32_t)cs.size(),
+                                                   total, (uint32_t)max_recs, any_mesh, q0, cnt + 0, SQ, cnt + 2, stp);
+        else
+            k_generate<false><<<g, kBlock, 0, st>>>(di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(),
+                                                    (uint32_t)cs.size(), total, (uint32_t)max_recs, any_mesh, q0,
+                                                    cnt + 0, SQ, cnt + 2, stp);
+        ++launches;
+        EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
+        EAR_HIP_TRY(c, hipStreamSynchronize(st));
+        if (pinned[2]) {  // mesh-source connections at k = 0
+            uint32_t nsh = pinned[2];
+            uint32_t gsh = grid_for(nsh, cap_s, cfg_s.block);
+            EventPair& e = mk(ev_shw);
+            hipEventRecord(e.a, st);
+            hipLaunchKernelGGL(fs, dim3(gsh), dim3(cfg_s.block), 0, st, di, gsp, gthreads, SQ,
+                               (const uint32_t*)(cnt + 2), stp);
+            hipEventRecord(e.b, st);
+            shadow_q += nsh;
+            ++launches;
             k_reset2<<<1, 64, 0, st>>>(cnt + 2, cnt + 3);
         }
         int cur = 0;
-        EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 4, hipMemcpyDeviceToHost, st));
-        EAR_HIP_TRY(c, hipStreamSynchronize(st));
         uint32_t n_live = pinned[0];
-        for (uint32_t k = 1; k <= D->max_bounces && n_live > 0; ++k) {
+        for (uint32_t k = 1; k <= max_bounces && n_live > 0; ++k) {
             Queue qi = cur == 0 ? q0 : q1;
             Queue qn = cur == 0 ? q1 : q0;
             uint32_t g1 = grid_for(n_live, cap_e, cfg_e.block);
             EventPair& e1 = mk(ev_ext);
             hipEventRecord(e1.a, st);
-            hipLaunchKernelGGL(fe, dim3(g1), dim3(cfg_e.block), 0, st, T, qi, (const uint32_t*)(cnt + cur),
-                               hits.as<int2>(), stp);
-            uint32_t gsd = persistent_grid(c, n_live, waves);
+            hipLaunchKernelGGL(fe, dim3(g1), dim3(cfg_e.block), 0, st, di, gsp, gthreads, qi,
+                               (const uint32_t*)(cnt + cur), hits.as<int2>(), stp);
             hipEventRecord(e1.b, st);
+            uint32_t gsd = persistent_grid(c, n_live, waves);
             EventPair& e2 = mk(ev_sha);
             hipEventRecord(e2.a, st);
-            k_shade<<<gsd, kBlock, 0, st>>>(P, s->tris, s->nm, s->mat_band, qi, cnt + cur, hits.as<int2>(), qn,
-                                           cnt + (1 - cur), SQ, cnt + 2, stp, logh.as<int32_t>(),
-                                           logn.as<uint32_t>());
+            if (batch)
+                k_shade<true><<<gsd, kBlock, 0, st>>>(di, qi, cnt + cur, hits.as<int2>(), qn, cnt + (1 - cur), SQ,
+                                                      cnt + 2, (uint32_t)max_recs, stp);
+            else
+                k_shade<false><<<gsd, kBlock, 0, st>>>(di, qi, cnt + cur, hits.as<int2>(), qn, cnt + (1 - cur), SQ,
+                                                       cnt + 2, (uint32_t)max_recs, stp);
             hipEventRecord(e2.b, st);
             EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
             EAR_HIP_TRY(c, hipStreamSynchronize(st));
@@ -1174,8 +1353,8 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
                 uint32_t g2 = grid_for(nsh, cap_s, cfg_s.block);
                 EventPair& e3 = mk(ev_shw);
                 hipEventRecord(e3.a, st);
-                hipLaunchKernelGGL(fs, dim3(g2), dim3(cfg_s.block), 0, st, P, T, SQ, (const uint32_t*)(cnt + 2), H,
-                                   stp);
+                hipLaunchKernelGGL(fs, dim3(g2), dim3(cfg_s.block), 0, st, di, gsp, gthreads, SQ,
+                                   (const uint32_t*)(cnt + 2), stp);
                 hipEventRecord(e3.b, st);
                 shadow_q += nsh;
                 ++launches;
@@ -1186,34 +1365,37 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
             cur = 1 - cur;
             ++bounces;
         }
-        // make sure both ray counters are zero for the next chunk
         k_reset2<<<1, 64, 0, st>>>(cnt + 0, cnt + 1);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) status = hip_status(c, le, "tracer launch");
     }
     if (status != EAR_OK) return status;
     if (width == 4)
-        k_direct<4><<<1, kBlock, 0, st>>>(P, T, dbands.as<int>(), n_bands, direct.as<DirectRamp>());
+        k_direct<4><<<n_items, kBlock, 0, st>>>(di, gsp, gthreads);
     else
-        k_direct<2><<<1, kBlock, 0, st>>>(P, T, dbands.as<int>(), n_bands, direct.as<DirectRamp>());
-    bool want_value = out && (out->value || out->abs_value);
-    if (want_value) TRY(value.alloc(cells * 8));
-    float finv = 1.0f / (float)std::max<uint32_t>(1, D->n_rays);   // Recorder::Multiply(1.0f / amount)
-    float g2 = D->source.gain * D->source.gain;                      // Recorder::Multiply(gain * gain)
-    k_finalize<<<n_bands * nch * P.hist_kinds, kFinBlock, 0, st>>>(
-        hist.as<unsigned long long>(), D->hist_bins, nch, P.hist_kinds, dbands.as<int>(), direct.as<DirectRamp>(),
-        (double)finv, (double)g2, want_value ? value.as<double>() : nullptr);
+        k_direct<2><<<n_items, kBlock, 0, st>>>(di, gsp, gthreads);
+    std::vector<uint4> rows;
+    for (uint32_t i = 0; i < n_items; ++i)
+        for (uint32_t b = 0; b < 3; ++b)
+            if ((descs[i]->band_mask >> b) & 1u)
+                for (uint32_t ch = 0; ch < inst[i].P.n_channels; ++ch)
+                    for (uint32_t kind = 0; kind < inst[i].P.hist_kinds; ++kind) rows.push_back(make_uint4(i, b, ch, kind));
+    TRY(drows.alloc(sizeof(uint4) * rows.size()));
+    EAR_HIP_TRY(c, hipMemcpyAsync(drows.p, rows.data(), sizeof(uint4) * rows.size(), hipMemcpyHostToDevice, st));
+    k_finalize<<<(uint32_t)rows.size(), kFinBlock, 0, st>>>(di, drows.as<uint4>());
     EAR_HIP_TRY(c, hipGetLastError());
     EAR_HIP_TRY(c, hipEventRecord(t1, st));
     EAR_HIP_TRY(c, hipStreamSynchronize(st));
 
-    // copy out [band slot][channel][bin] rows for the rendered bands
-    if (out) {
-        size_t nb = D->hist_bins;
-        std::vector<unsigned long long> rowbuf;
-        std::vector<double> vbuf;
-        for (uint32_t bi = 0; bi < n_bands; ++bi) {
-            int b = bands[bi];
+    // copy out [band slot][channel][bin] rows of every rendered band
+    for (uint32_t i = 0; i < n_items; ++i) {
+        ear_hist* out = outs[i];
+        if (!out) continue;
+        const RenderParams& P = inst[i].P;
+        const size_t nb = P.n_bins;
+        const uint32_t nch = P.n_channels;
+        for (int b = 0; b < 3; ++b) {
+            if (!((descs[i]->band_mask >> b) & 1u)) continue;
             for (uint32_t ch = 0; ch < nch; ++ch) {
                 for (uint32_t kind = 0; kind < P.hist_kinds; ++kind) {
                     size_t roff = (((size_t)b * nch + ch) * P.hist_kinds + kind) * nb;
@@ -1221,22 +1403,24 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
                     int64_t* raw = kind == 0 ? out->raw : out->abs_raw;
                     double* val = kind == 0 ? out->value : out->abs_value;
                     if (raw)
-                        EAR_HIP_TRY(c, hipMemcpyAsync(raw + 2 * ooff, hist.as<unsigned long long>() + 2 * roff,
+                        EAR_HIP_TRY(c, hipMemcpyAsync(raw + 2 * ooff, hist.as<unsigned long long>() + 2 * (hist_off[i] + roff),
                                                       nb * 16, hipMemcpyDeviceToHost, st));
-                    if (val && want_value)
-                        EAR_HIP_TRY(c, hipMemcpyAsync(val + ooff, value.as<double>() + roff, nb * 8,
+                    if (val && inst[i].value)
+                        EAR_HIP_TRY(c, hipMemcpyAsync(val + ooff, value.as<double>() + value_off[i] + roff, nb * 8,
                                                       hipMemcpyDeviceToHost, st));
                 }
             }
         }
-        if (want_log) {
+        if (P.trace_rays) {
             std::vector<int32_t> lh((size_t)3 * P.trace_rays * P.trace_segs);
             std::vector<uint32_t> ln((size_t)3 * P.trace_rays);
-            EAR_HIP_TRY(c, hipMemcpyAsync(lh.data(), logh.p, lh.size() * 4, hipMemcpyDeviceToHost, st));
-            EAR_HIP_TRY(c, hipMemcpyAsync(ln.data(), logn.p, ln.size() * 4, hipMemcpyDeviceToHost, st));
+            EAR_HIP_TRY(c, hipMemcpyAsync(lh.data(), logh.as<int32_t>() + logh_off[i], lh.size() * 4,
+                                          hipMemcpyDeviceToHost, st));
+            EAR_HIP_TRY(c, hipMemcpyAsync(ln.data(), logn.as<uint32_t>() + logn_off[i], ln.size() * 4,
+                                          hipMemcpyDeviceToHost, st));
             EAR_HIP_TRY(c, hipStreamSynchronize(st));
-            for (uint32_t bi = 0; bi < n_bands; ++bi) {
-                int b = bands[bi];
+            for (int b = 0; b < 3; ++b) {
+                if (!((descs[i]->band_mask >> b) & 1u)) continue;
                 size_t per = (size_t)P.trace_rays * P.trace_segs;
                 if (out->trace_hits) {
                     for (size_t j = 0; j < per; ++j) {
@@ -1249,8 +1433,8 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
                                 (size_t)P.trace_rays * 4);
             }
         }
-        EAR_HIP_TRY(c, hipStreamSynchronize(st));
     }
+    EAR_HIP_TRY(c, hipStreamSynchronize(st));
 
     if (st_out) {
         unsigned long long hs[ST_N];
@@ -1294,6 +1478,13 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* D, ear_hist
     hipEventDestroy(t0);
     hipEventDestroy(t1);
     return EAR_OK;
+}
+
+ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* d, ear_hist* out, ear_stats* st) {
+    DeviceScene* sc[1] = {s};
+    const ear_render_desc* dd[1] = {d};
+    ear_hist* oo[1] = {out};
+    return run_batch(c, 1, sc, dd, oo, st);
 }
 
 }  // namespace ear

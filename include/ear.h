@@ -210,6 +210,18 @@ typedef struct {
 ear_status ear_render(ear_ctx* ctx, ear_scene* scene, const ear_render_desc* desc, ear_hist* out,
                       ear_stats* stats);
 
+/* A batch of renders in one persistent wavefront (the reference runs its
+ * SceneContexts on a thread pool, src/EAR.cpp:193-207): every item is a
+ * (scene, description, output) triple; the rays of all items share the
+ * device queues.  Each item's histograms are bitwise identical to rendering it
+ * alone.  All scenes must have the same BVH width. */
+typedef struct {
+    ear_scene* scene;
+    const ear_render_desc* desc;
+    ear_hist* out;
+} ear_batch_item;
+ear_status ear_render_batch(ear_ctx* ctx, const ear_batch_item* items, uint32_t n_items, ear_stats* stats);
+
 /* Launch configuration of the tracer (block sizes, ray chunk, stack split);
  * values of 0 keep the library default.  Results do not depend on them
  * (integer histogram sums are order-independent). */
