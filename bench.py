@@ -12,9 +12,11 @@ emits, traces and splats every ray and finalizes the histograms.
 
 Prints one JSON line (rank 0, N=1).  `roofline` is for the closest-hit kernel
 (k_extend), priced at algorithmic bytes per query
-    24 (ray origin + direction) + 8 (hit record) + 64 * V + 36 * T
+    24 (ray origin + direction) + 8 (hit record) + B * V + 36 * T
 with V / T the mean BVH node visits / triangle tests per query counted by an
-instrumented pass over the same rays.  `cpu_baseline` is the CPU oracle (a
+instrumented pass over the same rays, and B the bytes of one node that a visit
+reads (112 for the 4-wide BVH: four child boxes and four child references; 56
+for the binary one: two child boxes and two child references).  `cpu_baseline` is the CPU oracle (a
 brute-force restatement of the reference algorithm) timed on this host on the
 first K rays of the same Philox streams.
 """
@@ -130,7 +132,7 @@ def main():
         st = ri.stats
         V = st["node_visits_closest"] / max(1, st["extend_queries"])
         T = st["tri_tests_closest"] / max(1, st["extend_queries"])
-        bpq = 24 + 8 + 64 * V + 36 * T
+        bpq = 24 + 8 + info["node_bytes"] * V + 36 * T
         achieved = bpq * ext_q / (ext_ms * 1e-3) / 1e9
         traffic = None
         if os.path.exists(a.traffic_file):
@@ -138,7 +140,8 @@ def main():
                 traffic = json.load(f).get("bytes_per_launch")
         roof = {"bound": "hbm", "kernel": "k_extend", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_query": round(bpq, 1), "node_visits_per_query": round(V, 3),
+                "bytes_per_query": round(bpq, 1), "node_bytes": info["node_bytes"],
+                "node_visits_per_query": round(V, 3),
                 "tri_tests_per_query": round(T, 3),
                 "shadow_node_visits_per_query": round(st["node_visits_shadow"] / max(1, st["shadow_queries"]), 3),
                 "shadow_tri_tests_per_query": round(st["tri_tests_shadow"] / max(1, st["shadow_queries"]), 3),

@@ -20,7 +20,7 @@ STATUS_NAMES = {
 EAR_MAT_HAS_REFR, EAR_MAT_HAS_SPEC = 1, 2
 EAR_REC_MONO, EAR_REC_STEREO = 0, 1
 EAR_SRC_POINT, EAR_SRC_MESH = 0, 1
-EAR_FLAG_ABS_HIST, EAR_FLAG_INSTRUMENT, EAR_FLAG_TRACE_LOG = 1, 2, 4
+EAR_FLAG_ABS_HIST, EAR_FLAG_INSTRUMENT, EAR_FLAG_TRACE_LOG, EAR_FLAG_ADJ_LOCAL = 1, 2, 4, 8
 EAR_FRAC_BITS = 88
 
 # every symbol declared in include/ear.h
@@ -28,6 +28,7 @@ EXPORTS = [
     "ear_ctx_create", "ear_ctx_destroy", "ear_ctx_set_memory_cap", "ear_status_string", "ear_ctx_last_error",
     "ear_abi_version", "ear_scene_upload", "ear_scene_destroy", "ear_scene_get_info", "ear_query_closest",
     "ear_query_occluded", "ear_render", "ear_render_batch", "ear_ctx_set_tuning", "ear_ctx_get_tuning",
+    "ear_render_backward",
 ]
 
 
@@ -118,6 +119,7 @@ def load(path: str = LIB_PATH):
     L.ear_query_occluded.argtypes = [vp, C.c_uint32, fp, fp, fp]
     L.ear_render.argtypes = [vp, vp, C.POINTER(ear_render_desc), C.POINTER(ear_hist), C.POINTER(ear_stats)]
     L.ear_render_batch.argtypes = [vp, C.POINTER(ear_batch_item), C.c_uint32, C.POINTER(ear_stats)]
+    L.ear_render_backward.argtypes = [vp, vp, C.POINTER(ear_render_desc), vp, vp, vp, vp]
     L.ear_ctx_set_tuning.argtypes = [vp, C.POINTER(ear_tuning)]
     L.ear_ctx_get_tuning.argtypes = [vp, C.POINTER(ear_tuning)]
     for name in EXPORTS:

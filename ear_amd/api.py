@@ -92,6 +92,7 @@ class Scene:
         lib = ctx.lib
         t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 3, 3)
         self.n_tris = len(t)
+        self.n_mat = len(materials)
         v0 = np.ascontiguousarray(t[:, 0, :])
         v1 = np.ascontiguousarray(t[:, 1, :])
         v2 = np.ascontiguousarray(t[:, 2, :])
@@ -276,3 +277,24 @@ def render_batch(ctx: Context, items) -> tuple[list, dict, float]:
     wall = time.perf_counter() - t0
     stats = {k: getattr(st, k) for k, _ in F.ear_stats._fields_}
     return [p.result({}, wall) for p in preps], stats, wall
+
+
+def render_backward(ctx: Context, scene: Scene, *, adj_value=None, adj_abs=None, local=False, **kw):
+    """Gradient of sum(adj_value * value) + sum(adj_abs * abs_value) of the
+    render `kw` describes with respect to each material's reflection
+    coefficient and specularity (ear_render_backward).  The adjoints have the
+    shape of RenderResult.value, (3, n_channels, n_bins).  Returns
+    (d/drefl, d/dspec), each (n_mat, 3); bands outside `bands` are zero.
+    local: EAR_FLAG_ADJ_LOCAL (direction tangents end at their vertex)."""
+    p = _Prepared(want_raw=False, want_value=False, **kw)
+    if local:
+        p.desc.flags |= F.EAR_FLAG_ADJ_LOCAL
+    nch = n_channels(kw["recorders"])
+    shape = (3, nch, p.desc.hist_bins)
+    a_v = None if adj_value is None else np.ascontiguousarray(np.broadcast_to(adj_value, shape), np.float64)
+    a_a = None if adj_abs is None else np.ascontiguousarray(np.broadcast_to(adj_abs, shape), np.float64)
+    g_r = np.zeros((scene.n_mat, 3), np.float64)
+    g_s = np.zeros((scene.n_mat, 3), np.float64)
+    _check(ctx.lib, ctx.lib.ear_render_backward(ctx.h, scene.h, C.byref(p.desc), _ptr(a_v), _ptr(a_a), _ptr(g_r),
+                                                _ptr(g_s)), ctx.h)
+    return g_r, g_s

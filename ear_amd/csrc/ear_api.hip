@@ -221,4 +221,23 @@ ear_status ear_render_batch(ear_ctx* c, const ear_batch_item* items, uint32_t n,
     }
 }
 
+ear_status ear_render_backward(ear_ctx* c, ear_scene* s, const ear_render_desc* d, const double* adj_value,
+                               const double* adj_abs, double* grad_refl, double* grad_spec) {
+    if (!c || !s || !d) return EAR_ERR_INVALID_ARGUMENT;
+    if (s->ctx != c) {
+        c->last_error = "scene belongs to another context";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    try {
+        return run_backward(c, &s->ds, d, adj_value, adj_abs, grad_refl, grad_spec);
+    } catch (const std::bad_alloc&) {
+        c->last_error = "host allocation failed";
+        return EAR_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        c->last_error = "internal exception";
+        return EAR_ERR_INTERNAL;
+    }
+}
+
 }  // extern "C"

@@ -129,6 +129,8 @@ void free_scene(Ctx* c, DeviceScene* s);
 
 // tracer.hip
 ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* d, ear_hist* out, ear_stats* st);
+ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* d, const double* adj_value,
+                        const double* adj_abs, double* grad_refl, double* grad_spec);
 ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const ear_render_desc* const* descs,
                      ear_hist* const* outs, ear_stats* st);
 ear_status run_query_closest(Ctx* c, DeviceScene* s, uint32_t n, const float* o, const float* d, float* t,

@@ -1488,3 +1488,479 @@ ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* d, ear_hist
 }
 
 }  // namespace ear
+
+// ===========================================================================
+// Adjoint: gradient of a loss on the histograms with respect to the rendered
+// bands' per-material reflection coefficient and specularity.
+// ===========================================================================
+namespace ear {
+namespace {
+
+constexpr int kAdMaxMat = 8;
+
+// Tangent state of the rays of a queue, SoA [q][field][slot] with fields
+// {o.xyz, d.xyz, dL, dlogI} per specularity parameter q, then
+// dlogI_refl[q][slot].
+struct Tangents {
+    float* t;  // [n_mat][8][cap]
+    float* r;  // [n_mat][cap]
+    uint32_t cap;
+    __device__ float& f(uint32_t q, int field, uint32_t slot) const {
+        return t[((size_t)q * 8 + field) * cap + slot];
+    }
+    __device__ float& refl(uint32_t q, uint32_t slot) const { return r[(size_t)q * cap + slot]; }
+};
+
+struct AdjArgs {
+    const double* adj_s;  // [3][n_ch][bins] dL / d raw (signed), or null
+    const double* adj_a;  // [3][n_ch][bins] dL / d raw (|.|), or null
+    double* grad;         // [3][2][n_mat]: band, {refl, spec}, material
+    uint32_t n_mat;
+};
+
+__device__ double ramp_adjoint(const double* adj, uint32_t nb, int s, int w, double per_c) {
+    double acc = 0.0;
+    for (int i = 0; i < w; ++i) {
+        int b = s + i;
+        if (b < 0 || (uint32_t)b >= nb) continue;
+        acc += adj[b] * ((double)(w - i) / (double)w) * per_c;
+    }
+    return acc;
+}
+
+// dL/dc of a contribution c at time t, distance dist, direction dir.
+__device__ double contribution_adjoint(const RenderParams& P, const AdjArgs& A, int band, const RecParams& R,
+                                       V3 dir, float c, float t, float dist) {
+    const uint32_t nb = P.n_bins, nch = P.n_channels;
+    float width = sqrtf(dist);
+    int w = (int)ceilf(width);
+    double per_c = 2.0 / (double)width;
+    double sgn = c < 0.0f ? -1.0 : 1.0;
+    double a = 0.0;
+    auto row = [&](const double* base, int ch) { return base + ((size_t)band * nch + ch) * nb; };
+    if (R.kind == EAR_REC_MONO) {
+        int s = (int)((double)t * 44100.0);
+        if (A.adj_s) a += ramp_adjoint(row(A.adj_s, R.channel), nb, s, w, per_c);
+        if (A.adj_a) a += sgn * ramp_adjoint(row(A.adj_a, R.channel), nb, s, w, per_c);
+    } else {
+        float dt = dot(dir, V3{R.rx, R.ry, R.rz});
+        float td = R.head_size / 343.0f;
+        int s_right = (int)((double)(t - dt * td) * 44100.0);
+        int s_left = (int)((double)(t + dt * td) * 44100.0);
+        float factor = powf_ear(R.head_abs[band], fabsf(dt) * R.head_size);
+        double gl = 1.0, gr = 1.0;
+        if (dt < 0.0f)
+            gr = (double)(factor * factor);
+        else
+            gl = (double)(factor * factor);
+        if (A.adj_s) {
+            a += gl * ramp_adjoint(row(A.adj_s, R.channel), nb, s_left, w, per_c);
+            a += gr * ramp_adjoint(row(A.adj_s, R.channel + 1), nb, s_right, w, per_c);
+        }
+        if (A.adj_a) {
+            a += sgn * gl * ramp_adjoint(row(A.adj_a, R.channel), nb, s_left, w, per_c);
+            a += sgn * gr * ramp_adjoint(row(A.adj_a, R.channel + 1), nb, s_right, w, per_c);
+        }
+    }
+    return a;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    return v;
+}
+
+// The primal of k_shade (same draws, same arithmetic) plus forward-mode
+// tangents of every specularity parameter through the reparameterised
+// direction v = normalize(h (1 - s) + r s) with h and the hit triangles held
+// fixed, and of log I with respect to every reflection coefficient.  Each
+// visible connection adds (dL/dc)(dc/dtheta) to the gradient.  Arrival-time
+// shifts and visibility changes are not differentiated.
+template <int WIDTH>
+__global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
+                                                      Queue qin, const uint32_t* __restrict__ n_in,
+                                                      const int2* __restrict__ hits, Tangents tin, Queue qout,
+                                                      uint32_t* __restrict__ n_out, Tangents tout, AdjArgs A,
+                                                      unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kLdsMin * kBlock];
+    const InstDev& In = inst[0];
+    const RenderParams& P = In.P;
+    const uint32_t nm = A.n_mat;
+    uint32_t n = *n_in;
+    uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    double g_refl[kAdMaxMat], g_spec[kAdMaxMat];
+    for (int q = 0; q < kAdMaxMat; ++q) g_refl[q] = g_spec[q] = 0.0;
+    int band_acc = -1;
+    uint32_t nvis = 0, ntri = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool cont = false;
+        float4 no{}, nd{};
+        uint4 nmeta{};
+        float dtan[kAdMaxMat][8];
+        float drefl[kAdMaxMat];
+        if (i < n) {
+            float4 ro = qin.o[i];
+            float4 rd = qin.d[i];
+            uint4 rm = qin.m[i];
+            int band = (int)(rm.z & 0xFFu);
+            band_acc = band;
+            uint32_t k = ((rm.z >> 8) & 0xFFFFu) + 1;
+            int2 h = hits[i];
+            if (h.y >= 0) {
+                float t = __int_as_float(h.x);
+                V3 o{ro.x, ro.y, ro.z};
+                V3 d{rd.x, rd.y, rd.z};
+                V3 p{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+                float4 nmv = In.nm[h.y];
+                V3 N{nmv.x, nmv.y, nmv.z};
+                V3 nrm = dot(N, d) > 0.0f ? V3{N.x * -1.0f, N.y * -1.0f, N.z * -1.0f} : N;
+                uint32_t mat = __float_as_uint(nmv.w);
+                float4 mb = In.mat_band[3 * mat + band];
+                Rng rng;
+                rng.init(P.seed, rm.x, P.ctx_word[band], P.instance, rm.y);
+                int bt = kReflect;
+                if (mb.x >= 0.0f) bt = (rng.next() <= mb.x) ? kReflect : kRefract;
+                float spec = mb.y;
+                if (bt == kRefract) nrm = neg(nrm);
+                V3 dist3 = sub(p, o);
+                float l = length(dist3);
+                float I = ro.w;
+                I *= powf_ear(P.air[band], l);
+                float L = rd.w + l;
+                I *= mb.z;
+                V3 prev = normalized(d);
+                const double ln_air = log((double)P.air[band]);
+                // tangents at the hit point (plane of the hit triangle)
+                const double Nd = (double)N.x * d.x + (double)N.y * d.y + (double)N.z * d.z;
+                const double dn = (double)length(d);
+                double dprev[kAdMaxMat][3];
+                for (uint32_t q = 0; q < nm; ++q) {
+                    double to[3] = {tin.f(q, 0, i), tin.f(q, 1, i), tin.f(q, 2, i)};
+                    double td[3] = {tin.f(q, 3, i), tin.f(q, 4, i), tin.f(q, 5, i)};
+                    double dL = tin.f(q, 6, i), dlogI = tin.f(q, 7, i);
+                    double Ndo = N.x * to[0] + N.y * to[1] + N.z * to[2];
+                    double Ndd = N.x * td[0] + N.y * td[1] + N.z * td[2];
+                    double dtt = -(Ndo + (double)t * Ndd) / Nd;
+                    double dp[3] = {to[0] + td[0] * t + (double)d.x * dtt, to[1] + td[1] * t + (double)d.y * dtt,
+                                    to[2] + td[2] * t + (double)d.z * dtt};
+                    double dl = ((double)dist3.x * (dp[0] - to[0]) + (double)dist3.y * (dp[1] - to[1]) +
+                                 (double)dist3.z * (dp[2] - to[2])) /
+                                (double)l;
+                    dL += dl;
+                    dlogI += ln_air * dl;
+                    double pd = ((double)d.x * td[0] + (double)d.y * td[1] + (double)d.z * td[2]) / dn;
+                    dprev[q][0] = (td[0] - (double)d.x / dn * pd) / dn;
+                    dprev[q][1] = (td[1] - (double)d.y / dn * pd) / dn;
+                    dprev[q][2] = (td[2] - (double)d.z / dn * pd) / dn;
+                    dtan[q][0] = (float)dp[0];
+                    dtan[q][1] = (float)dp[1];
+                    dtan[q][2] = (float)dp[2];
+                    dtan[q][3] = (float)td[0];  // incoming direction tangent, replaced below
+                    dtan[q][4] = (float)td[1];
+                    dtan[q][5] = (float)td[2];
+                    dtan[q][6] = (float)dL;
+                    dtan[q][7] = (float)dlogI;
+                    drefl[q] = tin.refl(q, i) + ((uint32_t)q == mat ? 1.0f / mb.z : 0.0f);
+                }
+                if (is_normal(I)) {
+                    for (uint32_t r = 0; r < P.n_rec; ++r) {
+                        const RecParams& R = P.rec[r];
+                        V3 dv = sub(V3{R.px, R.py, R.pz}, p);
+                        V3 lsdir = normalized(dv);
+                        float dtd = dot(lsdir, nrm);
+                        if (!(dtd > 0.0f)) continue;
+                        V3 rv{0, 0, 0};
+                        float diff, sf;
+                        if (bt == kReflect) {
+                            rv = reflect(prev, nrm);
+                            diff = -dot(nrm, prev);
+                            sf = fmaxf(0.0f, dot(rv, lsdir));
+                        } else {
+                            diff = dot(nrm, prev);
+                            sf = fmaxf(0.0f, dot(prev, lsdir));
+                        }
+                        float pw = powf_ear(sf, kExp);
+                        float factor = spec * kExpInt * pw + (1.0f - spec) * diff;
+                        float c = I;
+                        c *= factor;
+                        float ls = length(dv);
+                        float geo_air = powf_ear(P.air[band], ls);
+                        float geo_inv = 2.0f / (4.0f * kPi * ls * ls);
+                        c *= geo_air;
+                        c *= geo_inv;
+                        if (!is_normal(c)) continue;
+                        if (k & 1u) c *= -1.0f;
+                        float bt_ = 0.0f;
+                        int bs_ = -1;
+                        uint32_t bo_ = 0;
+                        Traversal T = make_trav(In, gstack, gthreads);
+                        if (traverse<kLdsMin, kBlock, WIDTH, true, false>(T, p, dv, bt_, bs_, bo_, lds + threadIdx.x,
+                                                                          gtid, nvis, ntri))
+                            continue;
+                        double a_c = contribution_adjoint(P, A, band, R, lsdir, c, (L + ls) / 343.0f, L + ls);
+                        if (a_c == 0.0) continue;
+                        const double cnof = ((k & 1u) ? -1.0 : 1.0) * (double)I * (double)geo_air * (double)geo_inv;
+                        const double lobe = (double)kExpInt * (double)pw;
+                        for (uint32_t q = 0; q < nm; ++q) {
+                            double dvv[3] = {-(double)dtan[q][0], -(double)dtan[q][1], -(double)dtan[q][2]};
+                            double dls = (dvv[0] * dv.x + dvv[1] * dv.y + dvv[2] * dv.z) / (double)ls;
+                            double ld = lsdir.x * dvv[0] + lsdir.y * dvv[1] + lsdir.z * dvv[2];
+                            double dld[3] = {(dvv[0] - lsdir.x * ld) / ls, (dvv[1] - lsdir.y * ld) / ls,
+                                             (dvv[2] - lsdir.z * ld) / ls};
+                            double pn = dprev[q][0] * nrm.x + dprev[q][1] * nrm.y + dprev[q][2] * nrm.z;
+                            double ddiff, dsf = 0.0;
+                            if (bt == kReflect) {
+                                ddiff = -pn;
+                                if (sf > 0.0f) {
+                                    double drv[3] = {dprev[q][0] - 2.0 * pn * nrm.x, dprev[q][1] - 2.0 * pn * nrm.y,
+                                                     dprev[q][2] - 2.0 * pn * nrm.z};
+                                    dsf = drv[0] * lsdir.x + drv[1] * lsdir.y + drv[2] * lsdir.z + dld[0] * rv.x +
+                                          dld[1] * rv.y + dld[2] * rv.z;
+                                }
+                            } else {
+                                ddiff = pn;
+                                if (sf > 0.0f)
+                                    dsf = dprev[q][0] * lsdir.x + dprev[q][1] * lsdir.y + dprev[q][2] * lsdir.z +
+                                          dld[0] * prev.x + dld[1] * prev.y + dld[2] * prev.z;
+                            }
+                            double dlobe = sf > 0.0f ? (double)kExpInt * 1000.0 * pow((double)sf, 999.0) * dsf : 0.0;
+                            double dfactor = (double)spec * dlobe + (1.0 - (double)spec) * ddiff;
+                            if (q == mat) dfactor += lobe - (double)diff;
+                            double dgeo = ln_air * dls - 2.0 * dls / (double)ls;
+                            double dc = cnof * dfactor + (double)c * ((double)dtan[q][7] + dgeo);
+                            g_spec[q] += a_c * dc;
+                            g_refl[q] += a_c * (double)c * (double)drefl[q];
+                        }
+                    }
+                    cont = !((double)I < P.cutoff) && k < P.max_bounces;
+                }
+                if (cont) {
+                    // hemisphere sample and blend exactly as sample_hemi_spec
+                    V3 hs = sample_hemi(rng, nrm);
+                    V3 rdir = bt == kRefract ? d : reflect(d, nrm);
+                    float gg = 1.0f - spec;
+                    V3 w{hs.x * gg + rdir.x * spec, hs.y * gg + rdir.y * spec, hs.z * gg + rdir.z * spec};
+                    float wl = length(w);
+                    V3 v = normalized(w);
+                    const bool local = (P.flags & EAR_FLAG_ADJ_LOCAL) != 0;
+                    for (uint32_t q = 0; q < nm; ++q) {
+                        if (local) {
+                            for (int f = 0; f < 8; ++f) dtan[q][f] = 0.0f;
+                            continue;
+                        }
+                        double td[3] = {dtan[q][3], dtan[q][4], dtan[q][5]};
+                        double dr[3];
+                        if (bt == kRefract) {
+                            dr[0] = td[0];
+                            dr[1] = td[1];
+                            dr[2] = td[2];
+                        } else {
+                            double tn = td[0] * nrm.x + td[1] * nrm.y + td[2] * nrm.z;
+                            dr[0] = td[0] - 2.0 * tn * nrm.x;
+                            dr[1] = td[1] - 2.0 * tn * nrm.y;
+                            dr[2] = td[2] - 2.0 * tn * nrm.z;
+                        }
+                        double dw[3] = {dr[0] * spec, dr[1] * spec, dr[2] * spec};
+                        if (q == mat) {
+                            dw[0] += (double)rdir.x - (double)hs.x;
+                            dw[1] += (double)rdir.y - (double)hs.y;
+                            dw[2] += (double)rdir.z - (double)hs.z;
+                        }
+                        double vdw = v.x * dw[0] + v.y * dw[1] + v.z * dw[2];
+                        dtan[q][3] = (float)((dw[0] - v.x * vdw) / wl);
+                        dtan[q][4] = (float)((dw[1] - v.y * vdw) / wl);
+                        dtan[q][5] = (float)((dw[2] - v.z * vdw) / wl);
+                    }
+                    no = make_float4(p.x, p.y, p.z, I);
+                    nd = make_float4(v.x, v.y, v.z, L);
+                    nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), 0u);
+                }
+            }
+        }
+        uint32_t slot = wave_append(n_out, cont);
+        if (cont) {
+            qout.o[slot] = no;
+            qout.d[slot] = nd;
+            qout.m[slot] = nmeta;
+            for (uint32_t q = 0; q < nm; ++q) {
+                for (int f = 0; f < 8; ++f) tout.f(q, f, slot) = dtan[q][f];
+                tout.refl(q, slot) = drefl[q];
+            }
+        }
+        wave_add_u64(stats + ST_SEGMENTS, i < n ? 1 : 0);
+    }
+    // one band per render call of the adjoint (enforced on the host)
+    int band = __shfl(band_acc, 0);
+    for (int l = 1; l < 64 && band < 0; ++l) band = __shfl(band_acc, l);
+    for (uint32_t q = 0; q < nm; ++q) {
+        double sr = wave_sum_d(g_refl[q]);
+        double ss = wave_sum_d(g_spec[q]);
+        if (lane_id() == 0 && band >= 0) {
+            if (sr != 0.0) atomicAdd(A.grad + ((size_t)band * 2 + 0) * nm + q, sr);
+            if (ss != 0.0) atomicAdd(A.grad + ((size_t)band * 2 + 1) * nm + q, ss);
+        }
+    }
+}
+
+__global__ void k_zero_f32(float* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+}  // namespace
+
+// Gradient of sum(adj_value * value) + sum(adj_abs * abs_value) with respect
+// to refl and spec of every material, for each band of the mask.  The bands
+// are replayed one after another: the tangent arrays and the per-wave
+// gradient partials belong to one band at a time.
+ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const double* adj_value,
+                        const double* adj_abs, double* grad_refl, double* grad_spec) {
+    RenderParams P;
+    TRY(make_params(c, D, &P));
+    if (D->source.kind != EAR_SRC_POINT) {
+        c->last_error = "the adjoint supports point sources";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (s->n_mat > (uint32_t)kAdMaxMat || s->n_mat == 0) {
+        c->last_error = "the adjoint supports 1..8 materials";
+        return EAR_ERR_CAPACITY;
+    }
+    if (!adj_value && !adj_abs) {
+        c->last_error = "no adjoint given";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    InstDev I;
+    std::memset(&I, 0, sizeof(I));
+    I.P = P;
+    I.nodes = s->nodes;
+    I.nodes4 = s->nodes4;
+    I.tris = s->tris;
+    I.nm = s->nm;
+    I.mat_band = s->mat_band;
+    I.n_tris = s->n_tris;
+    const uint32_t nm = s->n_mat;
+    const uint32_t nch = P.n_channels, nb = P.n_bins;
+    // value = raw * (1/N) * gain^2 (k_finalize), so dL/draw = dL/dvalue * (1/N) * gain^2
+    const double scale = (double)(1.0f / (float)std::max<uint32_t>(1, D->n_rays)) *
+                         (double)(D->source.gain * D->source.gain);
+    const size_t hn = (size_t)3 * nch * nb;
+    std::vector<double> as, aa;
+    if (adj_value) {
+        as.assign(adj_value, adj_value + hn);
+        for (double& x : as) x *= scale;
+    }
+    if (adj_abs) {
+        aa.assign(adj_abs, adj_abs + hn);
+        for (double& x : aa) x *= scale;
+    }
+    const ear_tuning& tu = c->tuning;
+    const uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 22);
+    const uint32_t cap = std::max<uint32_t>(1, std::min(chunk, D->n_rays));
+    hipStream_t st = c->stream;
+    DevBuf qo[2] = {DevBuf(c), DevBuf(c)}, qd[2] = {DevBuf(c), DevBuf(c)}, qm[2] = {DevBuf(c), DevBuf(c)};
+    DevBuf tt[2] = {DevBuf(c), DevBuf(c)}, tr[2] = {DevBuf(c), DevBuf(c)};
+    DevBuf hits(c), counters(c), stats(c), gs(c), dinst(c), dsegs(c), dsegoff(c), dadj_s(c), dadj_a(c), dgrad(c),
+        sqp(c), sqm(c);
+    for (int k = 0; k < 2; ++k) {
+        TRY(qo[k].alloc((size_t)cap * 16));
+        TRY(qd[k].alloc((size_t)cap * 16));
+        TRY(qm[k].alloc((size_t)cap * 16));
+        TRY(tt[k].alloc((size_t)cap * nm * 8 * 4));
+        TRY(tr[k].alloc((size_t)cap * nm * 4));
+    }
+    TRY(hits.alloc((size_t)cap * 8));
+    TRY(counters.alloc(16));
+    TRY(stats.alloc(ST_N * 8));
+    TRY(sqp.alloc(16));
+    TRY(sqm.alloc(16));
+    ExtendFn fe;
+    ShadowFn fs_unused;
+    const TravCfg cfg{256, 16, (int)s->width, false, false};
+    pick_kernels(cfg, &fe, &fs_unused);
+    const uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 32;
+    const uint32_t cap_e = (uint32_t)c->num_cus * (uint32_t)std::max(1, resident_blocks((const void*)fe, 256));
+    const uint32_t gmax = persistent_grid(c, 0xFFFFFFFFu, waves);
+    // every thread of either traversal launch owns one overflow stack
+    const uint32_t gthreads = std::max(cap_e * 256u, gmax * (uint32_t)kBlock);
+    TRY(gs.alloc((size_t)(kStackDepth - kLdsMin) * gthreads * 4));
+    TRY(dinst.alloc(sizeof(InstDev)));
+    TRY(dsegs.alloc(sizeof(GenSeg)));
+    TRY(dsegoff.alloc(4));
+    TRY(dgrad.alloc(sizeof(double) * 3 * 2 * nm));
+    if (adj_value) {
+        TRY(dadj_s.alloc(hn * 8));
+        EAR_HIP_TRY(c, hipMemcpyAsync(dadj_s.p, as.data(), hn * 8, hipMemcpyHostToDevice, st));
+    }
+    if (adj_abs) {
+        TRY(dadj_a.alloc(hn * 8));
+        EAR_HIP_TRY(c, hipMemcpyAsync(dadj_a.p, aa.data(), hn * 8, hipMemcpyHostToDevice, st));
+    }
+    EAR_HIP_TRY(c, hipMemcpyAsync(dinst.p, &I, sizeof(InstDev), hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemsetAsync(dgrad.p, 0, dgrad.n, st));
+    EAR_HIP_TRY(c, hipMemsetAsync(counters.p, 0, counters.n, st));
+    EAR_HIP_TRY(c, hipMemsetAsync(stats.p, 0, stats.n, st));
+    const AdjArgs A{adj_value ? dadj_s.as<double>() : nullptr, adj_abs ? dadj_a.as<double>() : nullptr,
+                    dgrad.as<double>(), nm};
+    const InstDev* di = dinst.as<InstDev>();
+    uint32_t* cnt = counters.as<uint32_t>();
+    unsigned long long* stp = stats.as<unsigned long long>();
+    uint32_t pinned[4];
+    Queue q[2] = {Queue{qo[0].as<float4>(), qd[0].as<float4>(), qm[0].as<uint4>()},
+                  Queue{qo[1].as<float4>(), qd[1].as<float4>(), qm[1].as<uint4>()}};
+    Tangents T[2] = {Tangents{tt[0].as<float>(), tr[0].as<float>(), cap},
+                     Tangents{tt[1].as<float>(), tr[1].as<float>(), cap}};
+    const ShadowQ SQ{sqp.as<float4>(), sqm.as<float4>()};
+    for (uint32_t band = 0; band < 3; ++band) {
+        if (!((P.band_mask >> band) & 1u)) continue;
+        for (uint32_t r0 = 0; r0 < D->n_rays; r0 += cap) {
+            const uint32_t n_chunk = std::min(cap, D->n_rays - r0);
+            const GenSeg g{0, band, r0, n_chunk};
+            const uint32_t off0 = 0;
+            EAR_HIP_TRY(c, hipMemcpyAsync(dsegs.p, &g, sizeof(g), hipMemcpyHostToDevice, st));
+            EAR_HIP_TRY(c, hipMemcpyAsync(dsegoff.p, &off0, 4, hipMemcpyHostToDevice, st));
+            k_generate<false><<<persistent_grid(c, n_chunk, waves), kBlock, 0, st>>>(
+                di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), 1u, n_chunk, P.n_rec, 0u, q[0], cnt + 0, SQ, cnt + 2,
+                stp);
+            k_zero_f32<<<1024, 256, 0, st>>>(T[0].t, (size_t)cap * nm * 8);
+            k_zero_f32<<<1024, 256, 0, st>>>(T[0].r, (size_t)cap * nm);
+            EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
+            EAR_HIP_TRY(c, hipStreamSynchronize(st));
+            uint32_t n_live = pinned[0];
+            int cur = 0;
+            for (uint32_t k = 1; k <= D->max_bounces && n_live > 0; ++k) {
+                hipLaunchKernelGGL(fe, dim3(grid_for(n_live, cap_e, 256)), dim3(256), 0, st, di, gs.as<int>(),
+                                   gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp);
+                const uint32_t gsd = persistent_grid(c, n_live, waves);
+                if (s->width == 4)
+                    k_shade_ad<4><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,
+                                                          hits.as<int2>(), T[cur], q[1 - cur], cnt + (1 - cur),
+                                                          T[1 - cur], A, stp);
+                else
+                    k_shade_ad<2><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,
+                                                          hits.as<int2>(), T[cur], q[1 - cur], cnt + (1 - cur),
+                                                          T[1 - cur], A, stp);
+                EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
+                EAR_HIP_TRY(c, hipStreamSynchronize(st));
+                n_live = pinned[1 - cur];
+                k_reset2<<<1, 64, 0, st>>>(cnt + cur, cnt + 2);
+                cur = 1 - cur;
+            }
+            k_reset2<<<1, 64, 0, st>>>(cnt + 0, cnt + 1);
+            EAR_HIP_TRY(c, hipGetLastError());
+        }
+    }
+    std::vector<double> g(3 * 2 * nm);
+    EAR_HIP_TRY(c, hipMemcpyAsync(g.data(), dgrad.p, g.size() * 8, hipMemcpyDeviceToHost, st));
+    EAR_HIP_TRY(c, hipStreamSynchronize(st));
+    for (uint32_t b = 0; b < 3; ++b) {
+        if (!((P.band_mask >> b) & 1u)) continue;
+        for (uint32_t m = 0; m < nm; ++m) {
+            if (grad_refl) grad_refl[(size_t)m * 3 + b] = g[((size_t)b * 2 + 0) * nm + m];
+            if (grad_spec) grad_spec[(size_t)m * 3 + b] = g[((size_t)b * 2 + 1) * nm + m];
+        }
+    }
+    return EAR_OK;
+}
+
+}  // namespace ear

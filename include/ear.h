@@ -146,6 +146,8 @@ typedef struct {
 #define EAR_FLAG_ABS_HIST 1u      /* also accumulate the |c| histogram (EDC / T60) */
 #define EAR_FLAG_INSTRUMENT 2u    /* count BVH node visits / triangle tests (slower) */
 #define EAR_FLAG_TRACE_LOG 4u     /* log per-ray hit triangle indices */
+#define EAR_FLAG_ADJ_LOCAL 8u     /* ear_render_backward: keep only each vertex's own
+                                     dependence on spec (see there) */
 
 typedef struct {
     uint32_t band_mask;     /* bit b renders band b (0 low, 1 mid, 2 high) as its own context */
@@ -221,6 +223,27 @@ typedef struct {
     ear_hist* out;
 } ear_batch_item;
 ear_status ear_render_batch(ear_ctx* ctx, const ear_batch_item* items, uint32_t n_items, ear_stats* stats);
+
+/* Adjoint of ear_render (no counterpart in the reference, which has no
+ * gradients; it serves inverse design: fitting materials to a measured
+ * decay).  For the loss L = sum(adj_value * value) + sum(adj_abs * abs_value)
+ * over the histograms ear_render would write for `desc` (adjoints laid out
+ * as ear_hist.value, [3][n_channels][hist_bins], either may be NULL), writes
+ * dL/drefl and dL/dspec as [n_mat][3] for every band of desc->band_mask
+ * (entries of other bands are left untouched).  The rays replay the render's
+ * paths exactly (same Philox stream); tangents follow the specular lobe's
+ * reparameterised directions with the hit triangles and hemisphere samples
+ * held fixed.  Arrival-time shifts and visibility changes are not
+ * differentiated.  Point sources, 1..8 materials.
+ * With EAR_FLAG_ADJ_LOCAL the direction tangents are not carried to later
+ * bounces: d/dspec keeps only the explicit blend term (lobe - diffuse) of the
+ * connection at each vertex.  That estimate is biased but of bounded variance;
+ * the full path derivative's variance grows with the bounce count (the
+ * 1000th-power lobe amplifies the accumulated tangents), so long renders
+ * (fitting) use the local form. */
+ear_status ear_render_backward(ear_ctx* ctx, ear_scene* scene, const ear_render_desc* desc,
+                               const double* adj_value, const double* adj_abs, double* grad_refl,
+                               double* grad_spec);
 
 /* Launch configuration of the tracer (block sizes, ray chunk, stack split);
  * values of 0 keep the library default.  Results do not depend on them

@@ -875,3 +875,349 @@ int or_render(const or_scene* s, const or_desc* D, or_out* out) {
     unprep(&ps);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* FP64 reference gradient of a render with respect to the per-material    */
+/* reflection coefficient and specularity of the rendered band.            */
+/*                                                                          */
+/* The primal path is exactly trace_ray's (binary32, same Philox stream).  */
+/* Alongside it, forward-mode tangents in binary64 follow each specularity */
+/* parameter through the reparameterised scattering direction             */
+/* v = normalize(h (1 - s) + r s) (Sample_Hemi, src/Distributions.h:71-75): */
+/* hemisphere samples h and the hit triangles are held fixed, so the        */
+/* tangent of a hit point follows the hit triangle's plane.  The reflection */
+/* coefficient enters through the retention factor (src/Material.cpp:59)   */
+/* and d retention / d refl = 1.  Each recorded contribution c contributes  */
+/* (dL/dc) (dc/dtheta), dL/dc = sum over its ramp bins of adj[bin] times   */
+/* the ramp weight (w - i)/w * 2/sqrt(d) (and ILD gains, stereo);          */
+/* adj_abs weighs |c| (sign(c) dc/dtheta).  Time-of-arrival shifts of a     */
+/* contribution are not differentiated (binning is piecewise constant).     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double o[3], d[3], dL, dlogI;
+} tan_t;
+
+typedef struct {
+    const rctx* C;
+    const double* adj_s; /* [n_ch][n_bins] or NULL */
+    const double* adj_a;
+    uint32_t begin, end;
+    uint32_t n_mat;
+    int local;      /* direction tangents are not carried past their vertex */
+    double* g_refl; /* [n_mat] */
+    double* g_spec;
+} gworker_t;
+
+static double ramp_adj(const double* adj, uint32_t nb, int s, int w, double A_per_c) {
+    double acc = 0.0;
+    for (int i = 0; i < w; i++) {
+        long long b = (long long)s + i;
+        if (b < 0 || b >= (long long)nb) continue;
+        acc += adj[b] * ((double)(w - i) / (double)w) * A_per_c;
+    }
+    return acc;
+}
+
+/* dL/dc for a contribution c of recorder r (signed and |c| adjoints) */
+static double contrib_adjoint(const gworker_t* G, int r, const float dir[3], float c, float t, float dist) {
+    const rctx* C = G->C;
+    const or_recorder* rec = &C->d->rec[r];
+    int ch = C->ch_of_rec[r];
+    uint32_t nb = C->d->n_bins;
+    float width = sqrtf(dist);
+    int w = (int)ceilf(width);
+    double per_c = 2.0 / (double)width; /* A = 2 c / sqrt(d) */
+    double sgn = c < 0.0f ? -1.0 : 1.0;
+    double a = 0.0;
+    if (rec->kind == OR_REC_MONO) {
+        int s = (int)((double)t * 44100.0);
+        if (G->adj_s) a += ramp_adj(G->adj_s + (size_t)ch * nb, nb, s, w, per_c);
+        if (G->adj_a) a += sgn * ramp_adj(G->adj_a + (size_t)ch * nb, nb, s, w, per_c);
+    } else {
+        float dt = dot3(dir, rec->right_ear);
+        float td = rec->head_size / 343.0f;
+        int s_right = (int)((double)(t - dt * td) * 44100.0);
+        int s_left = (int)((double)(t + dt * td) * 44100.0);
+        float factor = or_pow(rec->head_abs[C->d->band], fabsf(dt) * rec->head_size);
+        double gl = 1.0, gr = 1.0;
+        if (dt < 0.0f)
+            gr = (double)(factor * factor);
+        else
+            gl = (double)(factor * factor);
+        if (G->adj_s) {
+            a += gl * ramp_adj(G->adj_s + (size_t)ch * nb, nb, s_left, w, per_c);
+            a += gr * ramp_adj(G->adj_s + (size_t)(ch + 1) * nb, nb, s_right, w, per_c);
+        }
+        if (G->adj_a) {
+            a += sgn * gl * ramp_adj(G->adj_a + (size_t)ch * nb, nb, s_left, w, per_c);
+            a += sgn * gr * ramp_adj(G->adj_a + (size_t)(ch + 1) * nb, nb, s_right, w, per_c);
+        }
+    }
+    return a;
+}
+
+static inline double ddot(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline double dfdot(const double a[3], const float b[3]) {
+    return a[0] * (double)b[0] + a[1] * (double)b[1] + a[2] * (double)b[2];
+}
+
+static void grad_ray(const gworker_t* G, uint32_t ray, tan_t* T, double* dlogI_refl) {
+    const rctx* C = G->C;
+    const or_desc* D = C->d;
+    const int band = D->band;
+    const uint32_t nm = G->n_mat;
+    const double ln_air = log((double)D->air);
+    or_rng R;
+    rng_init(&R, D->seed, ray, D->ctx, D->instance);
+    float I = 1.0f, L = 0.0f;
+    float o[3], dd[3], prev[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+    int bt = OR_REFLECT, mat = -1;
+    double dprev[16][3];
+    for (uint32_t q = 0; q < nm; q++) {
+        memset(&T[q], 0, sizeof(tan_t));
+        dlogI_refl[q] = 0.0;
+    }
+    for (uint32_t k = 0; k <= D->max_bounces; k++) {
+        float spec = 0.0f;
+        float dnorm = 1.0f;
+        if (k == 0) {
+            if (D->src_kind == OR_SRC_MESH) {
+                emit_mesh(C, &R, o, dd);
+            } else {
+                o[0] = D->src_pos[0];
+                o[1] = D->src_pos[1];
+                o[2] = D->src_pos[2];
+                sample_sphere(&R, dd);
+            }
+        } else {
+            float t;
+            int32_t idx;
+            int hit = closest_hit_p(C->s, o, dd, &t, &idx);
+            if (!hit) break;
+            float p[3] = {o[0] + dd[0] * t, o[1] + dd[1] * t, o[2] + dd[2] * t};
+            const float* N = C->s->nrm + 3 * idx;
+            if (dot3(N, dd) > 0.0f) {
+                n[0] = N[0] * -1.0f;
+                n[1] = N[1] * -1.0f;
+                n[2] = N[2] * -1.0f;
+            } else {
+                n[0] = N[0];
+                n[1] = N[1];
+                n[2] = N[2];
+            }
+            mat = C->s->mat[idx];
+            bt = (int)material_bounce(C, &R, mat, band);
+            spec = (C->mats[mat].flags & OR_MAT_HAS_SPEC) ? C->mats[mat].spec[band] : 0.0f;
+            /* hemisphere sample h (rejection against the final normal) and the blend */
+            float nn[3] = {n[0], n[1], n[2]};
+            if (bt == OR_REFRACT) {
+                nn[0] = -n[0];
+                nn[1] = -n[1];
+                nn[2] = -n[2];
+            }
+            float h[3], rdir[3];
+            sample_hemi(&R, nn, h);
+            if (bt == OR_REFRACT)
+                memcpy(rdir, dd, sizeof(rdir));
+            else
+                reflect3(dd, nn, rdir);
+            float g = 1.0f - spec;
+            float wv[3] = {h[0] * g + rdir[0] * spec, h[1] * g + rdir[1] * spec, h[2] * g + rdir[2] * spec};
+            float wl = len3(wv);
+            float v[3] = {wv[0], wv[1], wv[2]};
+            normalize3(v);
+            /* tangents of the hit point (plane of the hit triangle) and the new direction */
+            double Nd = dfdot((const double[3]){N[0], N[1], N[2]}, dd);
+            float dist[3];
+            sub3(p, o, dist);
+            float l = len3(dist);
+            for (uint32_t q = 0; q < nm; q++) {
+                tan_t* tq = &T[q];
+                double Nv[3] = {N[0], N[1], N[2]};
+                double dt_ = -(ddot(Nv, tq->o) + (double)t * ddot(Nv, tq->d)) / Nd;
+                double dp[3];
+                for (int a = 0; a < 3; a++) dp[a] = tq->o[a] + tq->d[a] * (double)t + (double)dd[a] * dt_;
+                double dl = 0.0;
+                for (int a = 0; a < 3; a++) dl += (double)dist[a] * (dp[a] - tq->o[a]);
+                dl /= (double)l;
+                tq->dL += dl;
+                tq->dlogI += ln_air * dl;
+                /* prev = d / |d| */
+                double dn = (double)len3(dd);
+                double pd = 0.0;
+                for (int a = 0; a < 3; a++) pd += (double)dd[a] / dn * tq->d[a];
+                for (int a = 0; a < 3; a++) dprev[q][a] = (tq->d[a] - (double)dd[a] / dn * pd) / dn;
+                /* new direction */
+                double dr[3];
+                if (bt == OR_REFRACT) {
+                    for (int a = 0; a < 3; a++) dr[a] = tq->d[a];
+                } else {
+                    double dn2 = dfdot(tq->d, nn);
+                    for (int a = 0; a < 3; a++) dr[a] = tq->d[a] - 2.0 * dn2 * (double)nn[a];
+                }
+                double dw[3];
+                for (int a = 0; a < 3; a++) {
+                    dw[a] = dr[a] * (double)spec;
+                    if ((int)q == mat) dw[a] += (double)rdir[a] - (double)h[a];
+                }
+                double vdw = dfdot(dw, v);
+                for (int a = 0; a < 3; a++) {
+                    tq->o[a] = dp[a];
+                    tq->d[a] = G->local ? 0.0 : (dw[a] - (double)v[a] * vdw) / (double)wl;
+                }
+            }
+            (void)dnorm;
+            dlogI_refl[mat] += 1.0 / (double)C->ret[mat][band];
+            I *= or_pow(D->air, l);
+            L += l;
+            o[0] = p[0];
+            o[1] = p[1];
+            o[2] = p[2];
+            dd[0] = v[0];
+            dd[1] = v[1];
+            dd[2] = v[2];
+            I *= C->ret[mat][band];
+            n[0] = nn[0];
+            n[1] = nn[1];
+            n[2] = nn[2];
+        }
+        if (!is_normal_f(I)) break;
+        if (k > 0 || D->src_kind == OR_SRC_MESH) {
+            for (uint32_t r = 0; r < D->n_rec; r++) {
+                const float* RP = D->rec[r].pos;
+                float dv[3];
+                sub3(RP, o, dv);
+                if (any_hit_p(C->s, o, dv)) continue;
+                float lsdir[3] = {dv[0], dv[1], dv[2]};
+                normalize3(lsdir);
+                float dtt = k ? dot3(lsdir, n) : 1.0f;
+                if (!(dtt > 0.0f)) continue;
+                float c = I, factor = 1.0f, lobe = 0.0f, diff = 0.0f, sf = 0.0f;
+                float rv[3] = {0, 0, 0};
+                if (k) {
+                    if (bt == OR_REFLECT) {
+                        reflect3(prev, n, rv);
+                        diff = -dot3(n, prev);
+                        sf = fmaxf(0.0f, dot3(rv, lsdir));
+                    } else {
+                        diff = dot3(n, prev);
+                        sf = fmaxf(0.0f, dot3(prev, lsdir));
+                    }
+                    lobe = OR_EXP_INT * or_pow(sf, OR_EXP);
+                    factor = spec * OR_EXP_INT * or_pow(sf, OR_EXP) + (1.0f - spec) * diff;
+                    c *= factor;
+                }
+                float l = len3(dv);
+                float geo = or_pow(D->air, l) * (2.0f / (4.0f * OR_PI * l * l));
+                c *= or_pow(D->air, l);
+                c *= 2.0f / (4.0f * OR_PI * l * l);
+                if (!is_normal_f(c)) continue;
+                if (k % 2) c *= -1.0f;
+                double a_c = contrib_adjoint(G, (int)r, lsdir, c, (L + l) / 343.0f, L + l);
+                if (a_c == 0.0) continue;
+                double sgn_k = (k % 2) ? -1.0 : 1.0;
+                /* c = sgn_k * I * factor * geo(l) */
+                double cnof = sgn_k * (double)I * (double)geo; /* c without the factor */
+                for (uint32_t q = 0; q < nm; q++) {
+                    const tan_t* tq = &T[q];
+                    double dvv[3] = {-tq->o[0], -tq->o[1], -tq->o[2]};
+                    double dls = (dvv[0] * dv[0] + dvv[1] * dv[1] + dvv[2] * dv[2]) / (double)l;
+                    double dlsdir[3];
+                    double ld = 0.0;
+                    for (int a = 0; a < 3; a++) ld += (double)lsdir[a] * dvv[a];
+                    for (int a = 0; a < 3; a++) dlsdir[a] = (dvv[a] - (double)lsdir[a] * ld) / (double)l;
+                    double dfactor = 0.0;
+                    if (k) {
+                        double ddiff, dsf = 0.0;
+                        if (bt == OR_REFLECT) {
+                            double drv[3];
+                            double pn = ddot(dprev[q], (const double[3]){n[0], n[1], n[2]});
+                            for (int a = 0; a < 3; a++) drv[a] = dprev[q][a] - 2.0 * pn * (double)n[a];
+                            ddiff = -pn;
+                            if (sf > 0.0f) dsf = dfdot(drv, lsdir) + dfdot(dlsdir, rv);
+                        } else {
+                            ddiff = ddot(dprev[q], (const double[3]){n[0], n[1], n[2]});
+                            if (sf > 0.0f) dsf = dfdot(dprev[q], lsdir) + dfdot(dlsdir, prev);
+                        }
+                        double dlobe = sf > 0.0f ? (double)OR_EXP_INT * 1000.0 * pow((double)sf, 999.0) * dsf : 0.0;
+                        dfactor = (double)spec * dlobe + (1.0 - (double)spec) * ddiff;
+                        if ((int)q == mat) dfactor += (double)lobe - (double)diff;
+                    }
+                    double dgeo_rel = log((double)D->air) * dls - 2.0 * dls / (double)l;
+                    double dc = cnof * dfactor + (double)c * (tq->dlogI + dgeo_rel);
+                    G->g_spec[q] += a_c * dc;
+                }
+                for (uint32_t m = 0; m < nm; m++) G->g_refl[m] += a_c * (double)c * dlogI_refl[m];
+            }
+        }
+        if ((double)I < D->cutoff) break;
+        prev[0] = dd[0];
+        prev[1] = dd[1];
+        prev[2] = dd[2];
+        normalize3(prev);
+    }
+}
+
+static void* gworker(void* arg) {
+    gworker_t* G = (gworker_t*)arg;
+    tan_t T[16];
+    double dr[16];
+    for (uint32_t ray = G->begin; ray < G->end; ray++) grad_ray(G, ray, T, dr);
+    return NULL;
+}
+
+int or_gradient(const or_scene* s, const or_desc* D, const double* adj_signed, const double* adj_abs, int local,
+                double* grad_refl, double* grad_spec) {
+    if (s->n_mat > 16 || D->band < 0 || D->band > 2) return -1;
+    prep_scene ps;
+    if (!prep(s, &ps)) return -2;
+    float (*ret)[3] = (float (*)[3])calloc(s->n_mat ? s->n_mat : 1, sizeof(float[3]));
+    for (uint32_t m = 0; m < s->n_mat; m++) or_material_retention(&s->mats[m], ret[m]);
+    int* ch_of = (int*)calloc(D->n_rec ? D->n_rec : 1, sizeof(int));
+    int c = 0;
+    for (uint32_t r = 0; r < D->n_rec; r++) {
+        ch_of[r] = c;
+        c += D->rec[r].kind == OR_REC_STEREO ? 2 : 1;
+    }
+    rctx C;
+    memset(&C, 0, sizeof(C));
+    C.d = D;
+    C.s = &ps;
+    C.ret = (const float (*)[3])ret;
+    C.mats = s->mats;
+    C.n_ch = c;
+    C.ch_of_rec = ch_of;
+    uint32_t nt = D->n_threads ? D->n_threads : 1;
+    uint32_t nr = D->ray_end - D->ray_begin;
+    if (nt > nr && nr > 0) nt = nr;
+    gworker_t* W = (gworker_t*)calloc(nt, sizeof(gworker_t));
+    pthread_t* th = (pthread_t*)calloc(nt, sizeof(pthread_t));
+    for (uint32_t i = 0; i < nt; i++) {
+        W[i].C = &C;
+        W[i].adj_s = adj_signed;
+        W[i].adj_a = adj_abs;
+        W[i].n_mat = s->n_mat;
+        W[i].local = local;
+        W[i].begin = D->ray_begin + (uint32_t)(((uint64_t)nr * i) / nt);
+        W[i].end = D->ray_begin + (uint32_t)(((uint64_t)nr * (i + 1)) / nt);
+        W[i].g_refl = (double*)calloc(16, sizeof(double));
+        W[i].g_spec = (double*)calloc(16, sizeof(double));
+        pthread_create(&th[i], NULL, gworker, &W[i]);
+    }
+    for (uint32_t m = 0; m < s->n_mat; m++) grad_refl[m] = grad_spec[m] = 0.0;
+    for (uint32_t i = 0; i < nt; i++) {
+        pthread_join(th[i], NULL);
+        for (uint32_t m = 0; m < s->n_mat; m++) {
+            grad_refl[m] += W[i].g_refl[m];
+            grad_spec[m] += W[i].g_spec[m];
+        }
+        free(W[i].g_refl);
+        free(W[i].g_spec);
+    }
+    free(W);
+    free(th);
+    free(ret);
+    free(ch_of);
+    unprep(&ps);
+    return 0;
+}

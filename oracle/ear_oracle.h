@@ -131,6 +131,13 @@ void or_record_f32(float* buf, uint32_t n, float a, float t, float dist);
 /* ---- the per-context render ---- */
 int or_render(const or_scene* s, const or_desc* d, or_out* out);
 int or_num_channels(const or_desc* d);
+/* FP64 reference gradient of sum(adj_signed * H_signed + adj_abs * H_abs)
+ * (raw sums, before 1/N) with respect to the rendered band's per-material
+ * reflection coefficient and specularity; adjoints are [n_ch][n_bins] or NULL.
+ * local != 0: direction tangents are not carried past the vertex that
+ * sampled them (only each connection's explicit dependence on spec). */
+int or_gradient(const or_scene* s, const or_desc* d, const double* adj_signed, const double* adj_abs, int local,
+                double* grad_refl, double* grad_spec);
 double or_now(void);
 
 #ifdef __cplusplus

@@ -87,6 +87,8 @@ def lib():
         L.or_record_f32.argtypes = [fp, C.c_uint32, C.c_float, C.c_float, C.c_float]
         L.or_render.argtypes = [C.POINTER(Scene), C.POINTER(Desc), C.POINTER(Out)]
         L.or_num_channels.argtypes = [C.POINTER(Desc)]
+        L.or_gradient.argtypes = [C.POINTER(Scene), C.POINTER(Desc), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                  C.c_void_p]
         L.or_now.restype = C.c_double
         _lib = L
     return _lib
@@ -269,6 +271,40 @@ def render(scene: OracleScene, *, band, air, recorders, n_rays, src_pos=(0, 0, 0
         res.update(trace_hits=th, trace_nseg=tn, trace_ncon=tc, trace_od=tod)
     del src
     return res
+
+
+def gradient(scene: OracleScene, *, band, air, recorders, n_rays, adj_signed=None, adj_abs=None,
+             src_pos=(0, 0, 0), src_tris=None, max_bounces=50, cutoff=0.0, seed=0xEA2, ctx=None, instance=0,
+             n_bins=44100, n_threads=8, local=False):
+    """FP64 reference gradient of sum(adj * H) (raw sums) with respect to the
+    band's per-material reflection coefficient and specularity (or_gradient;
+    local: direction tangents end at their vertex)."""
+    if ctx is None:
+        ctx = band
+    recs = make_recorders(recorders)
+    d = Desc()
+    d.band, d.air, d.dry, d.gain = band, air, 1.0, 1.0
+    src = None
+    if src_tris is not None:
+        src = np.ascontiguousarray(src_tris, dtype=np.float32).reshape(-1, 3, 3)
+        d.src_kind, d.src_n_tris, d.src_v = SRC_MESH, len(src), src.ctypes.data
+    else:
+        d.src_kind = SRC_POINT
+    d.src_pos[:] = [float(x) for x in src_pos]
+    d.n_rec, d.rec = len(recorders), recs
+    d.ray_begin, d.ray_end, d.n_rays = 0, n_rays, n_rays
+    d.max_bounces, d.cutoff, d.seed = max_bounces, cutoff, seed
+    d.ctx, d.instance, d.n_bins, d.n_threads = ctx, instance, n_bins, n_threads
+    a_s = None if adj_signed is None else np.ascontiguousarray(adj_signed, np.float64)
+    a_a = None if adj_abs is None else np.ascontiguousarray(adj_abs, np.float64)
+    gr = np.zeros(max(scene.n_mat, 1), np.float64)
+    gs = np.zeros(max(scene.n_mat, 1), np.float64)
+    rc = lib().or_gradient(C.byref(scene.s), C.byref(d), None if a_s is None else a_s.ctypes.data,
+                           None if a_a is None else a_a.ctypes.data, int(local), gr.ctypes.data, gs.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"or_gradient failed: {rc}")
+    del src
+    return gr[:scene.n_mat].copy(), gs[:scene.n_mat].copy()
 
 
 def direct_ramp(direct, n_bins):
