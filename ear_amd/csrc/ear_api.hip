@@ -108,6 +108,10 @@ const char* ear_ctx_last_error(const ear_ctx* c) { return c ? c->last_error.c_st
 
 ear_status ear_ctx_set_tuning(ear_ctx* c, const ear_tuning* t) {
     if (!c || !t) return EAR_ERR_INVALID_ARGUMENT;
+    if (t->traversal > EAR_TRAVERSAL_FETCH) {
+        c->last_error = "unknown traversal mode";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
     c->tuning = *t;
     return EAR_OK;
 }

@@ -181,16 +181,42 @@ __device__ __forceinline__ void cswap(float& ka, int& va, float& kb, int& vb) {
     }
 }
 
-// Returns true on a hit; best_* hold the closest hit (ANY = false).
+// One ray's BVH walk as a resumable state machine: step() visits one node
+// (its child boxes, and the triangles of the leaf children it reaches) and
+// picks the next node, so a persistent lane can interleave walks of rays it
+// fetches one after another.  Closest hit (ANY = false): best_* hold the
+// nearest hit so far.  Any hit: `hit` is set when a blocker is found.
 template <int LDS, int BLOCK, int WIDTH, bool ANY, bool INSTR>
-__device__ bool traverse(const Traversal& T, V3 o, V3 d, float& best_t, int& best_s, uint32_t& best_o, int* lds,
-                         uint32_t gtid, uint32_t& nvis, uint32_t& ntri) {
-    if (T.n_tris == 0) return false;
-    SlabRay sr{o.x, o.y, o.z, safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
-    Stack<LDS, BLOCK> stk{lds, T.gstack, gtid, T.gthreads};
-    int node = 0;
-    bool found = false;
-    for (;;) {
+struct Walker {
+    SlabRay sr;
+    V3 o, d;
+    float best_t;
+    int best_s;
+    uint32_t best_o;
+    int node;
+    bool found;
+    Stack<LDS, BLOCK> stk;
+
+    __device__ __forceinline__ Walker(int* lds, int* gstack, uint32_t gtid, uint32_t gthreads)
+        : stk{lds, gstack, gtid, gthreads} {}
+
+    // returns false when the walk is already over (empty scene)
+    __device__ __forceinline__ bool start(const Traversal& T, V3 o_, V3 d_, float t_max) {
+        o = o_;
+        d = d_;
+        sr = SlabRay{o.x, o.y, o.z, safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+        best_t = t_max;
+        best_s = -1;
+        best_o = 0;
+        node = 0;
+        found = false;
+        stk.sp = 0;
+        return T.n_tris != 0;
+    }
+
+    // Visits `node`; returns true when the walk has ended (found / best_*
+    // final, or for ANY a blocker found: found = true).
+    __device__ __forceinline__ bool step(const Traversal& T, uint32_t& nvis, uint32_t& ntri) {
         if (INSTR) ++nvis;
         if (WIDTH == 2) {
             const BvhNode* N = T.nodes + node;
@@ -203,26 +229,32 @@ __device__ bool traverse(const Traversal& T, V3 o, V3 d, float& best_t, int& bes
             bool h0 = tn0 <= tf0 && tf0 >= 0.0f && tn0 <= lim;
             bool h1 = tn1 <= tf1 && tf1 >= 0.0f && tn1 <= lim;
             if (h0 && ch.x < 0) {
-                if (test_leaf<ANY, INSTR>(T, ch.x, o, d, best_t, best_s, best_o, found, ntri)) return true;
+                if (test_leaf<ANY, INSTR>(T, ch.x, o, d, best_t, best_s, best_o, found, ntri)) {
+                    found = true;
+                    return true;
+                }
                 h0 = false;
             }
             if (h1 && ch.y < 0) {
-                if (test_leaf<ANY, INSTR>(T, ch.y, o, d, best_t, best_s, best_o, found, ntri)) return true;
+                if (test_leaf<ANY, INSTR>(T, ch.y, o, d, best_t, best_s, best_o, found, ntri)) {
+                    found = true;
+                    return true;
+                }
                 h1 = false;
             }
             if (h0 && h1) {
                 bool first0 = tn0 <= tn1;
                 stk.push(first0 ? ch.y : ch.x);
                 node = first0 ? ch.x : ch.y;
-                continue;
+                return false;
             }
             if (h0) {
                 node = ch.x;
-                continue;
+                return false;
             }
             if (h1) {
                 node = ch.y;
-                continue;
+                return false;
             }
         } else {
             const Bvh4Node* N = T.nodes4 + node;
@@ -244,7 +276,10 @@ __device__ bool traverse(const Traversal& T, V3 o, V3 d, float& best_t, int& bes
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 if (h[k] && r[k] < 0) {
-                    if (test_leaf<ANY, INSTR>(T, r[k], o, d, best_t, best_s, best_o, found, ntri)) return true;
+                    if (test_leaf<ANY, INSTR>(T, r[k], o, d, best_t, best_s, best_o, found, ntri)) {
+                        found = true;
+                        return true;
+                    }
                     h[k] = false;
                 }
             }
@@ -279,13 +314,27 @@ __device__ bool traverse(const Traversal& T, V3 o, V3 d, float& best_t, int& bes
                 }
                 for (int k = nh - 1; k >= 1; --k) stk.push(val[k]);
                 node = val[0];
-                continue;
+                return false;
             }
         }
-        if (stk.sp == 0) break;
+        if (stk.sp == 0) return true;
         node = stk.pop();
+        return false;
     }
-    return found;
+};
+
+// Returns true on a hit; best_* hold the closest hit (ANY = false).
+template <int LDS, int BLOCK, int WIDTH, bool ANY, bool INSTR>
+__device__ bool traverse(const Traversal& T, V3 o, V3 d, float& best_t, int& best_s, uint32_t& best_o, int* lds,
+                         uint32_t gtid, uint32_t& nvis, uint32_t& ntri) {
+    Walker<LDS, BLOCK, WIDTH, ANY, INSTR> w(lds, T.gstack, gtid, T.gthreads);
+    if (!w.start(T, o, d, best_t)) return false;
+    while (!w.step(T, nvis, ntri)) {
+    }
+    best_t = w.best_t;
+    best_s = w.best_s;
+    best_o = w.best_o;
+    return w.found;
 }
 
 // ---------------------------------------------------------------------------
@@ -425,12 +474,21 @@ __device__ __forceinline__ Hist make_hist(const InstDev& I) {
 template <bool BATCH>
 __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__ inst, const GenSeg* __restrict__ segs,
                                                       const uint32_t* __restrict__ seg_off, uint32_t n_segs,
-                                                      uint32_t total, uint32_t max_rec, uint32_t any_mesh, Queue q,
+                                                      uint32_t total, const uint32_t* __restrict__ plan,
+                                                      uint32_t max_rec, uint32_t any_mesh, Queue q,
                                                       uint32_t* __restrict__ n_q, ShadowQ sq,
                                                       uint32_t* __restrict__ n_sq, unsigned long long* __restrict__ stats) {
+    // rays [first, first + total) of the segment table; a plan written on the
+    // device (k_plan) overrides the host's range
+    uint32_t first = 0;
+    if (plan) {
+        first = plan[0];
+        total = plan[1];
+    }
     for (uint32_t base = blockIdx.x * kBlock; base < total; base += gridDim.x * kBlock) {
-        uint32_t gid = base + threadIdx.x;
-        bool valid = gid < total;
+        uint32_t gid = first + base + threadIdx.x;
+        const uint32_t end = first + total;
+        bool valid = gid < end;
         bool live = false;
         float4 ro{}, rd{};
         uint4 rm{};
@@ -521,7 +579,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__
 template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
 __global__ void __launch_bounds__(BLOCK) k_extend(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
                                                    Queue q, const uint32_t* __restrict__ n_q, int2* __restrict__ hits,
-                                                   unsigned long long* __restrict__ stats) {
+                                                   unsigned long long* __restrict__ stats, uint32_t* /*fetch*/) {
     __shared__ int lds[LDS * BLOCK];
     uint32_t n = *n_q;
     uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -674,7 +732,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
 template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
 __global__ void __launch_bounds__(BLOCK) k_shadow(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
                                                    ShadowQ sq, const uint32_t* __restrict__ n_sq,
-                                                   unsigned long long* __restrict__ stats) {
+                                                   unsigned long long* __restrict__ stats, uint32_t* /*fetch*/) {
     __shared__ int lds[LDS * BLOCK];
     uint32_t n = *n_sq;
     uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -714,10 +772,170 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(const InstDev* __restrict__ in
     }
 }
 
+// Persistent variants of k_extend / k_shadow: a lane whose walk has ended
+// takes the next queue entry, so a wave keeps its lanes busy instead of
+// waiting for its slowest ray (Aila & Laine, "Understanding the Efficiency of
+// Ray Traversal on GPUs", HPG 2009).  A wave fetches when at least kRefill of
+// its lanes are idle, with one atomic on `fetch` (zero at launch) for all of
+// them.  Each lane keeps its own traversal stack across the rays it takes.
+constexpr int kRefill = 16;
+
+template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
+__global__ void __launch_bounds__(BLOCK) k_extend_f(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
+                                                     Queue q, const uint32_t* __restrict__ n_q,
+                                                     int2* __restrict__ hits, unsigned long long* __restrict__ stats,
+                                                     uint32_t* fetch) {
+    __shared__ int lds[LDS * BLOCK];
+    const uint32_t n = *n_q;
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t lane = lane_id();
+    uint32_t nvis = 0, ntri = 0;
+    Walker<LDS, BLOCK, WIDTH, false, INSTR> w(lds + threadIdx.x, gstack, gtid, gthreads);
+    Traversal T = make_trav(inst[0], gstack, gthreads);
+    bool active = false, exhausted = n == 0;
+    uint32_t slot = 0;
+    for (;;) {
+        uint64_t act = __ballot(active);
+        const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
+        if (!exhausted && n_idle >= (uint32_t)kRefill) {
+            const uint64_t idle = ~act;
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(fetch, n_idle);
+            base = __shfl(base, leader);
+            if (base + n_idle >= n) exhausted = true;
+            if (!active) {
+                const uint32_t mine = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (mine < n) {
+                    slot = mine;
+                    const float4 o = q.o[mine], d = q.d[mine];
+                    if (BATCH) T = make_trav(inst[q.m[mine].w], gstack, gthreads);
+                    if (w.start(T, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 1000000.0f))
+                        active = true;
+                    else
+                        hits[mine] = make_int2(__float_as_int(1000000.0f), -1);
+                }
+            }
+            act = __ballot(active);
+        }
+        if (act == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active && w.step(T, nvis, ntri)) {
+            hits[slot] = make_int2(__float_as_int(w.best_t), w.best_s);
+            active = false;
+        }
+    }
+    if (INSTR) {
+        wave_add_u64(stats + ST_NV_CL, nvis);
+        wave_add_u64(stats + ST_NT_CL, ntri);
+    }
+}
+
+template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
+__global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
+                                                     ShadowQ sq, const uint32_t* __restrict__ n_sq,
+                                                     unsigned long long* __restrict__ stats, uint32_t* fetch) {
+    __shared__ int lds[LDS * BLOCK];
+    const uint32_t n = *n_sq;
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t lane = lane_id();
+    uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0;
+    Walker<LDS, BLOCK, WIDTH, true, INSTR> w(lds + threadIdx.x, gstack, gtid, gthreads);
+    Traversal T = make_trav(inst[0], gstack, gthreads);
+    bool active = false, exhausted = n == 0;
+    uint32_t slot = 0;
+    for (;;) {
+        uint64_t act = __ballot(active);
+        const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
+        bool done = false;
+        if (!exhausted && n_idle >= (uint32_t)kRefill) {
+            const uint64_t idle = ~act;
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(fetch, n_idle);
+            base = __shfl(base, leader);
+            if (base + n_idle >= n) exhausted = true;
+            if (!active) {
+                const uint32_t mine = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (mine < n) {
+                    slot = mine;
+                    ++nsh;
+                    const float4 pc = sq.p[mine], m = sq.m[mine];
+                    const uint32_t packed = __float_as_uint(m.y);
+                    const uint32_t id = BATCH ? __float_as_uint(m.z) : 0;
+                    if (BATCH) T = make_trav(inst[id], gstack, gthreads);
+                    const RecParams& R = inst[id].P.rec[packed & 0xFFu];
+                    const V3 p{pc.x, pc.y, pc.z};
+                    if (w.start(T, p, sub(V3{R.px, R.py, R.pz}, p), 0.0f))
+                        active = true;
+                    else
+                        done = true;  // empty scene: visible
+                }
+            }
+            act = __ballot(active);
+        }
+        // the loop is left by the whole wave at once (the fetch above reads
+        // a leader lane)
+        if (__ballot(active || done) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active && w.step(T, nvis, ntri)) {
+            active = false;
+            done = !w.found;
+        }
+        if (done) {
+            const float4 pc = sq.p[slot], m = sq.m[slot];
+            const uint32_t packed = __float_as_uint(m.y);
+            const int r = (int)(packed & 0xFFu), band = (int)((packed >> 8) & 0xFFu);
+            const uint32_t id = BATCH ? __float_as_uint(m.z) : 0;
+            const InstDev& In = inst[id];
+            const RecParams& R = In.P.rec[r];
+            const V3 dv = sub(V3{R.px, R.py, R.pz}, V3{pc.x, pc.y, pc.z});
+            const float l = length(dv);
+            record(make_hist(In), band, R, normalized(dv), pc.w, (m.x + l) / 343.0f, m.x + l);
+            ++ncon;
+        }
+    }
+    wave_add_u64(stats + ST_SHADOW, nsh);
+    wave_add_u64(stats + ST_CONTRIB, ncon);
+    if (INSTR) {
+        wave_add_u64(stats + ST_NV_SH, nvis);
+        wave_add_u64(stats + ST_NT_SH, ntri);
+    }
+}
+
 __global__ void k_reset2(uint32_t* a, uint32_t* b) {
     if (threadIdx.x == 0) {
         *a = 0;
         *b = 0;
+    }
+}
+
+// Ray regeneration: tops the current queue up to `cap` with the next rays of
+// the segment table.  plan = {first ray, count} for k_generate; the cursor
+// counts the rays taken so far.
+__global__ void k_plan(uint32_t* cursor, uint32_t total, uint32_t cap, const uint32_t* n_q, uint32_t* plan) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint32_t taken = *cursor, live = *n_q;
+        const uint32_t room = cap > live ? cap - live : 0u;
+        const uint32_t take = min(room, total - taken);
+        plan[0] = taken;
+        plan[1] = take;
+        *cursor = taken + take;
+    }
+}
+
+// zeroes four counters (a kernel, not a memset node, so a captured bounce
+// loop replays it correctly)
+__global__ void k_reset4(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* d) {
+    if (threadIdx.x == 0) {
+        *a = 0;
+        *b = 0;
+        *c = 0;
+        *d = 0;
     }
 }
 
@@ -858,30 +1076,36 @@ __global__ void __launch_bounds__(kBlock) k_query(Traversal T, uint32_t n, const
 // width, instrumentation, batch) combination; the render picks one at run time.
 struct TravCfg {
     int block, lds, width;
-    bool instr, batch;
+    bool instr, batch, fetch;
 };
 
-using ExtendFn = void (*)(const InstDev*, int*, uint32_t, Queue, const uint32_t*, int2*, unsigned long long*);
-using ShadowFn = void (*)(const InstDev*, int*, uint32_t, ShadowQ, const uint32_t*, unsigned long long*);
+using ExtendFn = void (*)(const InstDev*, int*, uint32_t, Queue, const uint32_t*, int2*, unsigned long long*,
+                          uint32_t*);
+using ShadowFn = void (*)(const InstDev*, int*, uint32_t, ShadowQ, const uint32_t*, unsigned long long*, uint32_t*);
 
 template <int B, int L, int W, bool BA>
-void pick_t(bool instr, ExtendFn* e, ShadowFn* s) {
-    *e = instr ? (ExtendFn)k_extend<B, L, W, true, BA> : (ExtendFn)k_extend<B, L, W, false, BA>;
-    *s = instr ? (ShadowFn)k_shadow<B, L, W, true, BA> : (ShadowFn)k_shadow<B, L, W, false, BA>;
+void pick_t(bool instr, bool fetch, ExtendFn* e, ShadowFn* s) {
+    if (fetch) {
+        *e = instr ? (ExtendFn)k_extend_f<B, L, W, true, BA> : (ExtendFn)k_extend_f<B, L, W, false, BA>;
+        *s = instr ? (ShadowFn)k_shadow_f<B, L, W, true, BA> : (ShadowFn)k_shadow_f<B, L, W, false, BA>;
+    } else {
+        *e = instr ? (ExtendFn)k_extend<B, L, W, true, BA> : (ExtendFn)k_extend<B, L, W, false, BA>;
+        *s = instr ? (ShadowFn)k_shadow<B, L, W, true, BA> : (ShadowFn)k_shadow<B, L, W, false, BA>;
+    }
 }
 
 template <bool BA>
 void pick_b(const TravCfg& c, ExtendFn* e, ShadowFn* s) {
     int key = (c.block >= 256 ? 4 : 0) | (c.lds >= 32 ? 2 : 0) | (c.width == 4 ? 1 : 0);
     switch (key) {
-        case 0: pick_t<128, 16, 2, BA>(c.instr, e, s); break;
-        case 1: pick_t<128, 16, 4, BA>(c.instr, e, s); break;
-        case 2: pick_t<128, 32, 2, BA>(c.instr, e, s); break;
-        case 3: pick_t<128, 32, 4, BA>(c.instr, e, s); break;
-        case 4: pick_t<256, 16, 2, BA>(c.instr, e, s); break;
-        case 5: pick_t<256, 16, 4, BA>(c.instr, e, s); break;
-        case 6: pick_t<256, 32, 2, BA>(c.instr, e, s); break;
-        default: pick_t<256, 32, 4, BA>(c.instr, e, s); break;
+        case 0: pick_t<128, 16, 2, BA>(c.instr, c.fetch, e, s); break;
+        case 1: pick_t<128, 16, 4, BA>(c.instr, c.fetch, e, s); break;
+        case 2: pick_t<128, 32, 2, BA>(c.instr, c.fetch, e, s); break;
+        case 3: pick_t<128, 32, 4, BA>(c.instr, c.fetch, e, s); break;
+        case 4: pick_t<256, 16, 2, BA>(c.instr, c.fetch, e, s); break;
+        case 5: pick_t<256, 16, 4, BA>(c.instr, c.fetch, e, s); break;
+        case 6: pick_t<256, 32, 2, BA>(c.instr, c.fetch, e, s); break;
+        default: pick_t<256, 32, 4, BA>(c.instr, c.fetch, e, s); break;
     }
 }
 
@@ -1163,12 +1387,17 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         }
     }
 
+    if (total_rays > 0xFFFFFFFFull) {
+        c->last_error = "more than 2^32 - 1 rays in one call";
+        return EAR_ERR_CAPACITY;
+    }
     const ear_tuning& tu = c->tuning;
     uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 22);
     uint64_t cap64 = std::min<uint64_t>((uint64_t)chunk * 3, std::max<uint64_t>(1, total_rays));
     const uint32_t cap_rays = (uint32_t)cap64;
-    const TravCfg cfg_e{tu.block_extend == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch};
-    const TravCfg cfg_s{tu.block_shadow == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch};
+    const bool fetch = tu.traversal == EAR_TRAVERSAL_FETCH;
+    const TravCfg cfg_e{tu.block_extend == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch, fetch};
+    const TravCfg cfg_s{tu.block_shadow == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch, fetch};
     ExtendFn fe, fe_unused;
     ShadowFn fs, fs_unused;
     pick_kernels(cfg_e, &fe, &fs_unused);
@@ -1192,10 +1421,14 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         TRY(qm[k].alloc((size_t)cap_rays * 16));
     }
     TRY(hits.alloc((size_t)cap_rays * 8));
-    size_t cap_sh = (size_t)cap_rays * max_recs;
+    // a queue's shading emits <= cap * recorders requests; mesh sources add
+    // their k = 0 connections of the rays generated in the same iteration
+    size_t cap_sh = (size_t)cap_rays * std::max<size_t>(1, max_recs) * (any_mesh ? 2 : 1);
     TRY(so.alloc(cap_sh * 16));
     TRY(sm.alloc(cap_sh * 16));
-    TRY(counters.alloc(4 * 4));
+    // queue 0, queue 1, shadow queue, generation cursor, extend fetch, shadow
+    // fetch, plan {first, count}
+    TRY(counters.alloc(8 * 4));
     TRY(stats.alloc(ST_N * 8));
     TRY(hist.alloc(hist_cells * 16));
     uint32_t gthreads = std::max(cap_e * (uint32_t)cfg_e.block, cap_s * (uint32_t)cfg_s.block);
@@ -1273,101 +1506,121 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     EAR_HIP_TRY(c, hipEventCreate(&t1));
     EAR_HIP_TRY(c, hipEventRecord(t0, st));
 
+    // The bounce loop is device driven: every iteration tops the current
+    // queue up with new rays (k_plan + k_generate), extends and shades it
+    // into the other queue, and connects the shadow requests.  Kernels read
+    // their counts from device memory and run on fixed persistent grids, so
+    // the host only checks for the end every few iterations, and an
+    // iteration block can be captured once as a graph and replayed.
     uint32_t bounces = 0, launches = 0;
-    uint64_t extend_q = 0, shadow_q = 0;
     ear_status status = EAR_OK;
-    Queue q0{qo[0].as<float4>(), qd[0].as<float4>(), qm[0].as<uint4>()};
-    Queue q1{qo[1].as<float4>(), qd[1].as<float4>(), qm[1].as<uint4>()};
+    Queue qq[2] = {Queue{qo[0].as<float4>(), qd[0].as<float4>(), qm[0].as<uint4>()},
+                   Queue{qo[1].as<float4>(), qd[1].as<float4>(), qm[1].as<uint4>()}};
     ShadowQ SQ{so.as<float4>(), sm.as<float4>()};
-    size_t seg_i = 0;
-    uint32_t seg_pos = 0;
-    while (seg_i < all.size() && status == EAR_OK) {
-        // next chunk of at most cap_rays rays
-        std::vector<GenSeg> cs;
-        std::vector<uint32_t> off;
-        uint32_t total = 0;
-        while (seg_i < all.size() && total < cap_rays) {
-            GenSeg g = all[seg_i];
-            uint32_t take = std::min(g.count - seg_pos, cap_rays - total);
-            cs.push_back(GenSeg{g.inst, g.band, g.ray0 + seg_pos, take});
-            off.push_back(total);
-            total += take;
-            seg_pos += take;
-            if (seg_pos == g.count) {
-                ++seg_i;
-                seg_pos = 0;
-            }
+    {
+        std::vector<uint32_t> off(all.size());
+        uint64_t acc = 0;
+        for (size_t k = 0; k < all.size(); ++k) {
+            off[k] = (uint32_t)acc;
+            acc += all[k].count;
         }
-        EAR_HIP_TRY(c, hipMemcpyAsync(dsegs.p, cs.data(), sizeof(GenSeg) * cs.size(), hipMemcpyHostToDevice, st));
-        EAR_HIP_TRY(c, hipMemcpyAsync(dsegoff.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, st));
-        uint32_t g = persistent_grid(c, total, waves);
+        if (!all.empty()) {
+            EAR_HIP_TRY(c, hipMemcpyAsync(dsegs.p, all.data(), sizeof(GenSeg) * all.size(), hipMemcpyHostToDevice, st));
+            EAR_HIP_TRY(c, hipMemcpyAsync(dsegoff.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, st));
+        }
+    }
+    const uint32_t n_segs = (uint32_t)std::max<size_t>(1, all.size());
+    const uint32_t total32 = (uint32_t)total_rays;
+    const uint32_t g_gen = persistent_grid(c, cap_rays, waves);
+    const uint32_t g_shade = persistent_grid(c, cap_rays, waves);
+    const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
+    const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
+    const bool timed = tu.graph_iters == 0;
+    // one iteration; cur = the queue it consumes
+    auto iteration = [&](int cur) {
+        uint32_t* nq = cnt + cur;
+        k_plan<<<1, 64, 0, st>>>(cnt + 3, total32, cap_rays, nq, cnt + 6);
         if (batch)
-            k_generate<true><<<g, kBlock, 0, st>>>(di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), (uint32_t)cs.size(),
-                                                   total, (uint32_t)max_recs, any_mesh, q0, cnt + 0, SQ, cnt + 2, stp);
+            k_generate<true><<<g_gen, kBlock, 0, st>>>(di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), n_segs, 0u,
+                                                       cnt + 6, (uint32_t)max_recs, any_mesh, qq[cur], nq, SQ,
+                                                       cnt + 2, stp);
         else
-            k_generate<false><<<g, kBlock, 0, st>>>(di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(),
-                                                    (uint32_t)cs.size(), total, (uint32_t)max_recs, any_mesh, q0,
-                                                    cnt + 0, SQ, cnt + 2, stp);
-        ++launches;
-        EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
+            k_generate<false><<<g_gen, kBlock, 0, st>>>(di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), n_segs, 0u,
+                                                        cnt + 6, (uint32_t)max_recs, any_mesh, qq[cur], nq, SQ,
+                                                        cnt + 2, stp);
+        EventPair* e1 = timed ? &mk(ev_ext) : nullptr;
+        if (e1) hipEventRecord(e1->a, st);
+        hipLaunchKernelGGL(fe, dim3(g_ext), dim3(cfg_e.block), 0, st, di, gsp, gthreads, qq[cur], (const uint32_t*)nq,
+                           hits.as<int2>(), stp, cnt + 4);
+        if (e1) hipEventRecord(e1->b, st);
+        EventPair* e2 = timed ? &mk(ev_sha) : nullptr;
+        if (e2) hipEventRecord(e2->a, st);
+        if (batch)
+            k_shade<true><<<g_shade, kBlock, 0, st>>>(di, qq[cur], nq, hits.as<int2>(), qq[1 - cur], cnt + (1 - cur),
+                                                      SQ, cnt + 2, (uint32_t)max_recs, stp);
+        else
+            k_shade<false><<<g_shade, kBlock, 0, st>>>(di, qq[cur], nq, hits.as<int2>(), qq[1 - cur],
+                                                       cnt + (1 - cur), SQ, cnt + 2, (uint32_t)max_recs, stp);
+        if (e2) hipEventRecord(e2->b, st);
+        EventPair* e3 = timed ? &mk(ev_shw) : nullptr;
+        if (e3) hipEventRecord(e3->a, st);
+        hipLaunchKernelGGL(fs, dim3(g_shw), dim3(cfg_s.block), 0, st, di, gsp, gthreads, SQ, (const uint32_t*)(cnt + 2),
+                           stp, cnt + 5);
+        if (e3) hipEventRecord(e3->b, st);
+        k_reset4<<<1, 64, 0, st>>>(nq, cnt + 2, cnt + 4, cnt + 5);
+        launches += 6;
+        ++bounces;
+    };
+    // the render has ended once every ray was taken and no queue holds any
+    auto finished = [&](int cur, bool* done) -> ear_status {
+        EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 16, hipMemcpyDeviceToHost, st));
         EAR_HIP_TRY(c, hipStreamSynchronize(st));
-        if (pinned[2]) {  // mesh-source connections at k = 0
-            uint32_t nsh = pinned[2];
-            uint32_t gsh = grid_for(nsh, cap_s, cfg_s.block);
-            EventPair& e = mk(ev_shw);
-            hipEventRecord(e.a, st);
-            hipLaunchKernelGGL(fs, dim3(gsh), dim3(cfg_s.block), 0, st, di, gsp, gthreads, SQ,
-                               (const uint32_t*)(cnt + 2), stp);
-            hipEventRecord(e.b, st);
-            shadow_q += nsh;
-            ++launches;
-            k_reset2<<<1, 64, 0, st>>>(cnt + 2, cnt + 3);
-        }
-        int cur = 0;
-        uint32_t n_live = pinned[0];
-        for (uint32_t k = 1; k <= max_bounces && n_live > 0; ++k) {
-            Queue qi = cur == 0 ? q0 : q1;
-            Queue qn = cur == 0 ? q1 : q0;
-            uint32_t g1 = grid_for(n_live, cap_e, cfg_e.block);
-            EventPair& e1 = mk(ev_ext);
-            hipEventRecord(e1.a, st);
-            hipLaunchKernelGGL(fe, dim3(g1), dim3(cfg_e.block), 0, st, di, gsp, gthreads, qi,
-                               (const uint32_t*)(cnt + cur), hits.as<int2>(), stp);
-            hipEventRecord(e1.b, st);
-            uint32_t gsd = persistent_grid(c, n_live, waves);
-            EventPair& e2 = mk(ev_sha);
-            hipEventRecord(e2.a, st);
-            if (batch)
-                k_shade<true><<<gsd, kBlock, 0, st>>>(di, qi, cnt + cur, hits.as<int2>(), qn, cnt + (1 - cur), SQ,
-                                                      cnt + 2, (uint32_t)max_recs, stp);
-            else
-                k_shade<false><<<gsd, kBlock, 0, st>>>(di, qi, cnt + cur, hits.as<int2>(), qn, cnt + (1 - cur), SQ,
-                                                       cnt + 2, (uint32_t)max_recs, stp);
-            hipEventRecord(e2.b, st);
-            EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
-            EAR_HIP_TRY(c, hipStreamSynchronize(st));
-            uint32_t nsh = pinned[2];
-            extend_q += n_live;
-            launches += 2;
-            if (nsh) {
-                uint32_t g2 = grid_for(nsh, cap_s, cfg_s.block);
-                EventPair& e3 = mk(ev_shw);
-                hipEventRecord(e3.a, st);
-                hipLaunchKernelGGL(fs, dim3(g2), dim3(cfg_s.block), 0, st, di, gsp, gthreads, SQ,
-                                   (const uint32_t*)(cnt + 2), stp);
-                hipEventRecord(e3.b, st);
-                shadow_q += nsh;
-                ++launches;
+        *done = pinned[cur] == 0 && pinned[3] == total32;
+        return EAR_OK;
+    };
+    const uint32_t max_iters = max_bounces + 2 + (uint32_t)((total_rays + cap_rays - 1) / cap_rays) * (max_bounces + 1);
+    int cur = 0;
+    bool done = false;
+    if (timed) {
+        const uint32_t every = 4;
+        while (!done && status == EAR_OK) {
+            for (uint32_t k = 0; k < every; ++k) {
+                iteration(cur);
+                cur = 1 - cur;
             }
-            n_live = pinned[1 - cur];
-            k_reset2<<<1, 64, 0, st>>>(cnt + cur, cnt + 2);
-            ++launches;
-            cur = 1 - cur;
-            ++bounces;
+            hipError_t le = hipGetLastError();
+            if (le != hipSuccess) status = hip_status(c, le, "tracer launch");
+            if (status == EAR_OK) status = finished(cur, &done);
+            if (!done && bounces > max_iters) {
+                c->last_error = "bounce loop did not drain";
+                status = EAR_ERR_INTERNAL;
+            }
         }
-        k_reset2<<<1, 64, 0, st>>>(cnt + 0, cnt + 1);
-        hipError_t le = hipGetLastError();
-        if (le != hipSuccess) status = hip_status(c, le, "tracer launch");
+    } else {
+        const uint32_t per = std::max<uint32_t>(2, (tu.graph_iters + 1) & ~1u);  // even: replays start on queue 0
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        EAR_HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (uint32_t k = 0; k < per; ++k) iteration((int)(k & 1u));
+        hipError_t ce = hipStreamEndCapture(st, &graph);
+        if (ce == hipSuccess) ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ce != hipSuccess) status = hip_status(c, ce, "bounce-loop graph");
+        const uint32_t captured = bounces;
+        bounces = 0;
+        launches = 0;
+        while (!done && status == EAR_OK) {
+            hipError_t le = hipGraphLaunch(exec, st);
+            bounces += captured;
+            launches += 1;
+            if (le != hipSuccess) status = hip_status(c, le, "bounce-loop graph launch");
+            if (status == EAR_OK) status = finished(0, &done);
+            if (!done && bounces > max_iters + per) {
+                c->last_error = "bounce loop did not drain";
+                status = EAR_ERR_INTERNAL;
+            }
+        }
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
     }
     if (status != EAR_OK) return status;
     if (width == 4)
@@ -1467,8 +1720,8 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         st_out->shadow_ms = sum(ev_shw);
         st_out->extend_launches = (uint32_t)ev_ext.size();
         st_out->shadow_launches = (uint32_t)ev_shw.size();
-        st_out->extend_queries = extend_q;
-        st_out->shadow_queries = shadow_q;
+        st_out->extend_queries = hs[ST_SEGMENTS];
+        st_out->shadow_queries = hs[ST_SHADOW];
     }
     for (auto* v : {&ev_ext, &ev_sha, &ev_shw})
         for (auto& e : *v) {
@@ -1870,13 +2123,13 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const 
         TRY(tr[k].alloc((size_t)cap * nm * 4));
     }
     TRY(hits.alloc((size_t)cap * 8));
-    TRY(counters.alloc(16));
+    TRY(counters.alloc(32));
     TRY(stats.alloc(ST_N * 8));
     TRY(sqp.alloc(16));
     TRY(sqm.alloc(16));
     ExtendFn fe;
     ShadowFn fs_unused;
-    const TravCfg cfg{256, 16, (int)s->width, false, false};
+    const TravCfg cfg{256, 16, (int)s->width, false, false, false};
     pick_kernels(cfg, &fe, &fs_unused);
     const uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 32;
     const uint32_t cap_e = (uint32_t)c->num_cus * (uint32_t)std::max(1, resident_blocks((const void*)fe, 256));
@@ -1920,8 +2173,8 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const 
             EAR_HIP_TRY(c, hipMemcpyAsync(dsegs.p, &g, sizeof(g), hipMemcpyHostToDevice, st));
             EAR_HIP_TRY(c, hipMemcpyAsync(dsegoff.p, &off0, 4, hipMemcpyHostToDevice, st));
             k_generate<false><<<persistent_grid(c, n_chunk, waves), kBlock, 0, st>>>(
-                di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), 1u, n_chunk, P.n_rec, 0u, q[0], cnt + 0, SQ, cnt + 2,
-                stp);
+                di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), 1u, n_chunk, nullptr, P.n_rec, 0u, q[0], cnt + 0, SQ,
+                cnt + 2, stp);
             k_zero_f32<<<1024, 256, 0, st>>>(T[0].t, (size_t)cap * nm * 8);
             k_zero_f32<<<1024, 256, 0, st>>>(T[0].r, (size_t)cap * nm);
             EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
@@ -1930,7 +2183,7 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const 
             int cur = 0;
             for (uint32_t k = 1; k <= D->max_bounces && n_live > 0; ++k) {
                 hipLaunchKernelGGL(fe, dim3(grid_for(n_live, cap_e, 256)), dim3(256), 0, st, di, gs.as<int>(),
-                                   gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp);
+                                   gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp, cnt + 4);
                 const uint32_t gsd = persistent_grid(c, n_live, waves);
                 if (s->width == 4)
                     k_shade_ad<4><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,

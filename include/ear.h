@@ -252,11 +252,17 @@ typedef struct {
     uint32_t block_extend;
     uint32_t block_shade;
     uint32_t block_shadow;
-    uint32_t ray_chunk;      /* rays in flight per wavefront */
+    uint32_t ray_chunk;      /* queue capacity = 3 x ray_chunk rays in flight; finished
+                                rays are replaced by new ones every iteration */
     uint32_t waves_per_cu;   /* persistent grid size */
     uint32_t stack_lds;      /* traversal stack entries kept in LDS (16 or 32) */
-    uint32_t reserved[2];
+    uint32_t traversal;      /* EAR_TRAVERSAL_*: how BVH walks are assigned to lanes */
+    uint32_t graph_iters;    /* > 0: bounce-loop iterations captured in one HIP graph and
+                                replayed (per-kernel times are then not measured) */
 } ear_tuning;
+#define EAR_TRAVERSAL_DEFAULT 0u
+#define EAR_TRAVERSAL_STATIC 1u  /* one queue entry per lane per grid-stride step */
+#define EAR_TRAVERSAL_FETCH 2u   /* persistent lanes take the next entry when their walk ends */
 ear_status ear_ctx_set_tuning(ear_ctx* ctx, const ear_tuning* t);
 ear_status ear_ctx_get_tuning(const ear_ctx* ctx, ear_tuning* t);
 

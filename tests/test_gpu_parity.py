@@ -144,3 +144,75 @@ def test_stereo_and_mesh_source_parity(gpu_ctx, oracle):
             np.testing.assert_array_equal(g.trace_hits[b], o["trace_hits"])
             assert_hist_parity(api.raw_to_float(g.raw[b]), api.raw_to_float(g.abs_raw[b]), o["hist_signed"],
                                o["hist_abs"], n_rays * 41)
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_fetch_traversal_is_bitwise_identical(gpu_ctx, width):
+    """Persistent lanes that refill from the queue walk every ray exactly as
+    one-ray-per-lane launches do: same histograms, same node visits and
+    triangle tests, for both BVH widths, block sizes and stack splits, with a
+    stereo recorder and a mesh source (its k = 0 connections run the shadow
+    kernel before the first bounce)."""
+    s = scenes.hall(target_tris=20_000)
+    src_tris = np.array([[[4.0, 13.0, 1.0], [5.0, 13.0, 1.0], [4.0, 14.0, 1.2]]], np.float32)
+    kw = dict(recorders=s["recorders"], n_rays=20_000, src_pos=s["src"], air=s["air"], max_bounces=100,
+              cutoff=1e-8, n_bins=132300, instrument=True)
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], width=width) as sc:
+        res = []
+        for trav, blk, lds in [(1, 256, 16), (2, 256, 16), (2, 128, 32)]:
+            gpu_ctx.set_tuning(traversal=trav, block_extend=blk, block_shadow=blk, stack_lds=lds)
+            res.append(api.render(gpu_ctx, sc, **kw))
+            res.append(api.render(gpu_ctx, sc, src_tris=src_tris, **kw))
+        gpu_ctx.set_tuning()
+    keys = ("segments", "shadow_rays", "contributions", "node_visits_closest", "tri_tests_closest",
+            "node_visits_shadow", "tri_tests_shadow")
+    for i in range(2, len(res)):
+        ref = res[i % 2]
+        np.testing.assert_array_equal(ref.raw, res[i].raw)
+        np.testing.assert_array_equal(ref.abs_raw, res[i].abs_raw)
+        assert all(ref.stats[k] == res[i].stats[k] for k in keys), (ref.stats, res[i].stats)
+
+
+def test_fetch_traversal_batch(gpu_ctx):
+    rooms = [scenes.room(seed=30 + i, target_tris=[12, 2000, 9000][i % 3]) for i in range(5)]
+    objs = [api.Scene(gpu_ctx, r["tris"], r["mat"], r["materials"]) for r in rooms]
+    try:
+        items = [(sc, dict(recorders=r["recorders"], src_pos=r["src"], air=r["air"], max_bounces=60, cutoff=1e-7,
+                           n_bins=44100, n_rays=3000 + 500 * i, instance=i))
+                 for i, (r, sc) in enumerate(zip(rooms, objs))]
+        gpu_ctx.set_tuning(traversal=1)
+        a, _, _ = api.render_batch(gpu_ctx, items)
+        gpu_ctx.set_tuning(traversal=2)
+        b, _, _ = api.render_batch(gpu_ctx, items)
+        gpu_ctx.set_tuning()
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x.raw, y.raw)
+            np.testing.assert_array_equal(x.abs_raw, y.abs_raw)
+    finally:
+        for o in objs:
+            o.close()
+
+
+def test_graph_bounce_loop_is_bitwise_identical(gpu_ctx):
+    """The bounce loop captured once as a HIP graph (2 or 6 iterations) and
+    replayed until the queues drain, with a host synchronisation between
+    replays, gives the eager loop's histograms on every render; a small
+    queue (ray_chunk 3000) makes each render replay it many times and
+    regenerate rays in most iterations."""
+    s = scenes.hall(target_tris=20_000)
+    src_tris = np.array([[[4.0, 13.0, 1.0], [5.0, 13.0, 1.0], [4.0, 14.0, 1.2]]], np.float32)
+    kw = dict(recorders=s["recorders"], n_rays=20_000, src_pos=s["src"], air=s["air"], max_bounces=100,
+              cutoff=1e-8, n_bins=132300)
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
+        gpu_ctx.set_tuning()
+        ref = [api.render(gpu_ctx, sc, **kw), api.render(gpu_ctx, sc, src_tris=src_tris, **kw)]
+        for g, trav in ((2, 1), (6, 2)):
+            gpu_ctx.set_tuning(graph_iters=g, ray_chunk=3000, traversal=trav)
+            for rep in range(2):
+                for j, extra in enumerate(({}, {"src_tris": src_tris})):
+                    r = api.render(gpu_ctx, sc, **kw, **extra)
+                    np.testing.assert_array_equal(ref[j].raw, r.raw)
+                    np.testing.assert_array_equal(ref[j].abs_raw, r.abs_raw)
+                    np.testing.assert_array_equal(ref[j].value, r.value)
+                    assert r.stats["segments"] == ref[j].stats["segments"]
+        gpu_ctx.set_tuning()
