@@ -87,20 +87,23 @@ class ear_batch_item(C.Structure):
 class ear_tuning(C.Structure):
     _fields_ = [("block_extend", C.c_uint32), ("block_shade", C.c_uint32), ("block_shadow", C.c_uint32),
                 ("ray_chunk", C.c_uint32), ("waves_per_cu", C.c_uint32), ("stack_lds", C.c_uint32),
-                ("traversal", C.c_uint32), ("graph_iters", C.c_uint32)]
+                ("traversal", C.c_uint32), ("graph_iters", C.c_uint32), ("refill", C.c_uint32),
+                ("fetch_chunk", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
-EAR_TRAVERSAL_DEFAULT, EAR_TRAVERSAL_STATIC, EAR_TRAVERSAL_FETCH = 0, 1, 2
+EAR_TRAVERSAL_DEFAULT, EAR_TRAVERSAL_STATIC, EAR_TRAVERSAL_FETCH, EAR_TRAVERSAL_FETCH_ALL = 0, 1, 2, 3
 
 
 _lib = None
 
 
-def load(path: str = LIB_PATH):
-    """Loads libear_hip.so and declares the prototypes; raises if it is not built."""
+def load(path: str | None = None):
+    """Loads libear_hip.so (LIB_PATH unless a path is given) and declares the
+    prototypes; raises if it is not built."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or LIB_PATH
     if not os.path.exists(path):
         raise ImportError(f"{path} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
     L = C.CDLL(path)

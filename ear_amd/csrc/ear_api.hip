@@ -108,8 +108,12 @@ const char* ear_ctx_last_error(const ear_ctx* c) { return c ? c->last_error.c_st
 
 ear_status ear_ctx_set_tuning(ear_ctx* c, const ear_tuning* t) {
     if (!c || !t) return EAR_ERR_INVALID_ARGUMENT;
-    if (t->traversal > EAR_TRAVERSAL_FETCH) {
+    if (t->traversal > EAR_TRAVERSAL_FETCH_ALL) {
         c->last_error = "unknown traversal mode";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (t->refill > 64 || t->fetch_chunk > (1u << 20)) {
+        c->last_error = "refill must be 0..64 and fetch_chunk 0..2^20";
         return EAR_ERR_INVALID_ARGUMENT;
     }
     c->tuning = *t;

@@ -507,7 +507,7 @@ ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2
         c->last_error = "more than 2^24 - 1 triangles";
         return EAR_ERR_CAPACITY;
     }
-    if (leaf_size == 0) leaf_size = 4;
+    if (leaf_size == 0) leaf_size = 1;  // fastest on the 200k-triangle hall (node tests are cheaper than triangle tests)
     if (leaf_size > 16) {
         c->last_error = "leaf_size must be 1..16";
         return EAR_ERR_INVALID_ARGUMENT;

@@ -68,6 +68,16 @@ struct Traversal {
     uint32_t gthreads;  // persistent threads sharing gstack
 };
 
+// Lane refilling for the persistent traversal kernels: once at least `refill`
+// lanes of a wave are idle, the wave hands them queue entries from its claimed
+// run, claiming a new run with one atomic on `counter` (zero at launch) when
+// its run is used up: `chunk` entries, or (chunk = 0) one per idle lane.
+struct Fetch {
+    uint32_t* counter;
+    uint32_t refill;
+    uint32_t chunk;
+};
+
 // ---------------------------------------------------------------------------
 // wave helpers
 // ---------------------------------------------------------------------------
@@ -485,6 +495,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__
         first = plan[0];
         total = plan[1];
     }
+    uint32_t n_done = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < total; base += gridDim.x * kBlock) {
         uint32_t gid = first + base + threadIdx.x;
         const uint32_t end = first + total;
@@ -572,14 +583,15 @@ __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__
             q.d[slot] = rd;
             q.m[slot] = rm;
         }
-        wave_add_u64(stats + ST_RAYS, valid ? 1 : 0);
+        n_done += valid ? 1u : 0u;
     }
+    wave_add_u64(stats + ST_RAYS, n_done);
 }
 
 template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
 __global__ void __launch_bounds__(BLOCK) k_extend(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
                                                    Queue q, const uint32_t* __restrict__ n_q, int2* __restrict__ hits,
-                                                   unsigned long long* __restrict__ stats, uint32_t* /*fetch*/) {
+                                                   unsigned long long* __restrict__ stats, Fetch /*fetch*/) {
     __shared__ int lds[LDS * BLOCK];
     uint32_t n = *n_q;
     uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -613,6 +625,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
                                                    uint32_t* __restrict__ n_sq, uint32_t max_rec,
                                                    unsigned long long* __restrict__ stats) {
     uint32_t n = *n_in;
+    uint32_t n_done = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < n;
@@ -723,8 +736,9 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
             qout.d[slot] = nd;
             qout.m[slot] = nmeta;
         }
-        wave_add_u64(stats + ST_SEGMENTS, valid ? 1 : 0);
+        n_done += valid ? 1u : 0u;
     }
+    wave_add_u64(stats + ST_SEGMENTS, n_done);
 }
 
 // Scene::Connect + Mesh::LineIntersection (src/Scene.cpp:84-96,
@@ -732,7 +746,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
 template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
 __global__ void __launch_bounds__(BLOCK) k_shadow(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
                                                    ShadowQ sq, const uint32_t* __restrict__ n_sq,
-                                                   unsigned long long* __restrict__ stats, uint32_t* /*fetch*/) {
+                                                   unsigned long long* __restrict__ stats, Fetch /*fetch*/) {
     __shared__ int lds[LDS * BLOCK];
     uint32_t n = *n_sq;
     uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -775,16 +789,16 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(const InstDev* __restrict__ in
 // Persistent variants of k_extend / k_shadow: a lane whose walk has ended
 // takes the next queue entry, so a wave keeps its lanes busy instead of
 // waiting for its slowest ray (Aila & Laine, "Understanding the Efficiency of
-// Ray Traversal on GPUs", HPG 2009).  A wave fetches when at least kRefill of
-// its lanes are idle, with one atomic on `fetch` (zero at launch) for all of
-// them.  Each lane keeps its own traversal stack across the rays it takes.
-constexpr int kRefill = 16;
+// Ray Traversal on GPUs", HPG 2009).  A wave hands out entries when at least
+// fetch.refill of its lanes are idle, from a run of fetch.chunk entries it
+// claims with one atomic.  Each lane keeps its own traversal stack across the
+// rays it takes.
 
 template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
 __global__ void __launch_bounds__(BLOCK) k_extend_f(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
                                                      Queue q, const uint32_t* __restrict__ n_q,
                                                      int2* __restrict__ hits, unsigned long long* __restrict__ stats,
-                                                     uint32_t* fetch) {
+                                                     Fetch fetch) {
     __shared__ int lds[LDS * BLOCK];
     const uint32_t n = *n_q;
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -794,19 +808,28 @@ __global__ void __launch_bounds__(BLOCK) k_extend_f(const InstDev* __restrict__ 
     Traversal T = make_trav(inst[0], gstack, gthreads);
     bool active = false, exhausted = n == 0;
     uint32_t slot = 0;
+    uint32_t c_next = 0, c_end = 0;  // the wave's claimed entries [c_next, c_end)
     for (;;) {
         uint64_t act = __ballot(active);
         const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
-        if (!exhausted && n_idle >= (uint32_t)kRefill) {
+        if (!exhausted && (n_idle >= fetch.refill || act == 0)) {
+            if (c_next == c_end) {
+                // claim fetch.chunk entries, or exactly one per idle lane
+                const uint32_t want = fetch.chunk ? fetch.chunk : n_idle;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(fetch.counter, want);
+                base = __shfl(base, 0);
+                c_next = min(base, n);
+                c_end = min(base + want, n);
+                if (base + want >= n) exhausted = c_next == c_end;
+            }
             const uint64_t idle = ~act;
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(fetch, n_idle);
-            base = __shfl(base, leader);
-            if (base + n_idle >= n) exhausted = true;
+            const uint32_t avail = c_end - c_next;
+            const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+            const uint32_t mine = c_next + rank;
+            c_next += min(avail, n_idle);
             if (!active) {
-                const uint32_t mine = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (mine < n) {
+                if (rank < avail) {
                     slot = mine;
                     const float4 o = q.o[mine], d = q.d[mine];
                     if (BATCH) T = make_trav(inst[q.m[mine].w], gstack, gthreads);
@@ -836,7 +859,7 @@ __global__ void __launch_bounds__(BLOCK) k_extend_f(const InstDev* __restrict__ 
 template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
 __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
                                                      ShadowQ sq, const uint32_t* __restrict__ n_sq,
-                                                     unsigned long long* __restrict__ stats, uint32_t* fetch) {
+                                                     unsigned long long* __restrict__ stats, Fetch fetch) {
     __shared__ int lds[LDS * BLOCK];
     const uint32_t n = *n_sq;
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -846,20 +869,29 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
     Traversal T = make_trav(inst[0], gstack, gthreads);
     bool active = false, exhausted = n == 0;
     uint32_t slot = 0;
+    uint32_t c_next = 0, c_end = 0;  // the wave's claimed entries [c_next, c_end)
     for (;;) {
         uint64_t act = __ballot(active);
         const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
         bool done = false;
-        if (!exhausted && n_idle >= (uint32_t)kRefill) {
+        if (!exhausted && (n_idle >= fetch.refill || act == 0)) {
+            if (c_next == c_end) {
+                // claim fetch.chunk entries, or exactly one per idle lane
+                const uint32_t want = fetch.chunk ? fetch.chunk : n_idle;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(fetch.counter, want);
+                base = __shfl(base, 0);
+                c_next = min(base, n);
+                c_end = min(base + want, n);
+                if (base + want >= n) exhausted = c_next == c_end;
+            }
             const uint64_t idle = ~act;
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(fetch, n_idle);
-            base = __shfl(base, leader);
-            if (base + n_idle >= n) exhausted = true;
+            const uint32_t avail = c_end - c_next;
+            const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+            const uint32_t mine = c_next + rank;
+            c_next += min(avail, n_idle);
             if (!active) {
-                const uint32_t mine = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (mine < n) {
+                if (rank < avail) {
                     slot = mine;
                     ++nsh;
                     const float4 pc = sq.p[mine], m = sq.m[mine];
@@ -1079,9 +1111,8 @@ struct TravCfg {
     bool instr, batch, fetch;
 };
 
-using ExtendFn = void (*)(const InstDev*, int*, uint32_t, Queue, const uint32_t*, int2*, unsigned long long*,
-                          uint32_t*);
-using ShadowFn = void (*)(const InstDev*, int*, uint32_t, ShadowQ, const uint32_t*, unsigned long long*, uint32_t*);
+using ExtendFn = void (*)(const InstDev*, int*, uint32_t, Queue, const uint32_t*, int2*, unsigned long long*, Fetch);
+using ShadowFn = void (*)(const InstDev*, int*, uint32_t, ShadowQ, const uint32_t*, unsigned long long*, Fetch);
 
 template <int B, int L, int W, bool BA>
 void pick_t(bool instr, bool fetch, ExtendFn* e, ShadowFn* s) {
@@ -1392,12 +1423,15 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         return EAR_ERR_CAPACITY;
     }
     const ear_tuning& tu = c->tuning;
-    uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 22);
+    uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 23);
     uint64_t cap64 = std::min<uint64_t>((uint64_t)chunk * 3, std::max<uint64_t>(1, total_rays));
     const uint32_t cap_rays = (uint32_t)cap64;
-    const bool fetch = tu.traversal == EAR_TRAVERSAL_FETCH;
-    const TravCfg cfg_e{tu.block_extend == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch, fetch};
-    const TravCfg cfg_s{tu.block_shadow == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch, fetch};
+    // walks vary in length; refilling idle lanes keeps waves busy (both
+    // kernels by default)
+    const bool fetch_e = tu.traversal != EAR_TRAVERSAL_STATIC;
+    const bool fetch_s = tu.traversal == EAR_TRAVERSAL_FETCH_ALL || tu.traversal == EAR_TRAVERSAL_DEFAULT;
+    const TravCfg cfg_e{tu.block_extend == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch, fetch_e};
+    const TravCfg cfg_s{tu.block_shadow == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch, fetch_s};
     ExtendFn fe, fe_unused;
     ShadowFn fs, fs_unused;
     pick_kernels(cfg_e, &fe, &fs_unused);
@@ -1536,6 +1570,9 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
     const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
     const bool timed = tu.graph_iters == 0;
+    const uint32_t refill = tu.refill ? tu.refill : 32u;
+    const uint32_t fchunk = tu.fetch_chunk;  // 0: one entry per idle lane
+    const Fetch fq_e{cnt + 4, refill, fchunk}, fq_s{cnt + 5, refill, fchunk};
     // one iteration; cur = the queue it consumes
     auto iteration = [&](int cur) {
         uint32_t* nq = cnt + cur;
@@ -1551,7 +1588,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         EventPair* e1 = timed ? &mk(ev_ext) : nullptr;
         if (e1) hipEventRecord(e1->a, st);
         hipLaunchKernelGGL(fe, dim3(g_ext), dim3(cfg_e.block), 0, st, di, gsp, gthreads, qq[cur], (const uint32_t*)nq,
-                           hits.as<int2>(), stp, cnt + 4);
+                           hits.as<int2>(), stp, fq_e);
         if (e1) hipEventRecord(e1->b, st);
         EventPair* e2 = timed ? &mk(ev_sha) : nullptr;
         if (e2) hipEventRecord(e2->a, st);
@@ -1565,7 +1602,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         EventPair* e3 = timed ? &mk(ev_shw) : nullptr;
         if (e3) hipEventRecord(e3->a, st);
         hipLaunchKernelGGL(fs, dim3(g_shw), dim3(cfg_s.block), 0, st, di, gsp, gthreads, SQ, (const uint32_t*)(cnt + 2),
-                           stp, cnt + 5);
+                           stp, fq_s);
         if (e3) hipEventRecord(e3->b, st);
         k_reset4<<<1, 64, 0, st>>>(nq, cnt + 2, cnt + 4, cnt + 5);
         launches += 6;
@@ -1840,6 +1877,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
     const RenderParams& P = In.P;
     const uint32_t nm = A.n_mat;
     uint32_t n = *n_in;
+    uint32_t n_done = 0;
     uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     double g_refl[kAdMaxMat], g_spec[kAdMaxMat];
     for (int q = 0; q < kAdMaxMat; ++q) g_refl[q] = g_spec[q] = 0.0;
@@ -2041,8 +2079,9 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
                 tout.refl(q, slot) = drefl[q];
             }
         }
-        wave_add_u64(stats + ST_SEGMENTS, i < n ? 1 : 0);
+        n_done += i < n ? 1u : 0u;
     }
+    wave_add_u64(stats + ST_SEGMENTS, n_done);
     // one band per render call of the adjoint (enforced on the host)
     int band = __shfl(band_acc, 0);
     for (int l = 1; l < 64 && band < 0; ++l) band = __shfl(band_acc, l);
@@ -2183,7 +2222,8 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const 
             int cur = 0;
             for (uint32_t k = 1; k <= D->max_bounces && n_live > 0; ++k) {
                 hipLaunchKernelGGL(fe, dim3(grid_for(n_live, cap_e, 256)), dim3(256), 0, st, di, gs.as<int>(),
-                                   gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp, cnt + 4);
+                                   gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp,
+                                   Fetch{cnt + 4, 32u, 0u});
                 const uint32_t gsd = persistent_grid(c, n_live, waves);
                 if (s->width == 4)
                     k_shade_ad<4><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EAR_ABI_VERSION 1
+#define EAR_ABI_VERSION 2
 
 typedef enum {
     EAR_OK = 0,
@@ -78,7 +78,7 @@ typedef struct {
 } ear_material;
 
 typedef struct {
-    uint32_t leaf_size;  /* max triangles per BVH leaf, 1..16 (0 = library default) */
+    uint32_t leaf_size;  /* max triangles per BVH leaf, 1..16 (0 = library default, 1) */
     uint32_t width;      /* children per node: 2 or 4 (0 = library default, 4) */
     uint32_t reserved[2];
 } ear_bvh_opts;
@@ -252,17 +252,24 @@ typedef struct {
     uint32_t block_extend;
     uint32_t block_shade;
     uint32_t block_shadow;
-    uint32_t ray_chunk;      /* queue capacity = 3 x ray_chunk rays in flight; finished
-                                rays are replaced by new ones every iteration */
+    uint32_t ray_chunk;      /* queue capacity = 3 x ray_chunk rays in flight (default
+                                2^23); finished rays are replaced every iteration */
     uint32_t waves_per_cu;   /* persistent grid size */
     uint32_t stack_lds;      /* traversal stack entries kept in LDS (16 or 32) */
     uint32_t traversal;      /* EAR_TRAVERSAL_*: how BVH walks are assigned to lanes */
     uint32_t graph_iters;    /* > 0: bounce-loop iterations captured in one HIP graph and
                                 replayed (per-kernel times are then not measured) */
+    uint32_t refill;         /* refilling traversal: idle lanes (of 64) that trigger a refill
+                                (default 32) */
+    uint32_t fetch_chunk;    /* refilling traversal: queue entries a wave claims per atomic
+                                (0: one per idle lane) */
+    uint32_t reserved[2];
 } ear_tuning;
-#define EAR_TRAVERSAL_DEFAULT 0u
-#define EAR_TRAVERSAL_STATIC 1u  /* one queue entry per lane per grid-stride step */
-#define EAR_TRAVERSAL_FETCH 2u   /* persistent lanes take the next entry when their walk ends */
+#define EAR_TRAVERSAL_DEFAULT 0u   /* = EAR_TRAVERSAL_FETCH_ALL */
+#define EAR_TRAVERSAL_STATIC 1u    /* one queue entry per lane per grid-stride step */
+#define EAR_TRAVERSAL_FETCH 2u     /* closest hit: persistent lanes take the next entry when
+                                      their walk ends; shadow rays one per lane */
+#define EAR_TRAVERSAL_FETCH_ALL 3u /* both kernels refill lanes */
 ear_status ear_ctx_set_tuning(ear_ctx* ctx, const ear_tuning* t);
 ear_status ear_ctx_get_tuning(const ear_ctx* ctx, ear_tuning* t);
 

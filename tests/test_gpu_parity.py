@@ -150,17 +150,20 @@ def test_stereo_and_mesh_source_parity(gpu_ctx, oracle):
 def test_fetch_traversal_is_bitwise_identical(gpu_ctx, width):
     """Persistent lanes that refill from the queue walk every ray exactly as
     one-ray-per-lane launches do: same histograms, same node visits and
-    triangle tests, for both BVH widths, block sizes and stack splits, with a
-    stereo recorder and a mesh source (its k = 0 connections run the shadow
-    kernel before the first bounce)."""
+    triangle tests, for both BVH widths, block sizes, stack splits, refill
+    thresholds (1..64 idle lanes) and claim sizes (1..4096 entries), with a
+    stereo recorder and a mesh source (its k = 0 connections are shadow
+    requests of the first iteration)."""
     s = scenes.hall(target_tris=20_000)
     src_tris = np.array([[[4.0, 13.0, 1.0], [5.0, 13.0, 1.0], [4.0, 14.0, 1.2]]], np.float32)
     kw = dict(recorders=s["recorders"], n_rays=20_000, src_pos=s["src"], air=s["air"], max_bounces=100,
               cutoff=1e-8, n_bins=132300, instrument=True)
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], width=width) as sc:
         res = []
-        for trav, blk, lds in [(1, 256, 16), (2, 256, 16), (2, 128, 32)]:
-            gpu_ctx.set_tuning(traversal=trav, block_extend=blk, block_shadow=blk, stack_lds=lds)
+        for trav, blk, lds, refill, chunk in [(1, 256, 16, 0, 0), (2, 256, 16, 1, 1), (3, 128, 32, 64, 4096),
+                                              (3, 256, 16, 0, 0)]:
+            gpu_ctx.set_tuning(traversal=trav, block_extend=blk, block_shadow=blk, stack_lds=lds, refill=refill,
+                               fetch_chunk=chunk)
             res.append(api.render(gpu_ctx, sc, **kw))
             res.append(api.render(gpu_ctx, sc, src_tris=src_tris, **kw))
         gpu_ctx.set_tuning()
@@ -182,7 +185,7 @@ def test_fetch_traversal_batch(gpu_ctx):
                  for i, (r, sc) in enumerate(zip(rooms, objs))]
         gpu_ctx.set_tuning(traversal=1)
         a, _, _ = api.render_batch(gpu_ctx, items)
-        gpu_ctx.set_tuning(traversal=2)
+        gpu_ctx.set_tuning(traversal=3)
         b, _, _ = api.render_batch(gpu_ctx, items)
         gpu_ctx.set_tuning()
         for x, y in zip(a, b):
@@ -206,7 +209,7 @@ def test_graph_bounce_loop_is_bitwise_identical(gpu_ctx):
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
         gpu_ctx.set_tuning()
         ref = [api.render(gpu_ctx, sc, **kw), api.render(gpu_ctx, sc, src_tris=src_tris, **kw)]
-        for g, trav in ((2, 1), (6, 2)):
+        for g, trav in ((2, 1), (6, 3)):
             gpu_ctx.set_tuning(graph_iters=g, ray_chunk=3000, traversal=trav)
             for rep in range(2):
                 for j, extra in enumerate(({}, {"src_tris": src_tris})):
