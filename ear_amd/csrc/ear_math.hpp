@@ -214,12 +214,13 @@ EAR_HD V3 sample_hemi(Rng& r, V3 n) {
         return v;
     }
 }
-EAR_HD V3 sample_hemi_spec(Rng& r, V3 n, V3 refl, float f) {
-    V3 v = sample_hemi(r, n);
+// The specular blend of Sample_Hemi (src/Distributions.h:71-75).
+EAR_HD V3 blend_spec(V3 v, V3 refl, float f) {
     float g = 1.0f - f;
     V3 w{v.x * g + refl.x * f, v.y * g + refl.y * f, v.z * g + refl.z * f};
     return normalized(w);
 }
+EAR_HD V3 sample_hemi_spec(Rng& r, V3 n, V3 refl, float f) { return blend_spec(sample_hemi(r, n), refl, f); }
 
 // ---------------------------------------------------------------------------
 // 128-bit two's-complement fixed point (value * 2^88) for the histograms.

@@ -103,6 +103,30 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint64
     if (lane_id() == 0 && v) atomicAdd(counter, (unsigned long long)v);
 }
 
+// Lane refilling (see Fetch): call with the whole wave when !exhausted and at
+// least f.refill lanes are idle (or all are); `act` is the ballot of busy
+// lanes.  Returns the queue entry handed to this lane, or ~0u.
+__device__ __forceinline__ uint32_t take_entries(const Fetch& f, uint32_t n, uint64_t act, uint32_t lane,
+                                                 uint32_t& c_next, uint32_t& c_end, bool& exhausted) {
+    const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
+    if (c_next == c_end) {
+        // claim f.chunk entries, or exactly one per idle lane
+        const uint32_t want = f.chunk ? f.chunk : n_idle;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(f.counter, want);
+        base = __shfl(base, 0);
+        c_next = min(base, n);
+        c_end = min(base + want, n);
+        if (base + want >= n) exhausted = c_next == c_end;
+    }
+    const uint64_t idle = ~act;
+    const uint32_t avail = c_end - c_next;
+    const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+    const uint32_t mine = c_next + rank;
+    c_next += min(avail, n_idle);
+    return (((act >> lane) & 1ull) == 0 && rank < avail) ? mine : 0xFFFFFFFFu;
+}
+
 // ---------------------------------------------------------------------------
 // BVH traversal
 // ---------------------------------------------------------------------------
@@ -813,31 +837,15 @@ __global__ void __launch_bounds__(BLOCK) k_extend_f(const InstDev* __restrict__ 
         uint64_t act = __ballot(active);
         const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
         if (!exhausted && (n_idle >= fetch.refill || act == 0)) {
-            if (c_next == c_end) {
-                // claim fetch.chunk entries, or exactly one per idle lane
-                const uint32_t want = fetch.chunk ? fetch.chunk : n_idle;
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(fetch.counter, want);
-                base = __shfl(base, 0);
-                c_next = min(base, n);
-                c_end = min(base + want, n);
-                if (base + want >= n) exhausted = c_next == c_end;
-            }
-            const uint64_t idle = ~act;
-            const uint32_t avail = c_end - c_next;
-            const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-            const uint32_t mine = c_next + rank;
-            c_next += min(avail, n_idle);
-            if (!active) {
-                if (rank < avail) {
-                    slot = mine;
-                    const float4 o = q.o[mine], d = q.d[mine];
-                    if (BATCH) T = make_trav(inst[q.m[mine].w], gstack, gthreads);
-                    if (w.start(T, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 1000000.0f))
-                        active = true;
-                    else
-                        hits[mine] = make_int2(__float_as_int(1000000.0f), -1);
-                }
+            const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted);
+            if (mine != 0xFFFFFFFFu) {
+                slot = mine;
+                const float4 o = q.o[mine], d = q.d[mine];
+                if (BATCH) T = make_trav(inst[q.m[mine].w], gstack, gthreads);
+                if (w.start(T, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 1000000.0f))
+                    active = true;
+                else
+                    hits[mine] = make_int2(__float_as_int(1000000.0f), -1);
             }
             act = __ballot(active);
         }
@@ -875,41 +883,25 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
         const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
         bool done = false;
         if (!exhausted && (n_idle >= fetch.refill || act == 0)) {
-            if (c_next == c_end) {
-                // claim fetch.chunk entries, or exactly one per idle lane
-                const uint32_t want = fetch.chunk ? fetch.chunk : n_idle;
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(fetch.counter, want);
-                base = __shfl(base, 0);
-                c_next = min(base, n);
-                c_end = min(base + want, n);
-                if (base + want >= n) exhausted = c_next == c_end;
-            }
-            const uint64_t idle = ~act;
-            const uint32_t avail = c_end - c_next;
-            const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-            const uint32_t mine = c_next + rank;
-            c_next += min(avail, n_idle);
-            if (!active) {
-                if (rank < avail) {
-                    slot = mine;
-                    ++nsh;
-                    const float4 pc = sq.p[mine], m = sq.m[mine];
-                    const uint32_t packed = __float_as_uint(m.y);
-                    const uint32_t id = BATCH ? __float_as_uint(m.z) : 0;
-                    if (BATCH) T = make_trav(inst[id], gstack, gthreads);
-                    const RecParams& R = inst[id].P.rec[packed & 0xFFu];
-                    const V3 p{pc.x, pc.y, pc.z};
-                    if (w.start(T, p, sub(V3{R.px, R.py, R.pz}, p), 0.0f))
-                        active = true;
-                    else
-                        done = true;  // empty scene: visible
-                }
+            const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted);
+            if (mine != 0xFFFFFFFFu) {
+                slot = mine;
+                ++nsh;
+                const float4 pc = sq.p[mine], m = sq.m[mine];
+                const uint32_t packed = __float_as_uint(m.y);
+                const uint32_t id = BATCH ? __float_as_uint(m.z) : 0;
+                if (BATCH) T = make_trav(inst[id], gstack, gthreads);
+                const RecParams& R = inst[id].P.rec[packed & 0xFFu];
+                const V3 p{pc.x, pc.y, pc.z};
+                if (w.start(T, p, sub(V3{R.px, R.py, R.pz}, p), 0.0f))
+                    active = true;
+                else
+                    done = true;  // empty scene: visible
             }
             act = __ballot(active);
         }
-        // the loop is left by the whole wave at once (the fetch above reads
-        // a leader lane)
+        // the loop is left by the whole wave at once (take_entries is a
+        // wave-wide step)
         if (__ballot(active || done) == 0) {
             if (exhausted) break;
             continue;
@@ -960,15 +952,10 @@ __global__ void k_plan(uint32_t* cursor, uint32_t total, uint32_t cap, const uin
     }
 }
 
-// zeroes four counters (a kernel, not a memset node, so a captured bounce
-// loop replays it correctly)
-__global__ void k_reset4(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* d) {
-    if (threadIdx.x == 0) {
-        *a = 0;
-        *b = 0;
-        *c = 0;
-        *d = 0;
-    }
+// zeroes counters c[i] for the bits i of mask (a kernel, not a memset node,
+// so a captured bounce loop replays it correctly)
+__global__ void k_reset_mask(uint32_t* c, uint32_t mask) {
+    if (threadIdx.x < 32 && ((mask >> threadIdx.x) & 1u)) c[threadIdx.x] = 0;
 }
 
 // Direct sound (src/Scene.cpp:299-311): one block per instance, one thread
@@ -1460,9 +1447,9 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     size_t cap_sh = (size_t)cap_rays * std::max<size_t>(1, max_recs) * (any_mesh ? 2 : 1);
     TRY(so.alloc(cap_sh * 16));
     TRY(sm.alloc(cap_sh * 16));
-    // queue 0, queue 1, shadow queue, generation cursor, extend fetch, shadow
-    // fetch, plan {first, count}
-    TRY(counters.alloc(8 * 4));
+    // [0] queue 0, [1] queue 1, [2] shadow queue, [3] generation cursor,
+    // [4] extend fetch, [5] shadow fetch, [6..7] plan {first, count}
+    TRY(counters.alloc(16 * 4));
     TRY(stats.alloc(ST_N * 8));
     TRY(hist.alloc(hist_cells * 16));
     uint32_t gthreads = std::max(cap_e * (uint32_t)cfg_e.block, cap_s * (uint32_t)cfg_s.block);
@@ -1604,7 +1591,8 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         hipLaunchKernelGGL(fs, dim3(g_shw), dim3(cfg_s.block), 0, st, di, gsp, gthreads, SQ, (const uint32_t*)(cnt + 2),
                            stp, fq_s);
         if (e3) hipEventRecord(e3->b, st);
-        k_reset4<<<1, 64, 0, st>>>(nq, cnt + 2, cnt + 4, cnt + 5);
+        // the consumed queue, the shadow queue and the fetch counters
+        k_reset_mask<<<1, 64, 0, st>>>(cnt, (1u << cur) | 0x34u);
         launches += 6;
         ++bounces;
     };
