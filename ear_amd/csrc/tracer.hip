@@ -383,9 +383,8 @@ __device__ __forceinline__ void acc128(unsigned long long* cell, I128 q) {
 }
 
 // Ramp of values (w - i) * A / w at bins s + i, i in [0, w), bins < 0 and
-// >= nb dropped; h points at the channel's signed row, habs (or null) at its
-// |.| row.
-__device__ void splat_ramp(unsigned long long* h, unsigned long long* habs, uint32_t nb, int s, int w, float A) {
+// >= nb dropped, added to the second-difference row h.
+__device__ void splat_ramp(unsigned long long* h, uint32_t nb, int s, int w, float A) {
     if (w <= 0) return;
     double S = (double)A / (double)w;
     int ww = w;
@@ -402,18 +401,13 @@ __device__ void splat_ramp(unsigned long long* h, unsigned long long* habs, uint
     acc128(h + 2 * (size_t)s0, aq);
     if (s0 + 1 < nb) acc128(h + 2 * (size_t)(s0 + 1), p1);
     if (s0 + (uint32_t)ww + 1 < nb) acc128(h + 2 * (size_t)(s0 + ww + 1), sq);
-    if (habs) {
-        if (sq.hi < 0) {
-            sq = i128_neg(sq);
-            aq = i128_neg(aq);
-            p1 = i128_neg(p1);
-        }
-        acc128(habs + 2 * (size_t)s0, aq);
-        if (s0 + 1 < nb) acc128(habs + 2 * (size_t)(s0 + 1), p1);
-        if (s0 + (uint32_t)ww + 1 < nb) acc128(habs + 2 * (size_t)(s0 + ww + 1), sq);
-    }
 }
 
+// Row layout of the accumulators.  Without the |c| histogram one signed row
+// per channel.  With it, contributions are split by sign: row 0 takes the
+// ramps of c >= 0, row 1 the ramps of |c| for c < 0, so each contribution
+// updates one row; the finalize step turns them into signed = P - N and
+// |.| = P + N (exact: quantisation is odd-symmetric, the sums are integers).
 struct Hist {
     unsigned long long* cells;  // [band slot][channel][kind][bin] x {lo, hi}
     uint32_t n_bins, n_ch, kinds;
@@ -425,13 +419,14 @@ struct Hist {
 // Recorder::Record: MonoRecorder (src/MonoRecorder.cpp:83-97) or
 // StereoRecorder (src/StereoRecorder.cpp:97-130).
 __device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, float a, float t, float dist) {
-    bool ab = H.kinds > 1;
+    const bool split = H.kinds > 1;
+    const uint32_t kind = split && a < 0.0f ? 1u : 0u;
     if (R.kind == EAR_REC_MONO) {
         int s = (int)((double)t * 44100.0);
         float width = sqrtf(dist);
         float ampl = 2.0f * a / width;
         int w = (int)ceilf(width);
-        splat_ramp(H.row(band, R.channel, 0), ab ? H.row(band, R.channel, 1) : nullptr, H.n_bins, s, w, ampl);
+        splat_ramp(H.row(band, R.channel, kind), H.n_bins, s, w, split ? fabsf(ampl) : ampl);
     } else {
         float dt = dot(dir, V3{R.rx, R.ry, R.rz});
         float td = R.head_size / 343.0f;
@@ -446,9 +441,8 @@ __device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, floa
         else
             al *= factor * factor;
         int w = (int)ceilf(width);
-        splat_ramp(H.row(band, R.channel, 0), ab ? H.row(band, R.channel, 1) : nullptr, H.n_bins, s_left, w, al);
-        splat_ramp(H.row(band, R.channel + 1, 0), ab ? H.row(band, R.channel + 1, 1) : nullptr, H.n_bins, s_right, w,
-                   ar);
+        splat_ramp(H.row(band, R.channel, kind), H.n_bins, s_left, w, split ? fabsf(al) : al);
+        splat_ramp(H.row(band, R.channel + 1, kind), H.n_bins, s_right, w, split ? fabsf(ar) : ar);
     }
 }
 
@@ -1045,19 +1039,45 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(const InstDev* __restric
         }
         __syncthreads();
     }
-    if (In.value) {
-        const DirectRamp dr = In.direct[(size_t)band * n_ch + ch];
-        double S = dr.visible ? (double)dr.a / (double)dr.w : 0.0;
-        for (uint32_t j = b; j < e; ++j) {
-            double v = (double)(int64_t)c[2 * j + 1] * 18446744073709551616.0 + (double)c[2 * j];
-            v = v * 3.2311742677852644e-27;  // 2^-88
-            v = v * In.inv_n;
-            if (dr.visible && (int)j >= dr.s && (int)j < dr.s + dr.w) {
-                double dv = (double)(dr.w - ((int)j - dr.s)) * S;
-                v += kind == 1 ? fabs(dv) : dv;
-            }
-            In.value[roff + j] = v * In.gain2;
+}
+
+// Per (instance, band, channel) and bin: the sign-split rows (if any) become
+// the signed and |.| sums, then the finished track: 2^-88, 1/N, the direct
+// sound (src/Scene.cpp:299-311) and gain^2 (src/Scene.cpp:313-316).
+__global__ void __launch_bounds__(256) k_values(const InstDev* __restrict__ inst, const uint4* __restrict__ rows) {
+    const uint4 rw = rows[blockIdx.y];
+    const InstDev& In = inst[rw.x];
+    const RenderParams& P = In.P;
+    const int band = (int)rw.y;
+    const uint32_t ch = rw.z, nb = P.n_bins, n_ch = P.n_channels, kinds = P.hist_kinds;
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= nb) return;
+    const size_t r0 = (((size_t)band * n_ch + ch) * kinds) * nb;
+    unsigned long long* c0 = In.hist + 2 * r0;
+    I128 v[2];
+    v[0] = I128{c0[2 * j], (int64_t)c0[2 * j + 1]};
+    if (kinds > 1) {
+        unsigned long long* c1 = c0 + 2 * (size_t)nb;
+        const I128 pos = v[0], neg = I128{c1[2 * j], (int64_t)c1[2 * j + 1]};
+        v[0] = i128_add(pos, i128_neg(neg));
+        v[1] = i128_add(pos, neg);
+        c0[2 * j] = v[0].lo;
+        c0[2 * j + 1] = (unsigned long long)v[0].hi;
+        c1[2 * j] = v[1].lo;
+        c1[2 * j + 1] = (unsigned long long)v[1].hi;
+    }
+    if (!In.value) return;
+    const DirectRamp dr = In.direct[(size_t)band * n_ch + ch];
+    const double S = dr.visible ? (double)dr.a / (double)dr.w : 0.0;
+    for (uint32_t kind = 0; kind < kinds; ++kind) {
+        double x = (double)v[kind].hi * 18446744073709551616.0 + (double)v[kind].lo;
+        x = x * 3.2311742677852644e-27;  // 2^-88
+        x = x * In.inv_n;
+        if (dr.visible && (int)j >= dr.s && (int)j < dr.s + dr.w) {
+            const double dv = (double)(dr.w - ((int)j - dr.s)) * S;
+            x += kind == 1 ? fabs(dv) : dv;
         }
+        In.value[r0 + (size_t)kind * nb + j] = x * In.gain2;
     }
 }
 
@@ -1435,7 +1455,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     hipStream_t st = c->stream;
     DevBuf qo[2] = {DevBuf(c), DevBuf(c)}, qd[2] = {DevBuf(c), DevBuf(c)}, qm[2] = {DevBuf(c), DevBuf(c)};
     DevBuf hits(c), so(c), sm(c), counters(c), stats(c), hist(c), gs(c), direct(c), value(c), logh(c), logn(c),
-        dinst(c), dsegs(c), dsegoff(c), drows(c), msrc(c);
+        dinst(c), dsegs(c), dsegoff(c), drows(c), dvrows(c), msrc(c);
     for (int k = 0; k < 2; ++k) {
         TRY(qo[k].alloc((size_t)cap_rays * 16));
         TRY(qd[k].alloc((size_t)cap_rays * 16));
@@ -1661,6 +1681,19 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     TRY(drows.alloc(sizeof(uint4) * rows.size()));
     EAR_HIP_TRY(c, hipMemcpyAsync(drows.p, rows.data(), sizeof(uint4) * rows.size(), hipMemcpyHostToDevice, st));
     k_finalize<<<(uint32_t)rows.size(), kFinBlock, 0, st>>>(di, drows.as<uint4>());
+    {
+        // one row per (instance, band, channel) for the combination step
+        std::vector<uint4> vrows;
+        uint32_t max_nb = 1;
+        for (const uint4& r : rows)
+            if (r.w == 0) {
+                vrows.push_back(r);
+                max_nb = std::max(max_nb, inst[r.x].P.n_bins);
+            }
+        TRY(dvrows.alloc(sizeof(uint4) * vrows.size()));
+        EAR_HIP_TRY(c, hipMemcpyAsync(dvrows.p, vrows.data(), sizeof(uint4) * vrows.size(), hipMemcpyHostToDevice, st));
+        k_values<<<dim3((max_nb + 255) / 256, (uint32_t)vrows.size()), 256, 0, st>>>(di, dvrows.as<uint4>());
+    }
     EAR_HIP_TRY(c, hipGetLastError());
     EAR_HIP_TRY(c, hipEventRecord(t1, st));
     EAR_HIP_TRY(c, hipStreamSynchronize(st));
