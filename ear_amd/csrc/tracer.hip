@@ -215,11 +215,17 @@ __device__ __forceinline__ void cswap(float& ka, int& va, float& kb, int& vb) {
     }
 }
 
-// One ray's BVH walk as a resumable state machine: step() visits one node
-// (its child boxes, and the triangles of the leaf children it reaches) and
-// picks the next node, so a persistent lane can interleave walks of rays it
-// fetches one after another.  Closest hit (ANY = false): best_* hold the
-// nearest hit so far.  Any hit: `hit` is set when a blocker is found.
+// One ray's BVH walk as a resumable state machine, so that a persistent lane
+// can interleave the walks of rays it fetches one after another.  4-wide
+// tree: a step either tests one pending leaf (nearest first; for the closest
+// hit a leaf whose box starts beyond the best hit so far is dropped) or
+// visits one internal node, whose hit children are sorted by entry distance:
+// leaves become pending, the nearest internal child is visited next and the
+// others are pushed.  One leaf per step keeps a wave's triangle tests in one
+// code path instead of one per child slot.  2-wide tree: a step visits one
+// node and tests its leaf children at once.  Closest hit (ANY = false):
+// best_* hold the nearest hit so far (order independent: ties go to the lower
+// original index).  Any hit: `found` is set when a blocker is found.
 template <int LDS, int BLOCK, int WIDTH, bool ANY, bool INSTR>
 struct Walker {
     SlabRay sr;
@@ -227,8 +233,11 @@ struct Walker {
     float best_t;
     int best_s;
     uint32_t best_o;
-    int node;
+    int node;  // next internal node to visit, -1: pop one
     bool found;
+    int npl;                 // pending leaves (4-wide tree), sorted by entry distance
+    int pl0, pl1, pl2, pl3;  // their references
+    float pt0, pt1, pt2, pt3;
     Stack<LDS, BLOCK> stk;
 
     __device__ __forceinline__ Walker(int* lds, int* gstack, uint32_t gtid, uint32_t gthreads)
@@ -244,116 +253,127 @@ struct Walker {
         best_o = 0;
         node = 0;
         found = false;
+        npl = 0;
         stk.sp = 0;
         return T.n_tris != 0;
     }
 
-    // Visits `node`; returns true when the walk has ended (found / best_*
-    // final, or for ANY a blocker found: found = true).
+    __device__ __forceinline__ bool exhausted() const { return npl == 0 && node < 0 && stk.sp == 0; }
+
+    // One step; returns true when the walk has ended (found / best_* final,
+    // or for ANY a blocker found: found = true).
     __device__ __forceinline__ bool step(const Traversal& T, uint32_t& nvis, uint32_t& ntri) {
-        if (INSTR) ++nvis;
-        if (WIDTH == 2) {
-            const BvhNode* N = T.nodes + node;
-            float4 a = N->a, b = N->b, c = N->c;
-            int4 ch = N->d;
-            float tn0, tf0, tn1, tf1;
-            slab(sr, a.x, a.y, a.z, a.w, b.x, b.y, tn0, tf0);
-            slab(sr, b.z, b.w, c.x, c.y, c.z, c.w, tn1, tf1);
-            float lim = ANY ? 1.0f : best_t;
-            bool h0 = tn0 <= tf0 && tf0 >= 0.0f && tn0 <= lim;
-            bool h1 = tn1 <= tf1 && tf1 >= 0.0f && tn1 <= lim;
-            if (h0 && ch.x < 0) {
-                if (test_leaf<ANY, INSTR>(T, ch.x, o, d, best_t, best_s, best_o, found, ntri)) {
+        if (WIDTH == 2) return step2(T, nvis, ntri);
+        if (npl > 0) {
+            const int ref = pl0;
+            const float tn = pt0;
+            pl0 = pl1;
+            pl1 = pl2;
+            pl2 = pl3;
+            pt0 = pt1;
+            pt1 = pt2;
+            pt2 = pt3;
+            --npl;
+            if (ANY || tn <= best_t) {
+                if (test_leaf<ANY, INSTR>(T, ref, o, d, best_t, best_s, best_o, found, ntri)) {
                     found = true;
                     return true;
                 }
-                h0 = false;
             }
-            if (h1 && ch.y < 0) {
-                if (test_leaf<ANY, INSTR>(T, ch.y, o, d, best_t, best_s, best_o, found, ntri)) {
-                    found = true;
-                    return true;
-                }
-                h1 = false;
-            }
-            if (h0 && h1) {
-                bool first0 = tn0 <= tn1;
-                stk.push(first0 ? ch.y : ch.x);
-                node = first0 ? ch.x : ch.y;
-                return false;
-            }
-            if (h0) {
-                node = ch.x;
-                return false;
-            }
-            if (h1) {
-                node = ch.y;
-                return false;
-            }
-        } else {
-            const Bvh4Node* N = T.nodes4 + node;
-            float4 lx = N->lox, hx = N->hix, ly = N->loy, hy = N->hiy, lz = N->loz, hz = N->hiz;
-            int4 rf = N->ref;
-            float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-            float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-            float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-            int r[4] = {rf.x, rf.y, rf.z, rf.w};
-            float tn[4];
-            bool h[4];
-            float lim = ANY ? 1.0f : best_t;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                float tf;
-                slab(sr, lxa[k], lya[k], lza[k], hxa[k], hya[k], hza[k], tn[k], tf);
-                h[k] = tn[k] <= tf && tf >= 0.0f && tn[k] <= lim && r[k] != kEmpty4;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (h[k] && r[k] < 0) {
-                    if (test_leaf<ANY, INSTR>(T, r[k], o, d, best_t, best_s, best_o, found, ntri)) {
-                        found = true;
-                        return true;
-                    }
-                    h[k] = false;
-                }
-            }
-            // internal children still worth a visit, nearest first
-            float key[4];
-            int val[4];
-            int nh = 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                bool keep = h[k] && (ANY || tn[k] <= best_t);
-                key[k] = keep ? tn[k] : INFINITY;
-                val[k] = r[k];
-                nh += keep ? 1 : 0;
-            }
-            if (nh > 0) {
-                if (!ANY) {
-                    cswap(key[0], val[0], key[1], val[1]);
-                    cswap(key[2], val[2], key[3], val[3]);
-                    cswap(key[0], val[0], key[2], val[2]);
-                    cswap(key[1], val[1], key[3], val[3]);
-                    cswap(key[1], val[1], key[2], val[2]);
-                } else {
-                    // compact the kept children to the front
-                    int j = 0;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        if (key[k] != INFINITY) {
-                            val[j] = val[k];
-                            ++j;
-                        }
-                    }
-                }
-                for (int k = nh - 1; k >= 1; --k) stk.push(val[k]);
-                node = val[0];
-                return false;
-            }
+            return exhausted();
         }
-        if (stk.sp == 0) return true;
-        node = stk.pop();
-        return false;
+        if (node < 0) node = stk.pop();  // not exhausted: the stack holds a node
+        if (INSTR) ++nvis;
+        const Bvh4Node* N = T.nodes4 + node;
+        float4 lx = N->lox, hx = N->hix, ly = N->loy, hy = N->hiy, lz = N->loz, hz = N->hiz;
+        int4 rf = N->ref;
+        float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+        float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+        float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+        int r[4] = {rf.x, rf.y, rf.z, rf.w};
+        float key[4];
+        const float lim = ANY ? 1.0f : best_t;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float tn, tf;
+            slab(sr, lxa[k], lya[k], lza[k], hxa[k], hya[k], hza[k], tn, tf);
+            const bool h = tn <= tf && tf >= 0.0f && tn <= lim && r[k] != kEmpty4;
+            key[k] = h ? tn : INFINITY;
+        }
+        // nearest first
+        cswap(key[0], r[0], key[1], r[1]);
+        cswap(key[2], r[2], key[3], r[3]);
+        cswap(key[0], r[0], key[2], r[2]);
+        cswap(key[1], r[1], key[3], r[3]);
+        cswap(key[1], r[1], key[2], r[2]);
+        // leaves to the pending list (nearest first)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (key[k] == INFINITY || r[k] >= 0) continue;
+            if (npl == 0) {
+                pl0 = r[k];
+                pt0 = key[k];
+            } else if (npl == 1) {
+                pl1 = r[k];
+                pt1 = key[k];
+            } else if (npl == 2) {
+                pl2 = r[k];
+                pt2 = key[k];
+            } else {
+                pl3 = r[k];
+                pt3 = key[k];
+            }
+            ++npl;
+        }
+        // internal children: the nearest is visited next, the others pushed
+        // farthest first
+        node = -1;
+#pragma unroll
+        for (int k = 3; k >= 0; --k) {
+            if (key[k] == INFINITY || r[k] < 0) continue;
+            if (node >= 0) stk.push(node);
+            node = r[k];
+        }
+        return exhausted();
+    }
+
+    __device__ __forceinline__ bool step2(const Traversal& T, uint32_t& nvis, uint32_t& ntri) {
+        if (node < 0) node = stk.pop();
+        if (INSTR) ++nvis;
+        const BvhNode* N = T.nodes + node;
+        float4 a = N->a, b = N->b, c = N->c;
+        int4 ch = N->d;
+        float tn0, tf0, tn1, tf1;
+        slab(sr, a.x, a.y, a.z, a.w, b.x, b.y, tn0, tf0);
+        slab(sr, b.z, b.w, c.x, c.y, c.z, c.w, tn1, tf1);
+        float lim = ANY ? 1.0f : best_t;
+        bool h0 = tn0 <= tf0 && tf0 >= 0.0f && tn0 <= lim;
+        bool h1 = tn1 <= tf1 && tf1 >= 0.0f && tn1 <= lim;
+        if (h0 && ch.x < 0) {
+            if (test_leaf<ANY, INSTR>(T, ch.x, o, d, best_t, best_s, best_o, found, ntri)) {
+                found = true;
+                return true;
+            }
+            h0 = false;
+        }
+        if (h1 && ch.y < 0) {
+            if (test_leaf<ANY, INSTR>(T, ch.y, o, d, best_t, best_s, best_o, found, ntri)) {
+                found = true;
+                return true;
+            }
+            h1 = false;
+        }
+        node = -1;
+        if (h0 && h1) {
+            bool first0 = tn0 <= tn1;
+            stk.push(first0 ? ch.y : ch.x);
+            node = first0 ? ch.x : ch.y;
+        } else if (h0) {
+            node = ch.x;
+        } else if (h1) {
+            node = ch.y;
+        }
+        return node < 0 && stk.sp == 0;
     }
 };
 
