@@ -130,6 +130,20 @@ __device__ __forceinline__ uint32_t take_entries(const Fetch& f, uint32_t n, uin
 // ---------------------------------------------------------------------------
 // BVH traversal
 // ---------------------------------------------------------------------------
+// Loads through a pointer known to address global memory (the scene arrays
+// reach the kernels inside structs, where the compiler would otherwise emit
+// flat loads that also wait on the LDS counter).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldg(const float4* p) {
+    const f32x4 v = *(const __attribute__((address_space(1))) f32x4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 ldg(const int4* p) {
+    const i32x4 v = *(const __attribute__((address_space(1))) i32x4*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ float safe_inv(float x) {
     float y = fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x);
     return 1.0f / y;
@@ -158,16 +172,16 @@ struct Stack {
     int* gstack;
     uint32_t gtid, gthreads;
     int sp = 0;
+    __device__ __forceinline__ int* slot(int i) const {
+        return i < LDS ? lds + i * BLOCK : gstack + (size_t)(i - LDS) * gthreads + gtid;
+    }
     __device__ __forceinline__ void push(int v) {
-        if (sp < LDS)
-            lds[sp * BLOCK] = v;
-        else
-            gstack[(size_t)(sp - LDS) * gthreads + gtid] = v;
+        *slot(sp) = v;
         ++sp;
     }
     __device__ __forceinline__ int pop() {
         --sp;
-        return sp < LDS ? lds[sp * BLOCK] : gstack[(size_t)(sp - LDS) * gthreads + gtid];
+        return *slot(sp);
     }
 };
 
@@ -183,7 +197,7 @@ __device__ __forceinline__ bool test_leaf(const Traversal& T, int32_t cref, V3 o
     uint32_t first = u & 0x00FFFFFFu;
     uint32_t cnt = ((u >> 24) & 0x7Fu) + 1;
     for (uint32_t i = first; i < first + cnt; ++i) {
-        float4 t0 = T.tris[3 * i], t1 = T.tris[3 * i + 1], t2 = T.tris[3 * i + 2];
+        float4 t0 = ldg(T.tris + 3 * i), t1 = ldg(T.tris + 3 * i + 1), t2 = ldg(T.tris + 3 * i + 2);
         V3 v0{t0.x, t0.y, t0.z}, e1{t0.w, t1.x, t1.y}, e2{t1.z, t1.w, t2.x};
         float t;
         if (INSTR) ++ntri;
@@ -204,6 +218,13 @@ __device__ __forceinline__ bool test_leaf(const Traversal& T, int32_t cref, V3 o
     return false;
 }
 
+// a_k by masks (a chain of selects on k would be turned into an indexed
+// array in scratch memory)
+__device__ __forceinline__ uint32_t sel4(int k, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) {
+    return (a0 & (0u - (uint32_t)(k == 0))) | (a1 & (0u - (uint32_t)(k == 1))) | (a2 & (0u - (uint32_t)(k == 2))) |
+           (a3 & (0u - (uint32_t)(k == 3)));
+}
+
 __device__ __forceinline__ void cswap(float& ka, int& va, float& kb, int& vb) {
     if (kb < ka) {
         float tk = ka;
@@ -220,8 +241,8 @@ __device__ __forceinline__ void cswap(float& ka, int& va, float& kb, int& vb) {
 // tree: a step either tests one pending leaf (nearest first; for the closest
 // hit a leaf whose box starts beyond the best hit so far is dropped) or
 // visits one internal node, whose hit children are sorted by entry distance:
-// leaves become pending, the nearest internal child is visited next and the
-// others are pushed.  One leaf per step keeps a wave's triangle tests in one
+// leaves become pending (a bit mask over the sorted children), the nearest
+// internal child is visited next and the others are pushed.  One leaf per step keeps a wave's triangle tests in one
 // code path instead of one per child slot.  2-wide tree: a step visits one
 // node and tests its leaf children at once.  Closest hit (ANY = false):
 // best_* hold the nearest hit so far (order independent: ties go to the lower
@@ -235,9 +256,9 @@ struct Walker {
     uint32_t best_o;
     int node;  // next internal node to visit, -1: pop one
     bool found;
-    int npl;                 // pending leaves (4-wide tree), sorted by entry distance
-    int pl0, pl1, pl2, pl3;  // their references
-    float pt0, pt1, pt2, pt3;
+    uint32_t pend;             // 4-wide tree: bit k set = child k of the last node is a leaf to test
+    int pr0, pr1, pr2, pr3;    // that node's children, sorted by entry distance
+    float pk0, pk1, pk2, pk3;  // and their entry distances
     Stack<LDS, BLOCK> stk;
 
     __device__ __forceinline__ Walker(int* lds, int* gstack, uint32_t gtid, uint32_t gthreads)
@@ -253,27 +274,23 @@ struct Walker {
         best_o = 0;
         node = 0;
         found = false;
-        npl = 0;
+        pend = 0;
         stk.sp = 0;
         return T.n_tris != 0;
     }
 
-    __device__ __forceinline__ bool exhausted() const { return npl == 0 && node < 0 && stk.sp == 0; }
+    __device__ __forceinline__ bool exhausted() const { return pend == 0 && node < 0 && stk.sp == 0; }
 
     // One step; returns true when the walk has ended (found / best_* final,
     // or for ANY a blocker found: found = true).
     __device__ __forceinline__ bool step(const Traversal& T, uint32_t& nvis, uint32_t& ntri) {
         if (WIDTH == 2) return step2(T, nvis, ntri);
-        if (npl > 0) {
-            const int ref = pl0;
-            const float tn = pt0;
-            pl0 = pl1;
-            pl1 = pl2;
-            pl2 = pl3;
-            pt0 = pt1;
-            pt1 = pt2;
-            pt2 = pt3;
-            --npl;
+        if (pend) {
+            const int k = __builtin_ctz(pend);  // nearest pending leaf
+            pend &= pend - 1u;
+            const int ref = (int)sel4(k, (uint32_t)pr0, (uint32_t)pr1, (uint32_t)pr2, (uint32_t)pr3);
+            const float tn = __uint_as_float(sel4(k, __float_as_uint(pk0), __float_as_uint(pk1),
+                                                  __float_as_uint(pk2), __float_as_uint(pk3)));
             if (ANY || tn <= best_t) {
                 if (test_leaf<ANY, INSTR>(T, ref, o, d, best_t, best_s, best_o, found, ntri)) {
                     found = true;
@@ -285,8 +302,9 @@ struct Walker {
         if (node < 0) node = stk.pop();  // not exhausted: the stack holds a node
         if (INSTR) ++nvis;
         const Bvh4Node* N = T.nodes4 + node;
-        float4 lx = N->lox, hx = N->hix, ly = N->loy, hy = N->hiy, lz = N->loz, hz = N->hiz;
-        int4 rf = N->ref;
+        float4 lx = ldg(&N->lox), hx = ldg(&N->hix), ly = ldg(&N->loy), hy = ldg(&N->hiy), lz = ldg(&N->loz),
+               hz = ldg(&N->hiz);
+        int4 rf = ldg(&N->ref);
         float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
         float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
         float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
@@ -306,25 +324,17 @@ struct Walker {
         cswap(key[0], r[0], key[2], r[2]);
         cswap(key[1], r[1], key[3], r[3]);
         cswap(key[1], r[1], key[2], r[2]);
-        // leaves to the pending list (nearest first)
+        // leaves become pending (tested nearest first)
+        pr0 = r[0];
+        pr1 = r[1];
+        pr2 = r[2];
+        pr3 = r[3];
+        pk0 = key[0];
+        pk1 = key[1];
+        pk2 = key[2];
+        pk3 = key[3];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (key[k] == INFINITY || r[k] >= 0) continue;
-            if (npl == 0) {
-                pl0 = r[k];
-                pt0 = key[k];
-            } else if (npl == 1) {
-                pl1 = r[k];
-                pt1 = key[k];
-            } else if (npl == 2) {
-                pl2 = r[k];
-                pt2 = key[k];
-            } else {
-                pl3 = r[k];
-                pt3 = key[k];
-            }
-            ++npl;
-        }
+        for (int k = 0; k < 4; ++k) pend |= (key[k] != INFINITY && r[k] < 0) ? (1u << k) : 0u;
         // internal children: the nearest is visited next, the others pushed
         // farthest first
         node = -1;
@@ -341,8 +351,8 @@ struct Walker {
         if (node < 0) node = stk.pop();
         if (INSTR) ++nvis;
         const BvhNode* N = T.nodes + node;
-        float4 a = N->a, b = N->b, c = N->c;
-        int4 ch = N->d;
+        float4 a = ldg(&N->a), b = ldg(&N->b), c = ldg(&N->c);
+        int4 ch = ldg(&N->d);
         float tn0, tf0, tn1, tf1;
         slab(sr, a.x, a.y, a.z, a.w, b.x, b.y, tn0, tf0);
         slab(sr, b.z, b.w, c.x, c.y, c.z, c.w, tn1, tf1);
