@@ -37,13 +37,17 @@ class ear_material(C.Structure):
 
 
 class ear_bvh_opts(C.Structure):
-    _fields_ = [("leaf_size", C.c_uint32), ("width", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+    _fields_ = [("leaf_size", C.c_uint32), ("width", C.c_uint32), ("builder", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+EAR_BVH_DEFAULT, EAR_BVH_LBVH, EAR_BVH_SAH = 0, 1, 2
 
 
 class ear_scene_info(C.Structure):
     _fields_ = [("n_tris", C.c_uint32), ("n_nodes", C.c_uint32), ("max_depth", C.c_uint32),
                 ("leaf_size", C.c_uint32), ("bounds", C.c_float * 6), ("build_ms", C.c_double),
-                ("width", C.c_uint32), ("node_bytes", C.c_uint32), ("collapse_ms", C.c_double)]
+                ("width", C.c_uint32), ("node_bytes", C.c_uint32), ("collapse_ms", C.c_double),
+                ("builder", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class ear_recorder(C.Structure):

@@ -95,7 +95,7 @@ def make_materials(specs):
 class Scene:
     """ear_scene: the merged triangle soup on the device with its LBVH."""
 
-    def __init__(self, ctx: Context, tris, mat_ids, materials, leaf_size: int = 0, width: int = 0):
+    def __init__(self, ctx: Context, tris, mat_ids, materials, leaf_size: int = 0, width: int = 0, builder: int = 0):
         self.ctx = ctx
         lib = ctx.lib
         t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 3, 3)
@@ -106,7 +106,7 @@ class Scene:
         v2 = np.ascontiguousarray(t[:, 2, :])
         m = np.ascontiguousarray(mat_ids, dtype=np.uint16).reshape(-1)
         mats = make_materials(materials)
-        opts = F.ear_bvh_opts(leaf_size, width)
+        opts = F.ear_bvh_opts(leaf_size, width, builder)
         h = C.c_void_p()
         _check(lib, lib.ear_scene_upload(ctx.h, _ptr(v0), _ptr(v1), _ptr(v2), _ptr(m), self.n_tris, mats,
                                          len(materials), C.byref(opts), C.byref(h)), ctx.h)
@@ -128,7 +128,7 @@ class Scene:
         _check(self.ctx.lib, self.ctx.lib.ear_scene_get_info(self.h, C.byref(i)), self.ctx.h)
         return dict(n_tris=i.n_tris, n_nodes=i.n_nodes, max_depth=i.max_depth, leaf_size=i.leaf_size,
                     bounds=list(i.bounds), build_ms=i.build_ms, width=i.width, node_bytes=i.node_bytes,
-                    collapse_ms=i.collapse_ms)
+                    collapse_ms=i.collapse_ms, builder=i.builder)
 
     def query_closest(self, o, d):
         o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)

@@ -136,7 +136,7 @@ ear_status ear_scene_upload(ear_ctx* c, const float* v0, const float* v1, const 
     if (!s) return EAR_ERR_OUT_OF_MEMORY;
     s->ctx = c;
     ear_status st = build_scene(c, v0, v1, v2, mat_id, n_tris, mats, n_mat, opts ? opts->leaf_size : 0,
-                                 opts ? opts->width : 0, &s->ds);
+                                 opts ? opts->width : 0, opts ? opts->builder : 0, &s->ds);
     if (st != EAR_OK) {
         delete s;
         return st;
@@ -165,6 +165,7 @@ ear_status ear_scene_get_info(const ear_scene* s, ear_scene_info* info) {
     info->width = s->ds.width;
     info->node_bytes = s->ds.width == 4 ? 112 : 56;
     info->collapse_ms = s->ds.collapse_ms;
+    info->builder = s->ds.builder;
     if (s->ds.width == 4) {
         info->n_nodes = s->ds.n_nodes4;
         info->max_depth = s->ds.depth4 + 1;

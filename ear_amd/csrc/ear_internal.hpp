@@ -55,6 +55,7 @@ struct DeviceScene {
     double collapse_ms = 0;
     Bvh4Node* nodes4 = nullptr;
     uint32_t n_mat = 0;
+    uint32_t builder = EAR_BVH_LBVH;
     uint32_t leaf_size = 4;
     uint32_t max_depth = 0;
     float bounds[6] = {0, 0, 0, 0, 0, 0};
@@ -124,7 +125,7 @@ ear_status hip_status(Ctx* c, hipError_t e, const char* what);
 // lbvh.hip
 ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2, const uint16_t* mat_id,
                        uint32_t n_tris, const ear_material* mats, uint32_t n_mat, uint32_t leaf_size,
-                       uint32_t width, DeviceScene* out);
+                       uint32_t width, uint32_t builder, DeviceScene* out);
 void free_scene(Ctx* c, DeviceScene* s);
 
 // tracer.hip

@@ -367,6 +367,145 @@ struct Buf {
     }
 };
 
+// Binned surface-area-heuristic build on the host (Wald, "On fast
+// Construction of SAH-based Bounding Volume Hierarchies", RT 2007): each node
+// bins its triangles' centroids into kSahBins slabs along each axis and takes
+// the split plane with the least SAH cost sum(area x count); ranges of at most
+// leaf_size triangles become leaves.  Output is the same 2-wide node array the
+// device build emits (root at 0, padded child boxes, leaf references), plus
+// the triangle order of the leaves.
+constexpr int kSahBins = 32;
+
+struct HostBox {
+    float lo[3], hi[3];
+    void reset() {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = INFINITY;
+            hi[k] = -INFINITY;
+        }
+    }
+    void grow(const HostBox& b) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    double area() const {
+        double dx = std::max(0.0, (double)hi[0] - lo[0]), dy = std::max(0.0, (double)hi[1] - lo[1]),
+               dz = std::max(0.0, (double)hi[2] - lo[2]);
+        return dx * dy + dy * dz + dz * dx;
+    }
+};
+
+struct SahBuild {
+    const std::vector<HostBox>& tb;
+    const std::vector<float>& cen;  // [n][3]
+    std::vector<uint32_t>& order;
+    std::vector<BvhNode>& nodes;
+    uint32_t leaf_size;
+    float pad;
+    uint32_t max_depth = 0;
+
+    void padded(const HostBox& b, float out[6]) const {
+        for (int k = 0; k < 3; ++k) {
+            out[k] = b.lo[k] - (pad + std::fabs(b.lo[k]) * 1e-6f);
+            out[3 + k] = b.hi[k] + (pad + std::fabs(b.hi[k]) * 1e-6f);
+        }
+    }
+    HostBox bounds(uint32_t b, uint32_t e) const {
+        HostBox r;
+        r.reset();
+        for (uint32_t i = b; i < e; ++i) r.grow(tb[order[i]]);
+        return r;
+    }
+    // splits [b, e) in place; returns the first index of the right part
+    uint32_t split(uint32_t b, uint32_t e) {
+        const uint32_t n = e - b;
+        float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = b; i < e; ++i)
+            for (int k = 0; k < 3; ++k) {
+                clo[k] = std::min(clo[k], cen[3 * order[i] + k]);
+                chi[k] = std::max(chi[k], cen[3 * order[i] + k]);
+            }
+        double best = INFINITY;
+        int best_axis = -1, best_bin = 0;
+        for (int k = 0; k < 3; ++k) {
+            const float ext = chi[k] - clo[k];
+            if (!(ext > 0.0f)) continue;
+            HostBox bb[kSahBins];
+            uint32_t bc[kSahBins] = {0};
+            for (int j = 0; j < kSahBins; ++j) bb[j].reset();
+            const float scale = (float)kSahBins / ext;
+            for (uint32_t i = b; i < e; ++i) {
+                int j = (int)((cen[3 * order[i] + k] - clo[k]) * scale);
+                j = std::min(std::max(j, 0), kSahBins - 1);
+                bb[j].grow(tb[order[i]]);
+                ++bc[j];
+            }
+            // sweep: right-to-left areas, then left-to-right costs
+            double ra[kSahBins];
+            uint32_t rc[kSahBins];
+            HostBox acc;
+            acc.reset();
+            uint32_t cnt = 0;
+            for (int j = kSahBins - 1; j > 0; --j) {
+                acc.grow(bb[j]);
+                cnt += bc[j];
+                ra[j] = acc.area();
+                rc[j] = cnt;
+            }
+            acc.reset();
+            cnt = 0;
+            for (int j = 0; j < kSahBins - 1; ++j) {
+                acc.grow(bb[j]);
+                cnt += bc[j];
+                if (cnt == 0 || rc[j + 1] == 0) continue;
+                const double cost = acc.area() * cnt + ra[j + 1] * rc[j + 1];
+                if (cost < best) {
+                    best = cost;
+                    best_axis = k;
+                    best_bin = j;
+                }
+            }
+        }
+        if (best_axis < 0) return b + n / 2;  // all centroids coincide: split by count
+        const int k = best_axis;
+        const float scale = (float)kSahBins / (chi[k] - clo[k]);
+        uint32_t* mid = std::partition(order.data() + b, order.data() + e, [&](uint32_t t) {
+            int j = (int)((cen[3 * t + k] - clo[k]) * scale);
+            j = std::min(std::max(j, 0), kSahBins - 1);
+            return j <= best_bin;
+        });
+        uint32_t m = (uint32_t)(mid - order.data());
+        if (m == b || m == e) m = b + n / 2;
+        return m;
+    }
+    // reference of the subtree over [b, e) (a leaf or a new internal node)
+    int32_t build(uint32_t b, uint32_t e, uint32_t depth, HostBox* box_out) {
+        HostBox box = bounds(b, e);
+        *box_out = box;
+        max_depth = std::max(max_depth, depth);
+        if (e - b <= leaf_size) return (int32_t)(kLeafFlag | ((e - b - 1) << 24) | b);
+        // below depth 32 split by count, so the tree stays within the
+        // traversal stack whatever the geometry
+        const uint32_t m = depth < 32 ? split(b, e) : b + (e - b) / 2;
+        const int32_t idx = (int32_t)nodes.size();
+        nodes.emplace_back();
+        HostBox lb, rb;
+        const int32_t l = build(b, m, depth + 1, &lb);
+        const int32_t r = build(m, e, depth + 1, &rb);
+        float pl[6], pr[6];
+        padded(lb, pl);
+        padded(rb, pr);
+        BvhNode& nd = nodes[idx];
+        nd.a = make_float4(pl[0], pl[1], pl[2], pl[3]);
+        nd.b = make_float4(pl[4], pl[5], pr[0], pr[1]);
+        nd.c = make_float4(pr[2], pr[3], pr[4], pr[5]);
+        nd.d = make_int4(l, r, 0, 0);
+        return idx;
+    }
+};
+
 }  // namespace
 
 void free_scene(Ctx* c, DeviceScene* s) {
@@ -487,10 +626,91 @@ static ear_status collapse_to_bvh4(Ctx* c, DeviceScene* s) {
     return EAR_OK;
 }
 
+// The SAH build of the 2-wide tree on the host, then the same device gather
+// of the triangles into leaf order as the LBVH path.
+static ear_status build_sah(Ctx* c, const float* v0, const float* v1, const float* v2, const uint16_t* mat_id,
+                            uint32_t n, uint32_t leaf_size, DeviceScene* s) {
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<HostBox> tb(n);
+    std::vector<float> cen(3 * (size_t)n);
+    HostBox scene;
+    scene.reset();
+    for (uint32_t i = 0; i < n; ++i) {
+        HostBox& b = tb[i];
+        for (int k = 0; k < 3; ++k) {
+            const float a = v0[3 * (size_t)i + k], bb = v1[3 * (size_t)i + k], cc = v2[3 * (size_t)i + k];
+            b.lo[k] = std::min(a, std::min(bb, cc));
+            b.hi[k] = std::max(a, std::max(bb, cc));
+            cen[3 * (size_t)i + k] = 0.5f * (b.lo[k] + b.hi[k]);
+        }
+        scene.grow(b);
+    }
+    float ext = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        s->bounds[k] = scene.lo[k];
+        s->bounds[3 + k] = scene.hi[k];
+        ext = std::max(ext, scene.hi[k] - scene.lo[k]);
+    }
+    const float pad = 1e-5f * ext + 1e-6f;
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::vector<BvhNode> nodes;
+    nodes.reserve(n);
+    SahBuild sb{tb, cen, order, nodes, leaf_size, pad};
+    HostBox root;
+    const int32_t r = sb.build(0, n, 1, &root);
+    if (r < 0) {
+        // the whole scene is one leaf: a root with that leaf and an empty child
+        float pb[6];
+        sb.padded(root, pb);
+        BvhNode nd;
+        nd.a = make_float4(pb[0], pb[1], pb[2], pb[3]);
+        nd.b = make_float4(pb[4], pb[5], 1e30f, 1e30f);
+        nd.c = make_float4(1e30f, 1e30f, 1e30f, 1e30f);
+        nd.d = make_int4(r, kEmptyChild, 0, 0);
+        nodes.push_back(nd);
+        sb.max_depth = 1;
+    }
+    s->n_nodes = (uint32_t)nodes.size();
+    s->max_depth = sb.max_depth;
+    if (s->max_depth >= (uint32_t)kStackDepth) {
+        c->last_error = "BVH deeper than the traversal stack";
+        return EAR_ERR_CAPACITY;
+    }
+    hipStream_t st = c->stream;
+    TRY(dev_alloc(c, (void**)&s->nodes, sizeof(BvhNode) * nodes.size()));
+    TRY(dev_alloc(c, (void**)&s->tris, sizeof(float4) * 3 * (size_t)n));
+    TRY(dev_alloc(c, (void**)&s->nm, sizeof(float4) * (size_t)n));
+    Buf dv0(c), dv1(c), dv2(c), dmat(c), vals(c);
+    const size_t n3 = 3 * (size_t)n;
+    TRY(dv0.alloc(n3 * 4));
+    TRY(dv1.alloc(n3 * 4));
+    TRY(dv2.alloc(n3 * 4));
+    TRY(dmat.alloc((size_t)n * 2));
+    TRY(vals.alloc((size_t)n * 4));
+    EAR_HIP_TRY(c, hipMemcpyAsync(s->nodes, nodes.data(), sizeof(BvhNode) * nodes.size(), hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(dv0.p, v0, n3 * 4, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(dv1.p, v1, n3 * 4, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(dv2.p, v2, n3 * 4, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(dmat.p, mat_id, (size_t)n * 2, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(vals.p, order.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+    k_gather<<<(n + 255) / 256, 256, 0, st>>>(n, vals.as<uint32_t>(), dv0.as<float>(), dv1.as<float>(),
+                                              dv2.as<float>(), dmat.as<uint16_t>(), s->tris, s->nm);
+    EAR_HIP_TRY(c, hipGetLastError());
+    EAR_HIP_TRY(c, hipStreamSynchronize(st));
+    s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return EAR_OK;
+}
+
 ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2, const uint16_t* mat_id,
                        uint32_t n, const ear_material* mats, uint32_t n_mat, uint32_t leaf_size, uint32_t width,
-                       DeviceScene* out) {
+                       uint32_t builder, DeviceScene* out) {
     if (width == 0) width = 4;
+    if (builder == EAR_BVH_DEFAULT) builder = EAR_BVH_LBVH;
+    if (builder != EAR_BVH_LBVH && builder != EAR_BVH_SAH) {
+        c->last_error = "unknown BVH builder";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
     if (width != 2 && width != 4) {
         c->last_error = "BVH width must be 2 or 4";
         return EAR_ERR_INVALID_ARGUMENT;
@@ -549,12 +769,30 @@ ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2
     s.n_tris = n;
     s.n_mat = n_mat;
     s.leaf_size = leaf_size;
+    s.builder = builder;
     s.mat_band_host = mb;
     TRY(dev_alloc(c, (void**)&s.mat_band, sizeof(float4) * mb.size()));
     hipStream_t st = c->stream;
     EAR_HIP_TRY(c, hipMemcpyAsync(s.mat_band, mb.data(), sizeof(float4) * mb.size(), hipMemcpyHostToDevice, st));
     if (n == 0) {
         EAR_HIP_TRY(c, hipStreamSynchronize(st));
+        *out = s;
+        return EAR_OK;
+    }
+
+    if (builder == EAR_BVH_SAH) {
+        ear_status bs = build_sah(c, v0, v1, v2, mat_id, n, leaf_size, &s);
+        if (bs != EAR_OK) {
+            free_scene(c, &s);
+            return bs;
+        }
+        if (width == 4) {
+            ear_status cs = collapse_to_bvh4(c, &s);
+            if (cs != EAR_OK) {
+                free_scene(c, &s);
+                return cs;
+            }
+        }
         *out = s;
         return EAR_OK;
     }

@@ -80,8 +80,12 @@ typedef struct {
 typedef struct {
     uint32_t leaf_size;  /* max triangles per BVH leaf, 1..16 (0 = library default, 1) */
     uint32_t width;      /* children per node: 2 or 4 (0 = library default, 4) */
-    uint32_t reserved[2];
+    uint32_t builder;    /* EAR_BVH_* (0 = library default) */
+    uint32_t reserved;
 } ear_bvh_opts;
+#define EAR_BVH_DEFAULT 0u
+#define EAR_BVH_LBVH 1u  /* device linear BVH: Morton order, Karras hierarchy */
+#define EAR_BVH_SAH 2u   /* host binned surface-area-heuristic build, then uploaded */
 
 /* Uploads the merged triangle soup of every MESH block (the reference merges
  * all meshes into meshes[0], src/Scene.cpp:103-106, src/Mesh.cpp:116-123)
@@ -100,10 +104,12 @@ typedef struct {
     uint32_t max_depth;   /* longest root-to-leaf path */
     uint32_t leaf_size;
     float bounds[6];      /* scene AABB lo xyz, hi xyz */
-    double build_ms;      /* device time of the LBVH build */
+    double build_ms;      /* build time: device time of the LBVH, host time of the SAH build */
     uint32_t width;       /* children per node */
     uint32_t node_bytes;  /* bytes fetched per node visit */
     double collapse_ms;   /* host time of the 2-wide -> 4-wide collapse */
+    uint32_t builder;     /* EAR_BVH_LBVH or EAR_BVH_SAH */
+    uint32_t reserved;
 } ear_scene_info;
 ear_status ear_scene_get_info(const ear_scene* scene, ear_scene_info* info);
 

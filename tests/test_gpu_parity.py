@@ -69,8 +69,8 @@ def test_shoebox_parity_bitexact_paths(gpu_ctx, oracle, shoebox):
         np.testing.assert_allclose(g.abs_value[b], expa, rtol=1e-9, atol=1e-12 * np.abs(expa).max())
 
 
-@pytest.mark.parametrize("width", [2, 4])
-def test_hall_queries_match_bruteforce(gpu_ctx, oracle, width):
+@pytest.mark.parametrize("width,builder,leaf", [(2, 1, 4), (4, 1, 4), (2, 2, 1), (4, 2, 3)])
+def test_hall_queries_match_bruteforce(gpu_ctx, oracle, width, builder, leaf):
     s = scenes.hall(target_tris=20000)
     rng = np.random.default_rng(7)
     n = 4000
@@ -83,8 +83,8 @@ def test_hall_queries_match_bruteforce(gpu_ctx, oracle, width):
     o[: n // 8, 2] = 0.0
     d[n // 8: n // 4] = np.eye(3, dtype=np.float32)[rng.integers(0, 3, n // 8)] * rng.choice([-1, 1], (n // 8, 1))
     osc = oracle.OracleScene(s["tris"], s["mat"], s["materials"])
-    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], leaf_size=4, width=width) as sc:
-        assert sc.info()["width"] == width
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], leaf_size=leaf, width=width, builder=builder) as sc:
+        assert sc.info()["width"] == width and sc.info()["builder"] == builder
         t, idx = sc.query_closest(o, d)
         q = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
         occ = sc.query_occluded(o, q)
@@ -96,14 +96,15 @@ def test_hall_queries_match_bruteforce(gpu_ctx, oracle, width):
         assert occ[i] == osc.any_hit(o[i], q[i])
 
 
-@pytest.mark.parametrize("leaf,width", [(1, 4), (2, 2), (8, 4), (16, 2)])
-def test_bvh_shape_does_not_change_results(gpu_ctx, shoebox, leaf, width):
+@pytest.mark.parametrize("leaf,width,builder", [(1, 4, 1), (2, 2, 1), (8, 4, 1), (16, 2, 1), (1, 4, 2), (3, 2, 2),
+                                               (16, 4, 2)])
+def test_bvh_shape_does_not_change_results(gpu_ctx, shoebox, leaf, width, builder):
     s = scenes.shoebox(tess=(6, 6, 3))
     kw = dict(recorders=s["recorders"], n_rays=2000, src_pos=s["src"], air=s["air"], max_bounces=50, cutoff=1e-6,
               n_bins=44100)
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], leaf_size=4) as a:
         ra = api.render(gpu_ctx, a, **kw)
-    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], leaf_size=leaf, width=width) as b:
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], leaf_size=leaf, width=width, builder=builder) as b:
         rb = api.render(gpu_ctx, b, **kw)
     np.testing.assert_array_equal(ra.raw, rb.raw)
     np.testing.assert_array_equal(ra.abs_raw, rb.abs_raw)
