@@ -16,6 +16,7 @@ import time
 # knob -> candidate values (0 = library default)
 LAUNCH_SPACE = {
     "traversal": [1, 2, 3],
+    "refill": [32, 48, 56],
     "block_extend": [128, 256],
     "block_shadow": [128, 256],
     "stack_lds": [16, 32],
@@ -23,6 +24,7 @@ LAUNCH_SPACE = {
     "graph_iters": [0, 8],
 }
 SCENE_SPACE = {
+    "builder": [1, 2],
     "width": [2, 4],
     "leaf_size": [1, 2, 4],
 }
@@ -81,16 +83,16 @@ def tune(ctx, tris, mat, materials, render_kw: dict, *, sample_rays: int | None 
     if sample_rays:
         kw["n_rays"] = sample_rays
     ctx.set_tuning()
-    scene_opts = {"width": 4, "leaf_size": 1}
+    scene_opts = {"builder": 2, "width": 4, "leaf_size": 2}
     scene_trials = []
     if tune_scene:
         def measure_scene(opts):
-            with api.Scene(ctx, tris, mat, materials, leaf_size=opts["leaf_size"], width=opts["width"]) as sc:
+            with api.Scene(ctx, tris, mat, materials, **opts) as sc:
                 t, _ = _timed_render(ctx, sc, kw, repeats)
             return t
 
         scene_opts, _, scene_trials = coordinate_descent(measure_scene, SCENE_SPACE, scene_opts, rounds)
-    with api.Scene(ctx, tris, mat, materials, leaf_size=scene_opts["leaf_size"], width=scene_opts["width"]) as sc:
+    with api.Scene(ctx, tris, mat, materials, **scene_opts) as sc:
         segs = {}
 
         def measure_launch(cfg):
@@ -123,11 +125,11 @@ def save(result: dict, path: str) -> None:
 
 def load(ctx, path: str) -> dict:
     """Applies a sidecar's launch tuning to ctx; returns its scene options
-    (width, leaf_size) for api.Scene."""
+    (builder, width, leaf_size) for api.Scene."""
     with open(path) as f:
         d = json.load(f)
     if d.get("version") != 1:
         raise ValueError(f"{path}: unknown autotune sidecar version {d.get('version')}")
-    known = set(LAUNCH_SPACE) | {"block_shade", "waves_per_cu"}
+    known = set(LAUNCH_SPACE) | {"block_shade", "waves_per_cu", "fetch_chunk"}
     ctx.set_tuning(**{k: int(v) for k, v in d["tuning"].items() if k in known})
     return dict(d.get("scene", {}))

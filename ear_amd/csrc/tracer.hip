@@ -1607,7 +1607,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
     const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
     const bool timed = tu.graph_iters == 0;
-    const uint32_t refill = tu.refill ? tu.refill : 32u;
+    const uint32_t refill = tu.refill ? tu.refill : 56u;
     const uint32_t fchunk = tu.fetch_chunk;  // 0: one entry per idle lane
     const Fetch fq_e{cnt + 4, refill, fchunk}, fq_s{cnt + 5, refill, fchunk};
     // one iteration; cur = the queue it consumes
@@ -2274,7 +2274,7 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const 
             for (uint32_t k = 1; k <= D->max_bounces && n_live > 0; ++k) {
                 hipLaunchKernelGGL(fe, dim3(grid_for(n_live, cap_e, 256)), dim3(256), 0, st, di, gs.as<int>(),
                                    gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp,
-                                   Fetch{cnt + 4, 32u, 0u});
+                                   Fetch{cnt + 4, 56u, 0u});
                 const uint32_t gsd = persistent_grid(c, n_live, waves);
                 if (s->width == 4)
                     k_shade_ad<4><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,

@@ -78,9 +78,9 @@ typedef struct {
 } ear_material;
 
 typedef struct {
-    uint32_t leaf_size;  /* max triangles per BVH leaf, 1..16 (0 = library default, 1) */
+    uint32_t leaf_size;  /* max triangles per BVH leaf, 1..16 (0 = library default, 2) */
     uint32_t width;      /* children per node: 2 or 4 (0 = library default, 4) */
-    uint32_t builder;    /* EAR_BVH_* (0 = library default) */
+    uint32_t builder;    /* EAR_BVH_* (0 = library default, EAR_BVH_SAH) */
     uint32_t reserved;
 } ear_bvh_opts;
 #define EAR_BVH_DEFAULT 0u
@@ -89,7 +89,7 @@ typedef struct {
 
 /* Uploads the merged triangle soup of every MESH block (the reference merges
  * all meshes into meshes[0], src/Scene.cpp:103-106, src/Mesh.cpp:116-123)
- * and builds the LBVH on the device.  Triangles are given as three vertex
+ * and builds its BVH (host SAH build by default, or the device LBVH).  Triangles are given as three vertex
  * arrays of n_tris xyz triples each, in the reference's triangle order; the
  * triangle index reported by queries is the position in that order.
  * n_tris may be 0 (the reference's Mesh::Empty(), src/EAR.cpp:160-163). */
@@ -266,7 +266,7 @@ typedef struct {
     uint32_t graph_iters;    /* > 0: bounce-loop iterations captured in one HIP graph and
                                 replayed (per-kernel times are then not measured) */
     uint32_t refill;         /* refilling traversal: idle lanes (of 64) that trigger a refill
-                                (default 32) */
+                                (default 56: fewer rays in flight keep more of the tree cached) */
     uint32_t fetch_chunk;    /* refilling traversal: queue entries a wave claims per atomic
                                 (0: one per idle lane) */
     uint32_t reserved[2];

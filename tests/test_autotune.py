@@ -48,7 +48,7 @@ def test_tuned_render_equals_default(gpu_ctx, tmp_path):
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
         ref = api.render(gpu_ctx, sc, **kw)
     opts = autotune.load(gpu_ctx, str(path))
-    assert set(opts) == {"width", "leaf_size"}
+    assert set(opts) == {"builder", "width", "leaf_size"}
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], **opts) as sc:
         r = api.render(gpu_ctx, sc, **kw)
     gpu_ctx.set_tuning()
