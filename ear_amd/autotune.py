@@ -16,7 +16,8 @@ import time
 # knob -> candidate values (0 = library default)
 LAUNCH_SPACE = {
     "traversal": [1, 2, 3],
-    "refill": [32, 48, 56],
+    "refill": [16, 24, 32, 48],
+    "fetch_segments": [16, 64],
     "block_extend": [128, 256],
     "block_shadow": [128, 256],
     "stack_lds": [16, 32],

@@ -112,8 +112,8 @@ ear_status ear_ctx_set_tuning(ear_ctx* c, const ear_tuning* t) {
         c->last_error = "unknown traversal mode";
         return EAR_ERR_INVALID_ARGUMENT;
     }
-    if (t->refill > 64 || t->fetch_chunk > (1u << 20)) {
-        c->last_error = "refill must be 0..64 and fetch_chunk 0..2^20";
+    if (t->refill > 64 || t->fetch_chunk > (1u << 20) || t->fetch_segments > 64) {
+        c->last_error = "refill must be 0..64, fetch_chunk 0..2^20, fetch_segments 0..64";
         return EAR_ERR_INVALID_ARGUMENT;
     }
     c->tuning = *t;

@@ -73,10 +73,13 @@ struct Traversal {
 // run, claiming a new run with one atomic on `counter` (zero at launch) when
 // its run is used up: `chunk` entries, or (chunk = 0) one per idle lane.
 struct Fetch {
-    uint32_t* counter;
+    uint32_t* counter;  // n_seg counters, one per 128-byte line (stride kSegStride)
     uint32_t refill;
     uint32_t chunk;
+    uint32_t n_seg;
 };
+constexpr uint32_t kSegStride = 32;  // uint32 words per counter line
+constexpr uint32_t kMaxSeg = 64;
 
 // ---------------------------------------------------------------------------
 // wave helpers
@@ -105,19 +108,36 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint64
 
 // Lane refilling (see Fetch): call with the whole wave when !exhausted and at
 // least f.refill lanes are idle (or all are); `act` is the ballot of busy
-// lanes.  Returns the queue entry handed to this lane, or ~0u.
+// lanes.  Returns the queue entry handed to this lane, or ~0u.  The queue
+// [0, n) is cut into f.n_seg segments with a counter each, so that claims of
+// many waves do not serialise on one address: a wave claims from its current
+// segment `seg` and moves on to the next one when that is used up; after
+// f.n_seg used-up segments in a row everything has been taken.
 __device__ __forceinline__ uint32_t take_entries(const Fetch& f, uint32_t n, uint64_t act, uint32_t lane,
-                                                 uint32_t& c_next, uint32_t& c_end, bool& exhausted) {
+                                                 uint32_t& c_next, uint32_t& c_end, bool& exhausted,
+                                                 uint32_t& seg) {
     const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
     if (c_next == c_end) {
         // claim f.chunk entries, or exactly one per idle lane
         const uint32_t want = f.chunk ? f.chunk : n_idle;
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(f.counter, want);
-        base = __shfl(base, 0);
-        c_next = min(base, n);
-        c_end = min(base + want, n);
-        if (base + want >= n) exhausted = c_next == c_end;
+        for (uint32_t fails = 0;;) {
+            const uint32_t lo = (uint32_t)(((uint64_t)n * seg) / f.n_seg);
+            const uint32_t hi = (uint32_t)(((uint64_t)n * (seg + 1)) / f.n_seg);
+            uint32_t old = 0;
+            if (lo < hi && lane == 0) old = atomicAdd(f.counter + seg * kSegStride, want);
+            old = __shfl(old, 0);
+            if (lo < hi && old < hi - lo) {
+                c_next = lo + old;
+                c_end = min(lo + old + want, hi);
+                break;
+            }
+            if (++fails == f.n_seg) {
+                exhausted = true;
+                c_next = c_end = 0;
+                break;
+            }
+            seg = seg + 1 == f.n_seg ? 0 : seg + 1;
+        }
     }
     const uint64_t idle = ~act;
     const uint32_t avail = c_end - c_next;
@@ -857,11 +877,12 @@ __global__ void __launch_bounds__(BLOCK) k_extend_f(const InstDev* __restrict__ 
     bool active = false, exhausted = n == 0;
     uint32_t slot = 0;
     uint32_t c_next = 0, c_end = 0;  // the wave's claimed entries [c_next, c_end)
+    uint32_t seg = ((blockIdx.x * BLOCK + threadIdx.x) >> 6) % fetch.n_seg;  // the wave's first segment
     for (;;) {
         uint64_t act = __ballot(active);
         const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
         if (!exhausted && (n_idle >= fetch.refill || act == 0)) {
-            const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted);
+            const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted, seg);
             if (mine != 0xFFFFFFFFu) {
                 slot = mine;
                 const float4 o = q.o[mine], d = q.d[mine];
@@ -902,12 +923,13 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
     bool active = false, exhausted = n == 0;
     uint32_t slot = 0;
     uint32_t c_next = 0, c_end = 0;  // the wave's claimed entries [c_next, c_end)
+    uint32_t seg = ((blockIdx.x * BLOCK + threadIdx.x) >> 6) % fetch.n_seg;  // the wave's first segment
     for (;;) {
         uint64_t act = __ballot(active);
         const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
         bool done = false;
         if (!exhausted && (n_idle >= fetch.refill || act == 0)) {
-            const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted);
+            const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted, seg);
             if (mine != 0xFFFFFFFFu) {
                 slot = mine;
                 ++nsh;
@@ -976,10 +998,12 @@ __global__ void k_plan(uint32_t* cursor, uint32_t total, uint32_t cap, const uin
     }
 }
 
-// zeroes counters c[i] for the bits i of mask (a kernel, not a memset node,
-// so a captured bounce loop replays it correctly)
-__global__ void k_reset_mask(uint32_t* c, uint32_t mask) {
+// End of a bounce-loop iteration: zeroes the counters c[i] for the bits i of
+// mask and the n_seg refill counters (a kernel, not memset nodes, so a
+// captured loop replays it correctly)
+__global__ void k_reset_iter(uint32_t* c, uint32_t mask, uint32_t* seg, uint32_t n_seg) {
     if (threadIdx.x < 32 && ((mask >> threadIdx.x) & 1u)) c[threadIdx.x] = 0;
+    for (uint32_t i = threadIdx.x; i < n_seg; i += blockDim.x) seg[i * kSegStride] = 0;
 }
 
 // Direct sound (src/Scene.cpp:299-311): one block per instance, one thread
@@ -1498,8 +1522,12 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     TRY(so.alloc(cap_sh * 16));
     TRY(sm.alloc(cap_sh * 16));
     // [0] queue 0, [1] queue 1, [2] shadow queue, [3] generation cursor,
-    // [4] extend fetch, [5] shadow fetch, [6..7] plan {first, count}
+    // [6..7] plan {first, count}
     TRY(counters.alloc(16 * 4));
+    // refill counters of the closest-hit and shadow kernels, one per line
+    DevBuf segc(c);
+    TRY(segc.alloc(2 * kMaxSeg * kSegStride * 4));
+    EAR_HIP_TRY(c, hipMemsetAsync(segc.p, 0, segc.n, st));
     TRY(stats.alloc(ST_N * 8));
     TRY(hist.alloc(hist_cells * 16));
     uint32_t gthreads = std::max(cap_e * (uint32_t)cfg_e.block, cap_s * (uint32_t)cfg_s.block);
@@ -1607,9 +1635,11 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
     const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
     const bool timed = tu.graph_iters == 0;
-    const uint32_t refill = tu.refill ? tu.refill : 56u;
+    const uint32_t refill = tu.refill ? tu.refill : 24u;
     const uint32_t fchunk = tu.fetch_chunk;  // 0: one entry per idle lane
-    const Fetch fq_e{cnt + 4, refill, fchunk}, fq_s{cnt + 5, refill, fchunk};
+    const uint32_t nseg = tu.fetch_segments ? std::min<uint32_t>(tu.fetch_segments, kMaxSeg) : 64u;
+    const Fetch fq_e{segc.as<uint32_t>(), refill, fchunk, nseg},
+        fq_s{segc.as<uint32_t>() + kMaxSeg * kSegStride, refill, fchunk, nseg};
     // one iteration; cur = the queue it consumes
     auto iteration = [&](int cur) {
         uint32_t* nq = cnt + cur;
@@ -1642,7 +1672,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
                            stp, fq_s);
         if (e3) hipEventRecord(e3->b, st);
         // the consumed queue, the shadow queue and the fetch counters
-        k_reset_mask<<<1, 64, 0, st>>>(cnt, (1u << cur) | 0x34u);
+        k_reset_iter<<<1, 64, 0, st>>>(cnt, (1u << cur) | 0x4u, segc.as<uint32_t>(), 2 * kMaxSeg);
         launches += 6;
         ++bounces;
     };
@@ -2274,7 +2304,7 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const 
             for (uint32_t k = 1; k <= D->max_bounces && n_live > 0; ++k) {
                 hipLaunchKernelGGL(fe, dim3(grid_for(n_live, cap_e, 256)), dim3(256), 0, st, di, gs.as<int>(),
                                    gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp,
-                                   Fetch{cnt + 4, 56u, 0u});
+                                   Fetch{cnt + 4, 56u, 0u, 1u});
                 const uint32_t gsd = persistent_grid(c, n_live, waves);
                 if (s->width == 4)
                     k_shade_ad<4><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,

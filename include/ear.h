@@ -266,10 +266,12 @@ typedef struct {
     uint32_t graph_iters;    /* > 0: bounce-loop iterations captured in one HIP graph and
                                 replayed (per-kernel times are then not measured) */
     uint32_t refill;         /* refilling traversal: idle lanes (of 64) that trigger a refill
-                                (default 56: fewer rays in flight keep more of the tree cached) */
+                                (default 24) */
     uint32_t fetch_chunk;    /* refilling traversal: queue entries a wave claims per atomic
                                 (0: one per idle lane) */
-    uint32_t reserved[2];
+    uint32_t fetch_segments; /* refilling traversal: queue segments with a claim counter each
+                                (1..64, default 64) */
+    uint32_t reserved;
 } ear_tuning;
 #define EAR_TRAVERSAL_DEFAULT 0u   /* = EAR_TRAVERSAL_FETCH_ALL */
 #define EAR_TRAVERSAL_STATIC 1u    /* one queue entry per lane per grid-stride step */

@@ -152,7 +152,8 @@ def test_fetch_traversal_is_bitwise_identical(gpu_ctx, width):
     """Persistent lanes that refill from the queue walk every ray exactly as
     one-ray-per-lane launches do: same histograms, same node visits and
     triangle tests, for both BVH widths, block sizes, stack splits, refill
-    thresholds (1..64 idle lanes) and claim sizes (1..4096 entries), with a
+    thresholds (1..64 idle lanes), claim sizes (1..4096 entries) and queue
+    segments (1..64), with a
     stereo recorder and a mesh source (its k = 0 connections are shadow
     requests of the first iteration)."""
     s = scenes.hall(target_tris=20_000)
@@ -161,10 +162,11 @@ def test_fetch_traversal_is_bitwise_identical(gpu_ctx, width):
               cutoff=1e-8, n_bins=132300, instrument=True)
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], width=width) as sc:
         res = []
-        for trav, blk, lds, refill, chunk in [(1, 256, 16, 0, 0), (2, 256, 16, 1, 1), (3, 128, 32, 64, 4096),
-                                              (3, 256, 16, 0, 0)]:
+        for trav, blk, lds, refill, chunk, segs in [(1, 256, 16, 0, 0, 0), (2, 256, 16, 1, 1, 1),
+                                                    (3, 128, 32, 64, 4096, 64), (3, 256, 16, 0, 0, 0),
+                                                    (3, 256, 16, 16, 0, 7)]:
             gpu_ctx.set_tuning(traversal=trav, block_extend=blk, block_shadow=blk, stack_lds=lds, refill=refill,
-                               fetch_chunk=chunk)
+                               fetch_chunk=chunk, fetch_segments=segs)
             res.append(api.render(gpu_ctx, sc, **kw))
             res.append(api.render(gpu_ctx, sc, src_tris=src_tris, **kw))
         gpu_ctx.set_tuning()
