@@ -100,6 +100,31 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     return base + (uint32_t)__popcll(below);
 }
 
+// Block-wide append: like wave_append with one atomic per block instead of
+// per wave (a queue counter is one address; per-wave atomics from every wave
+// of the GPU serialise there).  Every thread of the block must call it;
+// `sc` is block-shared scratch of BLOCK / 64 + 1 words that consecutive calls
+// must not share (the caller alternates two).
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred, uint32_t* sc) {
+    constexpr uint32_t kWaves = BLOCK / 64;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t mask = __ballot(pred);
+    if (lane == 0) sc[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            const uint32_t cw = sc[w];
+            sc[w] = tot;
+            tot += cw;
+        }
+        sc[kWaves] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    return sc[kWaves] + sc[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
 __device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint64_t v) {
     // reduce within the wave, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
@@ -692,6 +717,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
                                                    Queue qout, uint32_t* __restrict__ n_out, ShadowQ sq,
                                                    uint32_t* __restrict__ n_sq, uint32_t max_rec,
                                                    unsigned long long* __restrict__ stats) {
+    __shared__ uint32_t sc[4][kBlock / 64 + 1];  // scratch of the block-wide appends, alternated
     uint32_t n = *n_in;
     uint32_t n_done = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
@@ -780,7 +806,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
                     }
                 }
             }
-            uint32_t slot = wave_append(n_sq, emit);
+            uint32_t slot = block_append<kBlock>(n_sq, emit, sc[r & 1u]);
             if (emit) {
                 sq.p[slot] = make_float4(p.x, p.y, p.z, c);
                 sq.m[slot] = make_float4(L, __uint_as_float(r | ((uint32_t)band << 8)), __uint_as_float(id), 0.0f);
@@ -798,7 +824,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
             nd = make_float4(v.x, v.y, v.z, L);
             nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), id);
         }
-        uint32_t slot = wave_append(n_out, cont);
+        uint32_t slot = block_append<kBlock>(n_out, cont, sc[2 + ((base / (gridDim.x * kBlock)) & 1u)]);
         if (cont) {
             qout.o[slot] = no;
             qout.d[slot] = nd;
