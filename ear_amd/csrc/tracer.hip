@@ -458,7 +458,10 @@ __device__ __forceinline__ void acc128(unsigned long long* cell, I128 q) {
 }
 
 // Ramp of values (w - i) * A / w at bins s + i, i in [0, w), bins < 0 and
-// >= nb dropped, added to the second-difference row h.
+// >= nb dropped, added to the second-difference row h: +w*S at s, -(w+1)*S
+// at s+1, +S at s+w+1.  Each point is added as a magnitude to the row's "+"
+// or "-" part, so no add is negative and the high word is touched only on a
+// carry (one atomic per point instead of two for negative values).
 __device__ void splat_ramp(unsigned long long* h, uint32_t nb, int s, int w, float A) {
     if (w <= 0) return;
     double S = (double)A / (double)w;
@@ -469,13 +472,16 @@ __device__ void splat_ramp(unsigned long long* h, uint32_t nb, int s, int w, flo
         if (ww <= 0) return;
     }
     if ((uint32_t)s >= nb) return;
-    I128 sq = fixed_from_double(S);
+    const bool neg = S < 0.0;
+    I128 sq = fixed_from_double(neg ? -S : S);
     I128 aq = i128_mul_small(sq, (uint32_t)ww);
-    I128 p1 = i128_neg(i128_add(aq, sq));
+    I128 p1 = i128_add(aq, sq);
+    unsigned long long* hp = neg ? h + 2 * (size_t)nb : h;  // gets +w*S and +S
+    unsigned long long* hm = neg ? h : h + 2 * (size_t)nb;  // gets (w+1)*S
     uint32_t s0 = (uint32_t)s;
-    acc128(h + 2 * (size_t)s0, aq);
-    if (s0 + 1 < nb) acc128(h + 2 * (size_t)(s0 + 1), p1);
-    if (s0 + (uint32_t)ww + 1 < nb) acc128(h + 2 * (size_t)(s0 + ww + 1), sq);
+    acc128(hp + 2 * (size_t)s0, aq);
+    if (s0 + 1 < nb) acc128(hm + 2 * (size_t)(s0 + 1), p1);
+    if (s0 + (uint32_t)ww + 1 < nb) acc128(hp + 2 * (size_t)(s0 + ww + 1), sq);
 }
 
 // Row layout of the accumulators.  Without the |c| histogram one signed row
@@ -484,10 +490,11 @@ __device__ void splat_ramp(unsigned long long* h, uint32_t nb, int s, int w, flo
 // updates one row; the finalize step turns them into signed = P - N and
 // |.| = P + N (exact: quantisation is odd-symmetric, the sums are integers).
 struct Hist {
-    unsigned long long* cells;  // [band slot][channel][kind][bin] x {lo, hi}
+    unsigned long long* cells;  // [band slot][channel][kind][+/-][bin] x {lo, hi}
     uint32_t n_bins, n_ch, kinds;
+    // the row's "+" part; its "-" part follows at + 2 * n_bins
     __device__ unsigned long long* row(int band, int ch, int kind) const {
-        return cells + 2 * (size_t)n_bins * (((size_t)band * n_ch + ch) * kinds + kind);
+        return cells + 4 * (size_t)n_bins * (((size_t)band * n_ch + ch) * kinds + kind);
     }
 };
 
@@ -946,53 +953,58 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
     uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0;
     Walker<LDS, BLOCK, WIDTH, true, INSTR> w(lds + threadIdx.x, gstack, gtid, gthreads);
     Traversal T = make_trav(inst[0], gstack, gthreads);
-    bool active = false, exhausted = n == 0;
+    // a lane is walking (active), holds a visible connection not yet
+    // recorded (pending) or is free; pending lanes are recorded together when
+    // the wave refills, so the splat runs with many lanes at once
+    bool active = false, pending = false, exhausted = n == 0;
     uint32_t slot = 0;
     uint32_t c_next = 0, c_end = 0;  // the wave's claimed entries [c_next, c_end)
     uint32_t seg = ((blockIdx.x * BLOCK + threadIdx.x) >> 6) % fetch.n_seg;  // the wave's first segment
     for (;;) {
         uint64_t act = __ballot(active);
-        const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
-        bool done = false;
-        if (!exhausted && (n_idle >= fetch.refill || act == 0)) {
-            const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted, seg);
-            if (mine != 0xFFFFFFFFu) {
-                slot = mine;
-                ++nsh;
-                const float4 pc = sq.p[mine], m = sq.m[mine];
+        const uint32_t n_free = 64u - (uint32_t)__popcll(act);
+        if (n_free >= fetch.refill || act == 0) {
+            if (pending) {
+                const float4 pc = sq.p[slot], m = sq.m[slot];
                 const uint32_t packed = __float_as_uint(m.y);
+                const int r = (int)(packed & 0xFFu), band = (int)((packed >> 8) & 0xFFu);
                 const uint32_t id = BATCH ? __float_as_uint(m.z) : 0;
-                if (BATCH) T = make_trav(inst[id], gstack, gthreads);
-                const RecParams& R = inst[id].P.rec[packed & 0xFFu];
-                const V3 p{pc.x, pc.y, pc.z};
-                if (w.start(T, p, sub(V3{R.px, R.py, R.pz}, p), 0.0f))
-                    active = true;
-                else
-                    done = true;  // empty scene: visible
+                const InstDev& In = inst[id];
+                const RecParams& R = In.P.rec[r];
+                const V3 dv = sub(V3{R.px, R.py, R.pz}, V3{pc.x, pc.y, pc.z});
+                const float l = length(dv);
+                record(make_hist(In), band, R, normalized(dv), pc.w, (m.x + l) / 343.0f, m.x + l);
+                ++ncon;
+                pending = false;
+            }
+            if (!exhausted) {
+                const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted, seg);
+                if (mine != 0xFFFFFFFFu) {
+                    slot = mine;
+                    ++nsh;
+                    const float4 pc = sq.p[mine], m = sq.m[mine];
+                    const uint32_t packed = __float_as_uint(m.y);
+                    const uint32_t id = BATCH ? __float_as_uint(m.z) : 0;
+                    if (BATCH) T = make_trav(inst[id], gstack, gthreads);
+                    const RecParams& R = inst[id].P.rec[packed & 0xFFu];
+                    const V3 p{pc.x, pc.y, pc.z};
+                    if (w.start(T, p, sub(V3{R.px, R.py, R.pz}, p), 0.0f))
+                        active = true;
+                    else
+                        pending = true;  // empty scene: visible
+                }
             }
             act = __ballot(active);
         }
         // the loop is left by the whole wave at once (take_entries is a
         // wave-wide step)
-        if (__ballot(active || done) == 0) {
-            if (exhausted) break;
+        if (act == 0) {
+            if (exhausted && __ballot(pending) == 0) break;
             continue;
         }
         if (active && w.step(T, nvis, ntri)) {
             active = false;
-            done = !w.found;
-        }
-        if (done) {
-            const float4 pc = sq.p[slot], m = sq.m[slot];
-            const uint32_t packed = __float_as_uint(m.y);
-            const int r = (int)(packed & 0xFFu), band = (int)((packed >> 8) & 0xFFu);
-            const uint32_t id = BATCH ? __float_as_uint(m.z) : 0;
-            const InstDev& In = inst[id];
-            const RecParams& R = In.P.rec[r];
-            const V3 dv = sub(V3{R.px, R.py, R.pz}, V3{pc.x, pc.y, pc.z});
-            const float l = length(dv);
-            record(make_hist(In), band, R, normalized(dv), pc.w, (m.x + l) / 343.0f, m.x + l);
-            ++ncon;
+            pending = !w.found;
         }
     }
     wave_add_u64(stats + ST_SHADOW, nsh);
@@ -1092,13 +1104,20 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(const InstDev* __restric
     const RenderParams& P = In.P;
     const int band = (int)rw.y;
     const uint32_t ch = rw.z, kind = rw.w, nb = P.n_bins, n_ch = P.n_channels, kinds = P.hist_kinds;
-    size_t roff = (((size_t)band * n_ch + ch) * kinds + kind) * nb;
-    unsigned long long* c = In.hist + 2 * roff;
+    size_t roff = (((size_t)band * n_ch + ch) * kinds + kind) * 2 * nb;
+    unsigned long long* c = In.hist + 2 * roff;      // "+" part, receives the sums
+    unsigned long long* cm = c + 2 * (size_t)nb;     // "-" part
     uint32_t per = (nb + kFinBlock - 1) / kFinBlock;
     uint32_t b = threadIdx.x * per, e = min(nb, b + per);
+    // pass 0 sums (+) - (-), the second differences; pass 1 their prefix sums
+    auto elem = [&](int pass, uint32_t j) {
+        I128 v{c[2 * j], (int64_t)c[2 * j + 1]};
+        if (pass == 0) v = i128_add(v, i128_neg(I128{cm[2 * j], (int64_t)cm[2 * j + 1]}));
+        return v;
+    };
     for (int pass = 0; pass < 2; ++pass) {
         I128 sum{0, 0};
-        for (uint32_t j = b; j < e; ++j) sum = i128_add(sum, I128{c[2 * j], (int64_t)c[2 * j + 1]});
+        for (uint32_t j = b; j < e; ++j) sum = i128_add(sum, elem(pass, j));
         slo[threadIdx.x] = sum.lo;
         shi[threadIdx.x] = sum.hi;
         __syncthreads();
@@ -1113,7 +1132,7 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(const InstDev* __restric
         }
         I128 run = i128_add(I128{slo[threadIdx.x], shi[threadIdx.x]}, i128_neg(sum));
         for (uint32_t j = b; j < e; ++j) {
-            run = i128_add(run, I128{c[2 * j], (int64_t)c[2 * j + 1]});
+            run = i128_add(run, elem(pass, j));
             c[2 * j] = run.lo;
             c[2 * j + 1] = (unsigned long long)run.hi;
         }
@@ -1132,12 +1151,12 @@ __global__ void __launch_bounds__(256) k_values(const InstDev* __restrict__ inst
     const uint32_t ch = rw.z, nb = P.n_bins, n_ch = P.n_channels, kinds = P.hist_kinds;
     const uint32_t j = blockIdx.x * 256 + threadIdx.x;
     if (j >= nb) return;
-    const size_t r0 = (((size_t)band * n_ch + ch) * kinds) * nb;
-    unsigned long long* c0 = In.hist + 2 * r0;
+    const size_t r0 = (((size_t)band * n_ch + ch) * kinds) * nb;  // value rows
+    unsigned long long* c0 = In.hist + 4 * r0;                      // kind 0 "+" part
     I128 v[2];
     v[0] = I128{c0[2 * j], (int64_t)c0[2 * j + 1]};
     if (kinds > 1) {
-        unsigned long long* c1 = c0 + 2 * (size_t)nb;
+        unsigned long long* c1 = c0 + 4 * (size_t)nb;  // kind 1 "+" part
         const I128 pos = v[0], neg = I128{c1[2 * j], (int64_t)c1[2 * j + 1]};
         v[0] = i128_add(pos, i128_neg(neg));
         v[1] = i128_add(pos, neg);
@@ -1488,7 +1507,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         max_recs = std::max<size_t>(max_recs, D->n_recorders);
         size_t cells = (size_t)3 * I.P.n_channels * I.P.hist_kinds * I.P.n_bins;
         hist_off[i] = hist_cells;
-        hist_cells += cells;
+        hist_cells += 2 * cells;  // "+" and "-" parts
         value_off[i] = value_cells;
         if (outs[i] && (outs[i]->value || outs[i]->abs_value)) {
             value_cells += cells;
@@ -1800,7 +1819,8 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
                     int64_t* raw = kind == 0 ? out->raw : out->abs_raw;
                     double* val = kind == 0 ? out->value : out->abs_value;
                     if (raw)
-                        EAR_HIP_TRY(c, hipMemcpyAsync(raw + 2 * ooff, hist.as<unsigned long long>() + 2 * (hist_off[i] + roff),
+                        EAR_HIP_TRY(c, hipMemcpyAsync(raw + 2 * ooff,
+                                                      hist.as<unsigned long long>() + 2 * (hist_off[i] + 2 * roff),
                                                       nb * 16, hipMemcpyDeviceToHost, st));
                     if (val && inst[i].value)
                         EAR_HIP_TRY(c, hipMemcpyAsync(val + ooff, value.as<double>() + value_off[i] + roff, nb * 8,
