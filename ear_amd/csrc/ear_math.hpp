@@ -91,7 +91,51 @@ struct Rng {
         if (w == 0) refill();
         ++draw;
         uint32_t x = w == 0 ? buf[0] : w == 1 ? buf[1] : w == 2 ? buf[2] : buf[3];
-        return (float)(x >> 8) * (1.0f / 16777216.0f);
+        return to_unit(x);
+    }
+    EAR_HD static float to_unit(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+    // The next three uniforms, exactly as three next() calls, with at most one
+    // Philox block computed (a wave then runs one block evaluation per call
+    // instead of one per uniform whose lanes sit at different phases).
+    EAR_HD void next3(float& u1, float& u2, float& u3) {
+        const uint32_t ph = draw & 3u;
+        uint32_t nb[4] = {buf[0], buf[1], buf[2], buf[3]};
+        if (ph != 1u) {
+            // phase 0: the block of `draw` itself; phases 2, 3: the next one
+            nb[0] = (draw >> 2) + (ph == 0u ? 0u : 1u);
+            nb[1] = ray;
+            nb[2] = ctx;
+            nb[3] = inst;
+            philox4x32_10(nb, k0, k1);
+        }
+        uint32_t a, b, c;
+        if (ph == 0u) {
+            a = nb[0];
+            b = nb[1];
+            c = nb[2];
+        } else if (ph == 1u) {
+            a = buf[1];
+            b = buf[2];
+            c = buf[3];
+        } else if (ph == 2u) {
+            a = buf[2];
+            b = buf[3];
+            c = nb[0];
+        } else {
+            a = buf[3];
+            b = nb[0];
+            c = nb[1];
+        }
+        if (ph != 1u) {
+            buf[0] = nb[0];
+            buf[1] = nb[1];
+            buf[2] = nb[2];
+            buf[3] = nb[3];
+        }
+        draw += 3u;
+        u1 = to_unit(a);
+        u2 = to_unit(b);
+        u3 = to_unit(c);
     }
 };
 
@@ -198,9 +242,11 @@ EAR_HD bool mt_intersect(V3 v0, V3 e1, V3 e2, V3 o, V3 d, float& t) {
 // ---------------------------------------------------------------------------
 EAR_HD V3 sample_sphere(Rng& r) {
     for (;;) {
-        float f1 = r.next() * 2.0f - 1.0f;
-        float f2 = r.next() * 2.0f - 1.0f;
-        float f3 = r.next() * 2.0f - 1.0f;
+        float u1, u2, u3;
+        r.next3(u1, u2, u3);
+        float f1 = u1 * 2.0f - 1.0f;
+        float f2 = u2 * 2.0f - 1.0f;
+        float f3 = u3 * 2.0f - 1.0f;
         float l = f1 * f1 + f2 * f2 + f3 * f3;
         if (l < 0.001f || l > 1.0f) continue;
         float s = sqrtf(l);
