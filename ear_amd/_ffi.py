@@ -28,7 +28,7 @@ EXPORTS = [
     "ear_ctx_create", "ear_ctx_destroy", "ear_ctx_set_memory_cap", "ear_status_string", "ear_ctx_last_error",
     "ear_abi_version", "ear_scene_upload", "ear_scene_destroy", "ear_scene_get_info", "ear_query_closest",
     "ear_query_occluded", "ear_render", "ear_render_batch", "ear_ctx_set_tuning", "ear_ctx_get_tuning",
-    "ear_render_backward",
+    "ear_render_backward", "ear_convolve",
 ]
 
 
@@ -130,6 +130,7 @@ def load(path: str | None = None):
     L.ear_render.argtypes = [vp, vp, C.POINTER(ear_render_desc), C.POINTER(ear_hist), C.POINTER(ear_stats)]
     L.ear_render_batch.argtypes = [vp, C.POINTER(ear_batch_item), C.c_uint32, C.POINTER(ear_stats)]
     L.ear_render_backward.argtypes = [vp, vp, C.POINTER(ear_render_desc), vp, vp, vp, vp]
+    L.ear_convolve.argtypes = [vp, fp, C.c_uint32, fp, C.c_uint32, fp, C.c_uint32, fp]
     L.ear_ctx_set_tuning.argtypes = [vp, C.POINTER(ear_tuning)]
     L.ear_ctx_get_tuning.argtypes = [vp, C.POINTER(ear_tuning)]
     for name in EXPORTS:

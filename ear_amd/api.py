@@ -306,3 +306,18 @@ def render_backward(ctx: Context, scene: Scene, *, adj_value=None, adj_abs=None,
     _check(ctx.lib, ctx.lib.ear_render_backward(ctx.h, scene.h, C.byref(p.desc), _ptr(a_v), _ptr(a_a), _ptr(g_r),
                                                 _ptr(g_s)), ctx.h)
     return g_r, g_s
+
+
+def convolve(ctx: Context, ir, signal, ir2=None) -> np.ndarray:
+    """RecorderTrack::Process on the device (ear_convolve): the convolution of
+    the track `ir` with `signal`, or with a second track `ir2` the keyframe
+    cross-fade (signal faded out for ir, faded in for ir2).  Returns
+    max(len(ir), len(ir2)) + len(signal) + 1 samples."""
+    a = np.ascontiguousarray(ir, np.float32)
+    s = np.ascontiguousarray(signal, np.float32)
+    b = None if ir2 is None else np.ascontiguousarray(ir2, np.float32)
+    n_out = max(len(a), 0 if b is None else len(b)) + len(s) + 1
+    out = np.zeros(n_out, np.float32)
+    _check(ctx.lib, ctx.lib.ear_convolve(ctx.h, _ptr(a), len(a), _ptr(b), 0 if b is None else len(b), _ptr(s), len(s),
+                                         _ptr(out)), ctx.h)
+    return out

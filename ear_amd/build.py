@@ -23,7 +23,7 @@ BUILD = os.path.join(PKG, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("EAR_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["ear_api.hip", "lbvh.hip", "tracer.hip"]
+HIP_SOURCES = ["ear_api.hip", "lbvh.hip", "tracer.hip", "convolve.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
@@ -67,7 +67,8 @@ def build_hip(force: bool = True) -> str:
         for j in jobs:
             j.result()
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lhipfft",
+          "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

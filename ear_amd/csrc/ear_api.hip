@@ -249,4 +249,45 @@ ear_status ear_render_backward(ear_ctx* c, ear_scene* s, const ear_render_desc* 
     }
 }
 
+ear_status ear_convolve(ear_ctx* c, const float* ir, uint32_t n_ir, const float* ir2, uint32_t n_ir2,
+                        const float* signal, uint32_t n_sig, float* out) {
+    if (!c || !ir || !signal || !out || (n_ir2 && !ir2)) return EAR_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    const uint64_t n_out64 = (uint64_t)std::max(n_ir, ir2 ? n_ir2 : 0u) + n_sig + 1;
+    if (n_out64 > (1u << 30)) {
+        c->last_error = "convolution longer than 2^30 samples";
+        return EAR_ERR_CAPACITY;
+    }
+    const uint32_t n_out = (uint32_t)n_out64;
+    try {
+        if (!ir2) {
+            const float* tr[1] = {ir};
+            const uint32_t nt[1] = {n_ir};
+            const float* sg[1] = {signal};
+            return run_convolve(c, tr, nt, sg, 1, n_sig, out, n_out);
+        }
+        // the reference's fades: a running float factor from 1 down by 1/N
+        // (fade out) and from 0 up by 1/N (fade in), src/Recorder.cpp:172-181
+        std::vector<float> s_out(n_sig), s_in(n_sig);
+        const float df = 1.0f / (float)std::max<uint32_t>(1, n_sig);
+        float f_out = 1.0f, f_in = 0.0f;
+        for (uint32_t i = 0; i < n_sig; ++i) {
+            s_out[i] = signal[i] * f_out;
+            s_in[i] = signal[i] * f_in;
+            f_out += -df;
+            f_in += df;
+        }
+        const float* tr[2] = {ir, ir2};
+        const uint32_t nt[2] = {n_ir, n_ir2};
+        const float* sg[2] = {s_out.data(), s_in.data()};
+        return run_convolve(c, tr, nt, sg, 2, n_sig, out, n_out);
+    } catch (const std::bad_alloc&) {
+        c->last_error = "host allocation failed";
+        return EAR_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        c->last_error = "internal exception";
+        return EAR_ERR_INTERNAL;
+    }
+}
+
 }  // extern "C"

@@ -130,6 +130,8 @@ void free_scene(Ctx* c, DeviceScene* s);
 
 // tracer.hip
 ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* d, ear_hist* out, ear_stats* st);
+ear_status run_convolve(Ctx* c, const float* const* tracks, const uint32_t* n_track, const float* const* signals,
+                        uint32_t terms, uint32_t n_sig, float* out, uint32_t n_out);
 ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* d, const double* adj_value,
                         const double* adj_abs, double* grad_refl, double* grad_spec);
 ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const ear_render_desc* const* descs,
@@ -144,5 +146,38 @@ ear_status run_query_occluded(Ctx* c, DeviceScene* s, uint32_t n, const float* p
         hipError_t _e = (expr);                                     \
         if (_e != hipSuccess) return ::ear::hip_status(ctx, _e, #expr); \
     } while (0)
+
+// A device allocation owned by a scope (through dev_alloc / dev_free).
+struct DevBuf {
+    Ctx* c;
+    void* p = nullptr;
+    size_t n = 0;
+    explicit DevBuf(Ctx* cc) : c(cc) {}
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) dev_free(c, p, n);
+        p = nullptr;
+        n = 0;
+    }
+    ear_status alloc(size_t bytes) {
+        release();
+        n = bytes ? bytes : 16;
+        return dev_alloc(c, &p, n);
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+#ifndef TRY
+#define TRY(x)                       \
+    do {                             \
+        ear_status _s = (x);         \
+        if (_s != EAR_OK) return _s; \
+    } while (0)
+#endif
 
 }  // namespace ear

@@ -519,11 +519,7 @@ void free_scene(Ctx* c, DeviceScene* s) {
     s->tris = s->nm = s->mat_band = nullptr;
 }
 
-#define TRY(x)                               \
-    do {                                     \
-        ear_status _s = (x);                 \
-        if (_s != EAR_OK) return _s;         \
-    } while (0)
+// TRY: ear_internal.hpp
 
 // Collapse the 2-wide tree into a 4-wide one: each 4-wide node takes a 2-wide
 // node's two children and, while fewer than four, replaces the internal child

@@ -1266,33 +1266,6 @@ int resident_blocks(const void* fn, int block) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-struct DevBuf {
-    Ctx* c;
-    void* p = nullptr;
-    size_t n = 0;
-    explicit DevBuf(Ctx* cc) : c(cc) {}
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) dev_free(c, p, n);
-        p = nullptr;
-        n = 0;
-    }
-    ear_status alloc(size_t bytes) {
-        release();
-        n = bytes ? bytes : 16;
-        return dev_alloc(c, &p, n);
-    }
-    template <class T>
-    T* as() const {
-        return (T*)p;
-    }
-};
-
-#define TRY(x)                       \
-    do {                             \
-        ear_status _s = (x);         \
-        if (_s != EAR_OK) return _s; \
-    } while (0)
 
 uint32_t persistent_grid(Ctx* c, uint32_t work, uint32_t waves_per_cu, uint32_t block = kBlock) {
     uint32_t blocks_needed = (work + block - 1) / block;
