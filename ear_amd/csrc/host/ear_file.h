@@ -29,7 +29,7 @@ struct SettingValue {
 struct EarFile {
     std::map<std::string, SettingValue> settings;
     std::vector<float> keys;
-    float freq[3] = {0.3f, 2.0f, 6.0f};
+    float freq[3] = {0.2f, 1.0f, 2.0f};  // kHz, src/SoundFile.cpp:240-242
     std::vector<std::unique_ptr<Material>> materials;
     std::vector<std::unique_ptr<Mesh>> meshes;
     struct Source {

@@ -6,6 +6,8 @@
 #include <cstring>
 #include <sstream>
 
+#include "audio.h"
+
 namespace earhost {
 
 void throw_on_error(ear_status s, const ear_ctx* ctx, const char* what) {
@@ -178,9 +180,95 @@ float RecorderTrack::T60() const {
     return (float)(last_significant - direct_offset) / 44100.0f;
 }
 
+void RecorderTrack::Normalize(float M, float MAX) { Multiply(M / (MAX < 0 ? Maximum() : MAX)); }
+
+void RecorderTrack::reset_buffer() {
+    data.assign(3 * 44100, 0.0f);
+    first_sample = 3 * 44100 - 1;
+    real_length = 0;
+}
+
+size_t RecorderTrack::fresh_buffer_length(unsigned first, unsigned last) {
+    size_t L = 3 * 44100;
+    if (first >= L) L = (size_t)first + 44100;
+    while (last >= L) L += 44100;
+    return L;
+}
+
+void RecorderTrack::Add(const RecorderTrack* other) {
+    const unsigned len = other->getLength(0.0f);
+    data.reserve((size_t)len + 44100);
+    for (unsigned i = 0; i < len; ++i) (*this)[i] += other->at(i);
+}
+
+RecorderTrack* RecorderTrack::Process(ear_ctx* ctx, const SoundFile& section, const RecorderTrack* other) const {
+    // The FFT form of the reference (src/Recorder.cpp:145-243): each track's
+    // first getLength() samples are convolved with the (faded) section, the
+    // M + N + 1 results written from the section's offset into a fresh
+    // buffer; the cross-fade adds the second track's result, whose buffer
+    // length also bounds the sum.  ear_convolve returns the sum of both
+    // convolutions in one array.
+    const unsigned M1 = real_length, M2 = other ? other->real_length : 0u, N = section.sample_length;
+    std::vector<float> a(std::max(M1, 1u)), b(std::max(M2, 1u));
+    for (unsigned i = 0; i < M1; ++i) a[i] = at(i);
+    for (unsigned i = 0; i < M2; ++i) b[i] = other->at(i);
+    const float zero = 0.0f;
+    const unsigned n_out = std::max(M1, M2) + N + 1;
+    std::vector<float> c(n_out);
+    throw_on_error(ear_convolve(ctx, a.data(), M1, other ? b.data() : nullptr, M2, N ? section.data : &zero, N, c.data()),
+                   ctx, "ear_convolve");
+    const unsigned off = section.offset;
+    RecorderTrack* r = new RecorderTrack();
+    r->reset_buffer();
+    r->data.reserve((size_t)off + n_out + 2 * 44100);
+    const unsigned MN1 = M1 + N + 1;
+    for (unsigned i = 0; i < MN1; ++i) (*r)[i + off] = c[i];
+    if (other) {
+        // the second track's result, a buffer of its own holding M2 + N + 1
+        // samples from `off`, is added over that buffer's whole length
+        const unsigned MN2 = M2 + N + 1;
+        const size_t n = fresh_buffer_length(off, off + MN2 - 1);
+        r->data.reserve(n + 44100);
+        for (unsigned j = 0; j < n; ++j) (*r)[j] += (j >= off + MN1 && j < off + MN2) ? c[j - off] : 0.0f;
+    }
+    return r;
+}
+
 // ---------------------------------------------------------------- recorders
 Recorder::~Recorder() {
     for (RecorderTrack* t : tracks) delete t;
+    for (RecorderTrack* t : processed_tracks) delete t;
+}
+void Recorder::Process(ear_ctx* ctx, SoundFile* sf, float offset) {
+    for (RecorderTrack* t : tracks) {
+        std::unique_ptr<SoundFile> section = sf->Section(offset);
+        processed_tracks.push_back(t->Process(ctx, *section));
+    }
+    is_processed = true;
+}
+void Recorder::Process(ear_ctx* ctx, SoundFile* sf, Recorder* r, float offset, float length) {
+    for (size_t i = 0; i < tracks.size(); ++i) {
+        std::unique_ptr<SoundFile> section = sf->Section(offset, length);
+        processed_tracks.push_back(tracks[i]->Process(ctx, *section, r->tracks.at(i)));
+    }
+    is_processed = true;
+}
+void Recorder::Add(Recorder* r) {
+    if (trackCount() != r->trackCount()) throw std::runtime_error("Recorder::Add: track counts differ");
+    for (size_t i = 0; i < r->processed_tracks.size(); ++i) {
+        if (i == processed_tracks.size()) {
+            processed_tracks.push_back(new RecorderTrack());
+            processed_tracks.back()->reset_buffer();
+        }
+        processed_tracks[i]->Add(r->processed_tracks[i]);
+    }
+    is_processed = true;
+}
+void Recorder::Normalize(float M) {
+    const std::vector<RecorderTrack*>& ts = save_processed ? processed_tracks : tracks;
+    float mx = -1e9f;
+    for (const RecorderTrack* t : ts) mx = std::max(mx, t->Maximum());
+    for (RecorderTrack* t : ts) t->Normalize(M, mx);
 }
 void Recorder::Multiply(float f) {
     for (RecorderTrack* t : tracks) t->Multiply(f);
@@ -226,6 +314,8 @@ Vec3 MonoRecorder::getLocation(int i) const { return (i >= 0 && animation) ? ani
 Recorder* MonoRecorder::getBlankCopy(int) const {
     MonoRecorder* r = new MonoRecorder(location);
     r->animation = animation;
+    r->filename = filename;
+    r->save_processed = save_processed;
     return r;
 }
 void MonoRecorder::Record(const Vec3&, float a, float t, float dist, int, int) {
@@ -234,6 +324,10 @@ void MonoRecorder::Record(const Vec3&, float a, float t, float dist, int, int) {
     const float width = std::sqrt(dist);
     splat(tracks[0], s, 2.0f * a / width, (int)std::ceil(width), false);
     has_samples = true;
+}
+bool MonoRecorder::Save(const std::string& fn, bool norm, float norm_max) {
+    const RecorderTrack* t = save_processed ? processed_tracks.at(0) : tracks.at(0);
+    return wave_write_mono(fn, t->data.data(), std::min<size_t>(t->getLength(), t->data.size()), norm, norm_max);
 }
 ear_recorder MonoRecorder::to_abi(int kf) const {
     ear_recorder r{};
@@ -266,6 +360,8 @@ Recorder* StereoRecorder::getBlankCopy(int) const {
     r->head_absorption = head_absorption;
     r->animation = animation;
     r->right_ear_animation = right_ear_animation;
+    r->filename = filename;
+    r->save_processed = save_processed;
     return r;
 }
 void StereoRecorder::Record(const Vec3& dir, float a, float t, float dist, int band, int kf) {
@@ -288,6 +384,12 @@ void StereoRecorder::Record(const Vec3& dir, float a, float t, float dist, int b
     splat(tracks[1], s_right, ar, w, true);
     has_samples = true;
 }
+bool StereoRecorder::Save(const std::string& fn, bool norm, float) {
+    const RecorderTrack* l = save_processed ? processed_tracks.at(0) : tracks.at(0);
+    const RecorderTrack* r = save_processed ? processed_tracks.at(1) : tracks.at(1);
+    return wave_write_stereo(fn, l->data.data(), std::min<size_t>(l->getLength(), l->data.size()), r->data.data(),
+                             std::min<size_t>(r->getLength(), r->data.size()), norm);
+}
 ear_recorder StereoRecorder::to_abi(int kf) const {
     ear_recorder r{};
     r.kind = EAR_REC_STEREO;
@@ -300,6 +402,63 @@ ear_recorder StereoRecorder::to_abi(int kf) const {
 }
 
 Vec3 AbstractSoundFile::getLocation(int i) const { return (i >= 0 && animation) ? animation->Evaluate(i) : location; }
+
+// ---------------------------------------------------------------- sound files
+float SoundFile::f1 = 0.2f;  // src/SoundFile.cpp:240-242
+float SoundFile::f2 = 1.0f;
+float SoundFile::f3 = 2.0f;
+
+SoundFile::SoundFile(std::vector<float> s, unsigned off) {
+    SetSamples(std::move(s));
+    offset = off;
+}
+SoundFile::SoundFile(const float* d, unsigned n, unsigned off) : data(d), sample_length(n) { offset = off; }
+void SoundFile::SetSamples(std::vector<float> s) {
+    samples = std::move(s);
+    data = samples.data();
+    sample_length = (unsigned)samples.size();
+    for (auto& b : bands_) b.reset();
+}
+bool SoundFile::Load(const std::string& wav) {
+    std::vector<float> s;
+    if (!wave_read(wav, &s)) return false;
+    filename = wav;
+    SetSamples(std::move(s));
+    return true;
+}
+SoundFile* SoundFile::Band(int i) {
+    if (i < 0 || i > 2) throw std::out_of_range("SoundFile::Band: band index");
+    if (!bands_[0]) {
+        // src/SoundFile.cpp:149-160: the split signals start at offset 0
+        std::vector<float> lo(sample_length), mi(sample_length), hi(sample_length);
+        equalizer_split(data, sample_length, lo.data(), mi.data(), hi.data(), f1 * 1000.0f, f2 * 1000.0f,
+                        f3 * 1000.0f);
+        bands_[0].reset(new SoundFile(std::move(lo), 0));
+        bands_[1].reset(new SoundFile(std::move(mi), 0));
+        bands_[2].reset(new SoundFile(std::move(hi), 0));
+    }
+    return bands_[i].get();
+}
+std::unique_ptr<SoundFile> SoundFile::Section(unsigned start, unsigned length) const {
+    // src/SoundFile.cpp:164-168
+    if (start >= sample_length) return std::unique_ptr<SoundFile>(new SoundFile(nullptr, 0, 0));
+    return std::unique_ptr<SoundFile>(new SoundFile(data + start, std::min(length, sample_length - start), offset + start));
+}
+std::unique_ptr<SoundFile> SoundFile::Section(float start, float length) const {
+    // src/SoundFile.cpp:169-173: seconds at 44.1 kHz, truncated
+    const unsigned s = (unsigned)(int)(start * 44100.0f);
+    const unsigned l = length < 0 ? sample_length - s : (unsigned)(int)(length * 44100.0f);
+    return Section(s, l);
+}
+int TripleBandSoundFile::Load(const std::string wav[3]) {
+    for (int i = 0; i < 3; ++i) {
+        std::vector<float> s;
+        if (!wave_read(wav[i], &s)) return i;
+        files[i].reset(new SoundFile(std::move(s), 0));  // src/SoundFile.cpp:88-92: offset 0
+        filenames[i] = wav[i];
+    }
+    return -1;
+}
 
 // ---------------------------------------------------------------- Scene
 Scene::Scene() = default;
@@ -322,6 +481,11 @@ void Scene::addMesh(Mesh* m) {
 }
 
 void Scene::addMaterial(Material* m) { Mesh::materials[m->name] = m; }
+
+ear_ctx* Scene::context() {
+    if (!ctx_) throw_on_error(ear_ctx_create(options.device, &ctx_), nullptr, "ear_ctx_create");
+    return ctx_;
+}
 
 void Scene::ensure_device() {
     if (!ctx_) throw_on_error(ear_ctx_create(options.device, &ctx_), nullptr, "ear_ctx_create");
@@ -350,7 +514,7 @@ void Scene::ensure_device() {
         }
         mid[i] = it->second;
     }
-    ear_bvh_opts o{options.leaf_size, options.bvh_width, {0, 0}};
+    ear_bvh_opts o{options.leaf_size, options.bvh_width, options.bvh_builder, 0};
     throw_on_error(ear_scene_upload(ctx_, v0.data(), v1.data(), v2.data(), mid.data(), (uint32_t)n,
                                     mats.empty() ? nullptr : mats.data(), (uint32_t)mats.size(), &o, &dev_scene_),
                    ctx_, "ear_scene_upload");

@@ -100,7 +100,11 @@ class Animated {
     const T& Evaluate(int i) const { return frames.at((size_t)i); }
 };
 
+class SoundFile;
+
 // RecorderTrack (src/Recorder.h:40-134): a float impulse-response track.
+// Writes through operator[] grow the buffer as FloatBuffer does (to the
+// index plus one second) and raise real_length to the largest index written.
 class RecorderTrack {
    public:
     std::vector<float> data;
@@ -109,10 +113,26 @@ class RecorderTrack {
     float at(unsigned i) const { return i < data.size() ? data[i] : 0.0f; }
     float Maximum() const;
     void Multiply(float f);
+    void Normalize(float M, float MAX);  // multiplies by M / MAX (MAX < 0: by M / Maximum())
     void Power(float a);
     void Truncate(unsigned l);
     unsigned getLength(float tresh = -1.0f) const;
     float T60() const;  // src/Recorder.cpp:303-340
+    // The empty FloatBuffer of the reference (src/Recorder.cpp:43-48): three
+    // seconds of zeros, first_sample at its last index, real_length 0.
+    void reset_buffer();
+    // Length of a fresh buffer after writes at first, first + 1, ..., last.
+    static size_t fresh_buffer_length(unsigned first, unsigned last);
+    // src/Recorder.cpp:287-293: adds other[i] for i below other's buffer
+    // length (getLength(0)), growing this buffer as needed.
+    void Add(const RecorderTrack* other);
+    // RecorderTrack::Process (src/Recorder.cpp:145-243): the first
+    // real_length samples convolved with the section, written from the
+    // section's offset into a new track; with `other` the section is faded
+    // out for this track and in for `other`, and other's result is added
+    // (the keyframe cross-fade).  The convolution runs on the device
+    // (ear_convolve).
+    RecorderTrack* Process(ear_ctx* ctx, const SoundFile& section, const RecorderTrack* other = nullptr) const;
 };
 
 // Recorder (src/Recorder.h:145-221).  Record() is the reference's per-sample
@@ -121,7 +141,9 @@ class RecorderTrack {
 class Recorder {
    public:
     std::vector<RecorderTrack*> tracks;
-    bool has_samples = false;
+    std::vector<RecorderTrack*> processed_tracks;  // tracks convolved with the sound
+    std::string filename;                           // the listener's output file
+    bool has_samples = false, is_processed = false, save_processed = false;
     virtual ~Recorder();
     virtual int trackCount() const = 0;
     virtual Vec3 getLocation(int keyframe = -1) const = 0;
@@ -134,6 +156,17 @@ class Recorder {
     void Truncate(int len);
     // Replaces track `channel` with the finalized histogram of a render.
     void AddHistogram(int channel, const double* values, size_t n);
+    // src/Recorder.cpp:342-364: each track convolved with the section of sf
+    // from `offset` seconds (to its end, or `length` seconds with the
+    // cross-fade into r's tracks), appended to processed_tracks.
+    void Process(ear_ctx* ctx, SoundFile* sf, float offset = 0.0f);
+    void Process(ear_ctx* ctx, SoundFile* sf, Recorder* r, float offset, float length);
+    void Add(Recorder* r);    // processed tracks summed (src/Recorder.cpp:433-442)
+    void Normalize(float M);  // largest |sample| of all tracks to M (src/Recorder.cpp:444-461)
+    // Writes a WAV of the processed tracks when save_processed, else of the
+    // tracks (src/MonoRecorder.cpp:69-78, src/StereoRecorder.cpp:81-91).
+    virtual bool Save(const std::string& fn, bool norm = true, float norm_max = -1.0f) = 0;
+    bool Save() { return Save(filename, false); }
 };
 
 class MonoRecorder : public Recorder {
@@ -147,6 +180,8 @@ class MonoRecorder : public Recorder {
     Recorder* getBlankCopy(int secs = -1) const override;
     void Record(const Vec3& dir, float a, float t, float dist, int band, int kf) override;
     ear_recorder to_abi(int keyframe) const override;
+    bool Save(const std::string& fn, bool norm = true, float norm_max = 1.0f) override;
+    using Recorder::Save;
 };
 
 class StereoRecorder : public Recorder {
@@ -164,24 +199,38 @@ class StereoRecorder : public Recorder {
     Recorder* getBlankCopy(int secs = -1) const override;
     void Record(const Vec3& dir, float a, float t, float dist, int band, int kf) override;
     ear_recorder to_abi(int keyframe) const override;
+    bool Save(const std::string& fn, bool norm = true, float norm_max = -1.0f) override;
+    using Recorder::Save;
 };
 
-// AbstractSoundFile (src/SoundFile.h:37-65): location or emitting mesh, gain.
-// Sample data, band splitting and sections are not part of the tracer.
+// AbstractSoundFile (src/SoundFile.h:37-65): location or emitting mesh, gain,
+// and the band signals the impulse responses are convolved with.
 class AbstractSoundFile {
    public:
     Vec3 location{0, 0, 0};
     Animated<Vec3>* animation = nullptr;
     Mesh* mesh = nullptr;
     float gain = 1.0f;
+    unsigned offset = 0;  // samples; read from the file, carried by no band (see Band)
     std::string filename;
     virtual ~AbstractSoundFile() = default;
     Vec3 getLocation(int i = -1) const;
     bool isMeshSource() const { return mesh != nullptr; }
     float getGain() const { return gain; }
+    virtual SoundFile* Band(int i) = 0;
 };
+
+// SoundFile (src/SoundFile.h:67-94, src/SoundFile.cpp): a mono signal at
+// 44.1 kHz.  Band(i) splits it once with the three-band equalizer at the
+// frequencies of SetEqBands (kHz; FREQ block, default 0.2 / 1 / 2); the band
+// signals start at offset 0, as the reference's.  Section views a span of the
+// samples and carries its start as offset.
 class SoundFile : public AbstractSoundFile {
    public:
+    std::vector<float> samples;    // owned samples (empty for a view)
+    const float* data = nullptr;   // samples.data() or a span of another file's
+    unsigned sample_length = 0;
+    static float f1, f2, f3;
     SoundFile(const Vec3& loc, float g = 1.0f) {
         location = loc;
         gain = g;
@@ -190,6 +239,40 @@ class SoundFile : public AbstractSoundFile {
         mesh = emitter;
         gain = g;
     }
+    SoundFile(std::vector<float> s, unsigned off);
+    SoundFile(const float* d, unsigned n, unsigned off);  // a view
+    SoundFile(const SoundFile&) = delete;
+    SoundFile& operator=(const SoundFile&) = delete;
+    void SetSamples(std::vector<float> s);
+    // Reads a WAV file (WaveFile::Load + ToFloat); false when it cannot.
+    bool Load(const std::string& wav);
+    SoundFile* Band(int i) override;
+    std::unique_ptr<SoundFile> Section(unsigned start, unsigned length) const;
+    std::unique_ptr<SoundFile> Section(float start, float length = -1.0f) const;
+    static void SetEqBands(float a, float b, float c) {
+        f1 = a;
+        f2 = b;
+        f3 = c;
+    }
+
+   private:
+    std::unique_ptr<SoundFile> bands_[3];
+};
+
+// TripleBandSoundFile (src/SoundFile.h:96-110): one WAV file per band, used
+// as given (no equalizer).
+class TripleBandSoundFile : public AbstractSoundFile {
+   public:
+    std::unique_ptr<SoundFile> files[3];
+    std::string filenames[3];
+    explicit TripleBandSoundFile(const Vec3& loc, float g = 1.0f) {
+        location = loc;
+        gain = g;
+    }
+    // Reads the three WAV files; returns the index of the first that cannot be
+    // read, or -1.
+    int Load(const std::string wav[3]);
+    SoundFile* Band(int i) override { return files[i].get(); }
 };
 
 // Run-time parameters the reference hard-codes or reads from SET.
@@ -199,7 +282,7 @@ struct RenderOptions {
     unsigned max_bounces = 999;   // src/Scene.cpp:143 (num_bounces < 1000)
     double energy_cutoff = 1e-8;  // src/Scene.cpp:275
     unsigned hist_bins = 3 * 44100;  // the reference's initial buffer (src/Recorder.h:36)
-    unsigned leaf_size = 0, bvh_width = 0;
+    unsigned leaf_size = 0, bvh_width = 0, bvh_builder = 0;  // 0: library defaults
 };
 
 // Scene (src/Scene.h:60-79).
@@ -222,6 +305,8 @@ class Scene {
     // the recorders of band b's context.
     void RenderBands(int sound, const float absorbtion_factor[3], int num_samples, float dry,
                      const std::vector<Recorder*> rec[3], int keyframeID = -1, unsigned band_mask = 7);
+    // The device context (created on first use), for the convolution stage.
+    ear_ctx* context();
     ear_stats last_stats{};
 
    private:

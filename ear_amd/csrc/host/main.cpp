@@ -1,12 +1,16 @@
 // main.cpp -- the EAR command line over the GPU tracer.
 //
-//   EAR render <file>     src/EAR.cpp:55-291: parse, render every
-//                         (sound, keyframe, band) context, Power(0.335),
-//                         threshold max/256, truncate; impulse responses are
-//                         written as raw float tracks to SET debugdir (the
-//                         FloatBuffer::Write format, src/Recorder.cpp:120-123).
-//                         Convolution with the sound files is not part of
-//                         this build.
+//   EAR render <file>     src/EAR.cpp:55-386: parse (reading the sources'
+//                         WAV files), render every (sound, keyframe, band)
+//                         context, Power(0.335), threshold max/256, truncate;
+//                         then convolve each response with its band of the
+//                         sound (keyframes cross-faded), sum per listener,
+//                         normalise to 0.8 and write the listener's WAV file.
+//                         SET debugdir adds the equalizer bands, the responses
+//                         (WAV and the raw FloatBuffer::Write format,
+//                         src/Recorder.cpp:120-123) and the per-context
+//                         results; SET noprocessing (int4 > 0) stops after
+//                         the responses.
 //   EAR calc T60 <file>   src/EAR.cpp:246-285, :412-423: mid band, first sound
 //                         and keyframe; prints T60 of the rendered response
 //                         beside the Sabine and Eyring predictions.
@@ -20,6 +24,7 @@
 #include <iostream>
 #include <sstream>
 
+#include "audio.h"
 #include "ear_file.h"
 #include "ear_host.h"
 
@@ -29,16 +34,33 @@ namespace {
 
 const char* kLoMiHi[] = {"low", "mid", "high"};
 
-Scene* build_scene(EarFile& F) {
-    Scene* scene = new Scene();
+// load_audio: read the sources' WAV files, as the reference's SSRC / 3SRC
+// readers do while parsing (src/SoundFile.cpp:33-42, :84-92).
+Scene* build_scene(EarFile& F, bool load_audio) {
+    std::unique_ptr<Scene> scene(new Scene());
     if (!F.keys.empty()) Keyframes::Set(F.keys);
+    SoundFile::SetEqBands(F.freq[0], F.freq[1], F.freq[2]);
     for (auto& m : F.materials) scene->addMaterial(m.get());
     for (auto& m : F.meshes) scene->addMesh(m.release());
     for (auto& s : F.sources) {
-        SoundFile* sf = new SoundFile(s.location, s.gain);
-        sf->filename = s.wav.empty() ? "" : s.wav[0];
-        if (!s.anim.empty()) sf->animation = new Animated<Vec3>(s.anim);
-        scene->addSoundSource(sf);
+        AbstractSoundFile* src;
+        if (s.wav.size() == 3) {
+            TripleBandSoundFile* t = new TripleBandSoundFile(s.location, s.gain);
+            scene->addSoundSource(t);
+            if (load_audio) {
+                const int bad = t->Load(s.wav.data());
+                if (bad >= 0) throw DatatypeException("Failed to open sound file " + s.wav[(size_t)bad]);
+            }
+            src = t;
+        } else {
+            SoundFile* sf = new SoundFile(s.location, s.gain);
+            scene->addSoundSource(sf);
+            sf->filename = s.wav.empty() ? "" : s.wav[0];
+            if (load_audio && !sf->Load(sf->filename)) throw DatatypeException("Failed to open sound file " + sf->filename);
+            src = sf;
+        }
+        src->offset = (unsigned)(s.offset * 44100.0f);
+        if (!s.anim.empty()) src->animation = new Animated<Vec3>(s.anim);
     }
     for (auto& l : F.listeners) {
         Recorder* r;
@@ -52,6 +74,7 @@ Scene* build_scene(EarFile& F) {
             if (!l.anim.empty()) mr->animation = new Animated<Vec3>(l.anim);
             r = mr;
         }
+        r->filename = l.wav;
         scene->addListener(r);
     }
     RenderOptions& o = scene->options;
@@ -60,7 +83,15 @@ Scene* build_scene(EarFile& F) {
     if (F.has("cutoff")) o.energy_cutoff = F.get_float("cutoff");
     if (F.has("irlength")) o.hist_bins = (unsigned)std::lround(F.get_float("irlength") * 44100.0f);
     if (F.has("device")) o.device = F.get_int("device");
-    return scene;
+    return scene.release();
+}
+
+std::string context_name(int sound, int keyframe, int band) {
+    std::stringstream ss;
+    ss << ".sound-" << sound;
+    if (keyframe != -1) ss << ".frame-" << (keyframe < 10 ? "0" : "") << keyframe;
+    ss << ".band-" << band;
+    return ss.str();
 }
 
 struct Ctx {
@@ -82,7 +113,22 @@ int Render(const std::string& filename, float* calc_T60 = nullptr, float* T60_Sa
     const Vec3 absorption = F.get_vec("absorption");
     const float dry_level = F.get_float("drylevel");
     const int num_samples = F.get_int("samples") / 10;  // Release build (src/EAR.cpp:78-82)
-    std::unique_ptr<Scene> scene(build_scene(F));
+    std::unique_ptr<Scene> scene(build_scene(F, true));
+    const bool has_debugdir = F.has("debugdir");
+    const std::string debugdir = has_debugdir ? F.get_string("debugdir") + "/" : "";
+    if (has_debugdir) {
+        // equalizer output (src/EAR.cpp:122-147)
+        for (size_t sf = 0; sf < scene->sources.size(); ++sf) {
+            for (int band = 0; band < 3; ++band) {
+                if (calc_T60 && band != 1) continue;
+                const SoundFile* b = scene->sources[sf]->Band(band);
+                std::stringstream ss;
+                ss << debugdir << "sound-" << sf << ".band-" << band << kLoMiHi[band] << ".wav";
+                wave_write_mono(ss.str(), b->data, b->sample_length, false);
+            }
+            if (calc_T60) break;
+        }
+    }
     if (scene->sources.empty()) {
         std::cout << std::endl << "No sound sources defined" << std::endl << std::endl;
         return 1;
@@ -127,18 +173,16 @@ int Render(const std::string& filename, float* calc_T60 = nullptr, float* T60_Sa
             for (RecorderTrack* t : r->tracks) mx = std::max(mx, t->Maximum());
         }
     const float treshold = mx / 256.0f;
-    const bool has_debugdir = F.has("debugdir");
     for (Ctx& c : ctxs) {
         int rec_id = 0;
         for (Recorder* r : c.recorders) {
             r->Truncate((int)r->getLength(treshold));
             if (has_debugdir) {
-                std::stringstream ss;
-                ss << F.get_string("debugdir") << "/response-" << rec_id << ".sound-" << c.sound;
-                if (c.keyframe != -1) ss << ".frame-" << (c.keyframe < 10 ? "0" : "") << c.keyframe;
-                ss << ".band-" << c.band << kLoMiHi[c.band] << ".bin";
+                const std::string base = debugdir + "response-" + std::to_string(rec_id) +
+                                         context_name(c.sound, c.keyframe, c.band) + kLoMiHi[c.band];
+                r->Save(base + ".wav", true, mx);
                 const RecorderTrack* t = r->tracks[0];
-                std::ofstream f(ss.str().c_str(), std::ios_base::binary);
+                std::ofstream f((base + ".bin").c_str(), std::ios_base::binary);
                 std::vector<float> d(t->data.begin(), t->data.end());
                 d.resize((size_t)t->real_length + 1, 0.0f);
                 f.write((const char*)d.data(), (std::streamsize)(sizeof(float) * d.size()));
@@ -147,7 +191,52 @@ int Render(const std::string& filename, float* calc_T60 = nullptr, float* T60_Sa
         }
     }
 
-    std::cout << std::endl << "Not processing data" << std::endl;
+    const bool noprocess = F.has("noprocessing") && F.get_int("noprocessing") > 0;
+    if (!noprocess && !calc_T60) {
+        std::cout << std::endl << "Processing data..." << std::endl;
+        ear_ctx* dev = scene->context();
+        // convolution of every context's responses with its band of the sound
+        // (src/EAR.cpp:297-327); a keyframe other than the last cross-fades
+        // into the same band's responses at the next keyframe
+        for (const Ctx& c : ctxs) {
+            SoundFile* band = scene->sources[(size_t)c.sound]->Band(c.band);
+            if (!keys) {
+                for (Recorder* r : c.recorders) r->Process(dev, band);
+                continue;
+            }
+            const float offset = keys->keys[(size_t)c.keyframe];
+            if (c.keyframe == (int)keys->keys.size() - 1) {
+                for (Recorder* r : c.recorders) r->Process(dev, band, offset);
+                continue;
+            }
+            const Ctx* next = nullptr;
+            for (const Ctx& d : ctxs)
+                if (d.sound == c.sound && d.keyframe == c.keyframe + 1 && d.band == c.band) next = &d;
+            const float length = keys->keys[(size_t)c.keyframe + 1] - offset;
+            for (size_t k = 0; k < c.recorders.size(); ++k)
+                c.recorders[k]->Process(dev, band, next->recorders.at(k), offset, length);
+        }
+        std::cout << "Merging result..." << std::endl;
+        // src/EAR.cpp:358-386.  The reference's Truncate(getLength(1e-6)) of
+        // the total acts on its unprocessed tracks only, so the saved file is
+        // the whole summed buffer.
+        for (size_t rec_id = 0; rec_id < scene->listeners.size(); ++rec_id) {
+            std::unique_ptr<Recorder> total(scene->listeners[rec_id]->getBlankCopy());
+            for (const Ctx& c : ctxs) {
+                Recorder* other = c.recorders[rec_id];
+                other->save_processed = true;
+                if (has_debugdir)
+                    other->Save(debugdir + "rec-" + std::to_string(rec_id) +
+                                context_name(c.sound, c.keyframe, c.band) + ".wav");
+                total->Add(other);
+            }
+            total->save_processed = true;
+            total->Normalize(0.8f);
+            if (!total->Save()) std::cout << "Failed to write " << total->filename << std::endl;
+        }
+    } else {
+        std::cout << std::endl << "Not processing data" << std::endl;
+    }
     if (calc_T60) {
         const Recorder* rec = ctxs.front().recorders.front();
         *calc_T60 = rec->tracks.front()->T60();
@@ -174,7 +263,7 @@ int Info(const std::string& filename) {
               << "\nsources " << F.sources.size() << "\nlisteners " << F.listeners.size() << "\nkeyframes "
               << F.keys.size() << "\nsettings " << F.settings.size() << std::endl;
     if (F.has("samples")) std::cout << "samples " << F.get_int("samples") << std::endl;
-    std::unique_ptr<Scene> scene(build_scene(F));
+    std::unique_ptr<Scene> scene(build_scene(F, false));
     if (!scene->meshes.empty()) {
         const Mesh* m = scene->meshes[0];
         std::cout.precision(9);

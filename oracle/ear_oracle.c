@@ -1221,3 +1221,65 @@ int or_gradient(const or_scene* s, const or_desc* D, const double* adj_signed, c
     unprep(&ps);
     return 0;
 }
+
+/* ======================================================================
+ * Auralization: Equalizer::Pass / Split (lib/equalizer/Equalizer.cpp:27-95).
+ * Single precision, each operation rounded in the order the reference
+ * writes it; tanf because the reference's C++ tan(float) is the float
+ * overload.  The coefficient expressions are the expansion of the squared
+ * Butterworth quadratic under the bilinear transform (checked in double
+ * against scipy's butter in the tests).
+ * ====================================================================== */
+void or_lr4_coefficients(float fc, int high_pass, float a[5], float b[5]) {
+    const float srate = 44100.0f, pi = 3.14285714285714f;
+    float wc = 2.0f * pi * srate;
+    float wc2 = wc * wc, wc3 = wc2 * wc, wc4 = wc2 * wc2;
+    float k = wc / tanf(pi * fc / srate);
+    float k2 = k * k, k3 = k2 * k, k4 = k2 * k2;
+    float r2 = sqrtf(2.0f);
+    float t1 = r2 * wc3 * k, t2 = r2 * wc * k3;
+    float den = 4.0f * wc2 * k2 + 2.0f * t1 + k4 + 2.0f * t2 + wc4;
+    b[0] = 1.0f;
+    b[1] = (4.0f * (wc4 + t1 - k4 - t2)) / den;
+    b[2] = (6.0f * wc4 - 8.0f * wc2 * k2 + 6.0f * k4) / den;
+    b[3] = (4.0f * (wc4 - t1 + t2 - k4)) / den;
+    b[4] = (k4 - 2.0f * t1 + wc4 - 2.0f * t2 + 4.0f * wc2 * k2) / den;
+    float g = high_pass ? k4 : wc4;
+    a[0] = g / den;
+    a[1] = (high_pass ? -4.0f : 4.0f) * g / den;
+    a[2] = 6.0f * g / den;
+    a[3] = a[1];
+    a[4] = a[0];
+}
+
+typedef struct {
+    float a[5], b[5], x[4], y[4];
+} or_lr4;
+
+static void lr4_init(or_lr4* f, float fc, int hp) {
+    memset(f, 0, sizeof(*f));
+    or_lr4_coefficients(fc, hp, f->a, f->b);
+}
+
+static float lr4_step(or_lr4* f, float in) {
+    float out = f->a[0] * in + f->a[1] * f->x[0] + f->a[2] * f->x[1] + f->a[3] * f->x[2] + f->a[4] * f->x[3] -
+                f->b[1] * f->y[0] - f->b[2] * f->y[1] - f->b[3] * f->y[2] - f->b[4] * f->y[3];
+    f->x[3] = f->x[2]; f->x[2] = f->x[1]; f->x[1] = f->x[0]; f->x[0] = in;
+    f->y[3] = f->y[2]; f->y[2] = f->y[1]; f->y[1] = f->y[0]; f->y[0] = out;
+    return out;
+}
+
+void or_equalizer_split(const float* x, uint32_t n, float* low, float* mid, float* high, float f1, float f2,
+                        float f3) {
+    const float fc1 = (f1 + f2) / 2.0f, fc2 = (f2 + f3) / 2.0f;
+    or_lr4 hp1, hp2, lp1, lp2;
+    lr4_init(&hp1, fc1, 1);
+    lr4_init(&hp2, fc2, 1);
+    lr4_init(&lp1, fc1, 0);
+    lr4_init(&lp2, fc2, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        high[i] = lr4_step(&hp2, x[i]);
+        low[i] = lr4_step(&lp1, x[i]);
+        mid[i] = lr4_step(&hp1, lr4_step(&lp2, x[i]));
+    }
+}

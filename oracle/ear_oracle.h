@@ -140,6 +140,14 @@ int or_gradient(const or_scene* s, const or_desc* d, const double* adj_signed, c
                 double* grad_refl, double* grad_spec);
 double or_now(void);
 
+/* ---- auralization: the three-band equalizer in single precision ---- */
+/* coefficients of one fourth-order Linkwitz-Riley section: numerator a[5],
+ * denominator b[5] (b[0] = 1) */
+void or_lr4_coefficients(float fc, int high_pass, float a[5], float b[5]);
+/* Equalizer::Split: low = LP(fc1), mid = HP(fc1) LP(fc2), high = HP(fc2) */
+void or_equalizer_split(const float* x, uint32_t n, float* low, float* mid, float* high, float f1, float f2,
+                        float f3);
+
 #ifdef __cplusplus
 }
 #endif

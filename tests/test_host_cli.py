@@ -116,13 +116,35 @@ def test_missing_sources_and_listeners(tmp_path):
     assert rc == 1 and "No sound sources defined" in out
 
 
+def test_missing_sound_file(tmp_path):
+    # the source's WAV file is read while parsing (src/SoundFile.cpp:33-42),
+    # before any device work
+    s = scenes.shoebox()
+    p = tmp_path / "m.EAR"
+    earfile.scene_to_ear(s, p, samples=1000, wav=str(tmp_path / "nope.wav"))
+    rc, out = run("render", str(p))
+    assert rc == 1 and "Failed to open sound file" in out
+
+
+def write_impulse_wav(path):
+    import wave
+
+    with wave.open(str(path), "wb") as w:
+        w.setnchannels(1)
+        w.setsampwidth(2)
+        w.setframerate(44100)
+        w.writeframes(np.array([16384] + [0] * 99, "<i2").tobytes())
+
+
 @pytest.mark.gpu
 def test_calc_t60_matches_python_front_end(tmp_path):
     from ear_amd import acoustics, api
 
     s = scenes.shoebox(spec=0.2)
     p = tmp_path / "box.EAR"
-    earfile.scene_to_ear(s, p, samples=200_000, seed=1234, maxbounces=200, irlength=2.0)
+    write_impulse_wav(tmp_path / "dry.wav")
+    earfile.scene_to_ear(s, p, samples=200_000, seed=1234, maxbounces=200, irlength=2.0,
+                         wav=str(tmp_path / "dry.wav"))
     rc, out = run("calc", "T60", str(p))
     assert rc == 0, out
     vals = {l.split(":")[0].strip(): float(l.split(":")[1].strip().rstrip("s")) for l in out.splitlines()
