@@ -123,7 +123,7 @@ class RecorderTrack {
     void reset_buffer();
     // Length of a fresh buffer after writes at first, first + 1, ..., last.
     static size_t fresh_buffer_length(unsigned first, unsigned last);
-    // src/Recorder.cpp:287-293: adds other[i] for i below other's buffer
+    // src/Recorder.cpp:294-301: adds other[i] for i below other's buffer
     // length (getLength(0)), growing this buffer as needed.
     void Add(const RecorderTrack* other);
     // RecorderTrack::Process (src/Recorder.cpp:145-243): the first

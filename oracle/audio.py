@@ -22,7 +22,7 @@ that pins it (tests/test_audio.py):
   of :172-181, and the FloatBuffer growth rules of src/Recorder.cpp:33-59.
   The convolution is pinned against np.convolve's direct sum; the buffer
   rules against hand-worked lengths.
-- add_track / normalize / listener_output: src/Recorder.cpp:287-293,
+- Track.add / listener_output: src/Recorder.cpp:294-301,
   :433-461, src/EAR.cpp:358-386.
 """
 from __future__ import annotations
