@@ -22,6 +22,7 @@ EAR_REC_MONO, EAR_REC_STEREO = 0, 1
 EAR_SRC_POINT, EAR_SRC_MESH = 0, 1
 EAR_FLAG_ABS_HIST, EAR_FLAG_INSTRUMENT, EAR_FLAG_TRACE_LOG, EAR_FLAG_ADJ_LOCAL = 1, 2, 4, 8
 EAR_FRAC_BITS = 88
+EAR_EST_NEE, EAR_EST_SPHERE = 0, 1
 
 # every symbol declared in include/ear.h
 EXPORTS = [
@@ -52,7 +53,7 @@ class ear_scene_info(C.Structure):
 
 class ear_recorder(C.Structure):
     _fields_ = [("kind", C.c_int32), ("pos", C.c_float * 3), ("right_ear", C.c_float * 3),
-                ("head_size", C.c_float), ("head_abs", C.c_float * 3)]
+                ("head_size", C.c_float), ("head_abs", C.c_float * 3), ("radius", C.c_float)]
 
 
 class ear_source(C.Structure):
@@ -66,7 +67,7 @@ class ear_render_desc(C.Structure):
                 ("max_bounces", C.c_uint32), ("energy_cutoff", C.c_double), ("seed", C.c_uint64),
                 ("instance", C.c_uint32), ("keyframe", C.c_int32), ("sound", C.c_uint32),
                 ("hist_bins", C.c_uint32), ("flags", C.c_uint32), ("trace_rays", C.c_uint32),
-                ("trace_segs", C.c_uint32)]
+                ("trace_segs", C.c_uint32), ("estimator", C.c_uint32)]
 
 
 class ear_hist(C.Structure):

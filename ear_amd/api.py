@@ -157,6 +157,7 @@ def make_recorders(recs):
         arr[i].right_ear[:] = [float(x) for x in r.get("right_ear", (1, 0, 0))]
         arr[i].head_size = float(r.get("head_size", 0.0))
         arr[i].head_abs[:] = [float(x) for x in r.get("head_abs", (1, 1, 1))]
+        arr[i].radius = float(r.get("radius", 0.0))
     return arr
 
 
@@ -201,7 +202,7 @@ class _Prepared:
     def __init__(self, *, recorders, n_rays, bands=(0, 1, 2), air=(0.99, 0.98, 0.95), dry=1.0, src_pos=(0, 0, 0),
                  src_tris=None, gain=1.0, max_bounces=999, cutoff=1e-8, seed=0xEA2, instance=0, keyframe=-1,
                  sound=0, n_bins=88200, abs_hist=True, instrument=False, trace_rays=0, trace_segs=0, want_raw=True,
-                 want_value=True):
+                 want_value=True, estimator=0):
         self.recs = make_recorders(recorders)
         nch = n_channels(recorders)
         d = F.ear_render_desc()
@@ -238,6 +239,7 @@ class _Prepared:
         d.flags = flags
         d.trace_rays = trace_rays
         d.trace_segs = trace_segs
+        d.estimator = estimator  # F.EAR_EST_NEE / F.EAR_EST_SPHERE (recorder "radius")
         self.desc = d
         self.bands = tuple(bands)
         self.value = np.zeros((3, nch, n_bins), np.float64) if want_value else None

@@ -237,6 +237,10 @@ ear_status ear_render_backward(ear_ctx* c, ear_scene* s, const ear_render_desc* 
         c->last_error = "scene belongs to another context";
         return EAR_ERR_INVALID_ARGUMENT;
     }
+    if (d->estimator != EAR_EST_NEE) {
+        c->last_error = "the backward pass covers the NEE estimator only";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
     if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
     try {
         return run_backward(c, &s->ds, d, adj_value, adj_abs, grad_refl, grad_spec);

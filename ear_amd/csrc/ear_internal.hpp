@@ -82,6 +82,10 @@ struct RecParams {
     float head_abs[3];
     int32_t kind;
     int32_t channel;  // first channel
+    // receiver spheres (EAR_EST_SPHERE): radius^2, 3 / (4 pi r^3), centres
+    // (mono: one; stereo: left and right channel)
+    float r2, inv_vol;
+    float cen[2][3];
 };
 
 struct RenderParams {
@@ -104,6 +108,7 @@ struct RenderParams {
     float src_total_area;
     uint32_t trace_rays, trace_segs;
     uint32_t flags;
+    uint32_t estimator;  // EAR_EST_*
     RecParams rec[kMaxRecorders];
 };
 

@@ -500,15 +500,22 @@ struct Hist {
 
 // Recorder::Record: MonoRecorder (src/MonoRecorder.cpp:83-97) or
 // StereoRecorder (src/StereoRecorder.cpp:97-130).
+// MonoRecorder::Record (src/MonoRecorder.cpp:83-97) into channel ch
+__device__ void record_channel(const Hist& H, int band, int ch, float a, float t, float dist) {
+    const bool split = H.kinds > 1;
+    const uint32_t kind = split && a < 0.0f ? 1u : 0u;
+    int s = (int)((double)t * 44100.0);
+    float width = sqrtf(dist);
+    float ampl = 2.0f * a / width;
+    int w = (int)ceilf(width);
+    splat_ramp(H.row(band, ch, kind), H.n_bins, s, w, split ? fabsf(ampl) : ampl);
+}
+
 __device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, float a, float t, float dist) {
     const bool split = H.kinds > 1;
     const uint32_t kind = split && a < 0.0f ? 1u : 0u;
     if (R.kind == EAR_REC_MONO) {
-        int s = (int)((double)t * 44100.0);
-        float width = sqrtf(dist);
-        float ampl = 2.0f * a / width;
-        int w = (int)ceilf(width);
-        splat_ramp(H.row(band, R.channel, kind), H.n_bins, s, w, split ? fabsf(ampl) : ampl);
+        record_channel(H, band, R.channel, a, t, dist);
     } else {
         float dt = dot(dir, V3{R.rx, R.ry, R.rz});
         float td = R.head_size / 343.0f;
@@ -526,6 +533,41 @@ __device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, floa
         splat_ramp(H.row(band, R.channel, kind), H.n_bins, s_left, w, split ? fabsf(al) : al);
         splat_ramp(H.row(band, R.channel + 1, kind), H.n_bins, s_right, w, split ? fabsf(ar) : ar);
     }
+}
+
+// Receiver spheres (EAR_EST_SPHERE): the segment o + s u, s in [0, seg),
+// deposits into every sphere it crosses I0 air^s0 (s1 - s0) / (4/3 pi r^3)
+// at distance L0 + s0, [s0, s1) being the chord clipped to the segment (the
+// track-length form of counting crossings).  j = reflections before the
+// segment: sign (-1)^j, and dry on a point source's first segment.  Returns
+// the number of deposits.
+__device__ uint32_t sphere_deposits(const Hist& H, const RenderParams& P, int band, V3 o, V3 u, float seg, float I0,
+                                    float L0, uint32_t j) {
+    uint32_t n = 0;
+    for (uint32_t r = 0; r < P.n_rec; ++r) {
+        const RecParams& R = P.rec[r];
+        const int nc = R.kind == EAR_REC_STEREO ? 2 : 1;
+        for (int e = 0; e < nc; ++e) {
+            const V3 oc = sub(o, V3{R.cen[e][0], R.cen[e][1], R.cen[e][2]});
+            const float b = dot(oc, u);
+            const float disc = b * b - (dot(oc, oc) - R.r2);
+            if (!(disc > 0.0f)) continue;
+            const float q = sqrtf(disc);
+            const float s0 = fmaxf(-b - q, 0.0f), s1 = fminf(-b + q, seg);
+            if (!(s1 > s0)) continue;
+            float c = I0;
+            c *= powf_ear(P.air[band], s0);
+            c *= s1 - s0;
+            c *= R.inv_vol;
+            if (j == 0 && P.src_kind == EAR_SRC_POINT) c *= P.dry;
+            if (!is_normal(c)) continue;
+            if (j & 1u) c *= -1.0f;
+            const float dist = L0 + s0;
+            record_channel(H, band, R.channel + e, c, dist / 343.0f, dist);
+            ++n;
+        }
+    }
+    return n;
 }
 
 // ---------------------------------------------------------------------------
@@ -660,7 +702,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__
             for (uint32_t r = 0; r < max_rec; ++r) {
                 bool emit = false;
                 float c = 0.0f;
-                if (valid && mesh_src && r < n_rec) {
+                if (valid && mesh_src && r < n_rec && inst[id].P.estimator == EAR_EST_NEE) {
                     const RecParams& R = inst[id].P.rec[r];
                     V3 dv = sub(V3{R.px, R.py, R.pz}, o);
                     float l = length(dv);
@@ -726,7 +768,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
                                                    unsigned long long* __restrict__ stats) {
     __shared__ uint32_t sc[4][kBlock / 64 + 1];  // scratch of the block-wide appends, alternated
     uint32_t n = *n_in;
-    uint32_t n_done = 0;
+    uint32_t n_done = 0, n_sph = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < n;
@@ -738,6 +780,8 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
         int bt = kReflect;
         int band = 0;
         uint32_t k = 0, id = 0, n_rec = 0;
+        float seg = __int_as_float(0x7f800000);  // segment length; +inf for a miss
+        bool nee = true;
         Rng rng;
         if (valid) {
             float4 ro = qin.o[i];
@@ -778,13 +822,20 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
                 I *= mb.z;
                 alive = is_normal(I);
                 prev = normalized(d);
+                seg = l;
+            }
+            if (P.estimator == EAR_EST_SPHERE) {
+                // hit or miss, the segment crosses the receiver spheres
+                n_sph += sphere_deposits(make_hist(In), P, band, V3{ro.x, ro.y, ro.z},
+                                         normalized(V3{rd.x, rd.y, rd.z}), seg, ro.w, rd.w, k - 1);
+                nee = false;
             }
         }
         // NEE to every recorder (src/Scene.cpp:185-268)
         for (uint32_t r = 0; r < max_rec; ++r) {
             bool emit = false;
             float c = 0.0f;
-            if (alive && r < n_rec) {
+            if (alive && nee && r < n_rec) {
                 const RenderParams& P = inst[id].P;
                 const RecParams& R = P.rec[r];
                 V3 dv = sub(V3{R.px, R.py, R.pz}, p);
@@ -840,6 +891,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
         n_done += valid ? 1u : 0u;
     }
     wave_add_u64(stats + ST_SEGMENTS, n_done);
+    wave_add_u64(stats + ST_CONTRIB, n_sph);
 }
 
 // Scene::Connect + Mesh::LineIntersection (src/Scene.cpp:84-96,
@@ -1059,7 +1111,7 @@ __global__ void k_direct(const InstDev* __restrict__ inst, int* gstack, uint32_t
     DirectRamp* o = In.direct + (size_t)band * P.n_channels + R.channel;
     int nch = R.kind == EAR_REC_STEREO ? 2 : 1;
     for (int c = 0; c < nch; ++c) o[c] = DirectRamp{0, 0, 0, 0.0f};
-    if (P.src_kind != EAR_SRC_POINT) return;
+    if (P.src_kind != EAR_SRC_POINT || P.estimator != EAR_EST_NEE) return;
     V3 rp{R.px, R.py, R.pz}, sp{P.src_pos[0], P.src_pos[1], P.src_pos[2]};
     float bt = 0.0f;
     int bs = -1;
@@ -1316,7 +1368,12 @@ ear_status make_params(Ctx* c, const ear_render_desc* D, RenderParams* P) {
     }
     for (int b = 0; b < 3; ++b)
         if (!std::isfinite(D->air[b])) return EAR_ERR_INVALID_ARGUMENT;
+    if (D->estimator > EAR_EST_SPHERE) {
+        c->last_error = "unknown estimator";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
     std::memset(P, 0, sizeof(*P));
+    P->estimator = D->estimator;
     P->seed = D->seed;
     P->instance = D->instance;
     for (int b = 0; b < 3; ++b) {
@@ -1350,6 +1407,27 @@ ear_status make_params(Ctx* c, const ear_render_desc* D, RenderParams* P) {
         Q.kind = R.kind;
         Q.channel = (int32_t)nch;
         nch += R.kind == EAR_REC_STEREO ? 2 : 1;
+        if (D->estimator == EAR_EST_SPHERE) {
+            // spheres: mono at pos; stereo head_size / 2 either side along
+            // the unit right-ear direction (left channel first)
+            const float rad = R.radius;
+            const float rn = sqrtf(R.right_ear[0] * R.right_ear[0] + R.right_ear[1] * R.right_ear[1] +
+                                   R.right_ear[2] * R.right_ear[2]);
+            if (!(rad > 0.0f) || !std::isfinite(rad) || (R.kind == EAR_REC_STEREO && !(rn > 0.0f))) {
+                c->last_error = "sphere estimator needs radius > 0 (and a right-ear direction for stereo)";
+                return EAR_ERR_INVALID_ARGUMENT;
+            }
+            Q.r2 = rad * rad;
+            Q.inv_vol = 3.0f / (4.0f * kPi * rad * rad * rad);
+            for (int k = 0; k < 3; ++k) Q.cen[0][k] = Q.cen[1][k] = R.pos[k];
+            if (R.kind == EAR_REC_STEREO) {
+                const float h = (0.5f * R.head_size) / rn;
+                for (int k = 0; k < 3; ++k) {
+                    Q.cen[0][k] = R.pos[k] - R.right_ear[k] * h;
+                    Q.cen[1][k] = R.pos[k] + R.right_ear[k] * h;
+                }
+            }
+        }
     }
     P->n_channels = nch;
     bool want_log = (D->flags & EAR_FLAG_TRACE_LOG) && D->trace_rays && D->trace_segs;

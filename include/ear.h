@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EAR_ABI_VERSION 2
+#define EAR_ABI_VERSION 3
 
 typedef enum {
     EAR_OK = 0,
@@ -136,6 +136,7 @@ typedef struct {
     float right_ear[3];  /* StereoRecorder::getRightEar(keyframe) */
     float head_size;
     float head_abs[3];   /* as stored by the StereoRecorder ctor: max(0,(1-h)^4) */
+    float radius;        /* EAR_EST_SPHERE: receiver sphere radius (m); unused otherwise */
 } ear_recorder;
 
 #define EAR_SRC_POINT 0 /* AbstractSoundFile with a location */
@@ -155,6 +156,22 @@ typedef struct {
 #define EAR_FLAG_ADJ_LOCAL 8u     /* ear_render_backward: keep only each vertex's own
                                      dependence on spec (see there) */
 
+/* Receiver estimators.  EAR_EST_NEE is the reference's: a connection from
+ * every hit point to every recorder, the shadow ray, the lobe-weighted
+ * contribution and the analytic direct sound (src/Scene.cpp:185-311).
+ * EAR_EST_SPHERE has no reference counterpart: each recorder is a sphere of
+ * `radius` (a stereo recorder two, centred head_size / 2 either side of pos
+ * along right_ear, one per channel, without the ITD/ILD model).  Every traced
+ * segment, including the first from the source and one that escapes, that
+ * crosses a sphere deposits I(s) * chord / (4/3 pi r^3) at the entry distance
+ * s (the track-length form of counting crossings; a full crossing averages
+ * I / (pi r^2)), with the a9 ramp of its channel, the sign (-1)^(reflections
+ * so far), and dry on the first segment of a point source.  No shadow rays,
+ * no analytic direct sound.  The expected deposit is the volume average of
+ * the intensity over the sphere, 1 / (4 pi d^2) in free field for r << d. */
+#define EAR_EST_NEE 0u
+#define EAR_EST_SPHERE 1u
+
 typedef struct {
     uint32_t band_mask;     /* bit b renders band b (0 low, 1 mid, 2 high) as its own context */
     float air[3];           /* absorbtion_factor per band = 1 - SET absorption[b] */
@@ -173,6 +190,7 @@ typedef struct {
     uint32_t flags;         /* EAR_FLAG_* */
     uint32_t trace_rays;    /* EAR_FLAG_TRACE_LOG: rays [0, trace_rays) of each band */
     uint32_t trace_segs;    /* segments logged per ray */
+    uint32_t estimator;     /* EAR_EST_NEE (default) or EAR_EST_SPHERE */
 } ear_render_desc;
 
 /* Histogram output, all host memory provided by the caller, laid out
@@ -240,7 +258,7 @@ ear_status ear_render_batch(ear_ctx* ctx, const ear_batch_item* items, uint32_t 
  * paths exactly (same Philox stream); tangents follow the specular lobe's
  * reparameterised directions with the hit triangles and hemisphere samples
  * held fixed.  Arrival-time shifts and visibility changes are not
- * differentiated.  Point sources, 1..8 materials.
+ * differentiated.  Point sources, 1..8 materials, EAR_EST_NEE.
  * With EAR_FLAG_ADJ_LOCAL the direction tangents are not carried to later
  * bounces: d/dspec keeps only the explicit blend term (lobe - diffuse) of the
  * connection at each vertex.  That estimate is biased but of bounded variance;

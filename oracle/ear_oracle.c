@@ -427,6 +427,10 @@ typedef struct {
     const float* src_nrm;
     int n_ch;
     const int* ch_of_rec; /* first channel of recorder r */
+    /* OR_EST_SPHERE: per recorder radius^2, 3 / (4 pi r^3), sphere centres */
+    const float* sph_r2;
+    const float* sph_inv_vol;
+    const float (*sph_cen)[2][3];
 } rctx;
 
 typedef struct {
@@ -509,6 +513,49 @@ static void record(const rctx* C, acc_t* acc, int r, const float dir[3], float a
     }
 }
 
+/* Receiver spheres (no reference counterpart): the segment o + s u, s in
+ * [0, seg), deposits into each sphere it crosses I0 air^s0 (s1 - s0) *
+ * 3 / (4 pi r^3) at distance L0 + s0, [s0, s1) the chord clipped to the
+ * segment; sign (-1)^j for j reflections so far, dry on a point source's
+ * first segment; each sphere is one channel with the mono ramp. */
+static void sphere_deposits(const rctx* C, acc_t* acc, const float o[3], const float u[3], float seg, float I0,
+                            float L0, uint32_t j) {
+    const or_desc* D = C->d;
+    uint32_t nb = D->n_bins;
+    for (uint32_t r = 0; r < D->n_rec; r++) {
+        int nc = D->rec[r].kind == OR_REC_STEREO ? 2 : 1;
+        for (int e = 0; e < nc; e++) {
+            float oc[3];
+            sub3(o, C->sph_cen[r][e], oc);
+            float b = dot3(oc, u);
+            float disc = b * b - (dot3(oc, oc) - C->sph_r2[r]);
+            if (!(disc > 0.0f)) continue;
+            float q = sqrtf(disc);
+            float s0 = fmaxf(-b - q, 0.0f), s1 = fminf(-b + q, seg);
+            if (!(s1 > s0)) continue;
+            float c = I0;
+            c *= or_pow(D->air, s0);
+            c *= s1 - s0;
+            c *= C->sph_inv_vol[r];
+            if (j == 0 && D->src_kind == OR_SRC_POINT) c *= D->dry;
+            if (!is_normal_f(c)) continue;
+            if (j & 1u) c *= -1.0f;
+            float dist = L0 + s0;
+            float t = dist / 343.0f;
+            int ch = C->ch_of_rec[r] + e;
+            int sb = (int)((double)t * 44100.0);
+            float width = sqrtf(dist);
+            float ampl = 2.0f * c / width;
+            int w = (int)ceilf(width);
+            acc->tally[r] += (double)c;
+            acc->tally_abs[r] += fabs((double)c);
+            add_ramp(C, acc, ch, sb, w, ampl);
+            if (acc->hf) ramp_f32(acc->hf + (size_t)ch * nb, nb, sb, w, ampl);
+            acc->contributions++;
+        }
+    }
+}
+
 /* emission from a mesh source: Mesh::SamplePoint + Triangle::SamplePoint +
  * 2-argument Sample_Hemi about the (unflipped) triangle normal. */
 static void emit_mesh(const rctx* C, or_rng* r, float o[3], float d[3]) {
@@ -574,6 +621,20 @@ static void trace_ray(const rctx* C, acc_t* acc, uint32_t ray, int32_t* log_hits
                 q[0] = o[0], q[1] = o[1], q[2] = o[2], q[3] = dd[0], q[4] = dd[1], q[5] = dd[2];
             }
             nseg++;
+            if (D->estimator == OR_EST_SPHERE) {
+                float u[3] = {dd[0], dd[1], dd[2]};
+                normalize3(u);
+                float seg = INFINITY;
+                if (hit) {
+                    float q[3], dq[3];
+                    q[0] = o[0] + dd[0] * t;
+                    q[1] = o[1] + dd[1] * t;
+                    q[2] = o[2] + dd[2] * t;
+                    sub3(q, o, dq);
+                    seg = len3(dq);
+                }
+                sphere_deposits(C, acc, o, u, seg, I, L, k - 1);
+            }
             if (!hit) break;
             float p[3];
             p[0] = o[0] + dd[0] * t;
@@ -618,7 +679,7 @@ static void trace_ray(const rctx* C, acc_t* acc, uint32_t ray, int32_t* log_hits
             I *= C->ret[mat][band];
         }
         if (!is_normal_f(I)) break;
-        if (k > 0 || mesh_src) {
+        if ((k > 0 || mesh_src) && D->estimator == OR_EST_NEE) {
             for (uint32_t r = 0; r < D->n_rec; r++) {
                 const float* RP = D->rec[r].pos;
                 float dv[3];
@@ -716,6 +777,28 @@ int or_render(const or_scene* s, const or_desc* D, or_out* out) {
     for (uint32_t i = 0; i < s->n_tris && rc == 0; i++)
         if (ps.mat[i] >= s->n_mat) rc = -4;
     int n_ch = or_num_channels(D);
+    if (D->estimator > OR_EST_SPHERE) rc = -6;
+    uint32_t nr_alloc = D->n_rec ? D->n_rec : 1;
+    float* sph_r2 = (float*)calloc(nr_alloc, sizeof(float));
+    float* sph_iv = (float*)calloc(nr_alloc, sizeof(float));
+    float (*sph_cen)[2][3] = (float (*)[2][3])calloc(nr_alloc, sizeof(float[2][3]));
+    for (uint32_t r = 0; r < D->n_rec && D->estimator == OR_EST_SPHERE; r++) {
+        const or_recorder* R = &D->rec[r];
+        float rad = R->radius;
+        float rn = sqrtf(R->right_ear[0] * R->right_ear[0] + R->right_ear[1] * R->right_ear[1] +
+                         R->right_ear[2] * R->right_ear[2]);
+        if (!(rad > 0.0f) || (R->kind == OR_REC_STEREO && !(rn > 0.0f))) rc = -6;
+        sph_r2[r] = rad * rad;
+        sph_iv[r] = 3.0f / (4.0f * OR_PI * rad * rad * rad);
+        for (int k = 0; k < 3; k++) sph_cen[r][0][k] = sph_cen[r][1][k] = R->pos[k];
+        if (R->kind == OR_REC_STEREO && rn > 0.0f) {
+            float h = (0.5f * R->head_size) / rn;
+            for (int k = 0; k < 3; k++) {
+                sph_cen[r][0][k] = R->pos[k] - R->right_ear[k] * h;
+                sph_cen[r][1][k] = R->pos[k] + R->right_ear[k] * h;
+            }
+        }
+    }
     int* ch_of = (int*)calloc(D->n_rec ? D->n_rec : 1, sizeof(int));
     {
         int c = 0;
@@ -742,6 +825,9 @@ int or_render(const or_scene* s, const or_desc* D, or_out* out) {
         free(ch_of);
         free(src_area);
         free(src_nrm);
+        free(sph_r2);
+        free(sph_iv);
+        free(sph_cen);
         unprep(&ps);
         return rc;
     }
@@ -757,6 +843,9 @@ int or_render(const or_scene* s, const or_desc* D, or_out* out) {
     C.src_nrm = src_nrm;
     C.n_ch = n_ch;
     C.ch_of_rec = ch_of;
+    C.sph_r2 = sph_r2;
+    C.sph_inv_vol = sph_iv;
+    C.sph_cen = (const float (*)[2][3])sph_cen;
 
     uint32_t nt = D->n_threads ? D->n_threads : 1;
     uint32_t nr = D->ray_end - D->ray_begin;
@@ -814,8 +903,9 @@ int or_render(const or_scene* s, const or_desc* D, or_out* out) {
     free(W);
     free(th);
 
-    /* direct sound, src/Scene.cpp:299-311: connect listener -> source */
-    if (D->src_kind == OR_SRC_POINT) {
+    /* direct sound, src/Scene.cpp:299-311: connect listener -> source
+     * (the sphere estimator collects the first segments instead) */
+    if (D->src_kind == OR_SRC_POINT && D->estimator == OR_EST_NEE) {
         for (uint32_t r = 0; r < D->n_rec; r++) {
             const or_recorder* rec = &D->rec[r];
             int ch = ch_of[r];
@@ -872,6 +962,9 @@ int or_render(const or_scene* s, const or_desc* D, or_out* out) {
     free(ch_of);
     free(src_area);
     free(src_nrm);
+    free(sph_r2);
+    free(sph_iv);
+    free(sph_cen);
     unprep(&ps);
     return 0;
 }

@@ -58,7 +58,13 @@ typedef struct {
     float right_ear[3];
     float head_size;
     float head_abs[3]; /* already max(0,(1-h)^4) as in the StereoRecorder ctor */
+    float radius;      /* OR_EST_SPHERE: receiver sphere radius */
 } or_recorder;
+
+/* receiver estimators: the reference's connections (NEE) or spheres that
+ * collect the traced segments crossing them (no reference counterpart) */
+#define OR_EST_NEE 0u
+#define OR_EST_SPHERE 1u
 
 typedef struct {
     uint32_t n_tris;
@@ -89,6 +95,7 @@ typedef struct {
     uint32_t n_bins;
     uint32_t n_threads;
     uint32_t trace_rays, trace_segs;
+    uint32_t estimator; /* OR_EST_* */
 } or_desc;
 
 typedef struct {

@@ -29,7 +29,7 @@ class Material(C.Structure):
 
 class Recorder(C.Structure):
     _fields_ = [("kind", C.c_int32), ("pos", C.c_float * 3), ("right_ear", C.c_float * 3),
-                ("head_size", C.c_float), ("head_abs", C.c_float * 3)]
+                ("head_size", C.c_float), ("head_abs", C.c_float * 3), ("radius", C.c_float)]
 
 
 class Scene(C.Structure):
@@ -44,7 +44,8 @@ class Desc(C.Structure):
                 ("ray_begin", C.c_uint32), ("ray_end", C.c_uint32), ("n_rays", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("cutoff", C.c_double), ("seed", C.c_uint64),
                 ("ctx", C.c_uint32), ("instance", C.c_uint32), ("n_bins", C.c_uint32),
-                ("n_threads", C.c_uint32), ("trace_rays", C.c_uint32), ("trace_segs", C.c_uint32)]
+                ("n_threads", C.c_uint32), ("trace_rays", C.c_uint32), ("trace_segs", C.c_uint32),
+                ("estimator", C.c_uint32)]
 
 
 class Out(C.Structure):
@@ -204,6 +205,7 @@ def make_recorders(recs):
         arr[i].right_ear[:] = [float(x) for x in r.get("right_ear", (1, 0, 0))]
         arr[i].head_size = float(r.get("head_size", 0.0))
         arr[i].head_abs[:] = [float(x) for x in r.get("head_abs", (1, 1, 1))]
+        arr[i].radius = float(r.get("radius", 0.0))
     return arr
 
 
@@ -214,8 +216,9 @@ def n_channels(recs):
 def render(scene: OracleScene, *, band, air, recorders, n_rays, src_pos=(0, 0, 0), src_tris=None,
            dry=1.0, gain=1.0, ray_begin=0, ray_end=None, max_bounces=999, cutoff=1e-8, seed=0xEA2,
            ctx=None, instance=0, n_bins=88200, n_threads=1, trace_rays=0, trace_segs=0,
-           want_f32=False, want_abs=True):
-    """One reference render context (sound, keyframe, band); returns a dict of raw sums."""
+           want_f32=False, want_abs=True, estimator=0):
+    """One reference render context (sound, keyframe, band); returns a dict of
+    raw sums.  estimator 1: receiver spheres (recorder "radius")."""
     if ray_end is None:
         ray_end = n_rays
     if ctx is None:
@@ -236,6 +239,7 @@ def render(scene: OracleScene, *, band, air, recorders, n_rays, src_pos=(0, 0, 0
     d.max_bounces, d.cutoff, d.seed = max_bounces, cutoff, seed
     d.ctx, d.instance, d.n_bins, d.n_threads = ctx, instance, n_bins, n_threads
     d.trace_rays, d.trace_segs = trace_rays, trace_segs
+    d.estimator = estimator
     hs = np.zeros((nch, n_bins), np.float64)
     ha = np.zeros((nch, n_bins), np.float64) if want_abs else None
     hf = np.zeros((nch, n_bins), np.float32) if want_f32 else None
