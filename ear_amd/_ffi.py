@@ -29,7 +29,7 @@ EXPORTS = [
     "ear_ctx_create", "ear_ctx_destroy", "ear_ctx_set_memory_cap", "ear_status_string", "ear_ctx_last_error",
     "ear_abi_version", "ear_scene_upload", "ear_scene_destroy", "ear_scene_get_info", "ear_query_closest",
     "ear_query_occluded", "ear_render", "ear_render_batch", "ear_ctx_set_tuning", "ear_ctx_get_tuning",
-    "ear_render_backward", "ear_convolve",
+    "ear_render_backward", "ear_convolve", "ear_render_backward_batch", "ear_scene_set_materials",
 ]
 
 
@@ -89,6 +89,11 @@ class ear_batch_item(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("desc", C.POINTER(ear_render_desc)), ("out", C.POINTER(ear_hist))]
 
 
+class ear_backward_item(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("desc", C.POINTER(ear_render_desc)), ("adj_value", C.c_void_p),
+                ("adj_abs", C.c_void_p), ("grad_refl", C.c_void_p), ("grad_spec", C.c_void_p)]
+
+
 class ear_tuning(C.Structure):
     _fields_ = [("block_extend", C.c_uint32), ("block_shade", C.c_uint32), ("block_shadow", C.c_uint32),
                 ("ray_chunk", C.c_uint32), ("waves_per_cu", C.c_uint32), ("stack_lds", C.c_uint32),
@@ -132,6 +137,8 @@ def load(path: str | None = None):
     L.ear_render_batch.argtypes = [vp, C.POINTER(ear_batch_item), C.c_uint32, C.POINTER(ear_stats)]
     L.ear_render_backward.argtypes = [vp, vp, C.POINTER(ear_render_desc), vp, vp, vp, vp]
     L.ear_convolve.argtypes = [vp, fp, C.c_uint32, fp, C.c_uint32, fp, C.c_uint32, fp]
+    L.ear_render_backward_batch.argtypes = [vp, C.POINTER(ear_backward_item), C.c_uint32]
+    L.ear_scene_set_materials.argtypes = [vp, C.POINTER(ear_material), C.c_uint32]
     L.ear_ctx_set_tuning.argtypes = [vp, C.POINTER(ear_tuning)]
     L.ear_ctx_get_tuning.argtypes = [vp, C.POINTER(ear_tuning)]
     for name in EXPORTS:

@@ -112,6 +112,11 @@ class Scene:
                                          len(materials), C.byref(opts), C.byref(h)), ctx.h)
         self.h = h
 
+    def set_materials(self, materials):
+        """Replaces the material table (same count) without rebuilding the BVH."""
+        mats = make_materials(materials)
+        _check(self.ctx.lib, self.ctx.lib.ear_scene_set_materials(self.h, mats, len(materials)), self.ctx.h)
+
     def close(self):
         if self.h:
             self.ctx.lib.ear_scene_destroy(self.h)
@@ -308,6 +313,30 @@ def render_backward(ctx: Context, scene: Scene, *, adj_value=None, adj_abs=None,
     _check(ctx.lib, ctx.lib.ear_render_backward(ctx.h, scene.h, C.byref(p.desc), _ptr(a_v), _ptr(a_a), _ptr(g_r),
                                                 _ptr(g_s)), ctx.h)
     return g_r, g_s
+
+
+def render_backward_batch(ctx: Context, items):
+    """The adjoint of many renders at once (ear_render_backward_batch).
+    items: sequence of (scene, kw) with kw as for render_backward (adj_value,
+    adj_abs, local and the render keywords).  Returns [(d/drefl, d/dspec)]."""
+    keep, arr = [], (F.ear_backward_item * max(1, len(items)))()
+    outs = []
+    for i, (scene, kw) in enumerate(items):
+        kw = dict(kw)
+        adj_value, adj_abs, local = kw.pop("adj_value", None), kw.pop("adj_abs", None), kw.pop("local", False)
+        p = _Prepared(want_raw=False, want_value=False, **kw)
+        if local:
+            p.desc.flags |= F.EAR_FLAG_ADJ_LOCAL
+        shape = (3, n_channels(kw["recorders"]), p.desc.hist_bins)
+        a_v = None if adj_value is None else np.ascontiguousarray(np.broadcast_to(adj_value, shape), np.float64)
+        a_a = None if adj_abs is None else np.ascontiguousarray(np.broadcast_to(adj_abs, shape), np.float64)
+        g_r = np.zeros((scene.n_mat, 3), np.float64)
+        g_s = np.zeros((scene.n_mat, 3), np.float64)
+        keep.append((p, a_v, a_a))
+        outs.append((g_r, g_s))
+        arr[i] = F.ear_backward_item(scene.h, C.pointer(p.desc), _ptr(a_v), _ptr(a_a), _ptr(g_r), _ptr(g_s))
+    _check(ctx.lib, ctx.lib.ear_render_backward_batch(ctx.h, arr, len(items)), ctx.h)
+    return outs
 
 
 def convolve(ctx: Context, ir, signal, ir2=None) -> np.ndarray:

@@ -153,6 +153,12 @@ ear_status ear_scene_destroy(ear_scene* s) {
     return EAR_OK;
 }
 
+ear_status ear_scene_set_materials(ear_scene* s, const ear_material* mats, uint32_t n_mat) {
+    if (!s) return EAR_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(s->ctx->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    return set_scene_materials(s->ctx, &s->ds, mats, n_mat);
+}
+
 ear_status ear_scene_get_info(const ear_scene* s, ear_scene_info* info) {
     if (!s || !info) return EAR_ERR_INVALID_ARGUMENT;
     std::memset(info, 0, sizeof(*info));
@@ -244,6 +250,42 @@ ear_status ear_render_backward(ear_ctx* c, ear_scene* s, const ear_render_desc* 
     if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
     try {
         return run_backward(c, &s->ds, d, adj_value, adj_abs, grad_refl, grad_spec);
+    } catch (const std::bad_alloc&) {
+        c->last_error = "host allocation failed";
+        return EAR_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        c->last_error = "internal exception";
+        return EAR_ERR_INTERNAL;
+    }
+}
+
+ear_status ear_render_backward_batch(ear_ctx* c, const ear_backward_item* items, uint32_t n_items) {
+    if (!c || !items || n_items == 0) return EAR_ERR_INVALID_ARGUMENT;
+    std::vector<DeviceScene*> sc(n_items);
+    std::vector<const ear_render_desc*> ds(n_items);
+    std::vector<const double*> av(n_items), aa(n_items);
+    std::vector<double*> gr(n_items), gsp(n_items);
+    for (uint32_t i = 0; i < n_items; ++i) {
+        const ear_backward_item& it = items[i];
+        if (!it.scene || !it.desc) return EAR_ERR_INVALID_ARGUMENT;
+        if (it.scene->ctx != c) {
+            c->last_error = "scene belongs to another context";
+            return EAR_ERR_INVALID_ARGUMENT;
+        }
+        if (it.desc->estimator != EAR_EST_NEE) {
+            c->last_error = "the backward pass covers the NEE estimator only";
+            return EAR_ERR_INVALID_ARGUMENT;
+        }
+        sc[i] = &it.scene->ds;
+        ds[i] = it.desc;
+        av[i] = it.adj_value;
+        aa[i] = it.adj_abs;
+        gr[i] = it.grad_refl;
+        gsp[i] = it.grad_spec;
+    }
+    if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    try {
+        return run_backward_batch(c, n_items, sc.data(), ds.data(), av.data(), aa.data(), gr.data(), gsp.data());
     } catch (const std::bad_alloc&) {
         c->last_error = "host allocation failed";
         return EAR_ERR_OUT_OF_MEMORY;

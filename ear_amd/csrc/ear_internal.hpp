@@ -132,6 +132,7 @@ ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2
                        uint32_t n_tris, const ear_material* mats, uint32_t n_mat, uint32_t leaf_size,
                        uint32_t width, uint32_t builder, DeviceScene* out);
 void free_scene(Ctx* c, DeviceScene* s);
+ear_status set_scene_materials(Ctx* c, DeviceScene* s, const ear_material* mats, uint32_t n_mat);
 
 // tracer.hip
 ear_status run_render(Ctx* c, DeviceScene* s, const ear_render_desc* d, ear_hist* out, ear_stats* st);
@@ -139,6 +140,9 @@ ear_status run_convolve(Ctx* c, const float* const* tracks, const uint32_t* n_tr
                         uint32_t terms, uint32_t n_sig, float* out, uint32_t n_out);
 ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* d, const double* adj_value,
                         const double* adj_abs, double* grad_refl, double* grad_spec);
+ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes,
+                              const ear_render_desc* const* descs, const double* const* adj_value,
+                              const double* const* adj_abs, double* const* grad_refl, double* const* grad_spec);
 ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const ear_render_desc* const* descs,
                      ear_hist* const* outs, ear_stats* st);
 ear_status run_query_closest(Ctx* c, DeviceScene* s, uint32_t n, const float* o, const float* d, float* t,

@@ -698,6 +698,46 @@ static ear_status build_sah(Ctx* c, const float* v0, const float* v1, const floa
     return EAR_OK;
 }
 
+// Materials: the Material ctor (src/Material.cpp:26-72) and Material::Bounce
+// (src/Material.cpp:76-83) folded into one row per band:
+// {refl / (refl + refr) or -1, spec, retention, refr}.
+ear_status material_table(Ctx* c, const ear_material* mats, uint32_t n_mat, std::vector<float4>* out) {
+    std::vector<float4>& mb = *out;
+    mb.assign(3 * (size_t)std::max<uint32_t>(1, n_mat), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (uint32_t m = 0; m < n_mat; ++m) {
+        for (int b = 0; b < 3; ++b) {
+            float fl = mats[m].refl[b];
+            float fr = (mats[m].flags & EAR_MAT_HAS_REFR) ? mats[m].refr[b] : 0.0f;
+            float sp = (mats[m].flags & EAR_MAT_HAS_SPEC) ? mats[m].spec[b] : 0.0f;
+            float a = 1.0f;
+            a -= fl - 1e-9f;
+            if (mats[m].flags & EAR_MAT_HAS_REFR) a -= fr - 1e-9f;
+            if (!(a >= 0.0f)) {
+                c->last_error = "Invalid material settings (refl + refr > 1)";
+                return EAR_ERR_INVALID_MATERIAL;
+            }
+            float ret = 1.0f - a;
+            float fl2 = ((double)fl < 0.0001 && (double)fr < 0.0001) ? -1.0f : fl / (fl + fr);
+            mb[3 * m + b] = make_float4(fl2, sp, ret, fr);
+        }
+    }
+    return EAR_OK;
+}
+
+ear_status set_scene_materials(Ctx* c, DeviceScene* s, const ear_material* mats, uint32_t n_mat) {
+    if (!mats || n_mat != s->n_mat) {
+        c->last_error = "the material count must equal the scene's";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    std::vector<float4> mb;
+    TRY(material_table(c, mats, n_mat, &mb));
+    s->mat_band_host = mb;
+    hipStream_t st = c->stream;
+    EAR_HIP_TRY(c, hipMemcpyAsync(s->mat_band, mb.data(), sizeof(float4) * mb.size(), hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipStreamSynchronize(st));
+    return EAR_OK;
+}
+
 ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2, const uint16_t* mat_id,
                        uint32_t n, const ear_material* mats, uint32_t n_mat, uint32_t leaf_size, uint32_t width,
                        uint32_t builder, DeviceScene* out) {
@@ -728,26 +768,8 @@ ear_status build_scene(Ctx* c, const float* v0, const float* v1, const float* v2
         c->last_error = "leaf_size must be 1..16";
         return EAR_ERR_INVALID_ARGUMENT;
     }
-    // materials: Material ctor (src/Material.cpp:26-72) and Material::Bounce
-    // (src/Material.cpp:76-83) folded into one row per band
-    std::vector<float4> mb(3 * (size_t)std::max<uint32_t>(1, n_mat));
-    for (uint32_t m = 0; m < n_mat; ++m) {
-        for (int b = 0; b < 3; ++b) {
-            float fl = mats[m].refl[b];
-            float fr = (mats[m].flags & EAR_MAT_HAS_REFR) ? mats[m].refr[b] : 0.0f;
-            float sp = (mats[m].flags & EAR_MAT_HAS_SPEC) ? mats[m].spec[b] : 0.0f;
-            float a = 1.0f;
-            a -= fl - 1e-9f;
-            if (mats[m].flags & EAR_MAT_HAS_REFR) a -= fr - 1e-9f;
-            if (!(a >= 0.0f)) {
-                c->last_error = "Invalid material settings (refl + refr > 1)";
-                return EAR_ERR_INVALID_MATERIAL;
-            }
-            float ret = 1.0f - a;
-            float fl2 = ((double)fl < 0.0001 && (double)fr < 0.0001) ? -1.0f : fl / (fl + fr);
-            mb[3 * m + b] = make_float4(fl2, sp, ret, fr);
-        }
-    }
+    std::vector<float4> mb;
+    TRY(material_table(c, mats, n_mat, &mb));
     for (uint32_t i = 0; i < n; ++i) {
         if (mat_id[i] >= n_mat) {
             c->last_error = "material id out of range";

@@ -2044,16 +2044,17 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 // fixed, and of log I with respect to every reflection coefficient.  Each
 // visible connection adds (dL/dc)(dc/dtheta) to the gradient.  Arrival-time
 // shifts and visibility changes are not differentiated.
-template <int WIDTH>
+// BATCH: the queue holds the rays of several instances (rm.w), each with
+// its own adjoint and gradient (adj[id]); the per-lane gradient partials are
+// then reduced per instance at the end of every grid-stride step.
+template <int WIDTH, bool BATCH>
 __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
                                                       Queue qin, const uint32_t* __restrict__ n_in,
                                                       const int2* __restrict__ hits, Tangents tin, Queue qout,
-                                                      uint32_t* __restrict__ n_out, Tangents tout, AdjArgs A,
+                                                      uint32_t* __restrict__ n_out, Tangents tout, AdjArgs A0,
+                                                      const AdjArgs* __restrict__ adj,
                                                       unsigned long long* __restrict__ stats) {
     __shared__ int lds[kLdsMin * kBlock];
-    const InstDev& In = inst[0];
-    const RenderParams& P = In.P;
-    const uint32_t nm = A.n_mat;
     uint32_t n = *n_in;
     uint32_t n_done = 0;
     uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -2061,6 +2062,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
     for (int q = 0; q < kAdMaxMat; ++q) g_refl[q] = g_spec[q] = 0.0;
     int band_acc = -1;
     uint32_t nvis = 0, ntri = 0;
+    uint32_t my_id = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         uint32_t i = base + threadIdx.x;
         bool cont = false;
@@ -2068,10 +2070,17 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
         uint4 nmeta{};
         float dtan[kAdMaxMat][8];
         float drefl[kAdMaxMat];
+        uint32_t nm = A0.n_mat;
         if (i < n) {
             float4 ro = qin.o[i];
             float4 rd = qin.d[i];
             uint4 rm = qin.m[i];
+            const uint32_t id = BATCH ? rm.w : 0u;
+            my_id = id;
+            const InstDev& In = inst[id];
+            const RenderParams& P = In.P;
+            const AdjArgs& A = BATCH ? adj[id] : A0;
+            nm = A.n_mat;
             int band = (int)(rm.z & 0xFFu);
             band_acc = band;
             uint32_t k = ((rm.z >> 8) & 0xFFFFu) + 1;
@@ -2243,9 +2252,32 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
                     }
                     no = make_float4(p.x, p.y, p.z, I);
                     nd = make_float4(v.x, v.y, v.z, L);
-                    nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), 0u);
+                    nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), id);
                 }
             }
+        }
+        if (BATCH) {
+            // reduce the step's partials per instance present in the wave
+            const uint32_t nmx = adj[my_id].n_mat;
+            uint64_t todo = __ballot(i < n);
+            while (todo) {
+                const int leader = __ffsll((unsigned long long)todo) - 1;
+                const uint32_t key = __shfl(my_id, leader);
+                const int kband = __shfl(band_acc, leader);
+                const bool mine = i < n && my_id == key;
+                const uint32_t nmk = __shfl(nmx, leader);
+                for (uint32_t q = 0; q < nmk; ++q) {
+                    const double sr = wave_sum_d(mine ? g_refl[q] : 0.0);
+                    const double ss = wave_sum_d(mine ? g_spec[q] : 0.0);
+                    if (lane_id() == 0) {
+                        double* gk = adj[key].grad;
+                        if (sr != 0.0) atomicAdd(gk + ((size_t)kband * 2 + 0) * nmk + q, sr);
+                        if (ss != 0.0) atomicAdd(gk + ((size_t)kband * 2 + 1) * nmk + q, ss);
+                    }
+                }
+                todo &= ~__ballot(mine);
+            }
+            for (int q = 0; q < kAdMaxMat; ++q) g_refl[q] = g_spec[q] = 0.0;
         }
         uint32_t slot = wave_append(n_out, cont);
         if (cont) {
@@ -2260,15 +2292,17 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
         n_done += i < n ? 1u : 0u;
     }
     wave_add_u64(stats + ST_SEGMENTS, n_done);
-    // one band per render call of the adjoint (enforced on the host)
+    if (BATCH) return;
+    // one band per launch (the host replays the bands one after another)
     int band = __shfl(band_acc, 0);
     for (int l = 1; l < 64 && band < 0; ++l) band = __shfl(band_acc, l);
+    const uint32_t nm = A0.n_mat;
     for (uint32_t q = 0; q < nm; ++q) {
         double sr = wave_sum_d(g_refl[q]);
         double ss = wave_sum_d(g_spec[q]);
         if (lane_id() == 0 && band >= 0) {
-            if (sr != 0.0) atomicAdd(A.grad + ((size_t)band * 2 + 0) * nm + q, sr);
-            if (ss != 0.0) atomicAdd(A.grad + ((size_t)band * 2 + 1) * nm + q, ss);
+            if (sr != 0.0) atomicAdd(A0.grad + ((size_t)band * 2 + 0) * nm + q, sr);
+            if (ss != 0.0) atomicAdd(A0.grad + ((size_t)band * 2 + 1) * nm + q, ss);
         }
     }
 }
@@ -2281,63 +2315,95 @@ __global__ void k_zero_f32(float* p, size_t n) {
 }  // namespace
 
 // Gradient of sum(adj_value * value) + sum(adj_abs * abs_value) with respect
-// to refl and spec of every material, for each band of the mask.  The bands
-// are replayed one after another: the tangent arrays and the per-wave
-// gradient partials belong to one band at a time.
-ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const double* adj_value,
-                        const double* adj_abs, double* grad_refl, double* grad_spec) {
-    RenderParams P;
-    TRY(make_params(c, D, &P));
-    if (D->source.kind != EAR_SRC_POINT) {
-        c->last_error = "the adjoint supports point sources";
-        return EAR_ERR_INVALID_ARGUMENT;
+// to refl and spec of every material, for each band of the mask, for every
+// item of a batch.  The bands are replayed one after another (the tangent
+// arrays and the per-wave partials belong to one band at a time); within a
+// band the rays of all items share the queues, in chunks of the ray chunk.
+ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes,
+                              const ear_render_desc* const* descs, const double* const* adj_value,
+                              const double* const* adj_abs, double* const* grad_refl, double* const* grad_spec) {
+    if (n_items == 0) return EAR_ERR_INVALID_ARGUMENT;
+    const bool batch = n_items > 1;
+    const int width = (int)scenes[0]->width;
+    std::vector<InstDev> inst(n_items);
+    uint32_t nm_max = 0, max_rec = 0, max_bounces = 0;
+    size_t adj_cells = 0;
+    std::vector<size_t> adj_off(n_items);
+    for (uint32_t i = 0; i < n_items; ++i) {
+        const DeviceScene* s = scenes[i];
+        const ear_render_desc* D = descs[i];
+        if (!s || !D) return EAR_ERR_INVALID_ARGUMENT;
+        if ((int)s->width != width) {
+            c->last_error = "all scenes of a batch must have the same BVH width";
+            return EAR_ERR_INVALID_ARGUMENT;
+        }
+        InstDev& I = inst[i];
+        std::memset(&I, 0, sizeof(I));
+        TRY(make_params(c, D, &I.P));
+        if (D->source.kind != EAR_SRC_POINT) {
+            c->last_error = "the adjoint supports point sources";
+            return EAR_ERR_INVALID_ARGUMENT;
+        }
+        if (s->n_mat > (uint32_t)kAdMaxMat || s->n_mat == 0) {
+            c->last_error = "the adjoint supports 1..8 materials";
+            return EAR_ERR_CAPACITY;
+        }
+        if (!adj_value[i] && !adj_abs[i]) {
+            c->last_error = "no adjoint given";
+            return EAR_ERR_INVALID_ARGUMENT;
+        }
+        I.nodes = s->nodes;
+        I.nodes4 = s->nodes4;
+        I.tris = s->tris;
+        I.nm = s->nm;
+        I.mat_band = s->mat_band;
+        I.n_tris = s->n_tris;
+        nm_max = std::max(nm_max, s->n_mat);
+        max_rec = std::max(max_rec, I.P.n_rec);
+        max_bounces = std::max(max_bounces, D->max_bounces);
+        adj_off[i] = adj_cells;
+        adj_cells += (size_t)3 * I.P.n_channels * I.P.n_bins;
     }
-    if (s->n_mat > (uint32_t)kAdMaxMat || s->n_mat == 0) {
-        c->last_error = "the adjoint supports 1..8 materials";
-        return EAR_ERR_CAPACITY;
-    }
-    if (!adj_value && !adj_abs) {
-        c->last_error = "no adjoint given";
-        return EAR_ERR_INVALID_ARGUMENT;
-    }
-    InstDev I;
-    std::memset(&I, 0, sizeof(I));
-    I.P = P;
-    I.nodes = s->nodes;
-    I.nodes4 = s->nodes4;
-    I.tris = s->tris;
-    I.nm = s->nm;
-    I.mat_band = s->mat_band;
-    I.n_tris = s->n_tris;
-    const uint32_t nm = s->n_mat;
-    const uint32_t nch = P.n_channels, nb = P.n_bins;
     // value = raw * (1/N) * gain^2 (k_finalize), so dL/draw = dL/dvalue * (1/N) * gain^2
-    const double scale = (double)(1.0f / (float)std::max<uint32_t>(1, D->n_rays)) *
-                         (double)(D->source.gain * D->source.gain);
-    const size_t hn = (size_t)3 * nch * nb;
-    std::vector<double> as, aa;
-    if (adj_value) {
-        as.assign(adj_value, adj_value + hn);
-        for (double& x : as) x *= scale;
-    }
-    if (adj_abs) {
-        aa.assign(adj_abs, adj_abs + hn);
-        for (double& x : aa) x *= scale;
+    std::vector<double> as(adj_cells, 0.0), aa(adj_cells, 0.0);
+    bool any_s = false, any_a = false;
+    for (uint32_t i = 0; i < n_items; ++i) {
+        const ear_render_desc* D = descs[i];
+        const double scale = (double)(1.0f / (float)std::max<uint32_t>(1, D->n_rays)) *
+                             (double)(D->source.gain * D->source.gain);
+        const size_t hn = (size_t)3 * inst[i].P.n_channels * inst[i].P.n_bins;
+        if (adj_value[i]) {
+            any_s = true;
+            for (size_t k = 0; k < hn; ++k) as[adj_off[i] + k] = adj_value[i][k] * scale;
+        }
+        if (adj_abs[i]) {
+            any_a = true;
+            for (size_t k = 0; k < hn; ++k) aa[adj_off[i] + k] = adj_abs[i][k] * scale;
+        }
     }
     const ear_tuning& tu = c->tuning;
     const uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 22);
-    const uint32_t cap = std::max<uint32_t>(1, std::min(chunk, D->n_rays));
+    uint64_t rays_any = 0;
+    for (uint32_t i = 0; i < n_items; ++i) rays_any = std::max<uint64_t>(rays_any, descs[i]->n_rays);
+    uint64_t rays_band_max = 0;
+    for (int b = 0; b < 3; ++b) {
+        uint64_t t = 0;
+        for (uint32_t i = 0; i < n_items; ++i)
+            if ((inst[i].P.band_mask >> b) & 1u) t += descs[i]->n_rays;
+        rays_band_max = std::max(rays_band_max, t);
+    }
+    const uint32_t cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunk, rays_band_max));
     hipStream_t st = c->stream;
     DevBuf qo[2] = {DevBuf(c), DevBuf(c)}, qd[2] = {DevBuf(c), DevBuf(c)}, qm[2] = {DevBuf(c), DevBuf(c)};
     DevBuf tt[2] = {DevBuf(c), DevBuf(c)}, tr[2] = {DevBuf(c), DevBuf(c)};
-    DevBuf hits(c), counters(c), stats(c), gs(c), dinst(c), dsegs(c), dsegoff(c), dadj_s(c), dadj_a(c), dgrad(c),
-        sqp(c), sqm(c);
+    DevBuf hits(c), counters(c), stats(c), gs(c), dinst(c), dsegs(c), dsegoff(c), dplan(c), dadj_s(c), dadj_a(c),
+        dgrad(c), dargs(c), sqp(c), sqm(c);
     for (int k = 0; k < 2; ++k) {
         TRY(qo[k].alloc((size_t)cap * 16));
         TRY(qd[k].alloc((size_t)cap * 16));
         TRY(qm[k].alloc((size_t)cap * 16));
-        TRY(tt[k].alloc((size_t)cap * nm * 8 * 4));
-        TRY(tr[k].alloc((size_t)cap * nm * 4));
+        TRY(tt[k].alloc((size_t)cap * nm_max * 8 * 4));
+        TRY(tr[k].alloc((size_t)cap * nm_max * 4));
     }
     TRY(hits.alloc((size_t)cap * 8));
     TRY(counters.alloc(32));
@@ -2346,7 +2412,7 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const 
     TRY(sqm.alloc(16));
     ExtendFn fe;
     ShadowFn fs_unused;
-    const TravCfg cfg{256, 16, (int)s->width, false, false, false};
+    const TravCfg cfg{256, 16, width, false, batch, false};
     pick_kernels(cfg, &fe, &fs_unused);
     const uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 32;
     const uint32_t cap_e = (uint32_t)c->num_cus * (uint32_t)std::max(1, resident_blocks((const void*)fe, 256));
@@ -2354,24 +2420,30 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const 
     // every thread of either traversal launch owns one overflow stack
     const uint32_t gthreads = std::max(cap_e * 256u, gmax * (uint32_t)kBlock);
     TRY(gs.alloc((size_t)(kStackDepth - kLdsMin) * gthreads * 4));
-    TRY(dinst.alloc(sizeof(InstDev)));
-    TRY(dsegs.alloc(sizeof(GenSeg)));
-    TRY(dsegoff.alloc(4));
-    TRY(dgrad.alloc(sizeof(double) * 3 * 2 * nm));
-    if (adj_value) {
-        TRY(dadj_s.alloc(hn * 8));
-        EAR_HIP_TRY(c, hipMemcpyAsync(dadj_s.p, as.data(), hn * 8, hipMemcpyHostToDevice, st));
+    TRY(dinst.alloc(sizeof(InstDev) * n_items));
+    TRY(dsegs.alloc(sizeof(GenSeg) * n_items));
+    TRY(dsegoff.alloc(4 * (size_t)n_items));
+    TRY(dplan.alloc(8));
+    TRY(dgrad.alloc(sizeof(double) * 3 * 2 * nm_max * n_items));
+    TRY(dargs.alloc(sizeof(AdjArgs) * n_items));
+    if (any_s) {
+        TRY(dadj_s.alloc(adj_cells * 8));
+        EAR_HIP_TRY(c, hipMemcpyAsync(dadj_s.p, as.data(), adj_cells * 8, hipMemcpyHostToDevice, st));
     }
-    if (adj_abs) {
-        TRY(dadj_a.alloc(hn * 8));
-        EAR_HIP_TRY(c, hipMemcpyAsync(dadj_a.p, aa.data(), hn * 8, hipMemcpyHostToDevice, st));
+    if (any_a) {
+        TRY(dadj_a.alloc(adj_cells * 8));
+        EAR_HIP_TRY(c, hipMemcpyAsync(dadj_a.p, aa.data(), adj_cells * 8, hipMemcpyHostToDevice, st));
     }
-    EAR_HIP_TRY(c, hipMemcpyAsync(dinst.p, &I, sizeof(InstDev), hipMemcpyHostToDevice, st));
+    std::vector<AdjArgs> args(n_items);
+    for (uint32_t i = 0; i < n_items; ++i)
+        args[i] = AdjArgs{adj_value[i] ? dadj_s.as<double>() + adj_off[i] : nullptr,
+                          adj_abs[i] ? dadj_a.as<double>() + adj_off[i] : nullptr,
+                          dgrad.as<double>() + (size_t)i * 3 * 2 * nm_max, scenes[i]->n_mat};
+    EAR_HIP_TRY(c, hipMemcpyAsync(dargs.p, args.data(), sizeof(AdjArgs) * n_items, hipMemcpyHostToDevice, st));
+    EAR_HIP_TRY(c, hipMemcpyAsync(dinst.p, inst.data(), sizeof(InstDev) * n_items, hipMemcpyHostToDevice, st));
     EAR_HIP_TRY(c, hipMemsetAsync(dgrad.p, 0, dgrad.n, st));
     EAR_HIP_TRY(c, hipMemsetAsync(counters.p, 0, counters.n, st));
     EAR_HIP_TRY(c, hipMemsetAsync(stats.p, 0, stats.n, st));
-    const AdjArgs A{adj_value ? dadj_s.as<double>() : nullptr, adj_abs ? dadj_a.as<double>() : nullptr,
-                    dgrad.as<double>(), nm};
     const InstDev* di = dinst.as<InstDev>();
     uint32_t* cnt = counters.as<uint32_t>();
     unsigned long long* stp = stats.as<unsigned long long>();
@@ -2381,36 +2453,60 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const 
     Tangents T[2] = {Tangents{tt[0].as<float>(), tr[0].as<float>(), cap},
                      Tangents{tt[1].as<float>(), tr[1].as<float>(), cap}};
     const ShadowQ SQ{sqp.as<float4>(), sqm.as<float4>()};
+    const AdjArgs A0 = args[0];
+    const AdjArgs* dA = dargs.as<AdjArgs>();
     for (uint32_t band = 0; band < 3; ++band) {
-        if (!((P.band_mask >> band) & 1u)) continue;
-        for (uint32_t r0 = 0; r0 < D->n_rays; r0 += cap) {
-            const uint32_t n_chunk = std::min(cap, D->n_rays - r0);
-            const GenSeg g{0, band, r0, n_chunk};
-            const uint32_t off0 = 0;
-            EAR_HIP_TRY(c, hipMemcpyAsync(dsegs.p, &g, sizeof(g), hipMemcpyHostToDevice, st));
-            EAR_HIP_TRY(c, hipMemcpyAsync(dsegoff.p, &off0, 4, hipMemcpyHostToDevice, st));
-            k_generate<false><<<persistent_grid(c, n_chunk, waves), kBlock, 0, st>>>(
-                di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), 1u, n_chunk, nullptr, P.n_rec, 0u, q[0], cnt + 0, SQ,
-                cnt + 2, stp);
-            k_zero_f32<<<1024, 256, 0, st>>>(T[0].t, (size_t)cap * nm * 8);
-            k_zero_f32<<<1024, 256, 0, st>>>(T[0].r, (size_t)cap * nm);
+        // this band's contexts as runs of rays
+        std::vector<GenSeg> segs;
+        std::vector<uint32_t> off;
+        uint32_t total = 0;
+        for (uint32_t i = 0; i < n_items; ++i) {
+            if (!((inst[i].P.band_mask >> band) & 1u) || descs[i]->n_rays == 0) continue;
+            segs.push_back(GenSeg{i, band, 0, descs[i]->n_rays});
+            off.push_back(total);
+            total += descs[i]->n_rays;
+        }
+        if (segs.empty()) continue;
+        EAR_HIP_TRY(c, hipMemcpyAsync(dsegs.p, segs.data(), sizeof(GenSeg) * segs.size(), hipMemcpyHostToDevice, st));
+        EAR_HIP_TRY(c, hipMemcpyAsync(dsegoff.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, st));
+        for (uint32_t r0 = 0; r0 < total; r0 += cap) {
+            const uint32_t n_chunk = std::min(cap, total - r0);
+            const uint32_t plan[2] = {r0, n_chunk};
+            EAR_HIP_TRY(c, hipMemcpyAsync(dplan.p, plan, 8, hipMemcpyHostToDevice, st));
+            if (batch)
+                k_generate<true><<<persistent_grid(c, n_chunk, waves), kBlock, 0, st>>>(
+                    di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), (uint32_t)segs.size(), n_chunk,
+                    dplan.as<uint32_t>(), max_rec, 0u, q[0], cnt + 0, SQ, cnt + 2, stp);
+            else
+                k_generate<false><<<persistent_grid(c, n_chunk, waves), kBlock, 0, st>>>(
+                    di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), (uint32_t)segs.size(), n_chunk,
+                    dplan.as<uint32_t>(), max_rec, 0u, q[0], cnt + 0, SQ, cnt + 2, stp);
+            k_zero_f32<<<1024, 256, 0, st>>>(T[0].t, (size_t)cap * nm_max * 8);
+            k_zero_f32<<<1024, 256, 0, st>>>(T[0].r, (size_t)cap * nm_max);
             EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
             EAR_HIP_TRY(c, hipStreamSynchronize(st));
             uint32_t n_live = pinned[0];
             int cur = 0;
-            for (uint32_t k = 1; k <= D->max_bounces && n_live > 0; ++k) {
+            for (uint32_t k = 1; k <= max_bounces && n_live > 0; ++k) {
                 hipLaunchKernelGGL(fe, dim3(grid_for(n_live, cap_e, 256)), dim3(256), 0, st, di, gs.as<int>(),
                                    gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp,
                                    Fetch{cnt + 4, 56u, 0u, 1u});
                 const uint32_t gsd = persistent_grid(c, n_live, waves);
-                if (s->width == 4)
-                    k_shade_ad<4><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,
-                                                          hits.as<int2>(), T[cur], q[1 - cur], cnt + (1 - cur),
-                                                          T[1 - cur], A, stp);
-                else
-                    k_shade_ad<2><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,
-                                                          hits.as<int2>(), T[cur], q[1 - cur], cnt + (1 - cur),
-                                                          T[1 - cur], A, stp);
+#define EAR_AD_LAUNCH(W, B)                                                                                       \
+    k_shade_ad<W, B><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur, hits.as<int2>(), \
+                                             T[cur], q[1 - cur], cnt + (1 - cur), T[1 - cur], A0, dA, stp)
+                if (width == 4) {
+                    if (batch)
+                        EAR_AD_LAUNCH(4, true);
+                    else
+                        EAR_AD_LAUNCH(4, false);
+                } else {
+                    if (batch)
+                        EAR_AD_LAUNCH(2, true);
+                    else
+                        EAR_AD_LAUNCH(2, false);
+                }
+#undef EAR_AD_LAUNCH
                 EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
                 EAR_HIP_TRY(c, hipStreamSynchronize(st));
                 n_live = pinned[1 - cur];
@@ -2421,17 +2517,26 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const 
             EAR_HIP_TRY(c, hipGetLastError());
         }
     }
-    std::vector<double> g(3 * 2 * nm);
+    std::vector<double> g((size_t)3 * 2 * nm_max * n_items);
     EAR_HIP_TRY(c, hipMemcpyAsync(g.data(), dgrad.p, g.size() * 8, hipMemcpyDeviceToHost, st));
     EAR_HIP_TRY(c, hipStreamSynchronize(st));
-    for (uint32_t b = 0; b < 3; ++b) {
-        if (!((P.band_mask >> b) & 1u)) continue;
-        for (uint32_t m = 0; m < nm; ++m) {
-            if (grad_refl) grad_refl[(size_t)m * 3 + b] = g[((size_t)b * 2 + 0) * nm + m];
-            if (grad_spec) grad_spec[(size_t)m * 3 + b] = g[((size_t)b * 2 + 1) * nm + m];
+    for (uint32_t i = 0; i < n_items; ++i) {
+        const uint32_t nm = scenes[i]->n_mat;
+        const double* gi = g.data() + (size_t)i * 3 * 2 * nm_max;
+        for (uint32_t b = 0; b < 3; ++b) {
+            if (!((inst[i].P.band_mask >> b) & 1u)) continue;
+            for (uint32_t m = 0; m < nm; ++m) {
+                if (grad_refl[i]) grad_refl[i][(size_t)m * 3 + b] = gi[((size_t)b * 2 + 0) * nm + m];
+                if (grad_spec[i]) grad_spec[i][(size_t)m * 3 + b] = gi[((size_t)b * 2 + 1) * nm + m];
+            }
         }
     }
     return EAR_OK;
+}
+
+ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* D, const double* adj_value,
+                        const double* adj_abs, double* grad_refl, double* grad_spec) {
+    return run_backward_batch(c, 1, &s, &D, &adj_value, &adj_abs, &grad_refl, &grad_spec);
 }
 
 }  // namespace ear

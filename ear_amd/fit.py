@@ -124,6 +124,56 @@ def fit_loop(forward, backward, target_log_edc: np.ndarray, alpha0, spec0, *, st
     return st
 
 
+def fit_rooms(ctx, rooms: list, target_abs_values: list, alpha0s: list, spec0s: list, *, steps: int,
+              lr: float = 0.004, decay: float = 5.0, momentum: float = 0.5, alpha_bounds=(0.01, 0.95),
+              spec_bounds=(0.0, 0.95), n_rays=1_000_000, max_bounces: int = 999, cutoff: float = 1e-8,
+              n_bins: int = 4096 * 11, seed: int = 0xEA2, n_coarse: int = N_COARSE) -> list:
+    """Many rooms fitted together: every step is one ear_render_batch of all
+    rooms and one ear_render_backward_batch, then one Adam step per room (the
+    same update as fit_loop).  The scenes are uploaded once; each step only
+    replaces their material tables.  n_rays: one count or one per room.
+    Returns one FitState per room."""
+    from . import api
+
+    k = len(rooms)
+    rays = list(n_rays) if np.ndim(n_rays) else [int(n_rays)] * k
+    kws = [dict(recorders=r["recorders"], src_pos=r["src"], air=r["air"], n_rays=rays[i], max_bounces=max_bounces,
+                cutoff=cutoff, n_bins=n_bins, seed=seed, abs_hist=True) for i, r in enumerate(rooms)]
+    targets = [log_edc(coarse_bins(t, n_coarse)) for t in target_abs_values]
+    masks = [decay_mask(t) for t in targets]
+    states = [FitState(np.array(a, np.float64), np.array(sp, np.float64)) for a, sp in zip(alpha0s, spec0s)]
+    opts = [Adam(lr, b1=momentum) for _ in range(k)]
+    scenes = [api.Scene(ctx, r["tris"], r["mat"], materials(st.alpha, st.spec)) for r, st in zip(rooms, states)]
+
+    def losses():
+        res, _, _ = api.render_batch(ctx, [(sc, dict(kw, want_raw=False)) for sc, kw in zip(scenes, kws)])
+        return [log_edc_loss(r.abs_value, t, m, n_coarse) for r, t, m in zip(res, targets, masks)]
+
+    try:
+        for t in range(steps):
+            for sc, st in zip(scenes, states):
+                sc.set_materials(materials(st.alpha, st.spec))
+            la = losses()
+            grads = api.render_backward_batch(ctx, [(sc, dict(kw, adj_abs=adj, local=True))
+                                                    for sc, kw, (_, adj) in zip(scenes, kws, la)])
+            for i, st in enumerate(states):
+                st.history.append((la[i][0], st.alpha.copy(), st.spec.copy()))
+                g_refl, g_spec = grads[i]
+                n_mat = len(st.alpha)
+                g = np.concatenate([-np.asarray(g_refl).reshape(-1), np.asarray(g_spec).sum(axis=1)])
+                x = opts[i].step(np.concatenate([st.alpha.reshape(-1), st.spec]), g, lr / (1.0 + t / decay))
+                st.alpha = np.clip(x[: 3 * n_mat].reshape(n_mat, 3), *alpha_bounds)
+                st.spec = np.clip(x[3 * n_mat:], *spec_bounds)
+        for sc, st in zip(scenes, states):
+            sc.set_materials(materials(st.alpha, st.spec))
+        for st, (loss, _) in zip(states, losses()):
+            st.history.append((loss, st.alpha.copy(), st.spec.copy()))
+        return states
+    finally:
+        for sc in scenes:
+            sc.close()
+
+
 def fit_room(ctx, room: dict, target_abs_value: np.ndarray, alpha0, spec0, *, steps: int, lr: float = 0.004,
              decay: float = 5.0,
              n_rays: int = 1_000_000, max_bounces: int = 999, cutoff: float = 1e-8, n_bins: int = 4096 * 11,
@@ -139,13 +189,14 @@ def fit_room(ctx, room: dict, target_abs_value: np.ndarray, alpha0, spec0, *, st
     scenes = {}
 
     def scene(mats):
-        key = repr(mats)
-        if key not in scenes:
-            for s in scenes.values():
-                s.close()
-            scenes.clear()
-            scenes[key] = api.Scene(ctx, room["tris"], room["mat"], mats)
-        return scenes[key]
+        # one upload (and BVH) for the whole fit; each step swaps the materials
+        if "s" not in scenes:
+            scenes["s"] = api.Scene(ctx, room["tris"], room["mat"], mats)
+            scenes["m"] = repr(mats)
+        elif scenes["m"] != repr(mats):
+            scenes["s"].set_materials(mats)
+            scenes["m"] = repr(mats)
+        return scenes["s"]
 
     def forward(mats):
         return api.render(ctx, scene(mats), **kw).abs_value
@@ -156,5 +207,5 @@ def fit_room(ctx, room: dict, target_abs_value: np.ndarray, alpha0, spec0, *, st
     try:
         return fit_loop(forward, backward, target, alpha0, spec0, steps=steps, lr=lr, decay=decay)
     finally:
-        for s in scenes.values():
-            s.close()
+        if "s" in scenes:
+            scenes["s"].close()

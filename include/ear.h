@@ -97,6 +97,10 @@ ear_status ear_scene_upload(ear_ctx* ctx, const float* v0, const float* v1, cons
                             const uint16_t* mat_id, uint32_t n_tris, const ear_material* mats,
                             uint32_t n_mat, const ear_bvh_opts* opts, ear_scene** out);
 ear_status ear_scene_destroy(ear_scene* scene);
+/* Replaces the scene's material table (same count as at upload) without
+ * rebuilding the BVH; no reference counterpart (its materials are fixed once
+ * parsed): fitting changes them every step. */
+ear_status ear_scene_set_materials(ear_scene* scene, const ear_material* mats, uint32_t n_mat);
 
 typedef struct {
     uint32_t n_tris;
@@ -268,6 +272,20 @@ ear_status ear_render_batch(ear_ctx* ctx, const ear_batch_item* items, uint32_t 
 ear_status ear_render_backward(ear_ctx* ctx, ear_scene* scene, const ear_render_desc* desc,
                                const double* adj_value, const double* adj_abs, double* grad_refl,
                                double* grad_spec);
+
+/* The adjoint of a batch (no reference counterpart): every item's gradients
+ * as ear_render_backward would write them, with the rays of all items sharing
+ * the device queues band by band (fitting many rooms at once).  All scenes
+ * must have the same BVH width. */
+typedef struct {
+    ear_scene* scene;
+    const ear_render_desc* desc;
+    const double* adj_value;  /* [3][n_channels][hist_bins] or NULL */
+    const double* adj_abs;    /* [3][n_channels][hist_bins] or NULL */
+    double* grad_refl;        /* [n_mat][3] */
+    double* grad_spec;        /* [n_mat][3] */
+} ear_backward_item;
+ear_status ear_render_backward_batch(ear_ctx* ctx, const ear_backward_item* items, uint32_t n_items);
 
 /* Auralization: the convolution of an impulse-response track with a sound
  * signal, RecorderTrack::Process (src/Recorder.cpp:145-243).  out[i], for i

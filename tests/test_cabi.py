@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-STRUCTS = ["ear_material", "ear_bvh_opts", "ear_scene_info", "ear_recorder", "ear_source", "ear_render_desc",
+STRUCTS = ["ear_material", "ear_bvh_opts", "ear_scene_info", "ear_recorder", "ear_source", "ear_render_desc", "ear_backward_item",
            "ear_hist", "ear_stats", "ear_tuning"]
 
 

@@ -142,3 +142,84 @@ def test_fit_room_recovers_area_weighted_absorption(gpu_ctx):
     print("fit losses", [round(h[0], 5) for h in st.history], "area-weighted alpha error", err0, err1)
     assert st.history[-1][0] < st.history[0][0] / 10
     assert np.abs(err1).max() < np.abs(err0).max() / 2.5
+
+
+def test_backward_batch_equals_single(gpu_ctx):
+    """ear_render_backward_batch over rooms of different shapes, material
+    counts, ray counts and bands: each item's gradient equals its own
+    ear_render_backward (the per-wave partials are summed in another order,
+    so to 1e-9 relative)."""
+    rooms = [scenes.room(seed=11, target_tris=12, kind="box"), scenes.room(seed=12, target_tris=400, kind="L"),
+             _scene(), scenes.room(seed=13, target_tris=2000, kind="box")]
+    specs = [dict(n_rays=30_000, bands=(0, 1, 2)), dict(n_rays=12_345, bands=(1,)), dict(n_rays=50_000, bands=(0, 2)),
+             dict(n_rays=7_000, bands=(0, 1, 2))]
+    rng = np.random.default_rng(9)
+    handles, items, singles = [], [], []
+    try:
+        for r, sp in zip(rooms, specs):
+            sc = api.Scene(gpu_ctx, r["tris"], r["mat"], r["materials"])
+            handles.append(sc)
+            nb = 4096
+            kw = dict(recorders=r["recorders"], src_pos=r["src"], air=r["air"], max_bounces=60, cutoff=1e-8,
+                      n_bins=nb, abs_hist=True, **sp)
+            a_v = rng.standard_normal((3, 1, nb))
+            a_a = rng.standard_normal((3, 1, nb))
+            items.append((sc, dict(kw, adj_value=a_v, adj_abs=a_a)))
+            singles.append(api.render_backward(gpu_ctx, sc, adj_value=a_v, adj_abs=a_a, **kw))
+        print("backward batch of", len(items), "rooms", flush=True)
+        got = api.render_backward_batch(gpu_ctx, items)
+        for (gr, gs), (wr, ws), sp in zip(got, singles, specs):
+            for g, w in ((gr, wr), (gs, ws)):
+                assert np.any(w[:, list(sp["bands"])] != 0)
+                np.testing.assert_allclose(g, w, rtol=1e-9, atol=1e-12 * np.abs(w).max())
+                off = [b for b in range(3) if b not in sp["bands"]]
+                assert np.all(g[:, off] == 0)
+    finally:
+        for sc in handles:
+            sc.close()
+
+
+def test_set_materials_equals_fresh_upload(gpu_ctx):
+    from ear_amd import fit
+
+    s = _scene()
+    kw = _kw(s, n_rays=20_000)
+    m2 = fit.materials(s["alpha"] * 1.3, np.full(6, 0.4))
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
+        sc.set_materials(m2)
+        a = api.render(gpu_ctx, sc, **kw)
+        with pytest.raises(api.EarError):
+            sc.set_materials(m2[:3])
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], m2) as sc:
+        b = api.render(gpu_ctx, sc, **kw)
+    assert np.array_equal(a.raw, b.raw) and np.array_equal(a.abs_raw, b.abs_raw)
+
+
+def test_fit_rooms_batched(gpu_ctx):
+    """fit.fit_rooms: two rooms fitted in lockstep by batched forward and
+    backward renders; a room fitted alone follows the same trajectory, and
+    every room's loss falls."""
+    from ear_amd import fit
+
+    rooms = [_scene(), scenes.room(seed=21, target_tris=200, kind="L")]
+    kw = dict(n_rays=40_000, max_bounces=100, cutoff=1e-8)
+    targets, a0s, s0s = [], [], []
+    for i, r in enumerate(rooms):
+        n_mat = len(r["materials"])
+        spec_t = np.full(n_mat, 0.3)
+        with api.Scene(gpu_ctx, r["tris"], r["mat"], fit.materials(r["alpha"], spec_t)) as sc:
+            targets.append(api.render(gpu_ctx, sc, recorders=r["recorders"], src_pos=r["src"], air=r["air"],
+                                      n_bins=4096 * 11, abs_hist=True, **kw).abs_value)
+        sign = np.random.default_rng(30 + i).choice([-1, 1], size=(n_mat, 4))
+        a0s.append(r["alpha"] * (1 + 0.2 * sign[:, :3]))
+        s0s.append(spec_t * (1 + 0.2 * sign[:, 3]))
+    states = fit.fit_rooms(gpu_ctx, rooms, targets, a0s, s0s, steps=8, **kw)
+    alone = fit.fit_room(gpu_ctx, rooms[0], targets[0], a0s[0], s0s[0], steps=8, **kw)
+    print("batched fit losses", [[round(h[0], 5) for h in st.history] for st in states], flush=True)
+    # 8 steps from +-20%: the shoebox's loss falls ~5x, the L room's ~2x
+    for st in states:
+        assert st.history[-1][0] < 0.6 * st.history[0][0]
+    for (l1, a1, s1), (l2, a2, s2) in zip(states[0].history, alone.history):
+        assert abs(l1 - l2) <= 1e-6 * max(abs(l2), 1e-12)
+        np.testing.assert_allclose(a1, a2, rtol=1e-6)
+        np.testing.assert_allclose(s1, s2, rtol=1e-6, atol=1e-9)
