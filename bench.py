@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-instrument", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batch (256 rooms) and forward + adjoint legs")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (PMC calibration) and exit")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_extend_hall200k.json"))
@@ -148,6 +150,10 @@ def main():
                 "avg_launch_ms": round(ext_ms / max(1, ext_launch), 4), "launches": ext_launch,
                 "queries_per_launch": round(ext_q / max(1, ext_launch), 1)}
 
+    extras = None
+    if not a.no_extras:
+        extras = {"batch": bench_batch(ctx), "adjoint": bench_adjoint(ctx)}
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(s, kw, a.cpu_seconds)
@@ -176,8 +182,123 @@ def main():
         "t60_mid_s": edc_t60,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_batch(ctx, n_rooms=256, seed=3):
+    """Batching: n_rooms box or L rooms of 12..50k triangles (log-uniform),
+    10k..2M rays each (log-uniform), 3 bands, 1 s histograms, in one
+    ear_render_batch, beside the same renders one call each."""
+    import numpy as np
+
+    from ear_amd import api, scenes
+
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    specs = []
+    for i in range(n_rooms):
+        nt = int(np.exp(rng.uniform(np.log(12), np.log(50_000))))
+        rays = int(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        specs.append((scenes.room(seed=1000 + i, target_tris=nt), rays))
+    log(f"batch: {n_rooms} rooms generated in {time.time() - t0:.1f}s")
+    t0 = time.perf_counter()
+    scs = [api.Scene(ctx, rm["tris"], rm["mat"], rm["materials"]) for rm, _ in specs]
+    upload_ms = 1e3 * (time.perf_counter() - t0)
+    items = [(sc, dict(recorders=rm["recorders"], src_pos=rm["src"], air=rm["air"], n_rays=rays, max_bounces=999,
+                       cutoff=1e-8, n_bins=44100, instance=i, abs_hist=False, want_raw=False))
+             for i, (sc, (rm, rays)) in enumerate(zip(scs, specs))]
+    api.render_batch(ctx, items[:16])  # warm-up (first-touch allocations)
+    _, st, wall = api.render_batch(ctx, items)
+    log(f"batch: {st['segments']} segments in {wall:.3f}s ({st['segments'] / wall:.3e} segments/s)")
+    t0 = time.perf_counter()
+    segs1 = 0
+    for sc, kw in items:
+        segs1 += api.render(ctx, sc, **kw).stats["segments"]
+    single = time.perf_counter() - t0
+    log(f"batch: one call each {single:.3f}s ({segs1 / single:.3e} segments/s)")
+    for sc in scs:
+        sc.close()
+    return {"workload": f"{n_rooms} box/L rooms, 12..50k triangles, 1e4..2e6 rays each, 3 bands, 1 s histograms",
+            "triangles": int(sum(len(rm["tris"]) for rm, _ in specs)),
+            "rays": int(3 * sum(r for _, r in specs)), "segments": int(st["segments"]),
+            "upload_ms": round(upload_ms, 1), "batch_s": round(wall, 4),
+            "batch_segments_per_s": round(st["segments"] / wall, 1), "single_calls_s": round(single, 4),
+            "single_calls_segments_per_s": round(segs1 / single, 1), "speedup": round(single / wall, 3)}
+
+
+def bench_adjoint(ctx, n_rooms=32):
+    """Differentiation: the 12 x 9 x 5 m shoebox (6 walls x 3 bands
+    absorption + 6 specularities), 1M rays, 4096-bin log-EDC loss against a
+    target rendered at the true parameters, evaluated at parameters off by
+    +-20%: forward, adjoint and their rate beside the forward alone; the EDC
+    error of the 1M-ray forward against 8M rays; then n_rooms rooms of
+    1k..100k triangles (250k rays each) through the batched forward and
+    adjoint."""
+    import numpy as np
+
+    from ear_amd import acoustics, api, fit, scenes
+
+    s = scenes.shoebox(12.0, 9.0, 5.0, spec=0.3, alpha_range=(0.1, 0.3), src=(3.0, 4.0, 2.0), rec=(8.0, 5.0, 1.7))
+    n_mat = len(s["materials"])
+    alpha_t, spec_t = s["alpha"], np.full(n_mat, 0.3)
+    sign = np.random.default_rng(5).choice([-1, 1], size=(n_mat, 4))
+    alpha0, spec0 = alpha_t * (1 + 0.2 * sign[:, :3]), spec_t * (1 + 0.2 * sign[:, 3])
+    kw = dict(recorders=s["recorders"], src_pos=s["src"], air=s["air"], n_rays=1_000_000, max_bounces=999,
+              cutoff=1e-8, n_bins=4096 * 11, abs_hist=True, want_raw=False)
+    out = {}
+    with api.Scene(ctx, s["tris"], s["mat"], fit.materials(alpha_t, spec_t)) as sc:
+        target = fit.log_edc(fit.coarse_bins(api.render(ctx, sc, **kw).abs_value))
+        mask = fit.decay_mask(target)
+        sc.set_materials(fit.materials(alpha0, spec0))
+        api.render(ctx, sc, **kw)  # warm-up
+        t0 = time.perf_counter()
+        r = api.render(ctx, sc, **kw)
+        t_fwd = time.perf_counter() - t0
+        loss, adj = fit.log_edc_loss(r.abs_value, target, mask)
+        api.render_backward(ctx, sc, adj_abs=adj, local=True, **kw)  # warm-up
+        t0 = time.perf_counter()
+        api.render_backward(ctx, sc, adj_abs=adj, local=True, **kw)
+        t_bwd = time.perf_counter() - t0
+        ref = api.render(ctx, sc, **dict(kw, n_rays=8_000_000, seed=0x5EED))
+        edc_err = max(acoustics.edc_error(r.abs_value[b, 0], ref.abs_value[b, 0]) for b in range(3))
+        segs = r.stats["segments"]
+    log(f"adjoint: forward {t_fwd * 1e3:.1f} ms, backward {t_bwd * 1e3:.1f} ms, {segs} segments, "
+        f"EDC error {edc_err:.4f}")
+    out.update({"workload": "shoebox 12x9x5 m, 18 absorption + 6 specularity parameters, 1M rays, 4096-bin log-EDC "
+                            "loss, parameters +-20% off the target's",
+                "segments": int(segs), "forward_ms": round(1e3 * t_fwd, 3), "backward_ms": round(1e3 * t_bwd, 3),
+                "forward_segments_per_s": round(segs / t_fwd, 1),
+                "forward_adjoint_segments_per_s": round(segs / (t_fwd + t_bwd), 1),
+                "forward_adjoint_vs_forward": round(t_fwd / (t_fwd + t_bwd), 4),
+                "edc_error_vs_8M_rays": round(edc_err, 5), "loss": round(loss, 6)})
+
+    rng = np.random.default_rng(17)
+    rooms = [scenes.room(seed=5000 + i, target_tris=int(np.exp(rng.uniform(np.log(1000), np.log(100_000)))))
+             for i in range(n_rooms)]
+    scs = [api.Scene(ctx, rm["tris"], rm["mat"], rm["materials"]) for rm in rooms]
+    kws = [dict(recorders=rm["recorders"], src_pos=rm["src"], air=rm["air"], n_rays=250_000, max_bounces=999,
+                cutoff=1e-8, n_bins=4096 * 11, abs_hist=True, want_raw=False) for rm in rooms]
+    fwd_items = [(sc, k) for sc, k in zip(scs, kws)]
+    res, _, _ = api.render_batch(ctx, fwd_items)  # warm-up; its histograms serve as targets
+    targets = [fit.log_edc(fit.coarse_bins(x.abs_value)) for x in res]
+    t0 = time.perf_counter()
+    res, st, _ = api.render_batch(ctx, fwd_items)
+    adjs = [fit.log_edc_loss(x.abs_value * 1.1, t, fit.decay_mask(t))[1] for x, t in zip(res, targets)]
+    t1 = time.perf_counter()
+    api.render_backward_batch(ctx, [(sc, dict(k, adj_abs=a, local=True)) for sc, k, a in zip(scs, kws, adjs)])
+    t2 = time.perf_counter()
+    for sc in scs:
+        sc.close()
+    bsegs = st["segments"]
+    log(f"adjoint batch: {n_rooms} rooms, {bsegs} segments, forward {t1 - t0:.3f}s, backward {t2 - t1:.3f}s")
+    out["batch"] = {"workload": f"{n_rooms} box/L rooms, 1k..100k triangles, 250k rays each, 3 bands, 4096-bin "
+                                f"log-EDC loss", "segments": int(bsegs), "forward_s": round(t1 - t0, 4),
+                    "backward_s": round(t2 - t1, 4),
+                    "forward_adjoint_segments_per_s": round(bsegs / (t2 - t0), 1),
+                    "forward_segments_per_s": round(bsegs / (t1 - t0), 1)}
+    return out
 
 
 def cpu_baseline(s, kw, target_s):

@@ -301,6 +301,7 @@ def render_backward(ctx: Context, scene: Scene, *, adj_value=None, adj_abs=None,
     shape of RenderResult.value, (3, n_channels, n_bins).  Returns
     (d/drefl, d/dspec), each (n_mat, 3); bands outside `bands` are zero.
     local: EAR_FLAG_ADJ_LOCAL (direction tangents end at their vertex)."""
+    kw = {k: v for k, v in kw.items() if k not in ("want_raw", "want_value")}  # no histograms come back
     p = _Prepared(want_raw=False, want_value=False, **kw)
     if local:
         p.desc.flags |= F.EAR_FLAG_ADJ_LOCAL
@@ -322,7 +323,7 @@ def render_backward_batch(ctx: Context, items):
     keep, arr = [], (F.ear_backward_item * max(1, len(items)))()
     outs = []
     for i, (scene, kw) in enumerate(items):
-        kw = dict(kw)
+        kw = {k: v for k, v in kw.items() if k not in ("want_raw", "want_value")}
         adj_value, adj_abs, local = kw.pop("adj_value", None), kw.pop("adj_abs", None), kw.pop("local", False)
         p = _Prepared(want_raw=False, want_value=False, **kw)
         if local:
