@@ -1965,6 +1965,11 @@ namespace ear {
 namespace {
 
 constexpr int kAdMaxMat = 8;
+// The gradient is accumulated into kGradCopies copies, a block adding into
+// copy blockIdx % kGradCopies, and the copies are summed at the end: the
+// per-wave partials of every launch would otherwise all land on the same
+// 2 n_mat addresses, and same-address atomics serialise in the L2.
+constexpr uint32_t kGradCopies = 128;
 
 // Tangent state of the rays of a queue, SoA [q][field][slot] with fields
 // {o.xyz, d.xyz, dL, dlogI} per specularity parameter q, then
@@ -1982,9 +1987,57 @@ struct Tangents {
 struct AdjArgs {
     const double* adj_s;  // [3][n_ch][bins] dL / d raw (signed), or null
     const double* adj_a;  // [3][n_ch][bins] dL / d raw (|.|), or null
-    double* grad;         // [3][2][n_mat]: band, {refl, spec}, material
+    double* grad;         // [3][2][n_mat]: band, {refl, spec}, material (copy 0)
     uint32_t n_mat;
+    uint32_t copy_stride;  // doubles from one gradient copy to the next
+    __device__ double* grad_copy() const { return grad + (size_t)(blockIdx.x % kGradCopies) * copy_stride; }
 };
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    return v;
+}
+
+// The end-of-launch gradient partials of a block (single instance, one
+// band): per-wave sums, then summed over the block's waves in LDS, one
+// atomic per value into the block's gradient copy.
+__device__ void block_grad_flush(const AdjArgs& A, int band_acc, const double* g_refl, const double* g_spec,
+                                 double* red /* LDS [kBlock / 64][2 * kAdMaxMat] */) {
+    int band = __shfl(band_acc, 0);
+    for (int l = 1; l < 64 && band < 0; ++l) band = __shfl(band_acc, l);
+    const uint32_t nm = A.n_mat;
+    const uint32_t wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < kAdMaxMat; ++q) {
+        const double sr = wave_sum_d((uint32_t)q < nm ? g_refl[q] : 0.0);
+        const double ss = wave_sum_d((uint32_t)q < nm ? g_spec[q] : 0.0);
+        if (lane_id() == 0) {
+            red[wave * 2 * kAdMaxMat + q] = sr;
+            red[wave * 2 * kAdMaxMat + kAdMaxMat + q] = ss;
+        }
+    }
+    // the band of the block: any wave's (one band per launch)
+    __shared__ int bband;
+    if (threadIdx.x == 0) bband = -1;
+    __syncthreads();
+    if (lane_id() == 0 && band >= 0) bband = band;
+    __syncthreads();
+    const int b = bband;
+    if (threadIdx.x < 2 * nm && b >= 0) {
+        const uint32_t q = threadIdx.x % nm, which = threadIdx.x / nm;  // 0 refl, 1 spec
+        double v = 0.0;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) v += red[w * 2 * kAdMaxMat + which * kAdMaxMat + q];
+        if (v != 0.0) atomicAdd(A.grad_copy() + ((size_t)b * 2 + which) * nm + q, v);
+    }
+}
+
+__global__ void k_sum_grad_copies(double* g, uint32_t n, uint32_t copy_stride) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (uint32_t c = 0; c < kGradCopies; ++c) v += g[(size_t)c * copy_stride + i];
+        g[i] = v;
+    }
+}
 
 __device__ double ramp_adjoint(const double* adj, uint32_t nb, int s, int w, double per_c) {
     double acc = 0.0;
@@ -2031,11 +2084,6 @@ __device__ double contribution_adjoint(const RenderParams& P, const AdjArgs& A, 
         }
     }
     return a;
-}
-
-__device__ __forceinline__ double wave_sum_d(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-    return v;
 }
 
 // The primal of k_shade (same draws, same arithmetic) plus forward-mode
@@ -2270,7 +2318,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
                     const double sr = wave_sum_d(mine ? g_refl[q] : 0.0);
                     const double ss = wave_sum_d(mine ? g_spec[q] : 0.0);
                     if (lane_id() == 0) {
-                        double* gk = adj[key].grad;
+                        double* gk = adj[key].grad_copy();
                         if (sr != 0.0) atomicAdd(gk + ((size_t)kband * 2 + 0) * nmk + q, sr);
                         if (ss != 0.0) atomicAdd(gk + ((size_t)kband * 2 + 1) * nmk + q, ss);
                     }
@@ -2293,18 +2341,177 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
     }
     wave_add_u64(stats + ST_SEGMENTS, n_done);
     if (BATCH) return;
-    // one band per launch (the host replays the bands one after another)
-    int band = __shfl(band_acc, 0);
-    for (int l = 1; l < 64 && band < 0; ++l) band = __shfl(band_acc, l);
-    const uint32_t nm = A0.n_mat;
-    for (uint32_t q = 0; q < nm; ++q) {
-        double sr = wave_sum_d(g_refl[q]);
-        double ss = wave_sum_d(g_spec[q]);
-        if (lane_id() == 0 && band >= 0) {
-            if (sr != 0.0) atomicAdd(A0.grad + ((size_t)band * 2 + 0) * nm + q, sr);
-            if (ss != 0.0) atomicAdd(A0.grad + ((size_t)band * 2 + 1) * nm + q, ss);
+    __shared__ double red[kBlock / 64][2 * kAdMaxMat];
+    block_grad_flush(A0, band_acc, g_refl, g_spec, &red[0][0]);
+}
+
+// The adjoint with EAR_FLAG_ADJ_LOCAL for every ray: no direction or
+// position tangents travel between vertices, so a ray carries only
+// dlogI/drefl (n_mat floats).  At each visible connection of a hit on
+// material m, d c / d spec_m = sign I air^l 2 / (4 pi l^2) (lobe - diffuse)
+// and d c / d refl_q = c dlogI/drefl_q.  Same primal and draws as k_shade;
+// the arrays are indexed with compile-time indices only (fully unrolled), so
+// nothing lives in scratch.
+template <int WIDTH, bool BATCH>
+__global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict__ inst, int* gstack,
+                                                       uint32_t gthreads, Queue qin, const uint32_t* __restrict__ n_in,
+                                                       const int2* __restrict__ hits, Tangents tin, Queue qout,
+                                                       uint32_t* __restrict__ n_out, Tangents tout, AdjArgs A0,
+                                                       const AdjArgs* __restrict__ adj,
+                                                       unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kLdsMin * kBlock];
+    const uint32_t n = *n_in;
+    uint32_t n_done = 0;
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    double g_refl[kAdMaxMat], g_spec[kAdMaxMat];
+#pragma unroll
+    for (int q = 0; q < kAdMaxMat; ++q) g_refl[q] = g_spec[q] = 0.0;
+    int band_acc = -1;
+    uint32_t my_id = 0, nvis = 0, ntri = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false;
+        float4 no{}, nd{};
+        uint4 nmeta{};
+        float drefl[kAdMaxMat];
+#pragma unroll
+        for (int q = 0; q < kAdMaxMat; ++q) drefl[q] = 0.0f;
+        uint32_t nm = A0.n_mat;
+        if (i < n) {
+            const float4 ro = qin.o[i];
+            const float4 rd = qin.d[i];
+            const uint4 rm = qin.m[i];
+            const uint32_t id = BATCH ? rm.w : 0u;
+            my_id = id;
+            const InstDev& In = inst[id];
+            const RenderParams& P = In.P;
+            const AdjArgs& A = BATCH ? adj[id] : A0;
+            nm = A.n_mat;
+            const int band = (int)(rm.z & 0xFFu);
+            band_acc = band;
+            const uint32_t k = ((rm.z >> 8) & 0xFFFFu) + 1;
+            const int2 h = hits[i];
+            if (h.y >= 0) {
+                const float t = __int_as_float(h.x);
+                const V3 o{ro.x, ro.y, ro.z};
+                const V3 d{rd.x, rd.y, rd.z};
+                const V3 p{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+                const float4 nmv = In.nm[h.y];
+                const V3 N{nmv.x, nmv.y, nmv.z};
+                V3 nrm = dot(N, d) > 0.0f ? V3{N.x * -1.0f, N.y * -1.0f, N.z * -1.0f} : N;
+                const uint32_t mat = __float_as_uint(nmv.w);
+                const float4 mb = In.mat_band[3 * mat + band];
+                Rng rng;
+                rng.init(P.seed, rm.x, P.ctx_word[band], P.instance, rm.y);
+                int bt = kReflect;
+                if (mb.x >= 0.0f) bt = (rng.next() <= mb.x) ? kReflect : kRefract;
+                const float spec = mb.y;
+                if (bt == kRefract) nrm = neg(nrm);
+                const float l = length(sub(p, o));
+                float I = ro.w;
+                I *= powf_ear(P.air[band], l);
+                const float L = rd.w + l;
+                I *= mb.z;
+                const V3 prev = normalized(d);
+#pragma unroll
+                for (int q = 0; q < kAdMaxMat; ++q)
+                    if ((uint32_t)q < nm) drefl[q] = tin.refl(q, i) + ((uint32_t)q == mat ? 1.0f / mb.z : 0.0f);
+                if (is_normal(I)) {
+                    for (uint32_t r = 0; r < P.n_rec; ++r) {
+                        const RecParams& R = P.rec[r];
+                        const V3 dv = sub(V3{R.px, R.py, R.pz}, p);
+                        const V3 lsdir = normalized(dv);
+                        if (!(dot(lsdir, nrm) > 0.0f)) continue;
+                        float diff, sf;
+                        if (bt == kReflect) {
+                            const V3 rv = reflect(prev, nrm);
+                            diff = -dot(nrm, prev);
+                            sf = fmaxf(0.0f, dot(rv, lsdir));
+                        } else {
+                            diff = dot(nrm, prev);
+                            sf = fmaxf(0.0f, dot(prev, lsdir));
+                        }
+                        const float pw = powf_ear(sf, kExp);
+                        const float factor = spec * kExpInt * pw + (1.0f - spec) * diff;
+                        float c = I;
+                        c *= factor;
+                        const float ls = length(dv);
+                        const float geo_air = powf_ear(P.air[band], ls);
+                        const float geo_inv = 2.0f / (4.0f * kPi * ls * ls);
+                        c *= geo_air;
+                        c *= geo_inv;
+                        if (!is_normal(c)) continue;
+                        if (k & 1u) c *= -1.0f;
+                        const double a_c = contribution_adjoint(P, A, band, R, lsdir, c, (L + ls) / 343.0f, L + ls);
+                        if (a_c == 0.0) continue;
+                        float bt_ = 0.0f;
+                        int bs_ = -1;
+                        uint32_t bo_ = 0;
+                        const Traversal T = make_trav(In, gstack, gthreads);
+                        if (traverse<kLdsMin, kBlock, WIDTH, true, false>(T, p, dv, bt_, bs_, bo_, lds + threadIdx.x,
+                                                                          gtid, nvis, ntri))
+                            continue;
+                        const double cnof = ((k & 1u) ? -1.0 : 1.0) * (double)I * (double)geo_air * (double)geo_inv;
+                        const double ds = a_c * cnof * ((double)kExpInt * (double)pw - (double)diff);
+                        const double ac = a_c * (double)c;
+#pragma unroll
+                        for (int q = 0; q < kAdMaxMat; ++q) {
+                            g_spec[q] += (uint32_t)q == mat ? ds : 0.0;
+                            g_refl[q] += ac * (double)drefl[q];
+                        }
+                    }
+                    cont = !((double)I < P.cutoff) && k < P.max_bounces;
+                }
+                if (cont) {
+                    const V3 rdir = bt == kRefract ? d : reflect(d, nrm);
+                    const V3 v = sample_hemi_spec(rng, nrm, rdir, spec);
+                    no = make_float4(p.x, p.y, p.z, I);
+                    nd = make_float4(v.x, v.y, v.z, L);
+                    nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), id);
+                }
+            }
         }
+        if (BATCH) {
+            // reduce the step's partials per instance present in the wave
+            const uint32_t nmx = adj[my_id].n_mat;
+            uint64_t todo = __ballot(i < n);
+            while (todo) {
+                const int leader = __ffsll((unsigned long long)todo) - 1;
+                const uint32_t key = __shfl(my_id, leader);
+                const int kband = __shfl(band_acc, leader);
+                const bool mine = i < n && my_id == key;
+                const uint32_t nmk = __shfl(nmx, leader);
+#pragma unroll
+                for (int q = 0; q < kAdMaxMat; ++q) {
+                    if ((uint32_t)q >= nmk) break;
+                    const double sr = wave_sum_d(mine ? g_refl[q] : 0.0);
+                    const double ss = wave_sum_d(mine ? g_spec[q] : 0.0);
+                    if (lane_id() == 0) {
+                        double* gk = adj[key].grad_copy();
+                        if (sr != 0.0) atomicAdd(gk + ((size_t)kband * 2 + 0) * nmk + q, sr);
+                        if (ss != 0.0) atomicAdd(gk + ((size_t)kband * 2 + 1) * nmk + q, ss);
+                    }
+                }
+                todo &= ~__ballot(mine);
+            }
+#pragma unroll
+            for (int q = 0; q < kAdMaxMat; ++q) g_refl[q] = g_spec[q] = 0.0;
+        }
+        const uint32_t slot = wave_append(n_out, cont);
+        if (cont) {
+            qout.o[slot] = no;
+            qout.d[slot] = nd;
+            qout.m[slot] = nmeta;
+#pragma unroll
+            for (int q = 0; q < kAdMaxMat; ++q)
+                if ((uint32_t)q < nm) tout.refl(q, slot) = drefl[q];
+        }
+        n_done += i < n ? 1u : 0u;
     }
+    wave_add_u64(stats + ST_SEGMENTS, n_done);
+    if (BATCH) return;
+    __shared__ double red[kBlock / 64][2 * kAdMaxMat];
+    block_grad_flush(A0, band_acc, g_refl, g_spec, &red[0][0]);
 }
 
 __global__ void k_zero_f32(float* p, size_t n) {
@@ -2424,7 +2631,8 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     TRY(dsegs.alloc(sizeof(GenSeg) * n_items));
     TRY(dsegoff.alloc(4 * (size_t)n_items));
     TRY(dplan.alloc(8));
-    TRY(dgrad.alloc(sizeof(double) * 3 * 2 * nm_max * n_items));
+    const uint32_t grad_n = 3 * 2 * nm_max * n_items;  // one copy
+    TRY(dgrad.alloc(sizeof(double) * (size_t)grad_n * kGradCopies));
     TRY(dargs.alloc(sizeof(AdjArgs) * n_items));
     if (any_s) {
         TRY(dadj_s.alloc(adj_cells * 8));
@@ -2438,7 +2646,7 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     for (uint32_t i = 0; i < n_items; ++i)
         args[i] = AdjArgs{adj_value[i] ? dadj_s.as<double>() + adj_off[i] : nullptr,
                           adj_abs[i] ? dadj_a.as<double>() + adj_off[i] : nullptr,
-                          dgrad.as<double>() + (size_t)i * 3 * 2 * nm_max, scenes[i]->n_mat};
+                          dgrad.as<double>() + (size_t)i * 3 * 2 * nm_max, scenes[i]->n_mat, grad_n};
     EAR_HIP_TRY(c, hipMemcpyAsync(dargs.p, args.data(), sizeof(AdjArgs) * n_items, hipMemcpyHostToDevice, st));
     EAR_HIP_TRY(c, hipMemcpyAsync(dinst.p, inst.data(), sizeof(InstDev) * n_items, hipMemcpyHostToDevice, st));
     EAR_HIP_TRY(c, hipMemsetAsync(dgrad.p, 0, dgrad.n, st));
@@ -2455,6 +2663,8 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     const ShadowQ SQ{sqp.as<float4>(), sqm.as<float4>()};
     const AdjArgs A0 = args[0];
     const AdjArgs* dA = dargs.as<AdjArgs>();
+    bool all_local = true;
+    for (uint32_t i = 0; i < n_items; ++i) all_local = all_local && (inst[i].P.flags & EAR_FLAG_ADJ_LOCAL);
     for (uint32_t band = 0; band < 3; ++band) {
         // this band's contexts as runs of rays
         std::vector<GenSeg> segs;
@@ -2481,7 +2691,7 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
                 k_generate<false><<<persistent_grid(c, n_chunk, waves), kBlock, 0, st>>>(
                     di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), (uint32_t)segs.size(), n_chunk,
                     dplan.as<uint32_t>(), max_rec, 0u, q[0], cnt + 0, SQ, cnt + 2, stp);
-            k_zero_f32<<<1024, 256, 0, st>>>(T[0].t, (size_t)cap * nm_max * 8);
+            if (!all_local) k_zero_f32<<<1024, 256, 0, st>>>(T[0].t, (size_t)cap * nm_max * 8);
             k_zero_f32<<<1024, 256, 0, st>>>(T[0].r, (size_t)cap * nm_max);
             EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
             EAR_HIP_TRY(c, hipStreamSynchronize(st));
@@ -2493,8 +2703,16 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
                                    Fetch{cnt + 4, 56u, 0u, 1u});
                 const uint32_t gsd = persistent_grid(c, n_live, waves);
 #define EAR_AD_LAUNCH(W, B)                                                                                       \
-    k_shade_ad<W, B><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur, hits.as<int2>(), \
-                                             T[cur], q[1 - cur], cnt + (1 - cur), T[1 - cur], A0, dA, stp)
+    do {                                                                                                          \
+        if (all_local)                                                                                            \
+            k_shade_adl<W, B><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,              \
+                                                      hits.as<int2>(), T[cur], q[1 - cur], cnt + (1 - cur),       \
+                                                      T[1 - cur], A0, dA, stp);                                   \
+        else                                                                                                      \
+            k_shade_ad<W, B><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,               \
+                                                     hits.as<int2>(), T[cur], q[1 - cur], cnt + (1 - cur),        \
+                                                     T[1 - cur], A0, dA, stp);                                    \
+    } while (0)
                 if (width == 4) {
                     if (batch)
                         EAR_AD_LAUNCH(4, true);
@@ -2517,7 +2735,10 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
             EAR_HIP_TRY(c, hipGetLastError());
         }
     }
-    std::vector<double> g((size_t)3 * 2 * nm_max * n_items);
+    k_sum_grad_copies<<<std::min<uint32_t>((grad_n + 255) / 256, 1024), 256, 0, st>>>(dgrad.as<double>(), grad_n,
+                                                                                      grad_n);
+    EAR_HIP_TRY(c, hipGetLastError());
+    std::vector<double> g(grad_n);
     EAR_HIP_TRY(c, hipMemcpyAsync(g.data(), dgrad.p, g.size() * 8, hipMemcpyDeviceToHost, st));
     EAR_HIP_TRY(c, hipStreamSynchronize(st));
     for (uint32_t i = 0; i < n_items; ++i) {
