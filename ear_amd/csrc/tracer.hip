@@ -125,10 +125,14 @@ __device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred, u
     return sc[kWaves] + sc[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
-__device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint64_t v) {
+// Statistics counters: [kStatCopies][ST_N], a block adding into copy
+// blockIdx % kStatCopies (the host sums the copies), so that the one atomic
+// per wave at the end of every launch does not pile up on one address.
+constexpr uint32_t kStatCopies = 64;
+__device__ __forceinline__ void stat_add(unsigned long long* stats, int idx, uint64_t v) {
     // reduce within the wave, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-    if (lane_id() == 0 && v) atomicAdd(counter, (unsigned long long)v);
+    if (lane_id() == 0 && v) atomicAdd(stats + (size_t)(blockIdx.x % kStatCopies) * ST_N + idx, (unsigned long long)v);
 }
 
 // Lane refilling (see Fetch): call with the whole wave when !exhausted and at
@@ -727,7 +731,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__
         }
         n_done += valid ? 1u : 0u;
     }
-    wave_add_u64(stats + ST_RAYS, n_done);
+    stat_add(stats, ST_RAYS, n_done);
 }
 
 template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
@@ -753,8 +757,8 @@ __global__ void __launch_bounds__(BLOCK) k_extend(const InstDev* __restrict__ in
         }
     }
     if (INSTR) {
-        wave_add_u64(stats + ST_NV_CL, nvis);
-        wave_add_u64(stats + ST_NT_CL, ntri);
+        stat_add(stats, ST_NV_CL, nvis);
+        stat_add(stats, ST_NT_CL, ntri);
     }
 }
 
@@ -890,8 +894,8 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
         }
         n_done += valid ? 1u : 0u;
     }
-    wave_add_u64(stats + ST_SEGMENTS, n_done);
-    wave_add_u64(stats + ST_CONTRIB, n_sph);
+    stat_add(stats, ST_SEGMENTS, n_done);
+    stat_add(stats, ST_CONTRIB, n_sph);
 }
 
 // Scene::Connect + Mesh::LineIntersection (src/Scene.cpp:84-96,
@@ -931,11 +935,11 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(const InstDev* __restrict__ in
             }
         }
     }
-    wave_add_u64(stats + ST_SHADOW, nsh);
-    wave_add_u64(stats + ST_CONTRIB, ncon);
+    stat_add(stats, ST_SHADOW, nsh);
+    stat_add(stats, ST_CONTRIB, ncon);
     if (INSTR) {
-        wave_add_u64(stats + ST_NV_SH, nvis);
-        wave_add_u64(stats + ST_NT_SH, ntri);
+        stat_add(stats, ST_NV_SH, nvis);
+        stat_add(stats, ST_NT_SH, ntri);
     }
 }
 
@@ -989,8 +993,8 @@ __global__ void __launch_bounds__(BLOCK) k_extend_f(const InstDev* __restrict__ 
         }
     }
     if (INSTR) {
-        wave_add_u64(stats + ST_NV_CL, nvis);
-        wave_add_u64(stats + ST_NT_CL, ntri);
+        stat_add(stats, ST_NV_CL, nvis);
+        stat_add(stats, ST_NT_CL, ntri);
     }
 }
 
@@ -1059,11 +1063,11 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
             pending = !w.found;
         }
     }
-    wave_add_u64(stats + ST_SHADOW, nsh);
-    wave_add_u64(stats + ST_CONTRIB, ncon);
+    stat_add(stats, ST_SHADOW, nsh);
+    stat_add(stats, ST_CONTRIB, ncon);
     if (INSTR) {
-        wave_add_u64(stats + ST_NV_SH, nvis);
-        wave_add_u64(stats + ST_NT_SH, ntri);
+        stat_add(stats, ST_NV_SH, nvis);
+        stat_add(stats, ST_NT_SH, ntri);
     }
 }
 
@@ -1624,7 +1628,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     DevBuf segc(c);
     TRY(segc.alloc(2 * kMaxSeg * kSegStride * 4));
     EAR_HIP_TRY(c, hipMemsetAsync(segc.p, 0, segc.n, st));
-    TRY(stats.alloc(ST_N * 8));
+    TRY(stats.alloc(ST_N * 8 * kStatCopies));
     TRY(hist.alloc(hist_cells * 16));
     uint32_t gthreads = std::max(cap_e * (uint32_t)cfg_e.block, cap_s * (uint32_t)cfg_s.block);
     gthreads = std::max<uint32_t>(gthreads, n_items * kBlock);
@@ -1905,8 +1909,11 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     EAR_HIP_TRY(c, hipStreamSynchronize(st));
 
     if (st_out) {
-        unsigned long long hs[ST_N];
-        EAR_HIP_TRY(c, hipMemcpy(hs, stats.p, sizeof(hs), hipMemcpyDeviceToHost));
+        std::vector<unsigned long long> hc((size_t)ST_N * kStatCopies);
+        EAR_HIP_TRY(c, hipMemcpy(hc.data(), stats.p, hc.size() * 8, hipMemcpyDeviceToHost));
+        unsigned long long hs[ST_N] = {};
+        for (uint32_t k = 0; k < kStatCopies; ++k)
+            for (int j = 0; j < ST_N; ++j) hs[j] += hc[(size_t)k * ST_N + j];
         std::memset(st_out, 0, sizeof(*st_out));
         st_out->segments = hs[ST_SEGMENTS];
         st_out->shadow_rays = hs[ST_SHADOW];
@@ -2339,7 +2346,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
         }
         n_done += i < n ? 1u : 0u;
     }
-    wave_add_u64(stats + ST_SEGMENTS, n_done);
+    stat_add(stats, ST_SEGMENTS, n_done);
     if (BATCH) return;
     __shared__ double red[kBlock / 64][2 * kAdMaxMat];
     block_grad_flush(A0, band_acc, g_refl, g_spec, &red[0][0]);
@@ -2508,7 +2515,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
         }
         n_done += i < n ? 1u : 0u;
     }
-    wave_add_u64(stats + ST_SEGMENTS, n_done);
+    stat_add(stats, ST_SEGMENTS, n_done);
     if (BATCH) return;
     __shared__ double red[kBlock / 64][2 * kAdMaxMat];
     block_grad_flush(A0, band_acc, g_refl, g_spec, &red[0][0]);
@@ -2614,7 +2621,7 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     }
     TRY(hits.alloc((size_t)cap * 8));
     TRY(counters.alloc(32));
-    TRY(stats.alloc(ST_N * 8));
+    TRY(stats.alloc(ST_N * 8 * kStatCopies));
     TRY(sqp.alloc(16));
     TRY(sqm.alloc(16));
     ExtendFn fe;
