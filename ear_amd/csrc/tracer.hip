@@ -35,6 +35,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kLdsMin = 16;  // smallest LDS part of the traversal stack; the overflow area covers the rest
+constexpr int kShadeBlock = 256;  // k_shade: its block-wide appends take one counter atomic per block step
 
 enum StatIdx {
     ST_SEGMENTS = 0,
@@ -765,15 +766,15 @@ __global__ void __launch_bounds__(BLOCK) k_extend(const InstDev* __restrict__ in
 // Scene::Bounce + Material::Bounce + the energy bookkeeping and NEE weight of
 // Scene::Render (src/Scene.cpp:49-82, 151-277, src/Material.cpp:76-83).
 template <bool BATCH>
-__global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ inst, Queue qin,
+__global__ void __launch_bounds__(kShadeBlock) k_shade(const InstDev* __restrict__ inst, Queue qin,
                                                    const uint32_t* __restrict__ n_in, const int2* __restrict__ hits,
                                                    Queue qout, uint32_t* __restrict__ n_out, ShadowQ sq,
                                                    uint32_t* __restrict__ n_sq, uint32_t max_rec,
                                                    unsigned long long* __restrict__ stats) {
-    __shared__ uint32_t sc[4][kBlock / 64 + 1];  // scratch of the block-wide appends, alternated
+    __shared__ uint32_t sc[4][kShadeBlock / 64 + 1];  // scratch of the block-wide appends, alternated
     uint32_t n = *n_in;
     uint32_t n_done = 0, n_sph = 0;
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < n;
         bool alive = false;  // hit and intensity normal: NEE runs
@@ -868,7 +869,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
                     }
                 }
             }
-            uint32_t slot = block_append<kBlock>(n_sq, emit, sc[r & 1u]);
+            uint32_t slot = block_append<kShadeBlock>(n_sq, emit, sc[r & 1u]);
             if (emit) {
                 sq.p[slot] = make_float4(p.x, p.y, p.z, c);
                 sq.m[slot] = make_float4(L, __uint_as_float(r | ((uint32_t)band << 8)), __uint_as_float(id), 0.0f);
@@ -886,7 +887,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(const InstDev* __restrict__ in
             nd = make_float4(v.x, v.y, v.z, L);
             nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), id);
         }
-        uint32_t slot = block_append<kBlock>(n_out, cont, sc[2 + ((base / (gridDim.x * kBlock)) & 1u)]);
+        uint32_t slot = block_append<kShadeBlock>(n_out, cont, sc[2 + ((base / (gridDim.x * kShadeBlock)) & 1u)]);
         if (cont) {
             qout.o[slot] = no;
             qout.d[slot] = nd;
@@ -1731,7 +1732,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const uint32_t n_segs = (uint32_t)std::max<size_t>(1, all.size());
     const uint32_t total32 = (uint32_t)total_rays;
     const uint32_t g_gen = persistent_grid(c, cap_rays, waves);
-    const uint32_t g_shade = persistent_grid(c, cap_rays, waves);
+    const uint32_t g_shade = persistent_grid(c, cap_rays, waves, kShadeBlock);
     const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
     const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
     const bool timed = tu.graph_iters == 0;
@@ -1760,10 +1761,10 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         EventPair* e2 = timed ? &mk(ev_sha) : nullptr;
         if (e2) hipEventRecord(e2->a, st);
         if (batch)
-            k_shade<true><<<g_shade, kBlock, 0, st>>>(di, qq[cur], nq, hits.as<int2>(), qq[1 - cur], cnt + (1 - cur),
+            k_shade<true><<<g_shade, kShadeBlock, 0, st>>>(di, qq[cur], nq, hits.as<int2>(), qq[1 - cur], cnt + (1 - cur),
                                                       SQ, cnt + 2, (uint32_t)max_recs, stp);
         else
-            k_shade<false><<<g_shade, kBlock, 0, st>>>(di, qq[cur], nq, hits.as<int2>(), qq[1 - cur],
+            k_shade<false><<<g_shade, kShadeBlock, 0, st>>>(di, qq[cur], nq, hits.as<int2>(), qq[1 - cur],
                                                        cnt + (1 - cur), SQ, cnt + 2, (uint32_t)max_recs, stp);
         if (e2) hipEventRecord(e2->b, st);
         EventPair* e3 = timed ? &mk(ev_shw) : nullptr;
