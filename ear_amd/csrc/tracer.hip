@@ -2625,10 +2625,19 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     TRY(stats.alloc(ST_N * 8 * kStatCopies));
     TRY(sqp.alloc(16));
     TRY(sqm.alloc(16));
+    // static closest-hit traversal by default (the replayed bands run one
+    // at a time, with fewer rays in flight than the forward's, where the
+    // refill bookkeeping does not pay); the refilling one on request
+    const bool fetch_e = tu.traversal == EAR_TRAVERSAL_FETCH || tu.traversal == EAR_TRAVERSAL_FETCH_ALL;
     ExtendFn fe;
     ShadowFn fs_unused;
-    const TravCfg cfg{256, 16, width, false, batch, false};
+    const TravCfg cfg{256, 16, width, false, batch, fetch_e};
     pick_kernels(cfg, &fe, &fs_unused);
+    DevBuf segc(c);
+    TRY(segc.alloc(kMaxSeg * kSegStride * 4));
+    EAR_HIP_TRY(c, hipMemsetAsync(segc.p, 0, segc.n, st));
+    const uint32_t nseg = tu.fetch_segments ? std::min<uint32_t>(tu.fetch_segments, kMaxSeg) : 64u;
+    const Fetch fq_e{segc.as<uint32_t>(), tu.refill ? tu.refill : 24u, tu.fetch_chunk, nseg};
     const uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 32;
     const uint32_t cap_e = (uint32_t)c->num_cus * (uint32_t)std::max(1, resident_blocks((const void*)fe, 256));
     const uint32_t gmax = persistent_grid(c, 0xFFFFFFFFu, waves);
@@ -2707,8 +2716,8 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
             int cur = 0;
             for (uint32_t k = 1; k <= max_bounces && n_live > 0; ++k) {
                 hipLaunchKernelGGL(fe, dim3(grid_for(n_live, cap_e, 256)), dim3(256), 0, st, di, gs.as<int>(),
-                                   gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp,
-                                   Fetch{cnt + 4, 56u, 0u, 1u});
+                                   gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp, fq_e);
+                if (fetch_e) k_reset_iter<<<1, 64, 0, st>>>(cnt, 0u, segc.as<uint32_t>(), nseg);
                 const uint32_t gsd = persistent_grid(c, n_live, waves);
 #define EAR_AD_LAUNCH(W, B)                                                                                       \
     do {                                                                                                          \
