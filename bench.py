@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-instrument", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batch (256 rooms) and forward + adjoint legs")
+    ap.add_argument("--legs", default="main,batch,adjoint",
+                    help="comma list of legs to run: main (the hall), batch, adjoint (for profiling one leg)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (PMC calibration) and exit")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_extend_hall200k.json"))
@@ -89,6 +91,18 @@ def main():
     import numpy as np
 
     from ear_amd import api, scenes
+
+    legs = set(a.legs.split(","))
+    if "main" not in legs:
+        ctx = api.Context(0)
+        extras = {}
+        if "batch" in legs:
+            extras["batch"] = bench_batch(ctx)
+        if "adjoint" in legs:
+            extras["adjoint"] = bench_adjoint(ctx)
+        ctx.close()
+        print(json.dumps({"legs": sorted(legs), "extras": extras}), flush=True)
+        return
 
     t0 = time.time()
     s = scenes.hall(target_tris=a.tris)
@@ -151,8 +165,12 @@ def main():
                 "queries_per_launch": round(ext_q / max(1, ext_launch), 1)}
 
     extras = None
-    if not a.no_extras:
-        extras = {"batch": bench_batch(ctx), "adjoint": bench_adjoint(ctx)}
+    if not a.no_extras and legs & {"batch", "adjoint"}:
+        extras = {}
+        if "batch" in legs:
+            extras["batch"] = bench_batch(ctx)
+        if "adjoint" in legs:
+            extras["adjoint"] = bench_adjoint(ctx)
 
     cpu = None
     if not a.no_cpu:
