@@ -2353,159 +2353,148 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
     block_grad_flush(A0, band_acc, g_refl, g_spec, &red[0][0]);
 }
 
+// Connection requests of the local adjoint: the hit point, the gradient
+// weights the connection adds when it is visible, and which recorder,
+// band, material and instance it belongs to.
+struct AdjShadowQ {
+    float4* p;  // {px, py, pz, d c / d spec_mat * dL/dc}
+    uint4* m;   // {rec | band << 8 | mat << 16, instance, -, -}
+    float4* w;  // [2 per request]: dL/dc * c * dlogI/drefl_q, q = 0..7
+};
+
 // The adjoint with EAR_FLAG_ADJ_LOCAL for every ray: no direction or
 // position tangents travel between vertices, so a ray carries only
-// dlogI/drefl (n_mat floats).  At each visible connection of a hit on
-// material m, d c / d spec_m = sign I air^l 2 / (4 pi l^2) (lobe - diffuse)
-// and d c / d refl_q = c dlogI/drefl_q.  Same primal and draws as k_shade;
-// the arrays are indexed with compile-time indices only (fully unrolled), so
-// nothing lives in scratch.
-template <int WIDTH, bool BATCH>
-__global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict__ inst, int* gstack,
-                                                       uint32_t gthreads, Queue qin, const uint32_t* __restrict__ n_in,
+// dlogI/drefl (n_mat floats).  At each connection of a hit on material m,
+// d c / d spec_m = sign I air^l 2 / (4 pi l^2) (lobe - diffuse) and
+// d c / d refl_q = c dlogI/drefl_q; times dL/dc they are queued as a request
+// whose visibility k_adj_vis decides, as k_shade queues the forward's.  Same
+// primal and draws as k_shade; arrays are indexed at compile time only.
+template <bool BATCH>
+__global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict__ inst, Queue qin,
+                                                       const uint32_t* __restrict__ n_in,
                                                        const int2* __restrict__ hits, Tangents tin, Queue qout,
-                                                       uint32_t* __restrict__ n_out, Tangents tout, AdjArgs A0,
+                                                       uint32_t* __restrict__ n_out, Tangents tout, AdjShadowQ aq,
+                                                       uint32_t* __restrict__ n_aq, uint32_t max_rec, AdjArgs A0,
                                                        const AdjArgs* __restrict__ adj,
                                                        unsigned long long* __restrict__ stats) {
-    __shared__ int lds[kLdsMin * kBlock];
+    __shared__ uint32_t sc[4][kBlock / 64 + 1];  // scratch of the block-wide appends, alternated
     const uint32_t n = *n_in;
     uint32_t n_done = 0;
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    double g_refl[kAdMaxMat], g_spec[kAdMaxMat];
-#pragma unroll
-    for (int q = 0; q < kAdMaxMat; ++q) g_refl[q] = g_spec[q] = 0.0;
-    int band_acc = -1;
-    uint32_t my_id = 0, nvis = 0, ntri = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         const uint32_t i = base + threadIdx.x;
-        bool cont = false;
+        bool cont = false, alive = false;
         float4 no{}, nd{};
         uint4 nmeta{};
         float drefl[kAdMaxMat];
 #pragma unroll
         for (int q = 0; q < kAdMaxMat; ++q) drefl[q] = 0.0f;
-        uint32_t nm = A0.n_mat;
+        uint32_t nm = A0.n_mat, id = 0, k = 0, mat = 0, n_rec = 0;
+        int band = 0, bt = kReflect;
+        float I = 0.0f, L = 0.0f, spec = 0.0f;
+        V3 p{0, 0, 0}, nrm{0, 0, 0}, prev{0, 0, 0};
         if (i < n) {
             const float4 ro = qin.o[i];
             const float4 rd = qin.d[i];
             const uint4 rm = qin.m[i];
-            const uint32_t id = BATCH ? rm.w : 0u;
-            my_id = id;
+            id = BATCH ? rm.w : 0u;
             const InstDev& In = inst[id];
             const RenderParams& P = In.P;
-            const AdjArgs& A = BATCH ? adj[id] : A0;
-            nm = A.n_mat;
-            const int band = (int)(rm.z & 0xFFu);
-            band_acc = band;
-            const uint32_t k = ((rm.z >> 8) & 0xFFFFu) + 1;
+            nm = (BATCH ? adj[id] : A0).n_mat;
+            n_rec = P.n_rec;
+            band = (int)(rm.z & 0xFFu);
+            k = ((rm.z >> 8) & 0xFFFFu) + 1;
             const int2 h = hits[i];
             if (h.y >= 0) {
                 const float t = __int_as_float(h.x);
                 const V3 o{ro.x, ro.y, ro.z};
                 const V3 d{rd.x, rd.y, rd.z};
-                const V3 p{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+                p = V3{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
                 const float4 nmv = In.nm[h.y];
                 const V3 N{nmv.x, nmv.y, nmv.z};
-                V3 nrm = dot(N, d) > 0.0f ? V3{N.x * -1.0f, N.y * -1.0f, N.z * -1.0f} : N;
-                const uint32_t mat = __float_as_uint(nmv.w);
+                nrm = dot(N, d) > 0.0f ? V3{N.x * -1.0f, N.y * -1.0f, N.z * -1.0f} : N;
+                mat = __float_as_uint(nmv.w);
                 const float4 mb = In.mat_band[3 * mat + band];
                 Rng rng;
                 rng.init(P.seed, rm.x, P.ctx_word[band], P.instance, rm.y);
-                int bt = kReflect;
                 if (mb.x >= 0.0f) bt = (rng.next() <= mb.x) ? kReflect : kRefract;
-                const float spec = mb.y;
+                spec = mb.y;
                 if (bt == kRefract) nrm = neg(nrm);
                 const float l = length(sub(p, o));
-                float I = ro.w;
+                I = ro.w;
                 I *= powf_ear(P.air[band], l);
-                const float L = rd.w + l;
+                L = rd.w + l;
                 I *= mb.z;
-                const V3 prev = normalized(d);
+                prev = normalized(d);
 #pragma unroll
                 for (int q = 0; q < kAdMaxMat; ++q)
                     if ((uint32_t)q < nm) drefl[q] = tin.refl(q, i) + ((uint32_t)q == mat ? 1.0f / mb.z : 0.0f);
-                if (is_normal(I)) {
-                    for (uint32_t r = 0; r < P.n_rec; ++r) {
-                        const RecParams& R = P.rec[r];
-                        const V3 dv = sub(V3{R.px, R.py, R.pz}, p);
-                        const V3 lsdir = normalized(dv);
-                        if (!(dot(lsdir, nrm) > 0.0f)) continue;
-                        float diff, sf;
-                        if (bt == kReflect) {
-                            const V3 rv = reflect(prev, nrm);
-                            diff = -dot(nrm, prev);
-                            sf = fmaxf(0.0f, dot(rv, lsdir));
-                        } else {
-                            diff = dot(nrm, prev);
-                            sf = fmaxf(0.0f, dot(prev, lsdir));
-                        }
-                        const float pw = powf_ear(sf, kExp);
-                        const float factor = spec * kExpInt * pw + (1.0f - spec) * diff;
-                        float c = I;
-                        c *= factor;
-                        const float ls = length(dv);
-                        const float geo_air = powf_ear(P.air[band], ls);
-                        const float geo_inv = 2.0f / (4.0f * kPi * ls * ls);
-                        c *= geo_air;
-                        c *= geo_inv;
-                        if (!is_normal(c)) continue;
-                        if (k & 1u) c *= -1.0f;
-                        const double a_c = contribution_adjoint(P, A, band, R, lsdir, c, (L + ls) / 343.0f, L + ls);
-                        if (a_c == 0.0) continue;
-                        float bt_ = 0.0f;
-                        int bs_ = -1;
-                        uint32_t bo_ = 0;
-                        const Traversal T = make_trav(In, gstack, gthreads);
-                        if (traverse<kLdsMin, kBlock, WIDTH, true, false>(T, p, dv, bt_, bs_, bo_, lds + threadIdx.x,
-                                                                          gtid, nvis, ntri))
-                            continue;
-                        const double cnof = ((k & 1u) ? -1.0 : 1.0) * (double)I * (double)geo_air * (double)geo_inv;
-                        const double ds = a_c * cnof * ((double)kExpInt * (double)pw - (double)diff);
-                        const double ac = a_c * (double)c;
-#pragma unroll
-                        for (int q = 0; q < kAdMaxMat; ++q) {
-                            g_spec[q] += (uint32_t)q == mat ? ds : 0.0;
-                            g_refl[q] += ac * (double)drefl[q];
-                        }
-                    }
+                alive = is_normal(I);
+                if (alive) {
                     cont = !((double)I < P.cutoff) && k < P.max_bounces;
-                }
-                if (cont) {
-                    const V3 rdir = bt == kRefract ? d : reflect(d, nrm);
-                    const V3 v = sample_hemi_spec(rng, nrm, rdir, spec);
-                    no = make_float4(p.x, p.y, p.z, I);
-                    nd = make_float4(v.x, v.y, v.z, L);
-                    nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), id);
-                }
-            }
-        }
-        if (BATCH) {
-            // reduce the step's partials per instance present in the wave
-            const uint32_t nmx = adj[my_id].n_mat;
-            uint64_t todo = __ballot(i < n);
-            while (todo) {
-                const int leader = __ffsll((unsigned long long)todo) - 1;
-                const uint32_t key = __shfl(my_id, leader);
-                const int kband = __shfl(band_acc, leader);
-                const bool mine = i < n && my_id == key;
-                const uint32_t nmk = __shfl(nmx, leader);
-#pragma unroll
-                for (int q = 0; q < kAdMaxMat; ++q) {
-                    if ((uint32_t)q >= nmk) break;
-                    const double sr = wave_sum_d(mine ? g_refl[q] : 0.0);
-                    const double ss = wave_sum_d(mine ? g_spec[q] : 0.0);
-                    if (lane_id() == 0) {
-                        double* gk = adj[key].grad_copy();
-                        if (sr != 0.0) atomicAdd(gk + ((size_t)kband * 2 + 0) * nmk + q, sr);
-                        if (ss != 0.0) atomicAdd(gk + ((size_t)kband * 2 + 1) * nmk + q, ss);
+                    if (cont) {
+                        const V3 rdir = bt == kRefract ? d : reflect(d, nrm);
+                        const V3 v = sample_hemi_spec(rng, nrm, rdir, spec);
+                        no = make_float4(p.x, p.y, p.z, I);
+                        nd = make_float4(v.x, v.y, v.z, L);
+                        nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), id);
                     }
                 }
-                todo &= ~__ballot(mine);
             }
-#pragma unroll
-            for (int q = 0; q < kAdMaxMat; ++q) g_refl[q] = g_spec[q] = 0.0;
         }
-        const uint32_t slot = wave_append(n_out, cont);
+        // connections to every recorder (src/Scene.cpp:185-268), queued with
+        // their gradient weights
+        for (uint32_t r = 0; r < max_rec; ++r) {
+            bool emit = false;
+            float ds = 0.0f, ac = 0.0f;
+            if (alive && r < n_rec) {
+                const RenderParams& P = inst[id].P;
+                const AdjArgs& A = BATCH ? adj[id] : A0;
+                const RecParams& R = P.rec[r];
+                const V3 dv = sub(V3{R.px, R.py, R.pz}, p);
+                const V3 lsdir = normalized(dv);
+                if (dot(lsdir, nrm) > 0.0f) {
+                    float diff, sf;
+                    if (bt == kReflect) {
+                        const V3 rv = reflect(prev, nrm);
+                        diff = -dot(nrm, prev);
+                        sf = fmaxf(0.0f, dot(rv, lsdir));
+                    } else {
+                        diff = dot(nrm, prev);
+                        sf = fmaxf(0.0f, dot(prev, lsdir));
+                    }
+                    const float pw = powf_ear(sf, kExp);
+                    const float factor = spec * kExpInt * pw + (1.0f - spec) * diff;
+                    float c = I;
+                    c *= factor;
+                    const float ls = length(dv);
+                    const float geo_air = powf_ear(P.air[band], ls);
+                    const float geo_inv = 2.0f / (4.0f * kPi * ls * ls);
+                    c *= geo_air;
+                    c *= geo_inv;
+                    if (is_normal(c)) {
+                        if (k & 1u) c *= -1.0f;
+                        const double a_c =
+                            contribution_adjoint(P, A, band, R, lsdir, c, (L + ls) / 343.0f, L + ls);
+                        if (a_c != 0.0) {
+                            const double cnof =
+                                ((k & 1u) ? -1.0 : 1.0) * (double)I * (double)geo_air * (double)geo_inv;
+                            ds = (float)(a_c * cnof * ((double)kExpInt * (double)pw - (double)diff));
+                            ac = (float)(a_c * (double)c);
+                            emit = true;
+                        }
+                    }
+                }
+            }
+            const uint32_t slot = block_append<kBlock>(n_aq, emit, sc[r & 1u]);
+            if (emit) {
+                aq.p[slot] = make_float4(p.x, p.y, p.z, ds);
+                aq.m[slot] = make_uint4(r | ((uint32_t)band << 8) | (mat << 16), id, 0u, 0u);
+                aq.w[2 * (size_t)slot] = make_float4(ac * drefl[0], ac * drefl[1], ac * drefl[2], ac * drefl[3]);
+                aq.w[2 * (size_t)slot + 1] =
+                    make_float4(ac * drefl[4], ac * drefl[5], ac * drefl[6], ac * drefl[7]);
+            }
+        }
+        const uint32_t slot = block_append<kBlock>(n_out, cont, sc[2 + ((base / (gridDim.x * kBlock)) & 1u)]);
         if (cont) {
             qout.o[slot] = no;
             qout.d[slot] = nd;
@@ -2517,9 +2506,124 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
         n_done += i < n ? 1u : 0u;
     }
     stat_add(stats, ST_SEGMENTS, n_done);
-    if (BATCH) return;
-    __shared__ double red[kBlock / 64][2 * kAdMaxMat];
-    block_grad_flush(A0, band_acc, g_refl, g_spec, &red[0][0]);
+}
+
+// Visibility of the local adjoint's connection requests (Scene::Connect,
+// src/Scene.cpp:84-96): the lane-refilling any-hit walk of k_shadow_f; a
+// visible request adds its weights to the gradient of its instance and band.
+template <int WIDTH, bool BATCH>
+__global__ void __launch_bounds__(kBlock) k_adj_vis(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
+                                                     AdjShadowQ aq, const uint32_t* __restrict__ n_aq, AdjArgs A0,
+                                                     const AdjArgs* __restrict__ adj,
+                                                     unsigned long long* __restrict__ stats, Fetch fetch) {
+    constexpr int LDS = 16;
+    __shared__ int lds[LDS * kBlock];
+    const uint32_t n = *n_aq;
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t lane = lane_id();
+    uint32_t nvis = 0, ntri = 0, nsh = 0;
+    Walker<LDS, kBlock, WIDTH, true, false> w(lds + threadIdx.x, gstack, gtid, gthreads);
+    Traversal T = make_trav(inst[0], gstack, gthreads);
+    double g_refl[kAdMaxMat], g_spec[kAdMaxMat];
+#pragma unroll
+    for (int q = 0; q < kAdMaxMat; ++q) g_refl[q] = g_spec[q] = 0.0;
+    uint32_t key = 0xFFFFFFFFu;  // instance of the accumulated partials
+    int kband = -1;
+    bool active = false, pending = false, exhausted = n == 0;
+    uint32_t slot = 0;
+    uint32_t c_next = 0, c_end = 0;
+    uint32_t seg = (gtid >> 6) % fetch.n_seg;
+    for (;;) {
+        uint64_t act = __ballot(active);
+        const uint32_t n_free = 64u - (uint32_t)__popcll(act);
+        if (n_free >= fetch.refill || act == 0) {
+            if (pending) {
+                const float4 pw = aq.p[slot];
+                const uint4 m = aq.m[slot];
+                const uint32_t id = BATCH ? m.y : 0u;
+                const int band = (int)((m.x >> 8) & 0xFFu);
+                const uint32_t mat = m.x >> 16;
+                if (BATCH && key != 0xFFFFFFFFu && (id != key || band != kband)) {
+                    // a batch lane moving to another instance or band hands its partials on
+                    const AdjArgs& A = adj[key];
+                    double* gk = A.grad_copy();
+#pragma unroll
+                    for (int q = 0; q < kAdMaxMat; ++q) {
+                        if ((uint32_t)q < A.n_mat) {
+                            if (g_refl[q] != 0.0) atomicAdd(gk + ((size_t)kband * 2 + 0) * A.n_mat + q, g_refl[q]);
+                            if (g_spec[q] != 0.0) atomicAdd(gk + ((size_t)kband * 2 + 1) * A.n_mat + q, g_spec[q]);
+                        }
+                        g_refl[q] = g_spec[q] = 0.0;
+                    }
+                }
+                key = id;
+                kband = band;
+                const float4 w0 = aq.w[2 * (size_t)slot], w1 = aq.w[2 * (size_t)slot + 1];
+                const float wq[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+                for (int q = 0; q < kAdMaxMat; ++q) {
+                    g_refl[q] += (double)wq[q];
+                    g_spec[q] += (uint32_t)q == mat ? (double)pw.w : 0.0;
+                }
+                pending = false;
+            }
+            if (!exhausted) {
+                const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted, seg);
+                if (mine != 0xFFFFFFFFu) {
+                    slot = mine;
+                    ++nsh;
+                    const float4 pw = aq.p[mine];
+                    const uint4 m = aq.m[mine];
+                    const uint32_t id = BATCH ? m.y : 0u;
+                    if (BATCH) T = make_trav(inst[id], gstack, gthreads);
+                    const RecParams& R = inst[id].P.rec[m.x & 0xFFu];
+                    const V3 p{pw.x, pw.y, pw.z};
+                    if (w.start(T, p, sub(V3{R.px, R.py, R.pz}, p), 0.0f))
+                        active = true;
+                    else
+                        pending = true;  // empty scene: visible
+                }
+            }
+            act = __ballot(active);
+        }
+        // the loop is left by the whole wave at once (take_entries is a
+        // wave-wide step)
+        if (act == 0) {
+            if (exhausted && __ballot(pending) == 0) break;
+            continue;
+        }
+        if (active && w.step(T, nvis, ntri)) {
+            active = false;
+            pending = !w.found;
+        }
+    }
+    stat_add(stats, ST_SHADOW, nsh);
+    if (!BATCH) {
+        __shared__ double red[kBlock / 64][2 * kAdMaxMat];
+        block_grad_flush(A0, kband, g_refl, g_spec, &red[0][0]);
+        return;
+    }
+    // what is left, reduced per instance and band present in the wave
+    uint64_t todo = __ballot(key != 0xFFFFFFFFu);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lk = __shfl(key, leader);
+        const int lb = __shfl(kband, leader);
+        const bool mine = key == lk && kband == lb;
+        const uint32_t nmk = adj[lk].n_mat;
+        double* gk = adj[lk].grad_copy();
+#pragma unroll
+        for (int q = 0; q < kAdMaxMat; ++q) {
+            if ((uint32_t)q >= nmk) break;
+            const double sr = wave_sum_d(mine ? g_refl[q] : 0.0);
+            const double ss = wave_sum_d(mine ? g_spec[q] : 0.0);
+            if (lane == 0) {
+                if (sr != 0.0) atomicAdd(gk + ((size_t)lb * 2 + 0) * nmk + q, sr);
+                if (ss != 0.0) atomicAdd(gk + ((size_t)lb * 2 + 1) * nmk + q, ss);
+            }
+        }
+        todo &= ~__ballot(mine);
+    }
 }
 
 __global__ void k_zero_f32(float* p, size_t n) {
@@ -2625,6 +2729,17 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     TRY(stats.alloc(ST_N * 8 * kStatCopies));
     TRY(sqp.alloc(16));
     TRY(sqm.alloc(16));
+    // connection requests of the local adjoint (one per ray and recorder)
+    bool all_local = true;
+    for (uint32_t i = 0; i < n_items; ++i) all_local = all_local && (inst[i].P.flags & EAR_FLAG_ADJ_LOCAL);
+    DevBuf aqp(c), aqm(c), aqw(c), segv(c);
+    const size_t cap_aq = all_local ? (size_t)cap * std::max<uint32_t>(1, max_rec) : 1;
+    TRY(aqp.alloc(cap_aq * 16));
+    TRY(aqm.alloc(cap_aq * 16));
+    TRY(aqw.alloc(cap_aq * 32));
+    TRY(segv.alloc(kMaxSeg * kSegStride * 4));
+    EAR_HIP_TRY(c, hipMemsetAsync(segv.p, 0, segv.n, st));
+    const AdjShadowQ AQ{aqp.as<float4>(), aqm.as<uint4>(), aqw.as<float4>()};
     // static closest-hit traversal by default (the replayed bands run one
     // at a time, with fewer rays in flight than the forward's, where the
     // refill bookkeeping does not pay); the refilling one on request
@@ -2680,8 +2795,13 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     const ShadowQ SQ{sqp.as<float4>(), sqm.as<float4>()};
     const AdjArgs A0 = args[0];
     const AdjArgs* dA = dargs.as<AdjArgs>();
-    bool all_local = true;
-    for (uint32_t i = 0; i < n_items; ++i) all_local = all_local && (inst[i].P.flags & EAR_FLAG_ADJ_LOCAL);
+    // the visibility kernel: lane refilling over the request queue
+    using VisFn = void (*)(const InstDev*, int*, uint32_t, AdjShadowQ, const uint32_t*, AdjArgs, const AdjArgs*,
+                           unsigned long long*, Fetch);
+    const VisFn fv = width == 4 ? (batch ? k_adj_vis<4, true> : k_adj_vis<4, false>)
+                                : (batch ? k_adj_vis<2, true> : k_adj_vis<2, false>);
+    const uint32_t cap_v = (uint32_t)c->num_cus * (uint32_t)std::max(1, resident_blocks((const void*)fv, kBlock));
+    const Fetch fq_v{segv.as<uint32_t>(), tu.refill ? tu.refill : 24u, tu.fetch_chunk, nseg};
     for (uint32_t band = 0; band < 3; ++band) {
         // this band's contexts as runs of rays
         std::vector<GenSeg> segs;
@@ -2719,29 +2839,36 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
                                    gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp, fq_e);
                 if (fetch_e) k_reset_iter<<<1, 64, 0, st>>>(cnt, 0u, segc.as<uint32_t>(), nseg);
                 const uint32_t gsd = persistent_grid(c, n_live, waves);
-#define EAR_AD_LAUNCH(W, B)                                                                                       \
-    do {                                                                                                          \
-        if (all_local)                                                                                            \
-            k_shade_adl<W, B><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,              \
-                                                      hits.as<int2>(), T[cur], q[1 - cur], cnt + (1 - cur),       \
-                                                      T[1 - cur], A0, dA, stp);                                   \
-        else                                                                                                      \
-            k_shade_ad<W, B><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur,               \
-                                                     hits.as<int2>(), T[cur], q[1 - cur], cnt + (1 - cur),        \
-                                                     T[1 - cur], A0, dA, stp);                                    \
-    } while (0)
-                if (width == 4) {
+                if (all_local) {
                     if (batch)
-                        EAR_AD_LAUNCH(4, true);
+                        k_shade_adl<true><<<gsd, kBlock, 0, st>>>(di, q[cur], cnt + cur, hits.as<int2>(), T[cur],
+                                                                  q[1 - cur], cnt + (1 - cur), T[1 - cur], AQ,
+                                                                  cnt + 2, max_rec, A0, dA, stp);
                     else
-                        EAR_AD_LAUNCH(4, false);
+                        k_shade_adl<false><<<gsd, kBlock, 0, st>>>(di, q[cur], cnt + cur, hits.as<int2>(), T[cur],
+                                                                   q[1 - cur], cnt + (1 - cur), T[1 - cur], AQ,
+                                                                   cnt + 2, max_rec, A0, dA, stp);
+                    const uint32_t n_req = (uint32_t)std::min<size_t>(cap_aq, (size_t)n_live * max_rec);
+                    hipLaunchKernelGGL(fv, dim3(grid_for(n_req, cap_v, kBlock)), dim3(kBlock), 0, st, di, gs.as<int>(),
+                                       gthreads, AQ, (const uint32_t*)(cnt + 2), A0, dA, stp, fq_v);
+                    k_reset_iter<<<1, 64, 0, st>>>(cnt, 0u, segv.as<uint32_t>(), nseg);
                 } else {
-                    if (batch)
-                        EAR_AD_LAUNCH(2, true);
-                    else
-                        EAR_AD_LAUNCH(2, false);
-                }
+#define EAR_AD_LAUNCH(W, B)                                                                                      \
+    k_shade_ad<W, B><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur, hits.as<int2>(), \
+                                             T[cur], q[1 - cur], cnt + (1 - cur), T[1 - cur], A0, dA, stp)
+                    if (width == 4) {
+                        if (batch)
+                            EAR_AD_LAUNCH(4, true);
+                        else
+                            EAR_AD_LAUNCH(4, false);
+                    } else {
+                        if (batch)
+                            EAR_AD_LAUNCH(2, true);
+                        else
+                            EAR_AD_LAUNCH(2, false);
+                    }
 #undef EAR_AD_LAUNCH
+                }
                 EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
                 EAR_HIP_TRY(c, hipStreamSynchronize(st));
                 n_live = pinned[1 - cur];
