@@ -2701,7 +2701,7 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
         }
     }
     const ear_tuning& tu = c->tuning;
-    const uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 22);
+    const uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 23);
     uint64_t rays_any = 0;
     for (uint32_t i = 0; i < n_items; ++i) rays_any = std::max<uint64_t>(rays_any, descs[i]->n_rays);
     uint64_t rays_band_max = 0;
@@ -2834,11 +2834,18 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
             EAR_HIP_TRY(c, hipStreamSynchronize(st));
             uint32_t n_live = pinned[0];
             int cur = 0;
-            for (uint32_t k = 1; k <= max_bounces && n_live > 0; ++k) {
-                hipLaunchKernelGGL(fe, dim3(grid_for(n_live, cap_e, 256)), dim3(256), 0, st, di, gs.as<int>(),
-                                   gthreads, q[cur], (const uint32_t*)(cnt + cur), hits.as<int2>(), stp, fq_e);
+            // bounces in groups of 4 between looks at the queue count: the
+            // kernels read their counts on the device, and a queue never grows
+            // within a chunk, so grids sized at a group's start cover it
+            uint32_t k = 1;
+            while (k <= max_bounces && n_live > 0) {
+              const uint32_t group = std::min<uint32_t>(4u, max_bounces - k + 1);
+              const uint32_t gsd = persistent_grid(c, n_live, waves);
+              const uint32_t ge = grid_for(n_live, cap_e, 256);
+              for (uint32_t gi = 0; gi < group; ++gi, ++k) {
+                hipLaunchKernelGGL(fe, dim3(ge), dim3(256), 0, st, di, gs.as<int>(), gthreads, q[cur],
+                                   (const uint32_t*)(cnt + cur), hits.as<int2>(), stp, fq_e);
                 if (fetch_e) k_reset_iter<<<1, 64, 0, st>>>(cnt, 0u, segc.as<uint32_t>(), nseg);
-                const uint32_t gsd = persistent_grid(c, n_live, waves);
                 if (all_local) {
                     if (batch)
                         k_shade_adl<true><<<gsd, kBlock, 0, st>>>(di, q[cur], cnt + cur, hits.as<int2>(), T[cur],
@@ -2869,11 +2876,12 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
                     }
 #undef EAR_AD_LAUNCH
                 }
-                EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
-                EAR_HIP_TRY(c, hipStreamSynchronize(st));
-                n_live = pinned[1 - cur];
                 k_reset2<<<1, 64, 0, st>>>(cnt + cur, cnt + 2);
                 cur = 1 - cur;
+              }
+              EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
+              EAR_HIP_TRY(c, hipStreamSynchronize(st));
+              n_live = pinned[cur];
             }
             k_reset2<<<1, 64, 0, st>>>(cnt + 0, cnt + 1);
             EAR_HIP_TRY(c, hipGetLastError());
