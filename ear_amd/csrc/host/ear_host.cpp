@@ -541,7 +541,10 @@ void Scene::RenderBands(int sound, const float absorbtion_factor[3], int num_sam
         if ((band_mask >> b) & 1u) any = &rec[b];
     if (!any || any->empty()) throw std::runtime_error("no recorders");
     std::vector<ear_recorder> recs;
-    for (Recorder* r : *any) recs.push_back(r->to_abi(keyframeID));
+    for (Recorder* r : *any) {
+        recs.push_back(r->to_abi(keyframeID));
+        recs.back().radius = options.receiver_radius;
+    }
     std::vector<float> mesh_v;
     ear_render_desc d{};
     d.band_mask = band_mask;
@@ -569,6 +572,7 @@ void Scene::RenderBands(int sound, const float absorbtion_factor[3], int num_sam
     d.keyframe = keyframeID;
     d.sound = (uint32_t)sound;
     d.hist_bins = options.hist_bins;
+    d.estimator = options.estimator;
     int nch = 0;
     for (const ear_recorder& r : recs) nch += r.kind == EAR_REC_STEREO ? 2 : 1;
     std::vector<double> value((size_t)3 * nch * d.hist_bins);

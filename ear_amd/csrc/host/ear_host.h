@@ -283,6 +283,8 @@ struct RenderOptions {
     double energy_cutoff = 1e-8;  // src/Scene.cpp:275
     unsigned hist_bins = 3 * 44100;  // the reference's initial buffer (src/Recorder.h:36)
     unsigned leaf_size = 0, bvh_width = 0, bvh_builder = 0;  // 0: library defaults
+    unsigned estimator = EAR_EST_NEE;  // EAR_EST_SPHERE: receiver spheres of receiver_radius
+    float receiver_radius = 0.0f;
 };
 
 // Scene (src/Scene.h:60-79).

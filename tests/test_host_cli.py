@@ -137,23 +137,29 @@ def write_impulse_wav(path):
 
 
 @pytest.mark.gpu
-def test_calc_t60_matches_python_front_end(tmp_path):
+@pytest.mark.parametrize("radius", [None, 0.5])
+def test_calc_t60_matches_python_front_end(tmp_path, radius):
+    """`EAR calc T60` against the same render through the C ABI; with SET
+    radius the receiver-sphere estimator (no direct-sound ramp)."""
     from ear_amd import acoustics, api
 
     s = scenes.shoebox(spec=0.2)
     p = tmp_path / "box.EAR"
     write_impulse_wav(tmp_path / "dry.wav")
+    extra = {} if radius is None else {"radius": float(radius)}
     earfile.scene_to_ear(s, p, samples=200_000, seed=1234, maxbounces=200, irlength=2.0,
-                         wav=str(tmp_path / "dry.wav"))
+                         wav=str(tmp_path / "dry.wav"), **extra)
     rc, out = run("calc", "T60", str(p))
     assert rc == 0, out
     vals = {l.split(":")[0].strip(): float(l.split(":")[1].strip().rstrip("s")) for l in out.splitlines()
             if l.startswith("T60_")}
     # same render through the C ABI directly: 20000 rays (samples / 10), mid band
+    recs = s["recorders"] if radius is None else [dict(r, radius=radius) for r in s["recorders"]]
     with api.Context(0) as ctx, api.Scene(ctx, s["tris"], s["mat"], s["materials"]) as sc:
         air = (0.99, 0.98, 0.95)
-        r = api.render(ctx, sc, recorders=s["recorders"], n_rays=20000, bands=(1,), air=air, src_pos=s["src"],
-                       seed=1234, max_bounces=200, cutoff=1e-8, n_bins=88200, abs_hist=False)
+        r = api.render(ctx, sc, recorders=recs, n_rays=20000, bands=(1,), air=air, src_pos=s["src"],
+                       seed=1234, max_bounces=200, cutoff=1e-8, n_bins=88200, abs_hist=False,
+                       estimator=0 if radius is None else 1)
     track = acoustics.power(r.value[1, 0].astype(np.float32))
     thr = np.abs(track).max() / 256.0
     nz = np.nonzero(np.abs(track) >= thr)[0]
