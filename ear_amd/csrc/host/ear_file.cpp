@@ -188,7 +188,18 @@ bool read_ear_file(const std::string& path, EarFile* F) {
                 if (F->keys.empty()) throw DatatypeException("Keyframe data not read");
                 s.anim = b.anim();
             } else if (b.peek() == "mesh") {
-                throw DatatypeException("mesh emitters in .EAR sources are not read (the reference cannot read them either)");
+                // an emitting mesh: Mesh::Mesh (src/Mesh.cpp:77-90) reads a
+                // material name and triangles from the block
+                Cursor mb = b.block("mesh");
+                s.mesh_material = mb.str();
+                if (mats.find(s.mesh_material) == mats.end())
+                    throw DatatypeException("mesh source uses material '" + s.mesh_material + "' before its MAT block");
+                while (mb.p < mb.end) {
+                    mb.expect("tri ");
+                    std::array<Vec3, 3> t{mb.vec(), mb.vec(), mb.vec()};
+                    s.mesh_tris.push_back(t);
+                }
+                if (s.mesh_tris.empty()) throw DatatypeException("mesh source without triangles");
             } else {
                 s.location = b.vec();
                 s.has_location = true;

@@ -39,6 +39,8 @@ struct EarFile {
         std::vector<Vec3> anim;
         float gain = 1.0f;
         float offset = 0.0f;
+        std::string mesh_material;                    // emitting mesh (SSRC/3SRC `mesh` element)
+        std::vector<std::array<Vec3, 3>> mesh_tris;
     };
     std::vector<Source> sources;
     struct Listener {

@@ -401,6 +401,10 @@ ear_recorder StereoRecorder::to_abi(int kf) const {
     return r;
 }
 
+AbstractSoundFile::~AbstractSoundFile() {
+    delete mesh;
+    delete animation;
+}
 Vec3 AbstractSoundFile::getLocation(int i) const { return (i >= 0 && animation) ? animation->Evaluate(i) : location; }
 
 // ---------------------------------------------------------------- sound files

@@ -213,7 +213,10 @@ class AbstractSoundFile {
     float gain = 1.0f;
     unsigned offset = 0;  // samples; read from the file, carried by no band (see Band)
     std::string filename;
-    virtual ~AbstractSoundFile() = default;
+    AbstractSoundFile() = default;
+    AbstractSoundFile(const AbstractSoundFile&) = delete;
+    AbstractSoundFile& operator=(const AbstractSoundFile&) = delete;
+    virtual ~AbstractSoundFile();  // owns mesh and animation (src/SoundFile.cpp:71-83)
     Vec3 getLocation(int i = -1) const;
     bool isMeshSource() const { return mesh != nullptr; }
     float getGain() const { return gain; }

@@ -63,6 +63,7 @@ Scene* build_scene(EarFile& F, bool load_audio) {
         }
         src->offset = (unsigned)(s.offset * 44100.0f);
         if (!s.anim.empty()) src->animation = new Animated<Vec3>(s.anim);
+        if (!s.mesh_tris.empty()) src->mesh = new Mesh(Mesh::materials.at(s.mesh_material), s.mesh_tris);
     }
     for (auto& l : F.listeners) {
         Recorder* r;

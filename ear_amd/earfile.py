@@ -70,9 +70,13 @@ class EarWriter:
     def mesh(self, material, tris):
         self.parts.append(block("MESH", [material] + [Tri([Vec(v) for v in t]) for t in tris]))
 
-    def source(self, wav, location=None, anim=None, gain=None, offset=None):
+    def source(self, wav, location=None, anim=None, gain=None, offset=None, mesh=None):
+        """SSRC block; mesh=(material name, triangles) makes an emitting mesh
+        (the reader's `mesh` element, src/SoundFile.cpp:50-53)."""
         items = [wav]
-        if anim is not None:
+        if mesh is not None:
+            items.append(block("mesh", [mesh[0]] + [Tri([Vec(v) for v in t]) for t in mesh[1]]))
+        elif anim is not None:
             items.append(block("anim", [Vec(p) for p in anim]))
         else:
             items.append(Vec(location))
@@ -105,8 +109,11 @@ class EarWriter:
             f.write(self.bytes())
 
 
-def scene_to_ear(s: dict, path: str, *, samples=10 ** 5, absorption=(0.01, 0.02, 0.05), wav="dry.wav", **extra):
-    """Writes a scenes.* dict as a .EAR file: one MAT + MESH block per material."""
+def scene_to_ear(s: dict, path: str, *, samples=10 ** 5, absorption=(0.01, 0.02, 0.05), wav="dry.wav",
+                 src_mesh=None, **extra):
+    """Writes a scenes.* dict as a .EAR file: one MAT + MESH block per
+    material; src_mesh=(material index, triangles) makes the source an
+    emitting mesh."""
     import numpy as np
 
     w = EarWriter()
@@ -118,7 +125,10 @@ def scene_to_ear(s: dict, path: str, *, samples=10 ** 5, absorption=(0.01, 0.02,
         sel = tris[np.asarray(s["mat"]) == i]
         if len(sel):
             w.mesh(f"m{i}", sel.tolist())
-    w.source(wav, location=s["src"])
+    if src_mesh is not None:
+        w.source(wav, mesh=(f"m{src_mesh[0]}", src_mesh[1]))
+    else:
+        w.source(wav, location=s["src"])
     for r in s["recorders"]:
         if r.get("kind", 0) == 1:
             raw = tuple(1.0 - h ** 0.25 for h in r["head_abs"])  # undo the ctor's max(0,(1-h)^4)

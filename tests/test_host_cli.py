@@ -116,6 +116,25 @@ def test_missing_sources_and_listeners(tmp_path):
     assert rc == 1 and "No sound sources defined" in out
 
 
+MESH_SRC = [[(4.0, 3.0, 1.0), (5.0, 3.0, 1.0), (4.0, 4.0, 1.2)], [(5.0, 3.0, 1.0), (5.0, 4.0, 1.2), (4.0, 4.0, 1.2)]]
+
+
+def test_mesh_source_is_read(tmp_path):
+    s = scenes.shoebox()
+    p = tmp_path / "ms.EAR"
+    earfile.scene_to_ear(s, p, src_mesh=(0, MESH_SRC))
+    d, _ = info(p)
+    assert d["sources"] == 1 and d["triangles"] == 12  # the emitter is not part of the room
+    w = earfile.EarWriter()
+    w.material("m", (0.5, 0.5, 0.5))
+    w.settings()
+    w.source("x.wav", mesh=("nope", MESH_SRC))
+    p2 = tmp_path / "bad.EAR"
+    w.save(p2)
+    rc, out = run("info", str(p2))
+    assert rc == 1 and "before its MAT block" in out
+
+
 def test_missing_sound_file(tmp_path):
     # the source's WAV file is read while parsing (src/SoundFile.cpp:33-42),
     # before any device work
@@ -137,10 +156,11 @@ def write_impulse_wav(path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("radius", [None, 0.5])
-def test_calc_t60_matches_python_front_end(tmp_path, radius):
+@pytest.mark.parametrize("radius,mesh", [(None, False), (0.5, False), (None, True)])
+def test_calc_t60_matches_python_front_end(tmp_path, radius, mesh):
     """`EAR calc T60` against the same render through the C ABI; with SET
-    radius the receiver-sphere estimator (no direct-sound ramp)."""
+    radius the receiver-sphere estimator (no direct-sound ramp); with an
+    emitting mesh as the source."""
     from ear_amd import acoustics, api
 
     s = scenes.shoebox(spec=0.2)
@@ -148,7 +168,7 @@ def test_calc_t60_matches_python_front_end(tmp_path, radius):
     write_impulse_wav(tmp_path / "dry.wav")
     extra = {} if radius is None else {"radius": float(radius)}
     earfile.scene_to_ear(s, p, samples=200_000, seed=1234, maxbounces=200, irlength=2.0,
-                         wav=str(tmp_path / "dry.wav"), **extra)
+                         wav=str(tmp_path / "dry.wav"), src_mesh=(0, MESH_SRC) if mesh else None, **extra)
     rc, out = run("calc", "T60", str(p))
     assert rc == 0, out
     vals = {l.split(":")[0].strip(): float(l.split(":")[1].strip().rstrip("s")) for l in out.splitlines()
@@ -159,7 +179,7 @@ def test_calc_t60_matches_python_front_end(tmp_path, radius):
         air = (0.99, 0.98, 0.95)
         r = api.render(ctx, sc, recorders=recs, n_rays=20000, bands=(1,), air=air, src_pos=s["src"],
                        seed=1234, max_bounces=200, cutoff=1e-8, n_bins=88200, abs_hist=False,
-                       estimator=0 if radius is None else 1)
+                       estimator=0 if radius is None else 1, src_tris=MESH_SRC if mesh else None)
     track = acoustics.power(r.value[1, 0].astype(np.float32))
     thr = np.abs(track).max() / 256.0
     nz = np.nonzero(np.abs(track) >= thr)[0]
