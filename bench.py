@@ -85,8 +85,8 @@ def main():
         copy_probe(a.copy_probe)
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this tracer runs on one GPU; --gpus must be 1 (sharding the ray ranges "
+                                   "over GPUs, with one integer all-reduce of the histograms, is not built)"}))
         sys.exit(2)
     import numpy as np
 
