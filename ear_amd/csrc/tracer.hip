@@ -847,17 +847,20 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(const InstDev* __restrict
                 V3 lsdir = normalized(dv);
                 float dt = dot(lsdir, nrm);
                 if (dt > 0.0f) {
-                    float factor;
+                    // factor = spec * 1001 * sf^1000 + (1 - spec) * diff; with
+                    // spec == 0 the lobe term is exactly +0 (sf^1000 is finite),
+                    // so its binary64 pow is skipped without changing a bit
+                    float diff, sf;
                     if (bt == kReflect) {
                         V3 rv = reflect(prev, nrm);
-                        float diff = -dot(nrm, prev);
-                        float sf = fmaxf(0.0f, dot(rv, lsdir));
-                        factor = spec * kExpInt * powf_ear(sf, kExp) + (1.0f - spec) * diff;
+                        diff = -dot(nrm, prev);
+                        sf = fmaxf(0.0f, dot(rv, lsdir));
                     } else {
-                        float diff = dot(nrm, prev);
-                        float sf = fmaxf(0.0f, dot(prev, lsdir));
-                        factor = spec * kExpInt * powf_ear(sf, kExp) + (1.0f - spec) * diff;
+                        diff = dot(nrm, prev);
+                        sf = fmaxf(0.0f, dot(prev, lsdir));
                     }
+                    const float lobe = spec != 0.0f ? spec * kExpInt * powf_ear(sf, kExp) : 0.0f;
+                    const float factor = lobe + (1.0f - spec) * diff;
                     c = I;
                     c *= factor;
                     float l = length(dv);

@@ -42,8 +42,10 @@ def shoebox():
     return scenes.shoebox()
 
 
-def test_shoebox_parity_bitexact_paths(gpu_ctx, oracle, shoebox):
-    s = shoebox
+@pytest.mark.parametrize("spec", [0.5, None])
+def test_shoebox_parity_bitexact_paths(gpu_ctx, oracle, spec):
+    # spec None: materials without specularity (the lobe term is exactly zero)
+    s = scenes.shoebox(spec=spec)
     n_rays, tr, ts = 3000, 256, 64
     kw = dict(max_bounces=50, cutoff=1e-6, n_bins=88200)
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
