@@ -86,6 +86,7 @@ struct RecParams {
     // (mono: one; stereo: left and right channel)
     float r2, inv_vol;
     float cen[2][3];
+    double l2_head_abs[3];  // pow_base_l2(head_abs[b])
 };
 
 struct RenderParams {
@@ -93,6 +94,7 @@ struct RenderParams {
     uint32_t instance;
     uint32_t ctx_word[3];  // Philox counter word 2 per band
     float air[3];
+    double l2_air[3];  // pow_base_l2(air[b])
     float dry;
     double cutoff;
     uint32_t max_bounces;

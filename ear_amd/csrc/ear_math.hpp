@@ -217,6 +217,28 @@ EAR_HD float powf_ear(float xf, float yf) {
     return (float)exp2_d(y * log2_pos(x));
 }
 
+// powf_ear(xf, yf) for a base whose log2_pos((double)xf) is known (l2x):
+// the same special cases and the same binary64 steps, without the logarithm
+// (air absorption and head shadow raise a per-band constant to many powers).
+EAR_HD float powf_ear_l2(float xf, double l2x, float yf) {
+    double x = (double)xf, y = (double)yf;
+    if (x != x || y != y) return __builtin_bit_cast(float, 0x7FC00000u);
+    if (y == 0.0 || x == 1.0) return 1.0f;
+    if (x == 0.0) return y > 0.0 ? 0.0f : __builtin_bit_cast(float, 0x7F800000u);
+    if (x < 0.0) return __builtin_bit_cast(float, 0x7FC00000u);
+    const double inf = __builtin_bit_cast(double, 0x7FF0000000000000ull);
+    if (x == inf) return y > 0.0 ? __builtin_bit_cast(float, 0x7F800000u) : 0.0f;
+    if (y == inf || y == -inf) return ((x < 1.0) == (y > 0.0)) ? 0.0f : __builtin_bit_cast(float, 0x7F800000u);
+    return (float)exp2_d(y * l2x);
+}
+
+// log2_pos of a base for powf_ear_l2 (0 where powf_ear_l2 does not use it)
+EAR_HD double pow_base_l2(float xf) {
+    const double x = (double)xf;
+    const double inf = __builtin_bit_cast(double, 0x7FF0000000000000ull);
+    return (x > 0.0 && x < inf && x != 1.0) ? log2_pos(x) : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Moeller-Trumbore, double sided (gmtl::intersectDoubleSided), on a triangle
 // stored as (v0, e1 = v1 - v0, e2 = v2 - v0).

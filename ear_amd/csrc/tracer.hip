@@ -529,7 +529,7 @@ __device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, floa
         float width = sqrtf(dist);
         float ampl = 2.0f * a / width;
         float al = ampl, ar = ampl;
-        float factor = powf_ear(R.head_abs[band], fabsf(dt) * R.head_size);
+        float factor = powf_ear_l2(R.head_abs[band], R.l2_head_abs[band], fabsf(dt) * R.head_size);
         if (dt < 0.0f)
             ar *= factor * factor;
         else
@@ -561,7 +561,7 @@ __device__ uint32_t sphere_deposits(const Hist& H, const RenderParams& P, int ba
             const float s0 = fmaxf(-b - q, 0.0f), s1 = fminf(-b + q, seg);
             if (!(s1 > s0)) continue;
             float c = I0;
-            c *= powf_ear(P.air[band], s0);
+            c *= powf_ear_l2(P.air[band], P.l2_air[band], s0);
             c *= s1 - s0;
             c *= R.inv_vol;
             if (j == 0 && P.src_kind == EAR_SRC_POINT) c *= P.dry;
@@ -712,7 +712,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__
                     V3 dv = sub(V3{R.px, R.py, R.pz}, o);
                     float l = length(dv);
                     c = 1.0f;
-                    c *= powf_ear(inst[id].P.air[band], l);
+                    c *= powf_ear_l2(inst[id].P.air[band], inst[id].P.l2_air[band], l);
                     c *= 2.0f / (4.0f * kPi * l * l);
                     emit = is_normal(c);
                 }
@@ -822,7 +822,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(const InstDev* __restrict
                 if (bt == kRefract) nrm = neg(nrm);
                 float l = length(sub(p, o));
                 I = ro.w;
-                I *= powf_ear(P.air[band], l);
+                I *= powf_ear_l2(P.air[band], P.l2_air[band], l);
                 L = rd.w + l;
                 I *= mb.z;
                 alive = is_normal(I);
@@ -864,7 +864,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(const InstDev* __restrict
                     c = I;
                     c *= factor;
                     float l = length(dv);
-                    c *= powf_ear(P.air[band], l);
+                    c *= powf_ear_l2(P.air[band], P.l2_air[band], l);
                     c *= 2.0f / (4.0f * kPi * l * l);
                     if (is_normal(c)) {
                         emit = true;
@@ -1131,7 +1131,7 @@ __global__ void k_direct(const InstDev* __restrict__ inst, int* gstack, uint32_t
     V3 dist = sub(rp, sp);
     float len = length(dist);
     V3 dir = normalized(dist);
-    float a = 1.0f / (4.0f * kPi * len * len) * powf_ear(P.air[band], len) * P.dry;
+    float a = 1.0f / (4.0f * kPi * len * len) * powf_ear_l2(P.air[band], P.l2_air[band], len) * P.dry;
     float tt = len / 343.0f;
     float width = sqrtf(len);
     float ampl = 2.0f * a / width;
@@ -1142,7 +1142,7 @@ __global__ void k_direct(const InstDev* __restrict__ inst, int* gstack, uint32_t
         float dt = dot(dir, V3{R.rx, R.ry, R.rz});
         float td = R.head_size / 343.0f;
         float al = ampl, ar = ampl;
-        float factor = powf_ear(R.head_abs[band], fabsf(dt) * R.head_size);
+        float factor = powf_ear_l2(R.head_abs[band], R.l2_head_abs[band], fabsf(dt) * R.head_size);
         if (dt < 0.0f)
             ar *= factor * factor;
         else
@@ -1387,6 +1387,7 @@ ear_status make_params(Ctx* c, const ear_render_desc* D, RenderParams* P) {
     for (int b = 0; b < 3; ++b) {
         P->ctx_word[b] = (uint32_t)b | ((D->sound & 0xFFu) << 8) | ((uint32_t)(D->keyframe + 1) << 16);
         P->air[b] = D->air[b];
+        P->l2_air[b] = pow_base_l2(D->air[b]);
     }
     P->dry = D->dry;
     P->cutoff = D->energy_cutoff;
@@ -1411,7 +1412,10 @@ ear_status make_params(Ctx* c, const ear_render_desc* D, RenderParams* P) {
         Q.ry = R.right_ear[1];
         Q.rz = R.right_ear[2];
         Q.head_size = R.head_size;
-        for (int b = 0; b < 3; ++b) Q.head_abs[b] = R.head_abs[b];
+        for (int b = 0; b < 3; ++b) {
+            Q.head_abs[b] = R.head_abs[b];
+            Q.l2_head_abs[b] = pow_base_l2(R.head_abs[b]);
+        }
         Q.kind = R.kind;
         Q.channel = (int32_t)nch;
         nch += R.kind == EAR_REC_STEREO ? 2 : 1;
@@ -2079,7 +2083,7 @@ __device__ double contribution_adjoint(const RenderParams& P, const AdjArgs& A, 
         float td = R.head_size / 343.0f;
         int s_right = (int)((double)(t - dt * td) * 44100.0);
         int s_left = (int)((double)(t + dt * td) * 44100.0);
-        float factor = powf_ear(R.head_abs[band], fabsf(dt) * R.head_size);
+        float factor = powf_ear_l2(R.head_abs[band], R.l2_head_abs[band], fabsf(dt) * R.head_size);
         double gl = 1.0, gr = 1.0;
         if (dt < 0.0f)
             gr = (double)(factor * factor);
@@ -2163,7 +2167,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
                 V3 dist3 = sub(p, o);
                 float l = length(dist3);
                 float I = ro.w;
-                I *= powf_ear(P.air[band], l);
+                I *= powf_ear_l2(P.air[band], P.l2_air[band], l);
                 float L = rd.w + l;
                 I *= mb.z;
                 V3 prev = normalized(d);
@@ -2222,7 +2226,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
                         float c = I;
                         c *= factor;
                         float ls = length(dv);
-                        float geo_air = powf_ear(P.air[band], ls);
+                        float geo_air = powf_ear_l2(P.air[band], P.l2_air[band], ls);
                         float geo_inv = 2.0f / (4.0f * kPi * ls * ls);
                         c *= geo_air;
                         c *= geo_inv;
@@ -2424,7 +2428,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
                 if (bt == kRefract) nrm = neg(nrm);
                 const float l = length(sub(p, o));
                 I = ro.w;
-                I *= powf_ear(P.air[band], l);
+                I *= powf_ear_l2(P.air[band], P.l2_air[band], l);
                 L = rd.w + l;
                 I *= mb.z;
                 prev = normalized(d);
@@ -2470,7 +2474,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
                     float c = I;
                     c *= factor;
                     const float ls = length(dv);
-                    const float geo_air = powf_ear(P.air[band], ls);
+                    const float geo_air = powf_ear_l2(P.air[band], P.l2_air[band], ls);
                     const float geo_inv = 2.0f / (4.0f * kPi * ls * ls);
                     c *= geo_air;
                     c *= geo_inv;
