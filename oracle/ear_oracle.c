@@ -24,6 +24,11 @@
  *   Sample_Sphere/Sample_Hemi src/Distributions.h:48-75
  *   MonoRecorder::Record      src/MonoRecorder.cpp:83-97
  *   StereoRecorder::Record    src/StereoRecorder.cpp:97-130
+ *   Equalizer::Pass / Split   lib/equalizer/Equalizer.cpp:27-95 -> or_lr4_coefficients,
+ *                                                     or_equalizer_split
+ * and, with no reference counterpart, the receiver-sphere estimator
+ * (sphere_deposits) and the FP64 gradient of a loss on the histograms
+ * (or_gradient).
  *   gmtl 0.6.1 (not vendored): intersectDoubleSided = Moeller-Trumbore 1997
  *   non-culling test with |det| < 1e-5 rejected; reflect(v,n) = v - 2(v.n)n;
  *   normalize(v) = v / sqrt(v.v); unitRandom -> Philox4x32-10 (Salmon et al.,
