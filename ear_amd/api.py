@@ -171,7 +171,7 @@ def n_channels(recs):
 
 
 def raw_to_float(raw: np.ndarray) -> np.ndarray:
-    """{lo, hi} int64 pairs (128-bit two's complement, 2^-88 units) -> float64."""
+    """{lo, hi} int64 pairs (128-bit two's complement, 2^-EAR_FRAC_BITS units) -> float64."""
     lo = raw[..., 0].view(np.uint64).astype(np.float64)
     hi = raw[..., 1].astype(np.float64)
     return (hi * 18446744073709551616.0 + lo) * FRAC_SCALE

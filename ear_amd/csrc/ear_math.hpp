@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ear.h"
+
 namespace ear {
 
 #define EAR_HD __host__ __device__ __forceinline__
@@ -291,7 +293,7 @@ EAR_HD V3 blend_spec(V3 v, V3 refl, float f) {
 EAR_HD V3 sample_hemi_spec(Rng& r, V3 n, V3 refl, float f) { return blend_spec(sample_hemi(r, n), refl, f); }
 
 // ---------------------------------------------------------------------------
-// 128-bit two's-complement fixed point (value * 2^88) for the histograms.
+// 128-bit two's-complement fixed point (value * 2^EAR_FRAC_BITS) for the histograms.
 // ---------------------------------------------------------------------------
 struct I128 {
     uint64_t lo;
@@ -319,11 +321,12 @@ EAR_HD I128 i128_mul_small(I128 a, uint32_t k) {
     r.hi = (int64_t)((uint64_t)a.hi * k + (p1 >> 32));
     return r;
 }
-// round-to-nearest-even of x * 2^88 as a 128-bit integer (|x| < 2^38)
+// round-to-nearest-even of x * 2^EAR_FRAC_BITS as a 128-bit integer (|x| < 2^46)
 EAR_HD I128 fixed_from_double(double x) {
     I128 r{0, 0};
     if (x == 0.0) return r;
-    double y = x * 309485009821345068724781056.0;  // 2^88, exact
+    static_assert(EAR_FRAC_BITS == 80, "the scale below is 2^80");
+    double y = x * 0x1p80;  // exact
     double a = y < 0 ? -y : y;
     // round to integer (ties to even) for a < 2^52; larger values are integers
     if (a < 4503599627370496.0) {

@@ -16,7 +16,7 @@
 //               and, when visible, Recorder::Record as a fixed-point ramp splat
 //   k_finalize  exact prefix sums of the second-difference histograms,
 //               1/N, direct sound (src/Scene.cpp:286-316), gain^2
-// Histograms are 128-bit two's-complement fixed point (2^-88 units).  A ramp of
+// Histograms are 128-bit two's-complement fixed point (2^-80 units).  A ramp of
 // width w and slope S (MonoRecorder::Record, src/MonoRecorder.cpp:83-97) is
 // stored as three second-difference points (+wS at s, -(w+1)S at s+1, +S at
 // s+w+1); integer sums are order independent, so results are bitwise
@@ -1153,7 +1153,7 @@ __global__ void k_direct(const InstDev* __restrict__ inst, int* gstack, uint32_t
 }
 
 // Two inclusive prefix sums over a second-difference row (128-bit, exact),
-// then value = (raw 2^-88 * inv_n + direct ramp) * gain2.  One block per row;
+// then value = (raw 2^-80 * inv_n + direct ramp) * gain2.  One block per row;
 // row r of the table is {instance, band, channel, kind}.
 constexpr int kFinBlock = 1024;
 __global__ void __launch_bounds__(kFinBlock) k_finalize(const InstDev* __restrict__ inst, const uint4* __restrict__ rows) {
@@ -1201,7 +1201,7 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(const InstDev* __restric
 }
 
 // Per (instance, band, channel) and bin: the sign-split rows (if any) become
-// the signed and |.| sums, then the finished track: 2^-88, 1/N, the direct
+// the signed and |.| sums, then the finished track: 2^-80, 1/N, the direct
 // sound (src/Scene.cpp:299-311) and gain^2 (src/Scene.cpp:313-316).
 __global__ void __launch_bounds__(256) k_values(const InstDev* __restrict__ inst, const uint4* __restrict__ rows) {
     const uint4 rw = rows[blockIdx.y];
@@ -1230,7 +1230,7 @@ __global__ void __launch_bounds__(256) k_values(const InstDev* __restrict__ inst
     const double S = dr.visible ? (double)dr.a / (double)dr.w : 0.0;
     for (uint32_t kind = 0; kind < kinds; ++kind) {
         double x = (double)v[kind].hi * 18446744073709551616.0 + (double)v[kind].lo;
-        x = x * 3.2311742677852644e-27;  // 2^-88
+        x = x * 0x1p-80;  // 2^-EAR_FRAC_BITS
         x = x * In.inv_n;
         if (dr.visible && (int)j >= dr.s && (int)j < dr.s + dr.w) {
             const double dv = (double)(dr.w - ((int)j - dr.s)) * S;

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EAR_ABI_VERSION 3
+#define EAR_ABI_VERSION 4
 
 typedef enum {
     EAR_OK = 0,
@@ -208,7 +208,7 @@ typedef struct {
  *              Scene::Render (src/Scene.cpp:286-316).
  *   abs_raw / abs_value: the same for |ramp| (EAR_FLAG_ABS_HIST).
  * Any pointer may be NULL to skip that output. */
-#define EAR_FRAC_BITS 88
+#define EAR_FRAC_BITS 80
 typedef struct {
     int64_t* raw;
     double* value;

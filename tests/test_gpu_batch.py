@@ -71,7 +71,7 @@ def test_config1_full_size_against_fp64_oracle(gpu_ctx, oracle):
                           n_threads=8, **kw)
         seg += o["segments"]
         gs = api.raw_to_float(g.raw[b])
-        tol = 1e-5 * o["hist_abs"] + 100_000 * 51 * 2.0 ** -88
+        tol = 1e-5 * o["hist_abs"] + 100_000 * 51 * api.FRAC_SCALE
         assert np.all(np.abs(gs - o["hist_signed"]) <= tol)
         assert np.all(gs[o["hist_abs"] == 0] == 0)
     assert g.stats["segments"] == seg

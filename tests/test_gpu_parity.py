@@ -3,18 +3,19 @@
 Parity bar (all on the same Philox stream):
   * per-ray hit-triangle sequence and segment count: bit-exact
   * segment / contribution counts: exact
-  * histograms: |H_gpu - H_fp64| <= 1e-5 * sum|c| per bin (+ the 2^-88
+  * histograms: |H_gpu - H_fp64| <= 1e-5 * sum|c| per bin (+ the 2^-80
     fixed-point quantum per splat), and bins with no contribution exactly 0
 """
 import numpy as np
 import pytest
 
+from ear_amd import _ffi as F
 from ear_amd import api, scenes
 
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-5
-QUANTUM = 2.0 ** -88
+QUANTUM = 2.0 ** -F.EAR_FRAC_BITS
 
 
 def oracle_bands(O, s, bands, n_rays, **kw):

@@ -13,7 +13,7 @@ from ear_amd import _ffi as F
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-5
-QUANTUM = 2.0 ** -88
+QUANTUM = 2.0 ** -F.EAR_FRAC_BITS
 
 
 def with_radius(recs, r):
