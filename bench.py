@@ -48,9 +48,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-instrument", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the batch (256 rooms) and forward + adjoint legs")
-    ap.add_argument("--legs", default="main,batch,adjoint",
-                    help="comma list of legs to run: main (the hall), batch, adjoint (for profiling one leg)")
+                    help="skip the batch (256 rooms), forward + adjoint and animated-hall legs")
+    ap.add_argument("--legs", default="main,batch,adjoint,animated",
+                    help="comma list of legs to run: main (the hall), batch, adjoint, animated (for profiling one leg)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (PMC calibration) and exit")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_extend_hall200k.json"))
@@ -100,6 +100,8 @@ def main():
             extras["batch"] = bench_batch(ctx)
         if "adjoint" in legs:
             extras["adjoint"] = bench_adjoint(ctx)
+        if "animated" in legs:
+            extras["animated"] = bench_animated(ctx)
         ctx.close()
         print(json.dumps({"legs": sorted(legs), "extras": extras}), flush=True)
         return
@@ -165,12 +167,14 @@ def main():
                 "queries_per_launch": round(ext_q / max(1, ext_launch), 1)}
 
     extras = None
-    if not a.no_extras and legs & {"batch", "adjoint"}:
+    if not a.no_extras and legs & {"batch", "adjoint", "animated"}:
         extras = {}
         if "batch" in legs:
             extras["batch"] = bench_batch(ctx)
         if "adjoint" in legs:
             extras["adjoint"] = bench_adjoint(ctx)
+        if "animated" in legs:
+            extras["animated"] = bench_animated(ctx)
 
     cpu = None
     if not a.no_cpu:
@@ -317,6 +321,41 @@ def bench_adjoint(ctx, n_rooms=32):
                     "forward_adjoint_segments_per_s": round(bsegs / (t2 - t0), 1),
                     "forward_segments_per_s": round(bsegs / (t1 - t0), 1)}
     return out
+
+
+def bench_animated(ctx, n_keys=64, rays_total=50_000_000, n_tris=500_000):
+    """Animated source: the ~500k-triangle hall with the source moving over the
+    stage along n_keys keyframes (one render each, src/Animated.h:89-91), a
+    stereo recorder, rays_total rays per band over all keyframes, 3 bands,
+    100 segments, 3 s histograms; all keyframes in one ear_render_batch over
+    the shared scene, beside one ear_render per keyframe."""
+    from ear_amd import api, scenes
+
+    t0 = time.time()
+    h = scenes.hall(target_tris=n_tris)
+    path = scenes.stage_path(n_keys)
+    rays = rays_total // n_keys
+    log(f"animated: {h['name']} generated in {time.time() - t0:.1f}s")
+    sc = api.Scene(ctx, h["tris"], h["mat"], h["materials"])
+    info = sc.info()
+    items = [(sc, dict(recorders=h["recorders"], src_pos=path[k], air=h["air"], n_rays=rays, max_bounces=100,
+                       cutoff=1e-8, n_bins=132_300, keyframe=k, abs_hist=False, want_raw=False))
+             for k in range(n_keys)]
+    api.render_batch(ctx, items[:2])  # warm-up (first-touch allocations)
+    _, st, wall = api.render_batch(ctx, items)
+    log(f"animated: {st['segments']} segments in {wall:.3f}s ({st['segments'] / wall:.3e} segments/s)")
+    t0 = time.perf_counter()
+    segs1 = 0
+    for _, kw in items:
+        segs1 += api.render(ctx, sc, **kw).stats["segments"]
+    single = time.perf_counter() - t0
+    log(f"animated: one call per keyframe {single:.3f}s ({segs1 / single:.3e} segments/s)")
+    sc.close()
+    return {"workload": f"hall {info['n_tris']} tris, source on a {n_keys}-keyframe path, stereo, 3 bands x "
+                        f"{rays} rays per keyframe ({n_keys * rays} per band), <= 100 segments, 3 s histograms",
+            "segments": int(st["segments"]), "bvh_build_ms": round(info["build_ms"], 1), "batch_s": round(wall, 4),
+            "batch_segments_per_s": round(st["segments"] / wall, 1), "single_calls_s": round(single, 4),
+            "single_calls_segments_per_s": round(segs1 / single, 1), "speedup": round(single / wall, 3)}
 
 
 def cpu_baseline(s, kw, target_s):

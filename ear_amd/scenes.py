@@ -167,6 +167,18 @@ def hall(*, seed=MASTER_SEED, target_tris=200_000, W=40.0, D=28.0, H=14.0, amp=1
                 recorders=[rec], air=EXPORTER_AIR)
 
 
+def stage_path(n_keys=64, *, W=40.0, D=28.0, z=1.5):
+    """Configuration 5's moving source: n_keys discrete positions on a closed
+    loop over the hall's stage (x < 0.3 W, clear of the seat blocks).  The
+    reference has no spline: keyframe i is position frames[i]
+    (src/Animated.h:89-91), as the exporter samples a curve once per frame."""
+    th = 2.0 * math.pi * np.arange(n_keys) / n_keys
+    x = 0.12 * W + 0.0625 * W * np.cos(th)
+    y = 0.5 * D + 0.32 * D * np.sin(th)
+    zz = z + 0.3 * np.sin(3.0 * th)
+    return [(float(a), float(b), float(c)) for a, b, c in zip(x, y, zz)]
+
+
 def room(*, seed, target_tris=12, kind=None, alpha_range=(0.02, 0.6)):
     """Configuration 3 instance: a box or L-shaped room whose walls are
     tessellated to reach ~target_tris triangles; seeded source/recorder."""

@@ -75,3 +75,26 @@ def test_config1_full_size_against_fp64_oracle(gpu_ctx, oracle):
         assert np.all(np.abs(gs - o["hist_signed"]) <= tol)
         assert np.all(gs[o["hist_abs"] == 0] == 0)
     assert g.stats["segments"] == seg
+
+
+def test_keyframes_in_one_batch_share_a_scene(gpu_ctx):
+    """A moving source: one item per keyframe, all on one scene.  Each item
+    equals its own ear_render, and the keyframe index alone selects the random
+    stream: two keyframes at the same position give different paths."""
+    h = scenes.hall(target_tris=20_000)
+    path = scenes.stage_path(6)
+    path[5] = path[4]
+    with api.Scene(gpu_ctx, h["tris"], h["mat"], h["materials"]) as sc:
+        items = [(sc, _kw(h, n_rays=3000, keyframe=k, src_pos=path[k], n_bins=132_300)) for k in range(6)]
+        res, stats, _ = api.render_batch(gpu_ctx, items)
+        seg = 0
+        for (_, kw), rb in zip(items, res):
+            rs = api.render(gpu_ctx, sc, **kw)
+            seg += rs.stats["segments"]
+            np.testing.assert_array_equal(rb.raw, rs.raw)
+            np.testing.assert_array_equal(rb.abs_raw, rs.abs_raw)
+        assert stats["segments"] == seg
+        assert not np.array_equal(res[4].raw, res[5].raw)
+        # same position, same expected response: the totals agree statistically
+        e4, e5 = res[4].abs_value.sum(), res[5].abs_value.sum()
+        assert abs(e4 / e5 - 1.0) < 0.1

@@ -265,3 +265,24 @@ def test_hall_full_size_is_launch_shape_independent(gpu_ctx):
             gpu_ctx.set_tuning()
     assert a.stats["segments"] == b.stats["segments"] > 9e8
     assert np.array_equal(a.raw, b.raw) and np.array_equal(a.abs_raw, b.abs_raw)
+
+
+def test_keyframe_and_sound_select_the_stream(gpu_ctx, oracle):
+    """A (sound, keyframe) context's paths against the oracle fed the same
+    Philox context word, band | sound << 8 | (keyframe + 1) << 16: the hit
+    sequences are bit-exact, so the device forms the word as the oracle does."""
+    s = scenes.shoebox(stereo=True)
+    n_rays, tr, ts = 1500, 128, 48
+    kw = dict(max_bounces=50, cutoff=1e-6, n_bins=88200)
+    sound, kf = 2, 5
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
+        g = api.render(gpu_ctx, sc, recorders=s["recorders"], n_rays=n_rays, src_pos=s["src"], air=s["air"],
+                       trace_rays=tr, trace_segs=ts, sound=sound, keyframe=kf, **kw)
+    osc = oracle.OracleScene(s["tris"], s["mat"], s["materials"])
+    for b in (0, 1, 2):
+        o = oracle.render(osc, band=b, air=s["air"][b], recorders=s["recorders"], n_rays=n_rays, src_pos=s["src"],
+                          ctx=b | sound << 8 | (kf + 1) << 16, n_threads=8, trace_rays=tr, trace_segs=ts, **kw)
+        np.testing.assert_array_equal(g.trace_hits[b], o["trace_hits"])
+        np.testing.assert_array_equal(g.trace_nseg[b], o["trace_nseg"])
+        assert_hist_parity(api.raw_to_float(g.raw[b]), api.raw_to_float(g.abs_raw[b]), o["hist_signed"],
+                           o["hist_abs"], n_rays * 51)
