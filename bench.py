@@ -15,8 +15,9 @@ Prints one JSON line (rank 0, N=1).  `roofline` is for the closest-hit kernel
     24 (ray origin + direction) + 8 (hit record) + B * V + 36 * T
 with V / T the mean BVH node visits / triangle tests per query counted by an
 instrumented pass over the same rays, and B the bytes of one node that a visit
-reads (112 for the 4-wide BVH: four child boxes and four child references; 56
-for the binary one: two child boxes and two child references).  `cpu_baseline` is the CPU oracle (a
+reads (64 for the 4-wide BVH: origin and scales, four child boxes quantised
+to 8 bits per plane, four child references; 56 for the binary one: two fp32
+child boxes and two child references).  `cpu_baseline` is the CPU oracle (a
 brute-force restatement of the reference algorithm) timed on this host on the
 first K rays of the same Philox streams.
 """
@@ -48,9 +49,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-instrument", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the batch (256 rooms), forward + adjoint and animated-hall legs")
-    ap.add_argument("--legs", default="main,batch,adjoint,animated",
-                    help="comma list of legs to run: main (the hall), batch, adjoint, animated (for profiling one leg)")
+                    help="skip the batch (256 rooms), forward + adjoint, animated-hall and fitting legs")
+    ap.add_argument("--legs", default="main,batch,adjoint,animated,fit",
+                    help="comma list of legs to run: main (the hall), batch, adjoint, animated, fit "
+                         "(for profiling one leg)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (PMC calibration) and exit")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_extend_hall200k.json"))
@@ -102,6 +104,8 @@ def main():
             extras["adjoint"] = bench_adjoint(ctx)
         if "animated" in legs:
             extras["animated"] = bench_animated(ctx)
+        if "fit" in legs:
+            extras["fit"] = bench_fit(ctx)
         ctx.close()
         print(json.dumps({"legs": sorted(legs), "extras": extras}), flush=True)
         return
@@ -167,7 +171,7 @@ def main():
                 "queries_per_launch": round(ext_q / max(1, ext_launch), 1)}
 
     extras = None
-    if not a.no_extras and legs & {"batch", "adjoint", "animated"}:
+    if not a.no_extras and legs & {"batch", "adjoint", "animated", "fit"}:
         extras = {}
         if "batch" in legs:
             extras["batch"] = bench_batch(ctx)
@@ -175,6 +179,8 @@ def main():
             extras["adjoint"] = bench_adjoint(ctx)
         if "animated" in legs:
             extras["animated"] = bench_animated(ctx)
+        if "fit" in legs:
+            extras["fit"] = bench_fit(ctx)
 
     cpu = None
     if not a.no_cpu:
@@ -356,6 +362,77 @@ def bench_animated(ctx, n_keys=64, rays_total=50_000_000, n_tris=500_000):
             "segments": int(st["segments"]), "bvh_build_ms": round(info["build_ms"], 1), "batch_s": round(wall, 4),
             "batch_segments_per_s": round(st["segments"] / wall, 1), "single_calls_s": round(single, 4),
             "single_calls_segments_per_s": round(segs1 / single, 1), "speedup": round(single / wall, 3)}
+
+
+def bench_fit(ctx, steps=200, n_rooms=32, room_steps=10):
+    """Inverse fitting by the adjoint.  The 12 x 9 x 5 m shoebox: 18 absorption
+    + 6 specularity parameters started +-20% off, 1M rays, `steps` Adam steps
+    on the 4096-bin log-EDC loss against a target rendered at the true
+    parameters with another seed.  Then n_rooms rooms of 1k..100k triangles
+    (box rooms 6 materials, L rooms 3; absorption per band + specularity)
+    fitted together by batched forward and adjoint renders, 250k rays each."""
+    import numpy as np
+
+    from ear_amd import api, fit, scenes
+
+    s = scenes.shoebox(12.0, 9.0, 5.0, spec=0.3, alpha_range=(0.1, 0.3), src=(3.0, 4.0, 2.0), rec=(8.0, 5.0, 1.7))
+    n_mat = len(s["materials"])
+    alpha_t, spec_t = s["alpha"], np.full(n_mat, 0.3)
+    sign = np.random.default_rng(5).choice([-1, 1], size=(n_mat, 4))
+    alpha0, spec0 = alpha_t * (1 + 0.2 * sign[:, :3]), spec_t * (1 + 0.2 * sign[:, 3])
+    kw = dict(recorders=s["recorders"], src_pos=s["src"], air=s["air"], n_rays=1_000_000, max_bounces=999,
+              cutoff=1e-8, n_bins=4096 * 11, abs_hist=True, want_raw=False)
+    with api.Scene(ctx, s["tris"], s["mat"], fit.materials(alpha_t, spec_t)) as sc:
+        target = api.render(ctx, sc, **dict(kw, seed=0x5EED)).abs_value
+    t0 = time.perf_counter()
+    st = fit.fit_room(ctx, s, target, alpha0, spec0, steps=steps, n_rays=1_000_000, max_bounces=999, cutoff=1e-8)
+    wall = time.perf_counter() - t0
+
+    # one receiver's decay fixes the area-weighted absorption of each band, not
+    # the six walls apart: report that quantity's relative error (max over bands)
+    tr = np.asarray(s["tris"], np.float64)
+    area_t = 0.5 * np.linalg.norm(np.cross(tr[:, 1] - tr[:, 0], tr[:, 2] - tr[:, 0]), axis=1)
+    area = np.array([area_t[np.asarray(s["mat"]) == m].sum() for m in range(n_mat)])
+
+    def err(a):
+        return float(np.max(np.abs(area @ a / (area @ alpha_t) - 1.0)))
+
+    out = {"workload": f"shoebox 12x9x5 m, 24 parameters +-20% off, 1M rays, {steps} Adam steps on the 4096-bin "
+                       f"log-EDC loss (target: the true parameters, another seed)",
+           "steps": steps, "wall_s": round(wall, 3), "ms_per_step": round(1e3 * wall / steps, 2),
+           "loss_first": round(st.history[0][0], 6), "loss_last": round(st.history[-1][0], 6),
+           "area_weighted_alpha_rel_error_first": round(err(st.history[0][1]), 5),
+           "area_weighted_alpha_rel_error_last": round(err(st.history[-1][1]), 5)}
+    log(f"fit: {steps} steps in {wall:.2f}s, loss {out['loss_first']} -> {out['loss_last']}, area-weighted alpha "
+        f"error {out['area_weighted_alpha_rel_error_first']} -> {out['area_weighted_alpha_rel_error_last']}")
+
+    rng = np.random.default_rng(17)
+    rooms = [scenes.room(seed=5000 + i, target_tris=int(np.exp(rng.uniform(np.log(1000), np.log(100_000)))))
+             for i in range(n_rooms)]
+    targets, a0s, s0s = [], [], []
+    for i, rm in enumerate(rooms):
+        nm = len(rm["materials"])
+        a_t = np.array([[1.0 - x for x in m[0]] for m in rm["materials"]])
+        sg = np.random.default_rng(100 + i).choice([-1, 1], size=(nm, 4))
+        with api.Scene(ctx, rm["tris"], rm["mat"], fit.materials(a_t, np.full(nm, 0.2))) as sc:
+            targets.append(api.render(ctx, sc, recorders=rm["recorders"], src_pos=rm["src"], air=rm["air"],
+                                      n_rays=250_000, max_bounces=999, cutoff=1e-8, n_bins=4096 * 11,
+                                      want_raw=False, seed=0x5EED).abs_value)
+        a0s.append(np.clip(a_t * (1 + 0.2 * sg[:, :3]), 0.01, 0.95))
+        s0s.append(np.full(nm, 0.2) * (1 + 0.2 * sg[:, 3]))
+    t0 = time.perf_counter()
+    sts = fit.fit_rooms(ctx, rooms, targets, a0s, s0s, steps=room_steps, n_rays=250_000)
+    wall = time.perf_counter() - t0
+    ratio = [x.history[-1][0] / max(x.history[0][0], 1e-30) for x in sts]
+    n_par = int(sum(4 * len(rm["materials"]) for rm in rooms))
+    out["rooms"] = {"workload": f"{n_rooms} box/L rooms, 1k..100k triangles, {n_par} parameters in all (+-20% off), "
+                                f"250k rays each, {room_steps} steps of batched forward + adjoint",
+                    "steps": room_steps, "wall_s": round(wall, 3), "s_per_step": round(wall / room_steps, 3),
+                    "loss_ratio_median": round(float(np.median(ratio)), 4),
+                    "loss_ratio_max": round(float(np.max(ratio)), 4)}
+    log(f"fit rooms: {n_rooms} rooms x {room_steps} steps in {wall:.2f}s, last / first loss: median "
+        f"{out['rooms']['loss_ratio_median']}, max {out['rooms']['loss_ratio_max']}")
+    return out
 
 
 def cpu_baseline(s, kw, target_s):

@@ -169,7 +169,7 @@ ear_status ear_scene_get_info(const ear_scene* s, ear_scene_info* info) {
     for (int k = 0; k < 6; ++k) info->bounds[k] = s->ds.bounds[k];
     info->build_ms = s->ds.build_ms;
     info->width = s->ds.width;
-    info->node_bytes = s->ds.width == 4 ? 112 : 56;
+    info->node_bytes = s->ds.width == 4 ? 64 : 56;
     info->collapse_ms = s->ds.collapse_ms;
     info->builder = s->ds.builder;
     if (s->ds.width == 4) {

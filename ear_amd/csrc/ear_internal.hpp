@@ -30,13 +30,21 @@ constexpr uint32_t kMaxTris = 1u << 24;
 // the bound for the tree it produced.
 constexpr int kStackDepth = 96;
 
-// BVH4 node, 128 B: child c's padded box in component c of each float4;
-// ref as in BvhNode; kEmpty4 marks an unused slot (its box lies at 1e30, so
-// the slab test never accepts it).
-struct alignas(128) Bvh4Node {
-    float4 lox, hix, loy, hiy, loz, hiz;
+// BVH4 node, 64 B: the four child boxes quantised to 8 bits per plane
+// against the node's own box, with a power-of-two scale per axis (Ylitie,
+// Karras, Laine, "Efficient incoherent ray traversal on GPUs through
+// compressed wide BVHs", HPG 2017).  A plane decodes as
+// org[axis] + q * 2^(e[axis] - 127): the product is exact, so the value
+// rounds once and host and device get the same float.  The build rounds every
+// lo plane down and every hi plane up, so a decoded box contains the exact
+// one: traversal may visit more nodes, never fewer, and hits are unchanged.
+// Byte c of each q word belongs to child c; ref as in BvhNode; kEmpty4 marks
+// an unused slot.
+struct alignas(64) Bvh4Node {
+    uint4 org;  // x, y, z: origin (float bits); w: biased exponents ex | ey << 8 | ez << 16
+    uint4 qa;   // qlo.x, qhi.x, qlo.y, qhi.y
+    uint4 qb;   // qlo.z, qhi.z, 0, 0
     int4 ref;
-    int4 pad;
 };
 constexpr int32_t kEmpty4 = 0x7FFFFFFF;
 constexpr int kMaxRecorders = 16;

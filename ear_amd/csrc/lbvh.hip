@@ -521,11 +521,74 @@ void free_scene(Ctx* c, DeviceScene* s) {
 
 // TRY: ear_internal.hpp
 
+// A quantised plane as the traversal decodes it: org + q * 2^(eb - 127).
+static float qplane(float org, uint32_t eb, uint32_t q) {
+    const uint32_t bits = eb << 23;
+    float scale;
+    std::memcpy(&scale, &bits, 4);
+    return org + (float)q * scale;
+}
+
+// Quantise the used children's boxes of one 4-wide node (Bvh4Node): per
+// axis the origin is the lowest lo plane and the scale the smallest power of
+// two for which every lo plane has a code at or below it and every hi plane a
+// code at or above it within 0..255.  Returns false if a scale beyond 2^127
+// would be needed (coordinates near the float range).
+static bool quantize_node(const float lo[3][4], const float hi[3][4], const int32_t ref[4], Bvh4Node& N) {
+    uint32_t q[6][4] = {};  // qlo.x, qhi.x, qlo.y, qhi.y, qlo.z, qhi.z
+    uint32_t org_bits[3], eb_all = 0;
+    for (int k = 0; k < 3; ++k) {
+        float mn = INFINITY, mx = -INFINITY;
+        for (int i = 0; i < 4; ++i)
+            if (ref[i] != kEmpty4) {
+                mn = std::min(mn, lo[k][i]);
+                mx = std::max(mx, hi[k][i]);
+            }
+        if (!(mn <= mx)) mn = mx = 0.0f;  // no used child (not produced by the collapse)
+        const double ext = (double)mx - (double)mn;
+        int e2 = ext > 0.0 ? (int)std::ceil(std::log2(ext / 255.0)) - 1 : -126;
+        e2 = std::max(e2, -126);
+        for (;; ++e2) {
+            if (e2 > 127) return false;
+            const uint32_t eb = (uint32_t)(e2 + 127);
+            const double sc = std::ldexp(1.0, e2);
+            bool ok = true;
+            for (int i = 0; i < 4 && ok; ++i) {
+                if (ref[i] == kEmpty4) continue;
+                double fl = std::floor(((double)lo[k][i] - (double)mn) / sc);
+                uint32_t ql = (uint32_t)std::min(255.0, std::max(0.0, fl));
+                while (ql > 0 && qplane(mn, eb, ql) > lo[k][i]) --ql;
+                double ce = std::ceil(((double)hi[k][i] - (double)mn) / sc);
+                uint32_t qh = (uint32_t)std::min(255.0, std::max(0.0, ce));
+                while (qh < 255 && qplane(mn, eb, qh) < hi[k][i]) ++qh;
+                if (qplane(mn, eb, ql) > lo[k][i] || qplane(mn, eb, qh) < hi[k][i]) {
+                    ok = false;
+                    break;
+                }
+                q[2 * k][i] = ql;
+                q[2 * k + 1][i] = qh;
+            }
+            if (ok) {
+                eb_all |= eb << (8 * k);
+                break;
+            }
+        }
+        std::memcpy(&org_bits[k], &mn, 4);
+    }
+    auto word = [&](int r) { return q[r][0] | q[r][1] << 8 | q[r][2] << 16 | q[r][3] << 24; };
+    N.org = make_uint4(org_bits[0], org_bits[1], org_bits[2], eb_all);
+    N.qa = make_uint4(word(0), word(1), word(2), word(3));
+    N.qb = make_uint4(word(4), word(5), 0u, 0u);
+    N.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+    return true;
+}
+
 // Collapse the 2-wide tree into a 4-wide one: each 4-wide node takes a 2-wide
 // node's two children and, while fewer than four, replaces the internal child
-// with the largest box surface area by its two children.  Boxes (already
-// padded) and leaf references carry over unchanged; nodes are laid out in
-// breadth-first order so siblings sit next to each other.
+// with the largest box surface area by its two children.  The (already
+// padded) boxes are quantised outward (quantize_node), leaf references carry
+// over unchanged; nodes are laid out in breadth-first order so siblings sit
+// next to each other.
 static ear_status collapse_to_bvh4(Ctx* c, DeviceScene* s) {
     auto t0 = std::chrono::steady_clock::now();
     std::vector<BvhNode> n2(s->n_nodes);
@@ -580,7 +643,8 @@ static ear_status collapse_to_bvh4(Ctx* c, DeviceScene* s) {
         float lo[3][4], hi[3][4];
         int32_t ref[4];
         for (int i = 0; i < 4; ++i) {
-            if (i < nch) {
+            // the 2-wide tree's placeholder child (a box at 1e30) is no child
+            if (i < nch && ch[i].b[0] < 1e29f) {
                 for (int k = 0; k < 3; ++k) {
                     lo[k][i] = ch[i].b[k];
                     hi[k][i] = ch[i].b[3 + k];
@@ -598,15 +662,10 @@ static ear_status collapse_to_bvh4(Ctx* c, DeviceScene* s) {
                 ref[i] = kEmpty4;
             }
         }
-        Bvh4Node& N = out[w.slot];
-        N.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
-        N.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
-        N.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
-        N.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
-        N.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
-        N.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
-        N.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
-        N.pad = make_int4(0, 0, 0, 0);
+        if (!quantize_node(lo, hi, ref, out[w.slot])) {
+            c->last_error = "scene coordinates too large for the quantised BVH nodes";
+            return EAR_ERR_INVALID_GEOMETRY;
+        }
     }
     if (3 * depth + 1 >= (uint32_t)kStackDepth) {
         c->last_error = "4-wide BVH deeper than the traversal stack";

@@ -193,6 +193,10 @@ __device__ __forceinline__ int4 ldg(const int4* p) {
     const i32x4 v = *(const __attribute__((address_space(1))) i32x4*)p;
     return make_int4(v.x, v.y, v.z, v.w);
 }
+__device__ __forceinline__ uint4 ldg(const uint4* p) {
+    const i32x4 v = *(const __attribute__((address_space(1))) i32x4*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
 
 __device__ __forceinline__ float safe_inv(float x) {
     float y = fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x);
@@ -352,12 +356,22 @@ struct Walker {
         if (node < 0) node = stk.pop();  // not exhausted: the stack holds a node
         if (INSTR) ++nvis;
         const Bvh4Node* N = T.nodes4 + node;
-        float4 lx = ldg(&N->lox), hx = ldg(&N->hix), ly = ldg(&N->loy), hy = ldg(&N->hiy), lz = ldg(&N->loz),
-               hz = ldg(&N->hiz);
+        const uint4 og = ldg(&N->org), qa = ldg(&N->qa), qb = ldg(&N->qb);
         int4 rf = ldg(&N->ref);
-        float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-        float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-        float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+        // decode the quantised child boxes: org + q * 2^(e - 127), exact product
+        const float ox = __uint_as_float(og.x), oy = __uint_as_float(og.y), oz = __uint_as_float(og.z);
+        const float sx = __uint_as_float((og.w & 0xFFu) << 23), sy = __uint_as_float(((og.w >> 8) & 0xFFu) << 23),
+                    sz = __uint_as_float(((og.w >> 16) & 0xFFu) << 23);
+        float lxa[4], hxa[4], lya[4], hya[4], lza[4], hza[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            lxa[k] = ox + (float)((qa.x >> (8 * k)) & 0xFFu) * sx;
+            hxa[k] = ox + (float)((qa.y >> (8 * k)) & 0xFFu) * sx;
+            lya[k] = oy + (float)((qa.z >> (8 * k)) & 0xFFu) * sy;
+            hya[k] = oy + (float)((qa.w >> (8 * k)) & 0xFFu) * sy;
+            lza[k] = oz + (float)((qb.x >> (8 * k)) & 0xFFu) * sz;
+            hza[k] = oz + (float)((qb.y >> (8 * k)) & 0xFFu) * sz;
+        }
         int r[4] = {rf.x, rf.y, rf.z, rf.w};
         float key[4];
         const float lim = ANY ? 1.0f : best_t;
