@@ -358,19 +358,22 @@ struct Walker {
         const Bvh4Node* N = T.nodes4 + node;
         const uint4 og = ldg(&N->org), qa = ldg(&N->qa), qb = ldg(&N->qb);
         int4 rf = ldg(&N->ref);
-        // decode the quantised child boxes: org + q * 2^(e - 127), exact product
+        // decode the quantised child boxes: org + q * 2^(e - 127).  The
+        // product is exact (q < 2^8, the scale a power of two, q * s normal or
+        // 0), so one fused multiply-add rounds to the same float as the
+        // host's multiply then add.
         const float ox = __uint_as_float(og.x), oy = __uint_as_float(og.y), oz = __uint_as_float(og.z);
         const float sx = __uint_as_float((og.w & 0xFFu) << 23), sy = __uint_as_float(((og.w >> 8) & 0xFFu) << 23),
                     sz = __uint_as_float(((og.w >> 16) & 0xFFu) << 23);
         float lxa[4], hxa[4], lya[4], hya[4], lza[4], hza[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            lxa[k] = ox + (float)((qa.x >> (8 * k)) & 0xFFu) * sx;
-            hxa[k] = ox + (float)((qa.y >> (8 * k)) & 0xFFu) * sx;
-            lya[k] = oy + (float)((qa.z >> (8 * k)) & 0xFFu) * sy;
-            hya[k] = oy + (float)((qa.w >> (8 * k)) & 0xFFu) * sy;
-            lza[k] = oz + (float)((qb.x >> (8 * k)) & 0xFFu) * sz;
-            hza[k] = oz + (float)((qb.y >> (8 * k)) & 0xFFu) * sz;
+            lxa[k] = __builtin_fmaf((float)((qa.x >> (8 * k)) & 0xFFu), sx, ox);
+            hxa[k] = __builtin_fmaf((float)((qa.y >> (8 * k)) & 0xFFu), sx, ox);
+            lya[k] = __builtin_fmaf((float)((qa.z >> (8 * k)) & 0xFFu), sy, oy);
+            hya[k] = __builtin_fmaf((float)((qa.w >> (8 * k)) & 0xFFu), sy, oy);
+            lza[k] = __builtin_fmaf((float)((qb.x >> (8 * k)) & 0xFFu), sz, oz);
+            hza[k] = __builtin_fmaf((float)((qb.y >> (8 * k)) & 0xFFu), sz, oz);
         }
         int r[4] = {rf.x, rf.y, rf.z, rf.w};
         float key[4];
