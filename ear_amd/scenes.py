@@ -86,7 +86,7 @@ def absorption_materials(alpha, spec=None):
 
 def shoebox(lx=10.0, ly=8.0, lz=4.0, *, seed=MASTER_SEED, alpha_range=(0.05, 0.4), spec=0.5,
             src=(2.0, 3.0, 1.5), rec=(7.0, 5.0, 1.7), tess=(1, 1, 1), stereo=False):
-    """Configuration 1: 10x8x4 m box, 12 triangles, one material per wall,
+    """The shoebox room: 10x8x4 m box, 12 triangles, one material per wall,
     alpha ~ U(0.05, 0.4) per wall and band, specularity 0.5."""
     rng = np.random.default_rng(seed)
     walls = box_walls(lx, ly, lz, tess=tess)
@@ -104,7 +104,7 @@ def stereo_recorder(pos, right_ear=(0.0, 1.0, 0.0), head_size=0.17, head_abs=EXP
 
 def hall(*, seed=MASTER_SEED, target_tris=200_000, W=40.0, D=28.0, H=14.0, amp=1.5, n_mat=8,
          alpha_range=(0.02, 0.6), spec=None):
-    """Configuration 2: procedural concert hall of ~target_tris triangles.
+    """Procedural concert hall of ~target_tris triangles (200k by default).
 
     Box shell W x D x H; the ceiling is z = H + amp sin(3*2pi x/W) sin(2*2pi y/D)
     (zero at the walls, so the shell is closed), tessellated into a grid;
@@ -168,7 +168,7 @@ def hall(*, seed=MASTER_SEED, target_tris=200_000, W=40.0, D=28.0, H=14.0, amp=1
 
 
 def stage_path(n_keys=64, *, W=40.0, D=28.0, z=1.5):
-    """Configuration 5's moving source: n_keys discrete positions on a closed
+    """A moving source for the hall: n_keys discrete positions on a closed
     loop over the hall's stage (x < 0.3 W, clear of the seat blocks).  The
     reference has no spline: keyframe i is position frames[i]
     (src/Animated.h:89-91), as the exporter samples a curve once per frame."""
@@ -180,7 +180,7 @@ def stage_path(n_keys=64, *, W=40.0, D=28.0, z=1.5):
 
 
 def room(*, seed, target_tris=12, kind=None, alpha_range=(0.02, 0.6)):
-    """Configuration 3 instance: a box or L-shaped room whose walls are
+    """One room of the batching workload: a box or L-shaped room whose walls are
     tessellated to reach ~target_tris triangles; seeded source/recorder."""
     rng = np.random.default_rng(seed)
     if kind is None:
