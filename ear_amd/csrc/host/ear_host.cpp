@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <sstream>
 
 #include "audio.h"
@@ -535,22 +536,33 @@ void Scene::Render(int band, int sound, float absorbtion_factor, int num_samples
     RenderBands(sound, air, num_samples, dry, per, keyframeID, 1u << band);
 }
 
-void Scene::RenderBands(int sound, const float absorbtion_factor[3], int num_samples, float dry,
-                        const std::vector<Recorder*> rec[3], int keyframeID, unsigned band_mask) {
-    ensure_device();
-    AbstractSoundFile* src = sources.at((size_t)sound);
+namespace {
+// The ABI description of one (sound, keyframe) render and the host buffer
+// its finished tracks come back in.  The descriptor points into this object,
+// so it stays where it was built.
+struct PreparedRender {
+    std::vector<ear_recorder> recs;
+    std::vector<float> mesh_v;
+    ear_render_desc d{};
+    std::vector<double> value;
+    ear_hist h{};
+    int nch = 0;
+};
+
+void prepare_render(const Scene& scene, PreparedRender& p, int sound, const float absorbtion_factor[3],
+                    int num_samples, float dry, const std::vector<Recorder*> rec[3], int keyframeID,
+                    unsigned band_mask) {
+    AbstractSoundFile* src = scene.sources.at((size_t)sound);
     // the recorders of every band context describe the same listeners
     const std::vector<Recorder*>* any = nullptr;
     for (int b = 0; b < 3; ++b)
         if ((band_mask >> b) & 1u) any = &rec[b];
     if (!any || any->empty()) throw std::runtime_error("no recorders");
-    std::vector<ear_recorder> recs;
     for (Recorder* r : *any) {
-        recs.push_back(r->to_abi(keyframeID));
-        recs.back().radius = options.receiver_radius;
+        p.recs.push_back(r->to_abi(keyframeID));
+        p.recs.back().radius = scene.options.receiver_radius;
     }
-    std::vector<float> mesh_v;
-    ear_render_desc d{};
+    ear_render_desc& d = p.d;
     d.band_mask = band_mask;
     for (int b = 0; b < 3; ++b) d.air[b] = absorbtion_factor[b];
     d.dry = dry;
@@ -558,39 +570,66 @@ void Scene::RenderBands(int sound, const float absorbtion_factor[3], int num_sam
         d.source.kind = EAR_SRC_MESH;
         for (const Triangle* t : src->mesh->tris)
             for (int v = 0; v < 3; ++v)
-                for (int k = 0; k < 3; ++k) mesh_v.push_back(t->mVerts[v][k]);
-        d.source.mesh_v = mesh_v.data();
+                for (int k = 0; k < 3; ++k) p.mesh_v.push_back(t->mVerts[v][k]);
+        d.source.mesh_v = p.mesh_v.data();
         d.source.mesh_n_tris = (uint32_t)src->mesh->tris.size();
     } else {
         d.source.kind = EAR_SRC_POINT;
-        Vec3 p = src->getLocation(keyframeID);
-        std::memcpy(d.source.pos, p.data(), sizeof(d.source.pos));
+        Vec3 pos = src->getLocation(keyframeID);
+        std::memcpy(d.source.pos, pos.data(), sizeof(d.source.pos));
     }
     d.source.gain = src->getGain();
-    d.recorders = recs.data();
-    d.n_recorders = (uint32_t)recs.size();
+    d.recorders = p.recs.data();
+    d.n_recorders = (uint32_t)p.recs.size();
     d.n_rays = (uint32_t)std::max(0, num_samples);
-    d.max_bounces = options.max_bounces;
-    d.energy_cutoff = options.energy_cutoff;
-    d.seed = options.seed;
+    d.max_bounces = scene.options.max_bounces;
+    d.energy_cutoff = scene.options.energy_cutoff;
+    d.seed = scene.options.seed;
     d.keyframe = keyframeID;
     d.sound = (uint32_t)sound;
-    d.hist_bins = options.hist_bins;
-    d.estimator = options.estimator;
-    int nch = 0;
-    for (const ear_recorder& r : recs) nch += r.kind == EAR_REC_STEREO ? 2 : 1;
-    std::vector<double> value((size_t)3 * nch * d.hist_bins);
-    ear_hist h{};
-    h.value = value.data();
-    throw_on_error(ear_render(ctx_, dev_scene_, &d, &h, &last_stats), ctx_, "ear_render");
+    d.hist_bins = scene.options.hist_bins;
+    d.estimator = scene.options.estimator;
+    for (const ear_recorder& r : p.recs) p.nch += r.kind == EAR_REC_STEREO ? 2 : 1;
+    p.value.assign((size_t)3 * p.nch * d.hist_bins, 0.0);
+    p.h.value = p.value.data();
+}
+
+// Recorder::AddHistogram of every rendered band's tracks.
+void deliver_render(const PreparedRender& p, const std::vector<Recorder*> rec[3], unsigned band_mask) {
     for (int b = 0; b < 3; ++b) {
         if (!((band_mask >> b) & 1u)) continue;
         int ch = 0;
         for (Recorder* r : rec[b]) {
             for (int k = 0; k < r->trackCount(); ++k, ++ch)
-                r->AddHistogram(k, value.data() + ((size_t)b * nch + ch) * d.hist_bins, d.hist_bins);
+                r->AddHistogram(k, p.value.data() + ((size_t)b * p.nch + ch) * p.d.hist_bins, p.d.hist_bins);
         }
     }
+}
+}  // namespace
+
+void Scene::RenderBands(int sound, const float absorbtion_factor[3], int num_samples, float dry,
+                        const std::vector<Recorder*> rec[3], int keyframeID, unsigned band_mask) {
+    ensure_device();
+    PreparedRender p;
+    prepare_render(*this, p, sound, absorbtion_factor, num_samples, dry, rec, keyframeID, band_mask);
+    throw_on_error(ear_render(ctx_, dev_scene_, &p.d, &p.h, &last_stats), ctx_, "ear_render");
+    deliver_render(p, rec, band_mask);
+}
+
+void Scene::RenderBatch(const float absorbtion_factor[3], int num_samples, float dry, std::vector<Job>& jobs) {
+    if (jobs.empty()) return;
+    ensure_device();
+    std::vector<std::unique_ptr<PreparedRender>> preps;
+    std::vector<ear_batch_item> items;
+    for (Job& j : jobs) {
+        preps.push_back(std::make_unique<PreparedRender>());
+        prepare_render(*this, *preps.back(), j.sound, absorbtion_factor, num_samples, dry, j.rec, j.keyframe,
+                       j.band_mask);
+        items.push_back(ear_batch_item{dev_scene_, &preps.back()->d, &preps.back()->h});
+    }
+    throw_on_error(ear_render_batch(ctx_, items.data(), (uint32_t)items.size(), &last_stats), ctx_,
+                   "ear_render_batch");
+    for (size_t k = 0; k < jobs.size(); ++k) deliver_render(*preps[k], jobs[k].rec, jobs[k].band_mask);
 }
 
 SceneContext::SceneContext(Scene* scn, int b, int sf, int s, float ab, float dr, int kf)

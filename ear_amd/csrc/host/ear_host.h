@@ -310,6 +310,19 @@ class Scene {
     // the recorders of band b's context.
     void RenderBands(int sound, const float absorbtion_factor[3], int num_samples, float dry,
                      const std::vector<Recorder*> rec[3], int keyframeID = -1, unsigned band_mask = 7);
+    // One (sound, keyframe) of a batch: rec[b] are the recorders of band b's
+    // context (bands outside band_mask are not rendered).
+    struct Job {
+        int sound = 0;
+        int keyframe = -1;
+        unsigned band_mask = 7;
+        std::vector<Recorder*> rec[3];
+    };
+    // Every job in one device call (ear_render_batch).  Each job gets the
+    // histograms RenderBands would give it (its rays run on their own random
+    // streams); the device stays full across jobs, which the tail of a single
+    // render's bounce loop leaves mostly idle.
+    void RenderBatch(const float absorbtion_factor[3], int num_samples, float dry, std::vector<Job>& jobs);
     // The device context (created on first use), for the convolution stage.
     ear_ctx* context();
     ear_stats last_stats{};

@@ -151,26 +151,31 @@ int Render(const std::string& filename, float* calc_T60 = nullptr, float* T60_Sa
     Keyframes* keys = Keyframes::Get();
     std::cout << "Rendering..." << std::endl;
 
-    // contexts per (sound, keyframe, band) as src/EAR.cpp:170-191, rendered
-    // three bands per device call
+    // contexts per (sound, keyframe, band) as src/EAR.cpp:170-191, all
+    // rendered in one device call (three bands per (sound, keyframe) item)
     std::vector<Ctx> ctxs;
     const float air[3] = {1.0f - absorption[0], 1.0f - absorption[1], 1.0f - absorption[2]};
+    std::vector<Scene::Job> jobs;
     for (int sound = 0; sound < (int)scene->sources.size(); ++sound) {
         for (int kf = keys ? 0 : -1; kf < (keys ? (int)keys->keys.size() : 0); ++kf) {
-            std::vector<Recorder*> per[3];
-            unsigned mask = 0;
+            Scene::Job j;
+            j.sound = sound;
+            j.keyframe = kf;
+            j.band_mask = 0;
             for (int band = 0; band < 3; ++band) {
                 if (calc_T60 && band != 1) continue;
-                for (Recorder* l : scene->listeners) per[band].push_back(l->getBlankCopy(4));
-                mask |= 1u << band;
+                for (Recorder* l : scene->listeners) j.rec[band].push_back(l->getBlankCopy(4));
+                j.band_mask |= 1u << band;
             }
-            scene->RenderBands(sound, air, num_samples, dry_level, per, kf, mask);
-            for (int band = 0; band < 3; ++band)
-                if ((mask >> band) & 1u) ctxs.push_back(Ctx{sound, kf, band, per[band]});
+            jobs.push_back(j);
             if (calc_T60) break;
         }
         if (calc_T60) break;
     }
+    scene->RenderBatch(air, num_samples, dry_level, jobs);
+    for (const Scene::Job& j : jobs)
+        for (int band = 0; band < 3; ++band)
+            if ((j.band_mask >> band) & 1u) ctxs.push_back(Ctx{j.sound, j.keyframe, band, j.rec[band]});
 
     // Power(0.335) and threshold truncation (src/EAR.cpp:210-244)
     float mx = 0.0f;
