@@ -156,3 +156,38 @@ def test_render_stereo_writes_two_channels(tmp_path):
     # the two ears hear different things (interaural delay and head shadow)
     assert np.abs(got[:, 0] - got[:, 1]).max() > 100
 
+
+
+def test_render_two_sources(tmp_path):
+    """Two sound sources: the CLI renders the (sound, band) contexts of both in
+    one batch (sound 0's bands, then sound 1's) and the listener output is the
+    sum of all six processed contexts."""
+    dry0 = write_dry(tmp_path / "dry0.wav", 12000, seed=11)
+    dry1 = write_dry(tmp_path / "dry1.wav", 9000, seed=12)
+    debug = tmp_path / "debug"
+    debug.mkdir()
+    s = scenes.shoebox(spec=0.2)
+    w = earfile.EarWriter()
+    for i, (refl, refr, spec) in enumerate(s["materials"]):
+        w.material(f"m{i}", refl, refr, spec)
+    w.settings(samples=20_000, seed=21, maxbounces=120, debugdir=str(debug))
+    tris = np.asarray(s["tris"], np.float64)
+    for i in range(len(s["materials"])):
+        sel = tris[np.asarray(s["mat"]) == i]
+        if len(sel):
+            w.mesh(f"m{i}", sel.tolist())
+    w.source(str(tmp_path / "dry0.wav"), location=s["src"])
+    w.source(str(tmp_path / "dry1.wav"), location=(8.0, 2.0, 2.5))
+    w.mono("out.wav", s["recorders"][0]["pos"])
+    p = tmp_path / "two.EAR"
+    w.save(p)
+    run(tmp_path, p)
+    tracks = []
+    for sound, dry in ((0, dry0), (1, dry1)):
+        bands = OA.equalizer_split_f32(dry)
+        for b in range(3):
+            tracks.append(OA.process_track(response(debug, 0, sound, b), bands[b], 0))
+    # the sources sit at different places: their responses differ
+    assert not np.array_equal(response(debug, 0, 0, 1), response(debug, 0, 1, 1))
+    got = read_wave(tmp_path / "out.wav")[:, 0]
+    check_close(got, OA.listener_output(tracks))
