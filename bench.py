@@ -292,6 +292,18 @@ def bench_adjoint(ctx, n_rooms=32):
         ref = api.render(ctx, sc, **dict(kw, n_rays=8_000_000, seed=0x5EED))
         edc_err = max(acoustics.edc_error(r.abs_value[b, 0], ref.abs_value[b, 0]) for b in range(3))
         segs = r.stats["segments"]
+        # the same forward + adjoint at 8M rays, where the EDC error is within 1%
+        kw4 = dict(kw, n_rays=8_000_000)
+        t0 = time.perf_counter()
+        r4 = api.render(ctx, sc, **kw4)
+        t4f = time.perf_counter() - t0
+        _, adj4 = fit.log_edc_loss(r4.abs_value, target, mask)
+        t0 = time.perf_counter()
+        api.render_backward(ctx, sc, adj_abs=adj4, local=True, **kw4)
+        t4b = time.perf_counter() - t0
+        ref16 = api.render(ctx, sc, **dict(kw, n_rays=16_000_000, seed=0x5EED))
+        edc4 = max(acoustics.edc_error(r4.abs_value[b, 0], ref16.abs_value[b, 0]) for b in range(3))
+        segs4 = r4.stats["segments"]
     log(f"adjoint: forward {t_fwd * 1e3:.1f} ms, backward {t_bwd * 1e3:.1f} ms, {segs} segments, "
         f"EDC error {edc_err:.4f}")
     out.update({"workload": "shoebox 12x9x5 m, 18 absorption + 6 specularity parameters, 1M rays, 4096-bin log-EDC "
@@ -300,7 +312,13 @@ def bench_adjoint(ctx, n_rooms=32):
                 "forward_segments_per_s": round(segs / t_fwd, 1),
                 "forward_adjoint_segments_per_s": round(segs / (t_fwd + t_bwd), 1),
                 "forward_adjoint_vs_forward": round(t_fwd / (t_fwd + t_bwd), 4),
-                "edc_error_vs_8M_rays": round(edc_err, 5), "loss": round(loss, 6)})
+                "edc_error_vs_8M_rays": round(edc_err, 5), "loss": round(loss, 6),
+                "at_8M_rays": {"segments": int(segs4), "forward_ms": round(1e3 * t4f, 3),
+                               "backward_ms": round(1e3 * t4b, 3),
+                               "forward_segments_per_s": round(segs4 / t4f, 1),
+                               "forward_adjoint_segments_per_s": round(segs4 / (t4f + t4b), 1),
+                               "edc_error_vs_16M_rays": round(edc4, 5)}})
+    log(f"adjoint at 8M rays: forward {t4f * 1e3:.1f} ms, backward {t4b * 1e3:.1f} ms, EDC error {edc4:.4f}")
 
     rng = np.random.default_rng(17)
     rooms = [scenes.room(seed=5000 + i, target_tris=int(np.exp(rng.uniform(np.log(1000), np.log(100_000)))))
@@ -317,6 +335,22 @@ def bench_adjoint(ctx, n_rooms=32):
     t1 = time.perf_counter()
     api.render_backward_batch(ctx, [(sc, dict(k, adj_abs=a, local=True)) for sc, k, a in zip(scs, kws, adjs)])
     t2 = time.perf_counter()
+    # accuracy of the 250k-ray rooms: EDC error against independent 2M-ray renders
+    refs, _, _ = api.render_batch(ctx, [(sc, dict(k, n_rays=2_000_000, seed=0x5EED)) for sc, k in fwd_items])
+    errs = [max(acoustics.edc_error(x.abs_value[b, 0], y.abs_value[b, 0]) for b in range(3))
+            for x, y in zip(res, refs)]
+    # the same batch at 2M rays per room: forward + adjoint rate, and the EDC
+    # error between two independent 2M-ray renders (each render's own error
+    # is about that over sqrt(2))
+    kws2 = [dict(k, n_rays=2_000_000) for k in kws]
+    t0b = time.perf_counter()
+    res2, st2, _ = api.render_batch(ctx, [(sc, k) for sc, k in zip(scs, kws2)])
+    adjs2 = [fit.log_edc_loss(x.abs_value * 1.1, t, fit.decay_mask(t))[1] for x, t in zip(res2, targets)]
+    t1b = time.perf_counter()
+    api.render_backward_batch(ctx, [(sc, dict(k, adj_abs=a, local=True)) for sc, k, a in zip(scs, kws2, adjs2)])
+    t2b = time.perf_counter()
+    errs2 = [max(acoustics.edc_error(x.abs_value[b, 0], y.abs_value[b, 0]) for b in range(3))
+             for x, y in zip(res2, refs)]
     for sc in scs:
         sc.close()
     bsegs = st["segments"]
@@ -325,7 +359,15 @@ def bench_adjoint(ctx, n_rooms=32):
                                 f"log-EDC loss", "segments": int(bsegs), "forward_s": round(t1 - t0, 4),
                     "backward_s": round(t2 - t1, 4),
                     "forward_adjoint_segments_per_s": round(bsegs / (t2 - t0), 1),
-                    "forward_segments_per_s": round(bsegs / (t1 - t0), 1)}
+                    "forward_segments_per_s": round(bsegs / (t1 - t0), 1),
+                    "edc_error_vs_2M_rays_median": round(float(np.median(errs)), 5),
+                    "edc_error_vs_2M_rays_max": round(float(np.max(errs)), 5),
+                    "at_2M_rays": {"segments": int(st2["segments"]), "forward_s": round(t1b - t0b, 4),
+                                   "backward_s": round(t2b - t1b, 4),
+                                   "forward_segments_per_s": round(st2["segments"] / (t1b - t0b), 1),
+                                   "forward_adjoint_segments_per_s": round(st2["segments"] / (t2b - t0b), 1),
+                                   "edc_error_between_two_seeds_median": round(float(np.median(errs2)), 5),
+                                   "edc_error_between_two_seeds_max": round(float(np.max(errs2)), 5)}}
     return out
 
 
