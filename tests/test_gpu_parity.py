@@ -99,6 +99,41 @@ def test_hall_queries_match_bruteforce(gpu_ctx, oracle, width, builder, leaf):
         assert occ[i] == osc.any_hit(o[i], q[i])
 
 
+@pytest.mark.parametrize("builder,leaf", [(1, 2), (2, 1)])
+@pytest.mark.parametrize("offset,scale", [((3.0e4, -1.5e4, 8.0e3), 1.0), ((0.0, 0.0, 0.0), 0.05)])
+def test_far_or_tiny_scene_queries_match_bruteforce(gpu_ctx, oracle, builder, leaf, offset, scale):
+    """The hall 30 km from the origin (float spacing there: 2 mm, against
+    triangles of a few cm) and shrunk 20x to 2 m (smaller quantisation
+    scales; much smaller scenes lose their triangles to the reference's
+    |det| < 1e-5 test on both sides): the 4-wide tree's quantised boxes and
+    the box padding still keep every closest and any hit of the brute-force
+    scan."""
+    s = scenes.hall(target_tris=20000)
+    tris = (s["tris"].astype(np.float64) * scale + np.asarray(offset)).astype(np.float32)
+    rng = np.random.default_rng(8)
+    n = 3000
+    lo = tris.reshape(-1, 3).min(0)
+    hi = tris.reshape(-1, 3).max(0)
+    o = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    d[: n // 8] = np.eye(3, dtype=np.float32)[rng.integers(0, 3, n // 8)] * rng.choice([-1, 1], (n // 8, 1))
+    osc = oracle.OracleScene(tris, s["mat"], s["materials"])
+    with api.Scene(gpu_ctx, tris, s["mat"], s["materials"], leaf_size=leaf, width=4, builder=builder) as sc:
+        t, idx = sc.query_closest(o, d)
+        q = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+        occ = sc.query_occluded(o, q)
+    hits = 0
+    for i in range(0, n, 5):
+        ot, oi = osc.closest_hit(o[i], d[i])
+        assert idx[i] == oi, (i, idx[i], oi)
+        if oi >= 0:
+            hits += 1
+            assert np.float32(t[i]) == np.float32(ot)
+        assert occ[i] == osc.any_hit(o[i], q[i])
+    assert hits > 100
+
+
 @pytest.mark.parametrize("leaf,width,builder", [(1, 4, 1), (2, 2, 1), (8, 4, 1), (16, 2, 1), (1, 4, 2), (3, 2, 2),
                                                (16, 4, 2)])
 def test_bvh_shape_does_not_change_results(gpu_ctx, shoebox, leaf, width, builder):
