@@ -182,9 +182,10 @@ def main():
         if "fit" in legs:
             extras["fit"] = bench_fit(ctx)
 
-    cpu = None
+    cpu = cpu_ref = None
     if not a.no_cpu:
         cpu = cpu_baseline(s, kw, a.cpu_seconds)
+        cpu_ref = cpu_reference_threading(s, kw, 200)
 
     sc.close()
     ctx.close()
@@ -210,6 +211,7 @@ def main():
         "t60_mid_s": edc_t60,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_reference_threading": cpu_ref,
         "extras": extras,
     }
     print(json.dumps(out), flush=True)
@@ -505,6 +507,36 @@ def cpu_baseline(s, kw, target_s):
     return {"value": round(segs / t, 2), "unit": "segments/s", "cores": nthreads, "kind": "port",
             "sample": f"first {k} rays of each of the 3 band contexts of the same workload ({segs} segments, "
                       f"{t:.1f} s), brute-force triangle scans as the reference"}
+
+
+def cpu_reference_threading(s, kw, k):
+    """The same CPU port with the reference's threading: one thread per band
+    context (src/EAR.cpp:193-207), the three contexts at once, each on its
+    first k rays."""
+    import threading
+
+    from oracle import oracle as O
+
+    osc = O.OracleScene(s["tris"], s["mat"], s["materials"])
+    segs = [0, 0, 0]
+
+    def one(b):
+        r = O.render(osc, band=b, air=s["air"][b], recorders=s["recorders"], n_rays=kw["n_rays"], ray_end=k,
+                     src_pos=s["src"], max_bounces=kw["max_bounces"], cutoff=kw["cutoff"], n_bins=kw["n_bins"],
+                     n_threads=1, want_abs=False)
+        segs[b] = int(r["segments"])
+
+    th = [threading.Thread(target=one, args=(b,)) for b in range(3)]
+    t0 = time.perf_counter()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    t = time.perf_counter() - t0
+    log(f"cpu baseline, reference threading: {3 * k} rays, {sum(segs)} segments in {t:.2f}s on 3 threads")
+    return {"value": round(sum(segs) / t, 2), "unit": "segments/s", "cores": 3, "kind": "port",
+            "sample": f"first {k} rays of each band context, one thread per context ({sum(segs)} segments, "
+                      f"{t:.1f} s)"}
 
 
 if __name__ == "__main__":
