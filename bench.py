@@ -147,6 +147,20 @@ def main():
         edc_t60 = acoustics.t60_iso(r.abs_value[1, 0])
     except Exception as e:  # pragma: no cover - reporting only
         log(f"t60 failed: {e}")
+    # accuracy: the same render on an independent random stream; the EDC and
+    # T60 differences between the two bound each render's own error (about
+    # the difference over sqrt(2))
+    accuracy = None
+    try:
+        r2 = api.render(ctx, sc, **dict(kw, seed=0x5EED))
+        edc_err = max(acoustics.edc_error(r.abs_value[b, c], r2.abs_value[b, c]) for b in range(3)
+                      for c in range(r.abs_value.shape[1]))
+        t60_b = acoustics.t60_iso(r2.abs_value[1, 0])
+        accuracy = {"edc_error_between_two_seeds": round(edc_err, 5),
+                    "t60_mid_s_other_seed": t60_b, "t60_rel_difference": round(abs(t60_b / edc_t60 - 1.0), 5)}
+        log(f"accuracy: EDC error between two seeds {edc_err:.4f}, T60 {edc_t60:.4f} / {t60_b:.4f} s")
+    except Exception as e:  # pragma: no cover - reporting only
+        log(f"accuracy check failed: {e}")
 
     roof = None
     if not a.no_instrument:
@@ -209,6 +223,7 @@ def main():
         "segments_per_step": segs // max(1, a.steps),
         "bvh_build_ms": round(info["build_ms"], 3),
         "t60_mid_s": edc_t60,
+        "accuracy": accuracy,
         "roofline": roof,
         "cpu_baseline": cpu,
         "cpu_baseline_reference_threading": cpu_ref,
