@@ -27,6 +27,15 @@
  * Histograms are accumulated two ways: in FP64 (exact ramp (w-i)*S per bin,
  * signed and |c|) and, single-threaded only, in the reference's own fp32
  * iterated-subtraction form.
+ *
+ * Parity unpinned (no reference output exists to check them against): the
+ * exact bits of gmtl's intersectDoubleSided (its epsilon, here |det| < 1e-5,
+ * and operation order), gmtl's unitRandom float mapping and the glibc rand()
+ * sequence (both replaced by Philox, u = (word >> 8) * 2^-24), and the
+ * operand order of gmtl's normal (here normalize(cross(v1 - v0, v2 - v0))).
+ * Each is fixed here once, and the device path is held bit-exact to it; the
+ * geometry itself is pinned by brute-force FP64 checks and closed forms in
+ * the tests.
  */
 #ifndef EAR_ORACLE_H
 #define EAR_ORACLE_H
