@@ -98,7 +98,7 @@ class ear_tuning(C.Structure):
     _fields_ = [("block_extend", C.c_uint32), ("block_shade", C.c_uint32), ("block_shadow", C.c_uint32),
                 ("ray_chunk", C.c_uint32), ("waves_per_cu", C.c_uint32), ("stack_lds", C.c_uint32),
                 ("traversal", C.c_uint32), ("graph_iters", C.c_uint32), ("refill", C.c_uint32),
-                ("fetch_chunk", C.c_uint32), ("fetch_segments", C.c_uint32), ("reserved", C.c_uint32)]
+                ("fetch_chunk", C.c_uint32), ("fetch_segments", C.c_uint32), ("tail_rays", C.c_uint32)]
 
 
 EAR_TRAVERSAL_DEFAULT, EAR_TRAVERSAL_STATIC, EAR_TRAVERSAL_FETCH, EAR_TRAVERSAL_FETCH_ALL = 0, 1, 2, 3

@@ -59,21 +59,23 @@ class Context:
         _check(self.lib, self.lib.ear_ctx_set_memory_cap(self.h, nbytes), self.h)
 
     def set_tuning(self, block_extend=0, block_shade=0, block_shadow=0, ray_chunk=0, waves_per_cu=0, stack_lds=0,
-                   traversal=0, graph_iters=0, refill=0, fetch_chunk=0, fetch_segments=0):
+                   traversal=0, graph_iters=0, refill=0, fetch_chunk=0, fetch_segments=0, tail_rays=0):
         """Launch configuration (0 = library default); traversal: 1 static
         lanes, 2 closest-hit lanes refilled from the queue, 3 both kernels
         refilled (default); refill / fetch_chunk: idle lanes that trigger a
         refill (default 24), entries claimed per atomic (default: one per
         idle lane) and queue segments with a counter each (default 64);
-        graph_iters > 0 replays the bounce loop as a captured HIP graph."""
+        graph_iters > 0 replays the bounce loop as a captured HIP graph;
+        tail_rays: the last <= tail_rays live rays go to one launch that
+        traces each to its end (default 2^20; 1 turns it off)."""
         t = F.ear_tuning(block_extend, block_shade, block_shadow, ray_chunk, waves_per_cu, stack_lds, traversal,
-                         graph_iters, refill, fetch_chunk, fetch_segments)
+                         graph_iters, refill, fetch_chunk, fetch_segments, tail_rays)
         _check(self.lib, self.lib.ear_ctx_set_tuning(self.h, C.byref(t)), self.h)
 
     def get_tuning(self) -> dict:
         t = F.ear_tuning()
         _check(self.lib, self.lib.ear_ctx_get_tuning(self.h, C.byref(t)), self.h)
-        return {k: getattr(t, k) for k, _ in F.ear_tuning._fields_ if k != "reserved"}
+        return {k: getattr(t, k) for k, _ in F.ear_tuning._fields_}
 
 
 def make_materials(specs):

@@ -46,7 +46,9 @@ enum StatIdx {
     ST_NV_SH = 5,
     ST_NT_SH = 6,
     ST_RAYS = 7,
-    ST_N = 8
+    ST_TAIL_SEG = 8,  // segments and shadow rays traced by k_tail (not by the
+    ST_TAIL_SH = 9,   // wavefront kernels whose launch times are measured)
+    ST_N = 10
 };
 
 struct Queue {
@@ -784,6 +786,70 @@ __global__ void __launch_bounds__(BLOCK) k_extend(const InstDev* __restrict__ in
 // Scene::Render (src/Scene.cpp:49-82, 151-277, src/Material.cpp:76-83).
 // Six waves per SIMD: the register cap that costs no spills (2% faster than
 // the five the compiler picks by itself).
+// The surface interaction of a segment from ro/rd that hit triangle h.y at
+// t = h.x (src/Scene.cpp:143-175): hit point, the normal facing the ray
+// (negated for REFRACT), Material::Bounce's uniform (its Philox stream
+// starts here), air over the segment and the material's retention.
+__device__ __forceinline__ void surface_hit(const InstDev& In, const RenderParams& P, int band, float4 ro, float4 rd,
+                                            uint4 rm, int2 h, Rng& rng, V3& p, V3& nrm, V3& prev, V3& d, float& I,
+                                            float& L, float& spec, float& seg, int& bt, bool& alive) {
+    float t = __int_as_float(h.x);
+    V3 o{ro.x, ro.y, ro.z};
+    d = V3{rd.x, rd.y, rd.z};
+    p = V3{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+    float4 nmv = In.nm[h.y];
+    V3 N{nmv.x, nmv.y, nmv.z};
+    nrm = dot(N, d) > 0.0f ? V3{N.x * -1.0f, N.y * -1.0f, N.z * -1.0f} : N;
+    uint32_t mat = __float_as_uint(nmv.w);
+    float4 mb = In.mat_band[3 * mat + band];  // {fl2 | -1, spec, retention, -}
+    rng.init(P.seed, rm.x, P.ctx_word[band], P.instance, rm.y);
+    bt = kReflect;
+    if (mb.x >= 0.0f) bt = (rng.next() <= mb.x) ? kReflect : kRefract;
+    spec = mb.y;
+    if (bt == kRefract) nrm = neg(nrm);
+    float l = length(sub(p, o));
+    I = ro.w;
+    I *= powf_ear_l2(P.air[band], P.l2_air[band], l);
+    L = rd.w + l;
+    I *= mb.z;
+    alive = is_normal(I);
+    prev = normalized(d);
+    seg = l;
+}
+
+// The next-event weight of the connection from hit point p to recorder R
+// (src/Scene.cpp:185-268): false if it faces away or is not FP_NORMAL;
+// c carries the sign of the k-th reflection.
+__device__ __forceinline__ bool nee_weight(const RenderParams& P, const RecParams& R, int band, V3 p, V3 nrm, V3 prev,
+                                           int bt, float spec, float I, uint32_t k, float& c) {
+    V3 dv = sub(V3{R.px, R.py, R.pz}, p);
+    V3 lsdir = normalized(dv);
+    float dt = dot(lsdir, nrm);
+    if (!(dt > 0.0f)) return false;
+    // factor = spec * 1001 * sf^1000 + (1 - spec) * diff; with spec == 0 the
+    // lobe term is exactly +0 (sf^1000 is finite), so its binary64 pow is
+    // skipped without changing a bit
+    float diff, sf;
+    if (bt == kReflect) {
+        V3 rv = reflect(prev, nrm);
+        diff = -dot(nrm, prev);
+        sf = fmaxf(0.0f, dot(rv, lsdir));
+    } else {
+        diff = dot(nrm, prev);
+        sf = fmaxf(0.0f, dot(prev, lsdir));
+    }
+    const float lobe = spec != 0.0f ? spec * kExpInt * powf_ear(sf, kExp) : 0.0f;
+    const float factor = lobe + (1.0f - spec) * diff;
+    c = I;
+    c *= factor;
+    float l = length(dv);
+    c *= powf_ear_l2(P.air[band], P.l2_air[band], l);
+    c *= 2.0f / (4.0f * kPi * l * l);
+    if (!is_normal(c)) return false;
+    if (k & 1u) c *= -1.0f;
+    return true;
+}
+
 template <bool BATCH>
 __global__ void __launch_bounds__(kShadeBlock, 6) k_shade(const InstDev* __restrict__ inst, Queue qin,
                                                    const uint32_t* __restrict__ n_in, const int2* __restrict__ hits,
@@ -824,30 +890,7 @@ __global__ void __launch_bounds__(kShadeBlock, 6) k_shade(const InstDev* __restr
                         h.y >= 0 ? (int32_t)__float_as_uint(In.tris[3 * h.y + 2].y) : -1;
                 In.log_nseg[(size_t)band * P.trace_rays + rm.x] = k;
             }
-            if (h.y >= 0) {
-                float t = __int_as_float(h.x);
-                V3 o{ro.x, ro.y, ro.z};
-                d = V3{rd.x, rd.y, rd.z};
-                p = V3{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
-                float4 nmv = In.nm[h.y];
-                V3 N{nmv.x, nmv.y, nmv.z};
-                nrm = dot(N, d) > 0.0f ? V3{N.x * -1.0f, N.y * -1.0f, N.z * -1.0f} : N;
-                uint32_t mat = __float_as_uint(nmv.w);
-                float4 mb = In.mat_band[3 * mat + band];  // {fl2 | -1, spec, retention, -}
-                rng.init(P.seed, rm.x, P.ctx_word[band], P.instance, rm.y);
-                bt = kReflect;
-                if (mb.x >= 0.0f) bt = (rng.next() <= mb.x) ? kReflect : kRefract;
-                spec = mb.y;
-                if (bt == kRefract) nrm = neg(nrm);
-                float l = length(sub(p, o));
-                I = ro.w;
-                I *= powf_ear_l2(P.air[band], P.l2_air[band], l);
-                L = rd.w + l;
-                I *= mb.z;
-                alive = is_normal(I);
-                prev = normalized(d);
-                seg = l;
-            }
+            if (h.y >= 0) surface_hit(In, P, band, ro, rd, rm, h, rng, p, nrm, prev, d, I, L, spec, seg, bt, alive);
             if (P.estimator == EAR_EST_SPHERE) {
                 // hit or miss, the segment crosses the receiver spheres
                 n_sph += sphere_deposits(make_hist(In), P, band, V3{ro.x, ro.y, ro.z},
@@ -861,35 +904,7 @@ __global__ void __launch_bounds__(kShadeBlock, 6) k_shade(const InstDev* __restr
             float c = 0.0f;
             if (alive && nee && r < n_rec) {
                 const RenderParams& P = inst[id].P;
-                const RecParams& R = P.rec[r];
-                V3 dv = sub(V3{R.px, R.py, R.pz}, p);
-                V3 lsdir = normalized(dv);
-                float dt = dot(lsdir, nrm);
-                if (dt > 0.0f) {
-                    // factor = spec * 1001 * sf^1000 + (1 - spec) * diff; with
-                    // spec == 0 the lobe term is exactly +0 (sf^1000 is finite),
-                    // so its binary64 pow is skipped without changing a bit
-                    float diff, sf;
-                    if (bt == kReflect) {
-                        V3 rv = reflect(prev, nrm);
-                        diff = -dot(nrm, prev);
-                        sf = fmaxf(0.0f, dot(rv, lsdir));
-                    } else {
-                        diff = dot(nrm, prev);
-                        sf = fmaxf(0.0f, dot(prev, lsdir));
-                    }
-                    const float lobe = spec != 0.0f ? spec * kExpInt * powf_ear(sf, kExp) : 0.0f;
-                    const float factor = lobe + (1.0f - spec) * diff;
-                    c = I;
-                    c *= factor;
-                    float l = length(dv);
-                    c *= powf_ear_l2(P.air[band], P.l2_air[band], l);
-                    c *= 2.0f / (4.0f * kPi * l * l);
-                    if (is_normal(c)) {
-                        emit = true;
-                        if (k & 1u) c *= -1.0f;
-                    }
-                }
+                emit = nee_weight(P, P.rec[r], band, p, nrm, prev, bt, spec, I, k, c);
             }
             uint32_t slot = block_append<kShadeBlock>(n_sq, emit, sc[r & 1u]);
             if (emit) {
@@ -919,6 +934,85 @@ __global__ void __launch_bounds__(kShadeBlock, 6) k_shade(const InstDev* __restr
     }
     stat_add(stats, ST_SEGMENTS, n_done);
     stat_add(stats, ST_CONTRIB, n_sph);
+}
+
+// The tail of the bounce loop: once every ray has been emitted and few are
+// left, each lane carries its rays through all their remaining segments in
+// one launch (closest hit as k_extend, the surface interaction and
+// next-event weights as k_shade, visibility and splat as k_shadow, the next
+// direction as k_shade), instead of one launch of each kernel per segment,
+// whose time is then set by the latency of a few walks.  The same device
+// functions in the same order: every path, and so every histogram sum, is
+// the wavefront loop's.  A path ends by a miss, a non-normal intensity, the
+// cut-off or max_bounces, so every lane's loop ends.  Not used with the
+// sphere estimator, hit logging or counter instrumentation.
+template <int BLOCK, int LDS, int WIDTH, bool BATCH>
+__global__ void __launch_bounds__(BLOCK) k_tail(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
+                                                Queue q, const uint32_t* __restrict__ n_q,
+                                                unsigned long long* __restrict__ stats) {
+    __shared__ int lds[LDS * BLOCK];
+    const uint32_t n = *n_q;
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t n_seg = 0, nsh = 0, ncon = 0, nvis = 0, ntri = 0;
+    for (uint32_t i = gtid; i < n; i += gridDim.x * BLOCK) {
+        float4 ro = q.o[i], rd = q.d[i];
+        uint4 rm = q.m[i];
+        const uint32_t id = BATCH ? rm.w : 0;
+        const InstDev& In = inst[id];
+        const RenderParams& P = In.P;
+        const Traversal T = make_trav(In, gstack, gthreads);
+        const int band = (int)(rm.z & 0xFFu);
+        for (;;) {
+            const uint32_t k = ((rm.z >> 8) & 0xFFFFu) + 1;  // this segment's bounce number
+            float bt_t = 1000000.0f;
+            int bs = -1;
+            uint32_t bo = 0;
+            traverse<LDS, BLOCK, WIDTH, false, false>(T, V3{ro.x, ro.y, ro.z}, V3{rd.x, rd.y, rd.z}, bt_t, bs, bo,
+                                                      lds + threadIdx.x, gtid, nvis, ntri);
+            ++n_seg;
+            if (bs < 0) break;  // no hit ends the path (src/Scene.cpp:166)
+            Rng rng;
+            V3 p, nrm, prev, d;
+            float I = 0.0f, L = 0.0f, spec = 0.0f, seg = 0.0f;
+            int bt = kReflect;
+            bool alive = false;
+            surface_hit(In, P, band, ro, rd, rm, make_int2(__float_as_int(bt_t), bs), rng, p, nrm, prev, d, I, L, spec,
+                        seg, bt, alive);
+            if (!alive) break;
+            for (uint32_t r = 0; r < P.n_rec; ++r) {
+                float c = 0.0f;
+                const RecParams& R = P.rec[r];
+                if (!nee_weight(P, R, band, p, nrm, prev, bt, spec, I, k, c)) continue;
+                // Scene::Connect: the segment from the hit point to the recorder
+                V3 dv = sub(V3{R.px, R.py, R.pz}, p);
+                float st = 0.0f;
+                int ss = -1;
+                uint32_t so = 0;
+                ++nsh;
+                const bool occ = traverse<LDS, BLOCK, WIDTH, true, false>(T, p, dv, st, ss, so, lds + threadIdx.x,
+                                                                          gtid, nvis, ntri);
+                if (!occ) {
+                    float l = length(dv);
+                    V3 lsdir = normalized(dv);
+                    record(make_hist(In), band, R, lsdir, c, (L + l) / 343.0f, L + l);
+                    ++ncon;
+                }
+            }
+            if ((double)I < P.cutoff || k >= P.max_bounces) break;
+            // REFRACT: the negated normal and the incoming direction; REFLECT:
+            // the mirror direction about the ray-facing normal (src/Scene.cpp:65-74)
+            V3 rdir = bt == kRefract ? d : reflect(d, nrm);
+            V3 v = sample_hemi_spec(rng, nrm, rdir, spec);
+            ro = make_float4(p.x, p.y, p.z, I);
+            rd = make_float4(v.x, v.y, v.z, L);
+            rm = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), id);
+        }
+    }
+    stat_add(stats, ST_SEGMENTS, n_seg);
+    stat_add(stats, ST_SHADOW, nsh);
+    stat_add(stats, ST_CONTRIB, ncon);
+    stat_add(stats, ST_TAIL_SEG, n_seg);
+    stat_add(stats, ST_TAIL_SH, nsh);
 }
 
 // Scene::Connect + Mesh::LineIntersection (src/Scene.cpp:84-96,
@@ -1323,6 +1417,19 @@ void pick_b(const TravCfg& c, ExtendFn* e, ShadowFn* s) {
         case 6: pick_t<256, 32, 2, BA>(c.instr, c.fetch, e, s); break;
         default: pick_t<256, 32, 4, BA>(c.instr, c.fetch, e, s); break;
     }
+}
+
+using TailFn = void (*)(const InstDev*, int*, uint32_t, Queue, const uint32_t*, unsigned long long*);
+
+template <int B, int L>
+TailFn tail_for(int width, bool batch) {
+    if (width == 4) return batch ? (TailFn)k_tail<B, L, 4, true> : (TailFn)k_tail<B, L, 4, false>;
+    return batch ? (TailFn)k_tail<B, L, 2, true> : (TailFn)k_tail<B, L, 2, false>;
+}
+
+TailFn pick_tail(const TravCfg& c) {
+    if (c.block == 128) return c.lds == 32 ? tail_for<128, 32>(c.width, c.batch) : tail_for<128, 16>(c.width, c.batch);
+    return c.lds == 32 ? tail_for<256, 32>(c.width, c.batch) : tail_for<256, 16>(c.width, c.batch);
 }
 
 void pick_kernels(const TravCfg& c, ExtendFn* e, ShadowFn* s) {
@@ -1810,6 +1917,24 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         *done = pinned[cur] == 0 && pinned[3] == total32;
         return EAR_OK;
     };
+    // the tail: every ray emitted and few live ones left (queue cur, of
+    // pinned[cur] rays, read by finished); one launch traces them to their ends
+    // (tail_rays 0: 2^20 live rays, measured best on the 200k-triangle hall;
+    // 1: no tail launch)
+    const uint32_t tail_rays = tu.tail_rays == 0 ? (1u << 20) : (tu.tail_rays == 1 ? 0u : tu.tail_rays);
+    bool tail_ok = tail_rays > 0 && !instr && !any_log;
+    for (uint32_t i = 0; i < n_items; ++i) tail_ok = tail_ok && inst[i].P.estimator == EAR_EST_NEE;
+    const TailFn ft = tail_ok ? pick_tail(cfg_e) : nullptr;
+    auto tail = [&](int cur, bool* done) -> ear_status {
+        if (*done || !ft || pinned[3] != total32 || pinned[cur] > tail_rays) return EAR_OK;
+        hipLaunchKernelGGL(ft, dim3(g_ext), dim3(cfg_e.block), 0, st, di, gsp, gthreads, qq[cur],
+                           (const uint32_t*)(cnt + cur), stp);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) return hip_status(c, le, "tail launch");
+        launches += 1;
+        *done = true;
+        return EAR_OK;
+    };
     const uint32_t max_iters = max_bounces + 2 + (uint32_t)((total_rays + cap_rays - 1) / cap_rays) * (max_bounces + 1);
     int cur = 0;
     bool done = false;
@@ -1823,6 +1948,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
             hipError_t le = hipGetLastError();
             if (le != hipSuccess) status = hip_status(c, le, "tracer launch");
             if (status == EAR_OK) status = finished(cur, &done);
+            if (status == EAR_OK) status = tail(cur, &done);
             if (!done && bounces > max_iters) {
                 c->last_error = "bounce loop did not drain";
                 status = EAR_ERR_INTERNAL;
@@ -1969,8 +2095,8 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         st_out->shadow_ms = sum(ev_shw);
         st_out->extend_launches = (uint32_t)ev_ext.size();
         st_out->shadow_launches = (uint32_t)ev_shw.size();
-        st_out->extend_queries = hs[ST_SEGMENTS];
-        st_out->shadow_queries = hs[ST_SHADOW];
+        st_out->extend_queries = hs[ST_SEGMENTS] - hs[ST_TAIL_SEG];  // the timed k_extend launches' share
+        st_out->shadow_queries = hs[ST_SHADOW] - hs[ST_TAIL_SH];
     }
     for (auto* v : {&ev_ext, &ev_sha, &ev_shw})
         for (auto& e : *v) {

@@ -321,3 +321,50 @@ def test_keyframe_and_sound_select_the_stream(gpu_ctx, oracle):
         np.testing.assert_array_equal(g.trace_nseg[b], o["trace_nseg"])
         assert_hist_parity(api.raw_to_float(g.raw[b]), api.raw_to_float(g.abs_raw[b]), o["hist_signed"],
                            o["hist_abs"], n_rays * 51)
+
+
+@pytest.mark.parametrize("tail", [1 << 30, 50_000, 0])
+def test_tail_kernel_is_bitwise_identical(gpu_ctx, tail):
+    """tail_rays: once every ray is emitted, the live ones traced to their ends
+    in one launch.  The histograms are the bounce loop's (tail_rays = 1: no
+    tail launch) bit for bit, and so are the segment, shadow-ray and
+    contribution counts (single renders with up to 999 bounces, and a batch).
+    2^30 hands almost the whole render to the tail kernel, 50,000 switches
+    late, 0 is the default (2^20)."""
+    cases = [(scenes.shoebox(stereo=True, spec=0.5), dict(n_rays=30_000, max_bounces=999, cutoff=1e-8, n_bins=88200)),
+             (scenes.hall(target_tris=20_000), dict(n_rays=20_000, max_bounces=100, cutoff=1e-8, n_bins=132_300))]
+    for s, kw in cases:
+        print(f"tail {tail}: {s['name']}", flush=True)
+        with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
+            args = dict(recorders=s["recorders"], src_pos=s["src"], air=s["air"], **kw)
+            gpu_ctx.set_tuning(tail_rays=1)
+            a = api.render(gpu_ctx, sc, **args)
+            gpu_ctx.set_tuning(tail_rays=tail)
+            try:
+                b = api.render(gpu_ctx, sc, **args)
+            finally:
+                gpu_ctx.set_tuning()
+        np.testing.assert_array_equal(a.raw, b.raw)
+        np.testing.assert_array_equal(a.abs_raw, b.abs_raw)
+        for k in ("segments", "shadow_rays", "contributions"):
+            assert a.stats[k] == b.stats[k], k
+        assert b.stats["extend_queries"] < b.stats["segments"]  # the tail's segments are its own
+    rooms = [scenes.room(seed=40 + i, target_tris=[12, 2000, 9000][i]) for i in range(3)]
+    objs = [api.Scene(gpu_ctx, r["tris"], r["mat"], r["materials"]) for r in rooms]
+    try:
+        items = [(o, dict(recorders=r["recorders"], src_pos=r["src"], air=r["air"], n_rays=[3000, 8000, 5000][i],
+                          instance=i, max_bounces=300, cutoff=1e-8, n_bins=44100))
+                 for i, (r, o) in enumerate(zip(rooms, objs))]
+        gpu_ctx.set_tuning(tail_rays=1)
+        a, sa, _ = api.render_batch(gpu_ctx, items)
+        gpu_ctx.set_tuning(tail_rays=tail)
+        try:
+            b, sb, _ = api.render_batch(gpu_ctx, items)
+        finally:
+            gpu_ctx.set_tuning()
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x.raw, y.raw)
+        assert sa["segments"] == sb["segments"] and sa["contributions"] == sb["contributions"]
+    finally:
+        for o in objs:
+            o.close()
