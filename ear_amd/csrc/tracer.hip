@@ -1919,9 +1919,15 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     };
     // the tail: every ray emitted and few live ones left (queue cur, of
     // pinned[cur] rays, read by finished); one launch traces them to their ends
-    // (tail_rays 0: 2^20 live rays, measured best on the 200k-triangle hall;
-    // 1: no tail launch)
-    const uint32_t tail_rays = tu.tail_rays == 0 ? (1u << 20) : (tu.tail_rays == 1 ? 0u : tu.tail_rays);
+    // (tail_rays 0: 2^20 live rays, measured best on the 200k-triangle hall,
+    // or every ray when no scene has more than 64 triangles: their walks are
+    // so short that the queues between the wavefront kernels cost more than
+    // the phase divergence of one launch (12-triangle box, 3M rays: 3.25e9
+    // against 2.11e9 segments/s); 1: no tail launch)
+    uint32_t max_tris = 0;
+    for (uint32_t i = 0; i < n_items; ++i) max_tris = std::max(max_tris, inst[i].n_tris);
+    const uint32_t tail_default = max_tris <= 64 ? 0xFFFFFFFFu : (1u << 20);
+    const uint32_t tail_rays = tu.tail_rays == 0 ? tail_default : (tu.tail_rays == 1 ? 0u : tu.tail_rays);
     bool tail_ok = tail_rays > 0 && !instr && !any_log;
     for (uint32_t i = 0; i < n_items; ++i) tail_ok = tail_ok && inst[i].P.estimator == EAR_EST_NEE;
     const TailFn ft = tail_ok ? pick_tail(cfg_e) : nullptr;

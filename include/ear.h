@@ -321,8 +321,9 @@ typedef struct {
                                 (1..64, default 64) */
     uint32_t tail_rays;      /* once every ray is emitted and at most this many are live,
                                 one launch traces each through all its remaining segments
-                                (same histograms); 0: default 2^20, 1: never (the bounce
-                                loop to the end) */
+                                (same histograms); 0: default, 2^20, or every ray when no
+                                scene has more than 64 triangles; 1: never (the bounce loop
+                                to the end) */
 } ear_tuning;
 #define EAR_TRAVERSAL_DEFAULT 0u   /* = EAR_TRAVERSAL_FETCH_ALL */
 #define EAR_TRAVERSAL_STATIC 1u    /* one queue entry per lane per grid-stride step */
