@@ -331,7 +331,10 @@ def test_tail_kernel_is_bitwise_identical(gpu_ctx, tail):
     contribution counts (single renders with up to 999 bounces, and a batch).
     2^30 hands almost the whole render to the tail kernel, 50,000 switches
     late, 0 is the default (2^20)."""
+    emitter = np.array([[[4.0, 3.0, 1.0], [5.0, 3.0, 1.0], [4.0, 4.0, 1.2]]], np.float32)
     cases = [(scenes.shoebox(stereo=True, spec=0.5), dict(n_rays=30_000, max_bounces=999, cutoff=1e-8, n_bins=88200)),
+             (scenes.shoebox(spec=0.2), dict(n_rays=20_000, max_bounces=999, cutoff=1e-8, n_bins=88200,
+                                             src_tris=emitter)),
              (scenes.hall(target_tris=20_000), dict(n_rays=20_000, max_bounces=100, cutoff=1e-8, n_bins=132_300))]
     for s, kw in cases:
         print(f"tail {tail}: {s['name']}", flush=True)
