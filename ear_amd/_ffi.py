@@ -82,7 +82,8 @@ class ear_stats(C.Structure):
                 ("bounces_launched", C.c_uint32), ("launches", C.c_uint32), ("trace_ms", C.c_double),
                 ("extend_ms", C.c_double), ("shade_ms", C.c_double), ("shadow_ms", C.c_double),
                 ("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32), ("extend_queries", C.c_uint64),
-                ("shadow_queries", C.c_uint64)]
+                ("shadow_queries", C.c_uint64), ("dropped_ramps", C.c_uint64), ("bins_needed", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class ear_batch_item(C.Structure):

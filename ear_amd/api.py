@@ -67,7 +67,8 @@ class Context:
         idle lane) and queue segments with a counter each (default 64);
         graph_iters > 0 replays the bounce loop as a captured HIP graph;
         tail_rays: the last <= tail_rays live rays go to one launch that
-        traces each to its end (default 2^20; 1 turns it off)."""
+        traces each to its end (default: every ray when no scene of the call
+        has more than 64 triangles, else the last 2^20; 1 turns it off)."""
         t = F.ear_tuning(block_extend, block_shade, block_shadow, ray_chunk, waves_per_cu, stack_lds, traversal,
                          graph_iters, refill, fetch_chunk, fetch_segments, tail_rays)
         _check(self.lib, self.lib.ear_ctx_set_tuning(self.h, C.byref(t)), self.h)

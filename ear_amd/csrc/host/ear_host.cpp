@@ -551,7 +551,7 @@ struct PreparedRender {
 
 void prepare_render(const Scene& scene, PreparedRender& p, int sound, const float absorbtion_factor[3],
                     int num_samples, float dry, const std::vector<Recorder*> rec[3], int keyframeID,
-                    unsigned band_mask) {
+                    unsigned band_mask, unsigned hist_bins) {
     AbstractSoundFile* src = scene.sources.at((size_t)sound);
     // the recorders of every band context describe the same listeners
     const std::vector<Recorder*>* any = nullptr;
@@ -587,7 +587,7 @@ void prepare_render(const Scene& scene, PreparedRender& p, int sound, const floa
     d.seed = scene.options.seed;
     d.keyframe = keyframeID;
     d.sound = (uint32_t)sound;
-    d.hist_bins = scene.options.hist_bins;
+    d.hist_bins = hist_bins;
     d.estimator = scene.options.estimator;
     for (const ear_recorder& r : p.recs) p.nch += r.kind == EAR_REC_STEREO ? 2 : 1;
     p.value.assign((size_t)3 * p.nch * d.hist_bins, 0.0);
@@ -607,29 +607,51 @@ void deliver_render(const PreparedRender& p, const std::vector<Recorder*> rec[3]
 }
 }  // namespace
 
+unsigned Scene::grown_bins(const ear_stats& st, unsigned bins) const {
+    if (!options.grow_histogram || st.dropped_ramps == 0 || st.bins_needed <= bins) return 0;
+    // whole seconds, as FloatBuffer grows (src/Recorder.cpp:52-59)
+    const uint64_t need = ((uint64_t)st.bins_needed + 44099) / 44100 * 44100;
+    const uint64_t g = std::min<uint64_t>(need, options.max_hist_bins);
+    return g > bins ? (unsigned)g : 0u;
+}
+
 void Scene::RenderBands(int sound, const float absorbtion_factor[3], int num_samples, float dry,
                         const std::vector<Recorder*> rec[3], int keyframeID, unsigned band_mask) {
     ensure_device();
-    PreparedRender p;
-    prepare_render(*this, p, sound, absorbtion_factor, num_samples, dry, rec, keyframeID, band_mask);
-    throw_on_error(ear_render(ctx_, dev_scene_, &p.d, &p.h, &last_stats), ctx_, "ear_render");
-    deliver_render(p, rec, band_mask);
+    for (unsigned bins = options.hist_bins;;) {
+        PreparedRender p;
+        prepare_render(*this, p, sound, absorbtion_factor, num_samples, dry, rec, keyframeID, band_mask, bins);
+        throw_on_error(ear_render(ctx_, dev_scene_, &p.d, &p.h, &last_stats), ctx_, "ear_render");
+        const unsigned more = grown_bins(last_stats, bins);
+        if (!more) {
+            deliver_render(p, rec, band_mask);
+            return;
+        }
+        bins = more;
+    }
 }
 
 void Scene::RenderBatch(const float absorbtion_factor[3], int num_samples, float dry, std::vector<Job>& jobs) {
     if (jobs.empty()) return;
     ensure_device();
-    std::vector<std::unique_ptr<PreparedRender>> preps;
-    std::vector<ear_batch_item> items;
-    for (Job& j : jobs) {
-        preps.push_back(std::make_unique<PreparedRender>());
-        prepare_render(*this, *preps.back(), j.sound, absorbtion_factor, num_samples, dry, j.rec, j.keyframe,
-                       j.band_mask);
-        items.push_back(ear_batch_item{dev_scene_, &preps.back()->d, &preps.back()->h});
+    for (unsigned bins = options.hist_bins;;) {
+        std::vector<std::unique_ptr<PreparedRender>> preps;
+        std::vector<ear_batch_item> items;
+        for (Job& j : jobs) {
+            preps.push_back(std::make_unique<PreparedRender>());
+            prepare_render(*this, *preps.back(), j.sound, absorbtion_factor, num_samples, dry, j.rec, j.keyframe,
+                           j.band_mask, bins);
+            items.push_back(ear_batch_item{dev_scene_, &preps.back()->d, &preps.back()->h});
+        }
+        throw_on_error(ear_render_batch(ctx_, items.data(), (uint32_t)items.size(), &last_stats), ctx_,
+                       "ear_render_batch");
+        const unsigned more = grown_bins(last_stats, bins);
+        if (!more) {
+            for (size_t k = 0; k < jobs.size(); ++k) deliver_render(*preps[k], jobs[k].rec, jobs[k].band_mask);
+            return;
+        }
+        bins = more;
     }
-    throw_on_error(ear_render_batch(ctx_, items.data(), (uint32_t)items.size(), &last_stats), ctx_,
-                   "ear_render_batch");
-    for (size_t k = 0; k < jobs.size(); ++k) deliver_render(*preps[k], jobs[k].rec, jobs[k].band_mask);
 }
 
 SceneContext::SceneContext(Scene* scn, int b, int sf, int s, float ab, float dr, int kf)

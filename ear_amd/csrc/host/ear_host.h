@@ -285,6 +285,11 @@ struct RenderOptions {
     unsigned max_bounces = 999;   // src/Scene.cpp:143 (num_bounces < 1000)
     double energy_cutoff = 1e-8;  // src/Scene.cpp:275
     unsigned hist_bins = 3 * 44100;  // the reference's initial buffer (src/Recorder.h:36)
+    // The reference's tracks grow on a write past their end (src/Recorder.cpp:52-59):
+    // a render whose ramps ran past hist_bins is repeated with whole seconds of
+    // room for them (same seed, same paths), up to max_hist_bins.
+    bool grow_histogram = true;
+    unsigned max_hist_bins = 600 * 44100;
     unsigned leaf_size = 0, bvh_width = 0, bvh_builder = 0;  // 0: library defaults
     unsigned estimator = EAR_EST_NEE;  // EAR_EST_SPHERE: receiver spheres of receiver_radius
     float receiver_radius = 0.0f;
@@ -328,6 +333,8 @@ class Scene {
     ear_stats last_stats{};
 
    private:
+    // bins for a repeated render after `st` reported cut ramps at `bins`; 0: none needed
+    unsigned grown_bins(const ear_stats& st, unsigned bins) const;
     ear_ctx* ctx_ = nullptr;
     ear_scene* dev_scene_ = nullptr;
     size_t uploaded_tris_ = (size_t)-1;

@@ -17,9 +17,11 @@
 //   EAR test              src/EAR.cpp:424-425: returns 0.
 //   EAR info <file>       parses the file and prints a summary (no device).
 // Settings beyond the reference's: seed (int4), maxbounces (int4), cutoff
-// (flt4), irlength (flt4, seconds of impulse response), device (int4),
-// radius (flt4: receiver spheres of this radius instead of the reference's
-// point connections).
+// (flt4), irlength (flt4, initial seconds of impulse response; a render whose
+// ramps run past it is repeated with room for them, as the reference's
+// buffers grow, unless growir (int4) is 0), device (int4), radius (flt4:
+// receiver spheres of this radius instead of the reference's point
+// connections).
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -85,6 +87,7 @@ Scene* build_scene(EarFile& F, bool load_audio) {
     if (F.has("maxbounces")) o.max_bounces = (unsigned)F.get_int("maxbounces");
     if (F.has("cutoff")) o.energy_cutoff = F.get_float("cutoff");
     if (F.has("irlength")) o.hist_bins = (unsigned)std::lround(F.get_float("irlength") * 44100.0f);
+    if (F.has("growir")) o.grow_histogram = F.get_int("growir") != 0;
     if (F.has("device")) o.device = F.get_int("device");
     if (F.has("radius")) {
         o.estimator = EAR_EST_SPHERE;

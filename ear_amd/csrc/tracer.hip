@@ -34,6 +34,7 @@ namespace ear {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr uint32_t kMaxBounces = 0xFFFFu;  // the bounce number's field in a queue entry (bits 8..23)
 constexpr int kLdsMin = 16;  // smallest LDS part of the traversal stack; the overflow area covers the rest
 constexpr int kShadeBlock = 256;  // k_shade: its block-wide appends take one counter atomic per block step
 
@@ -48,7 +49,9 @@ enum StatIdx {
     ST_RAYS = 7,
     ST_TAIL_SEG = 8,  // segments and shadow rays traced by k_tail (not by the
     ST_TAIL_SH = 9,   // wavefront kernels whose launch times are measured)
-    ST_N = 10
+    ST_DROPPED = 10,  // ramps cut at the end of the histogram
+    ST_BINS = 11,     // the largest ramp end among them (a maximum, not a sum)
+    ST_N = 12
 };
 
 struct Queue {
@@ -136,6 +139,29 @@ __device__ __forceinline__ void stat_add(unsigned long long* stats, int idx, uin
     // reduce within the wave, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
     if (lane_id() == 0 && v) atomicAdd(stats + (size_t)(blockIdx.x % kStatCopies) * ST_N + idx, (unsigned long long)v);
+}
+
+// Ramps that run past the last bin of the histogram.  The reference grows its
+// buffer on such a write (FloatBuffer::operator[], src/Recorder.cpp:52-59);
+// here the late bins are not recorded, and each such ramp is counted with the
+// bin count that would have kept it whole, so that a caller can size the
+// histogram and render again.
+struct Drops {
+    uint32_t n = 0, end = 0;
+    __device__ __forceinline__ void add(uint32_t e) {
+        if (e) {
+            ++n;
+            end = max(end, e);
+        }
+    }
+};
+// All lanes of the wave must call it.
+__device__ __forceinline__ void stat_drops(unsigned long long* stats, const Drops& d) {
+    stat_add(stats, ST_DROPPED, d.n);
+    uint32_t e = d.end;
+    for (int off = 32; off > 0; off >>= 1) e = max(e, (uint32_t)__shfl_down(e, off));
+    if (lane_id() == 0 && e)
+        atomicMax(stats + (size_t)(blockIdx.x % kStatCopies) * ST_N + ST_BINS, (unsigned long long)e);
 }
 
 // Lane refilling (see Fetch): call with the whole wave when !exhausted and at
@@ -485,17 +511,21 @@ __device__ __forceinline__ void acc128(unsigned long long* cell, I128 q) {
 // >= nb dropped, added to the second-difference row h: +w*S at s, -(w+1)*S
 // at s+1, +S at s+w+1.  Each point is added as a magnitude to the row's "+"
 // or "-" part, so no add is negative and the high word is touched only on a
-// carry (one atomic per point instead of two for negative values).
-__device__ void splat_ramp(unsigned long long* h, uint32_t nb, int s, int w, float A) {
-    if (w <= 0) return;
+// carry (one atomic per point instead of two for negative values).  Returns
+// the ramp's end (one past its last bin) if bins at or past nb were cut, else
+// 0 (a start before bin 0 is the reference's own skip, src/StereoRecorder.cpp:92-95).
+__device__ uint32_t splat_ramp(unsigned long long* h, uint32_t nb, int s, int w, float A) {
+    if (w <= 0) return 0;
     double S = (double)A / (double)w;
     int ww = w;
     if (s < 0) {
         ww = w + s;
         s = 0;
-        if (ww <= 0) return;
+        if (ww <= 0) return 0;
     }
-    if ((uint32_t)s >= nb) return;
+    const uint32_t end = (uint32_t)s + (uint32_t)ww;
+    const uint32_t cut = end > nb ? end : 0u;
+    if ((uint32_t)s >= nb) return cut;
     const bool neg = S < 0.0;
     I128 sq = fixed_from_double(neg ? -S : S);
     I128 aq = i128_mul_small(sq, (uint32_t)ww);
@@ -506,6 +536,7 @@ __device__ void splat_ramp(unsigned long long* h, uint32_t nb, int s, int w, flo
     acc128(hp + 2 * (size_t)s0, aq);
     if (s0 + 1 < nb) acc128(hm + 2 * (size_t)(s0 + 1), p1);
     if (s0 + (uint32_t)ww + 1 < nb) acc128(hp + 2 * (size_t)(s0 + ww + 1), sq);
+    return cut;
 }
 
 // Row layout of the accumulators.  Without the |c| histogram one signed row
@@ -525,21 +556,21 @@ struct Hist {
 // Recorder::Record: MonoRecorder (src/MonoRecorder.cpp:83-97) or
 // StereoRecorder (src/StereoRecorder.cpp:97-130).
 // MonoRecorder::Record (src/MonoRecorder.cpp:83-97) into channel ch
-__device__ void record_channel(const Hist& H, int band, int ch, float a, float t, float dist) {
+__device__ void record_channel(const Hist& H, int band, int ch, float a, float t, float dist, Drops& dr) {
     const bool split = H.kinds > 1;
     const uint32_t kind = split && a < 0.0f ? 1u : 0u;
     int s = (int)((double)t * 44100.0);
     float width = sqrtf(dist);
     float ampl = 2.0f * a / width;
     int w = (int)ceilf(width);
-    splat_ramp(H.row(band, ch, kind), H.n_bins, s, w, split ? fabsf(ampl) : ampl);
+    dr.add(splat_ramp(H.row(band, ch, kind), H.n_bins, s, w, split ? fabsf(ampl) : ampl));
 }
 
-__device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, float a, float t, float dist) {
+__device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, float a, float t, float dist, Drops& dr) {
     const bool split = H.kinds > 1;
     const uint32_t kind = split && a < 0.0f ? 1u : 0u;
     if (R.kind == EAR_REC_MONO) {
-        record_channel(H, band, R.channel, a, t, dist);
+        record_channel(H, band, R.channel, a, t, dist, dr);
     } else {
         float dt = dot(dir, V3{R.rx, R.ry, R.rz});
         float td = R.head_size / 343.0f;
@@ -554,8 +585,8 @@ __device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, floa
         else
             al *= factor * factor;
         int w = (int)ceilf(width);
-        splat_ramp(H.row(band, R.channel, kind), H.n_bins, s_left, w, split ? fabsf(al) : al);
-        splat_ramp(H.row(band, R.channel + 1, kind), H.n_bins, s_right, w, split ? fabsf(ar) : ar);
+        dr.add(splat_ramp(H.row(band, R.channel, kind), H.n_bins, s_left, w, split ? fabsf(al) : al));
+        dr.add(splat_ramp(H.row(band, R.channel + 1, kind), H.n_bins, s_right, w, split ? fabsf(ar) : ar));
     }
 }
 
@@ -566,7 +597,7 @@ __device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, floa
 // segment: sign (-1)^j, and dry on a point source's first segment.  Returns
 // the number of deposits.
 __device__ uint32_t sphere_deposits(const Hist& H, const RenderParams& P, int band, V3 o, V3 u, float seg, float I0,
-                                    float L0, uint32_t j) {
+                                    float L0, uint32_t j, Drops& dr) {
     uint32_t n = 0;
     for (uint32_t r = 0; r < P.n_rec; ++r) {
         const RecParams& R = P.rec[r];
@@ -587,7 +618,7 @@ __device__ uint32_t sphere_deposits(const Hist& H, const RenderParams& P, int ba
             if (!is_normal(c)) continue;
             if (j & 1u) c *= -1.0f;
             const float dist = L0 + s0;
-            record_channel(H, band, R.channel + e, c, dist / 343.0f, dist);
+            record_channel(H, band, R.channel + e, c, dist / 343.0f, dist, dr);
             ++n;
         }
     }
@@ -859,6 +890,7 @@ __global__ void __launch_bounds__(kShadeBlock, 6) k_shade(const InstDev* __restr
     __shared__ uint32_t sc[4][kShadeBlock / 64 + 1];  // scratch of the block-wide appends, alternated
     uint32_t n = *n_in;
     uint32_t n_done = 0, n_sph = 0;
+    Drops drops;
     for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < n;
@@ -894,7 +926,7 @@ __global__ void __launch_bounds__(kShadeBlock, 6) k_shade(const InstDev* __restr
             if (P.estimator == EAR_EST_SPHERE) {
                 // hit or miss, the segment crosses the receiver spheres
                 n_sph += sphere_deposits(make_hist(In), P, band, V3{ro.x, ro.y, ro.z},
-                                         normalized(V3{rd.x, rd.y, rd.z}), seg, ro.w, rd.w, k - 1);
+                                         normalized(V3{rd.x, rd.y, rd.z}), seg, ro.w, rd.w, k - 1, drops);
                 nee = false;
             }
         }
@@ -934,6 +966,7 @@ __global__ void __launch_bounds__(kShadeBlock, 6) k_shade(const InstDev* __restr
     }
     stat_add(stats, ST_SEGMENTS, n_done);
     stat_add(stats, ST_CONTRIB, n_sph);
+    stat_drops(stats, drops);
 }
 
 // The tail of the bounce loop: once every ray has been emitted and few are
@@ -954,6 +987,7 @@ __global__ void __launch_bounds__(BLOCK) k_tail(const InstDev* __restrict__ inst
     const uint32_t n = *n_q;
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t n_seg = 0, nsh = 0, ncon = 0, nvis = 0, ntri = 0;
+    Drops drops;
     for (uint32_t i = gtid; i < n; i += gridDim.x * BLOCK) {
         float4 ro = q.o[i], rd = q.d[i];
         uint4 rm = q.m[i];
@@ -994,7 +1028,7 @@ __global__ void __launch_bounds__(BLOCK) k_tail(const InstDev* __restrict__ inst
                 if (!occ) {
                     float l = length(dv);
                     V3 lsdir = normalized(dv);
-                    record(make_hist(In), band, R, lsdir, c, (L + l) / 343.0f, L + l);
+                    record(make_hist(In), band, R, lsdir, c, (L + l) / 343.0f, L + l, drops);
                     ++ncon;
                 }
             }
@@ -1013,6 +1047,7 @@ __global__ void __launch_bounds__(BLOCK) k_tail(const InstDev* __restrict__ inst
     stat_add(stats, ST_CONTRIB, ncon);
     stat_add(stats, ST_TAIL_SEG, n_seg);
     stat_add(stats, ST_TAIL_SH, nsh);
+    stat_drops(stats, drops);
 }
 
 // Scene::Connect + Mesh::LineIntersection (src/Scene.cpp:84-96,
@@ -1025,6 +1060,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(const InstDev* __restrict__ in
     uint32_t n = *n_sq;
     uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0;
+    Drops drops;
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         uint32_t i = base + threadIdx.x;
         if (i < n) {
@@ -1047,13 +1083,14 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(const InstDev* __restrict__ in
                 float l = length(dv);
                 V3 lsdir = normalized(dv);
                 float L = m.x;
-                record(make_hist(In), band, R, lsdir, pc.w, (L + l) / 343.0f, L + l);
+                record(make_hist(In), band, R, lsdir, pc.w, (L + l) / 343.0f, L + l, drops);
                 ++ncon;
             }
         }
     }
     stat_add(stats, ST_SHADOW, nsh);
     stat_add(stats, ST_CONTRIB, ncon);
+    stat_drops(stats, drops);
     if (INSTR) {
         stat_add(stats, ST_NV_SH, nvis);
         stat_add(stats, ST_NT_SH, ntri);
@@ -1124,6 +1161,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t lane = lane_id();
     uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0;
+    Drops drops;
     Walker<LDS, BLOCK, WIDTH, true, INSTR> w(lds + threadIdx.x, gstack, gtid, gthreads);
     Traversal T = make_trav(inst[0], gstack, gthreads);
     // a lane is walking (active), holds a visible connection not yet
@@ -1146,7 +1184,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
                 const RecParams& R = In.P.rec[r];
                 const V3 dv = sub(V3{R.px, R.py, R.pz}, V3{pc.x, pc.y, pc.z});
                 const float l = length(dv);
-                record(make_hist(In), band, R, normalized(dv), pc.w, (m.x + l) / 343.0f, m.x + l);
+                record(make_hist(In), band, R, normalized(dv), pc.w, (m.x + l) / 343.0f, m.x + l, drops);
                 ++ncon;
                 pending = false;
             }
@@ -1182,6 +1220,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
     }
     stat_add(stats, ST_SHADOW, nsh);
     stat_add(stats, ST_CONTRIB, ncon);
+    stat_drops(stats, drops);
     if (INSTR) {
         stat_add(stats, ST_NV_SH, nvis);
         stat_add(stats, ST_NT_SH, ntri);
@@ -1220,7 +1259,8 @@ __global__ void k_reset_iter(uint32_t* c, uint32_t mask, uint32_t* seg, uint32_t
 // Direct sound (src/Scene.cpp:299-311): one block per instance, one thread
 // per (band, recorder).
 template <int WIDTH>
-__global__ void k_direct(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads) {
+__global__ void k_direct(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
+                         unsigned long long* __restrict__ stats) {
     __shared__ int lds[kLdsMin * kBlock];
     const InstDev& In = inst[blockIdx.x];
     const RenderParams& P = In.P;
@@ -1249,8 +1289,17 @@ __global__ void k_direct(const InstDev* __restrict__ inst, int* gstack, uint32_t
     float width = sqrtf(len);
     float ampl = 2.0f * a / width;
     int w = (int)ceilf(width);
+    // a direct ramp cut at the end of the histogram counts as a dropped ramp
+    auto cut = [&](int s) {
+        const int64_t e = (int64_t)s + (int64_t)w;  // one past the ramp's last bin
+        if (e > (int64_t)P.n_bins) {
+            atomicAdd(stats + ST_DROPPED, 1ull);
+            atomicMax(stats + ST_BINS, (unsigned long long)e);
+        }
+    };
     if (R.kind == EAR_REC_MONO) {
         o[0] = DirectRamp{1, (int)((double)tt * 44100.0), w, ampl};
+        cut(o[0].s);
     } else {
         float dt = dot(dir, V3{R.rx, R.ry, R.rz});
         float td = R.head_size / 343.0f;
@@ -1262,6 +1311,8 @@ __global__ void k_direct(const InstDev* __restrict__ inst, int* gstack, uint32_t
             al *= factor * factor;
         o[0] = DirectRamp{1, (int)((double)(tt + dt * td) * 44100.0), w, al};
         o[1] = DirectRamp{1, (int)((double)(tt - dt * td) * 44100.0), w, ar};
+        cut(o[0].s);
+        cut(o[1].s);
     }
 }
 
@@ -1502,6 +1553,12 @@ ear_status make_params(Ctx* c, const ear_render_desc* D, RenderParams* P) {
     }
     for (int b = 0; b < 3; ++b)
         if (!std::isfinite(D->air[b])) return EAR_ERR_INVALID_ARGUMENT;
+    if (D->max_bounces > kMaxBounces) {
+        // the bounce number travels in 16 bits of a queue entry; past them it
+        // would wrap, and a path in a lossless room would never end
+        c->last_error = "max_bounces above 65535";
+        return EAR_ERR_CAPACITY;
+    }
     if (D->estimator > EAR_EST_SPHERE) {
         c->last_error = "unknown estimator";
         return EAR_ERR_INVALID_ARGUMENT;
@@ -1988,9 +2045,9 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     }
     if (status != EAR_OK) return status;
     if (width == 4)
-        k_direct<4><<<n_items, kBlock, 0, st>>>(di, gsp, gthreads);
+        k_direct<4><<<n_items, kBlock, 0, st>>>(di, gsp, gthreads, stp);
     else
-        k_direct<2><<<n_items, kBlock, 0, st>>>(di, gsp, gthreads);
+        k_direct<2><<<n_items, kBlock, 0, st>>>(di, gsp, gthreads, stp);
     std::vector<uint4> rows;
     for (uint32_t i = 0; i < n_items; ++i)
         for (uint32_t b = 0; b < 3; ++b)
@@ -2072,7 +2129,8 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         EAR_HIP_TRY(c, hipMemcpy(hc.data(), stats.p, hc.size() * 8, hipMemcpyDeviceToHost));
         unsigned long long hs[ST_N] = {};
         for (uint32_t k = 0; k < kStatCopies; ++k)
-            for (int j = 0; j < ST_N; ++j) hs[j] += hc[(size_t)k * ST_N + j];
+            for (int j = 0; j < ST_N; ++j)
+                hs[j] = j == ST_BINS ? std::max(hs[j], hc[(size_t)k * ST_N + j]) : hs[j] + hc[(size_t)k * ST_N + j];
         std::memset(st_out, 0, sizeof(*st_out));
         st_out->segments = hs[ST_SEGMENTS];
         st_out->shadow_rays = hs[ST_SHADOW];
@@ -2103,6 +2161,8 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         st_out->shadow_launches = (uint32_t)ev_shw.size();
         st_out->extend_queries = hs[ST_SEGMENTS] - hs[ST_TAIL_SEG];  // the timed k_extend launches' share
         st_out->shadow_queries = hs[ST_SHADOW] - hs[ST_TAIL_SH];
+        st_out->dropped_ramps = hs[ST_DROPPED];
+        st_out->bins_needed = (uint32_t)std::min<unsigned long long>(hs[ST_BINS], 0xFFFFFFFFull);
     }
     for (auto* v : {&ev_ext, &ev_sha, &ev_shw})
         for (auto& e : *v) {

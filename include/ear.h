@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EAR_ABI_VERSION 4
+#define EAR_ABI_VERSION 5
 
 typedef enum {
     EAR_OK = 0,
@@ -184,7 +184,8 @@ typedef struct {
     const ear_recorder* recorders;
     uint32_t n_recorders;
     uint32_t n_rays;        /* num_samples */
-    uint32_t max_bounces;   /* closest-hit queries per ray; the reference loops to 999 */
+    uint32_t max_bounces;   /* closest-hit queries per ray; the reference loops to 999;
+                               at most 65535 (EAR_ERR_CAPACITY above) */
     double energy_cutoff;   /* path stops when intensity < cutoff; reference 1e-8 */
     uint64_t seed;          /* Philox key */
     uint32_t instance;      /* Philox counter word 3 (batch instance) */
@@ -235,6 +236,14 @@ typedef struct {
     uint32_t shadow_launches;
     uint64_t extend_queries;   /* rays handled by closest-hit launches */
     uint64_t shadow_queries;
+    /* Ramps cut at the end of the histogram: a contribution (or the direct
+     * sound) whose ramp runs past hist_bins loses its late bins.  The
+     * reference grows its buffer on such a write instead (FloatBuffer::
+     * operator[], src/Recorder.cpp:52-59); a caller that wants every bin
+     * renders again with hist_bins >= bins_needed (same seed: same paths). */
+    uint64_t dropped_ramps;
+    uint32_t bins_needed;      /* the largest ramp end among them; 0 if none was cut */
+    uint32_t reserved;
 } ear_stats;
 
 ear_status ear_render(ear_ctx* ctx, ear_scene* scene, const ear_render_desc* desc, ear_hist* out,

@@ -330,7 +330,8 @@ def test_tail_kernel_is_bitwise_identical(gpu_ctx, tail):
     tail launch) bit for bit, and so are the segment, shadow-ray and
     contribution counts (single renders with up to 999 bounces, and a batch).
     2^30 hands almost the whole render to the tail kernel, 50,000 switches
-    late, 0 is the default (2^20)."""
+    late, 0 is the default (the whole render for the 12-triangle boxes, the
+    last 2^20 live rays for the hall and the rooms of up to 9000 triangles)."""
     emitter = np.array([[[4.0, 3.0, 1.0], [5.0, 3.0, 1.0], [4.0, 4.0, 1.2]]], np.float32)
     cases = [(scenes.shoebox(stereo=True, spec=0.5), dict(n_rays=30_000, max_bounces=999, cutoff=1e-8, n_bins=88200)),
              (scenes.shoebox(spec=0.2), dict(n_rays=20_000, max_bounces=999, cutoff=1e-8, n_bins=88200,

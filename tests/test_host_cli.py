@@ -188,3 +188,35 @@ def test_calc_t60_matches_python_front_end(tmp_path, radius, mesh):
     V, S = 320.0, 304.0
     A = (1.0 - s["materials"][0][0][1]) * 80.0
     assert abs(vals["T60_sabine"] - 0.1611 * V / (A + 4 * 0.02 * V)) < 1e-4
+
+
+@pytest.mark.gpu
+def test_render_grows_the_response_past_three_seconds(tmp_path):
+    """A live room (absorption 0.01 per wall and band, air 0.999 per metre:
+    T60 about 10 s) keeps contributions above the 1e-8 cut-off for far
+    longer than the 3 s buffer the reference starts with; its buffers grow on
+    such writes (src/Recorder.cpp:52-59), and so does `EAR render`: the
+    responses equal those of a render given 20 s from the start, and run past
+    3 s.  SET growir 0 keeps the 3 s buffer (the late ramps are cut)."""
+    s = scenes.shoebox(spec=None, alpha_range=(0.01, 0.01))
+    write_impulse_wav(tmp_path / "dry.wav")
+
+    def responses(name, **extra):
+        debug = tmp_path / name
+        debug.mkdir()
+        p = tmp_path / f"{name}.EAR"
+        earfile.scene_to_ear(s, p, samples=20_000, seed=11, maxbounces=999, cutoff=1e-8,
+                             absorption=(0.001, 0.001, 0.001), wav=str(tmp_path / "dry.wav"), debugdir=str(debug),
+                             noprocessing=1, **extra)
+        rc, out = run("render", str(p))
+        assert rc == 0, out
+        return [np.fromfile(debug / f"response-0.sound-0.band-{b}{n}.bin", "<f4")
+                for b, n in enumerate(("low", "mid", "high"))]
+
+    grown = responses("grown")
+    wide = responses("wide", irlength=20.0)
+    kept = responses("kept", growir=0)
+    for b in range(3):
+        np.testing.assert_array_equal(grown[b], wide[b])
+        assert len(grown[b]) - 1 > 3 * 44100  # real_length past the 3 s buffer
+        assert len(kept[b]) - 1 <= 3 * 44100

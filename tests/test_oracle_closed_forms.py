@@ -192,3 +192,57 @@ def test_reference_fp32_accumulation_is_within_the_gate(oracle):
                       max_bounces=50, cutoff=1e-6, n_bins=88200, want_f32=True)
     err = np.abs(r["hist_f32"].astype(np.float64) - r["hist_signed"])
     assert np.all(err <= 1e-5 * r["hist_abs"] + 1e-30)
+
+
+@pytest.mark.parametrize("side", [1.0, -1.0])
+def test_stereo_nee_splat_matches_image_source(oracle, side):
+    """The stereo recorder's NEE splat (StereoRecorder::Record,
+    src/StereoRecorder.cpp:97-130) on a mirror plane (spec = 1, refl = 1,
+    air = 1): the connections that carry the energy leave the plane near the
+    specular point, so their direction is the image source's, fixed by the
+    geometry alone.  With dot = dir . right_ear of that direction:
+      * channel 0 (tracks[0], the left track StereoRecorder::Save writes) peaks
+        at (t + dot h / 343) 44100 and channel 1 at (t - dot h / 343) 44100, t
+        the image-source delay: the peaks are 2 h |dot| / 343 44100 bins
+        apart, the later one on the side the sign of dot names;
+      * the channel with dot's sign (left for dot >= 0) is scaled by
+        head_abs^(2 h |dot|), so the ratio of the channels' totals is that
+        factor (the lobe's spread of directions moves it by well under 1%).
+    A channel swap, a reversed ITD sign, a missing square on the ILD factor or
+    the incoming instead of the outgoing direction each fail one check."""
+    sc = oracle.OracleScene(plane(), np.zeros(2, np.uint16), [([1.0] * 3, [0.0] * 3, [1.0] * 3)])
+    h, band = 0.17, 1
+    habs = scenes.EXPORTER_HEAD_ABS
+    S = np.array([0.0, -3.0 * side, 2.0])
+    R = np.array([0.0, 3.0 * side, 1.5])
+    rec = dict(kind=1, pos=tuple(R), right_ear=(0.0, 1.0, 0.0), head_size=h, head_abs=habs)
+    img = S * np.array([1.0, 1.0, -1.0])
+    dist = float(np.linalg.norm(R - img))
+    dot = float((R - img)[1] / dist)  # propagation direction (towards R) . right_ear
+    n_bins = 4410
+    r = oracle.render(sc, band=band, air=1.0, recorders=[rec], n_rays=1_000_000, src_pos=tuple(S), max_bounces=3,
+                      n_bins=n_bins, n_threads=8, want_abs=False)
+    hist = r["hist_signed"]
+    assert hist.shape == (2, n_bins)
+    t = dist / 343.0
+    td = h / 343.0
+    peak = [int(np.argmax(np.abs(hist[c]))) for c in (0, 1)]
+    want = [(t + dot * td) * 44100.0, (t - dot * td) * 44100.0]
+    for c in (0, 1):
+        assert abs(peak[c] - want[c]) <= 1.5, (c, peak, want)
+    assert abs((peak[0] - peak[1]) - 2 * dot * td * 44100.0) <= 1.5
+    f2 = habs[band] ** (2 * h * abs(dot))
+    tot = hist.sum(axis=1)
+    assert tot[0] < 0 and tot[1] < 0  # one reflection: the sign of k = 1
+    far, near = (0, 1) if dot >= 0 else (1, 0)
+    ratio = tot[far] / tot[near]
+    assert abs(ratio / f2 - 1) < 0.01, (ratio, f2)
+    # the energy itself: the image source, 1 / (4 pi |S'R|^2) per ray, on the
+    # near channel (whose ramps are not scaled)
+    n_rays = 1_000_000
+    ramp_total = tot[near] / n_rays  # sum of A (w + 1) / 2 per ray, A = 2 c / sqrt(d)
+    w = math.ceil(math.sqrt(dist))
+    c_est = ramp_total / ((w + 1) / 2) * math.sqrt(dist) / 2
+    expect = -1.0 / (4 * math.pi * dist ** 2)
+    assert abs(c_est / expect - 1) < 0.2, (c_est, expect)
+
