@@ -30,6 +30,7 @@ EXPORTS = [
     "ear_abi_version", "ear_scene_upload", "ear_scene_destroy", "ear_scene_get_info", "ear_query_closest",
     "ear_query_occluded", "ear_render", "ear_render_batch", "ear_ctx_set_tuning", "ear_ctx_get_tuning",
     "ear_render_backward", "ear_convolve", "ear_render_backward_batch", "ear_scene_set_materials",
+    "ear_render_noalloc",
 ]
 
 
@@ -135,6 +136,7 @@ def load(path: str | None = None):
     L.ear_query_closest.argtypes = [vp, C.c_uint32, fp, fp, fp, fp]
     L.ear_query_occluded.argtypes = [vp, C.c_uint32, fp, fp, fp]
     L.ear_render.argtypes = [vp, vp, C.POINTER(ear_render_desc), C.POINTER(ear_hist), C.POINTER(ear_stats)]
+    L.ear_render_noalloc.argtypes = [vp, vp, C.POINTER(ear_render_desc), C.POINTER(ear_hist), C.POINTER(ear_stats)]
     L.ear_render_batch.argtypes = [vp, C.POINTER(ear_batch_item), C.c_uint32, C.POINTER(ear_stats)]
     L.ear_render_backward.argtypes = [vp, vp, C.POINTER(ear_render_desc), vp, vp, vp, vp]
     L.ear_convolve.argtypes = [vp, fp, C.c_uint32, fp, C.c_uint32, fp, C.c_uint32, fp]

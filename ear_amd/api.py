@@ -278,6 +278,21 @@ def render(ctx: Context, scene: Scene, **kw) -> RenderResult:
     return p.result({k: getattr(st, k) for k, _ in F.ear_stats._fields_}, wall)
 
 
+def render_noalloc(ctx: Context, scene: Scene, **kw) -> RenderResult:
+    """One render without device allocation (ear_render_noalloc): the same
+    arguments and results as render, every device buffer carved from the
+    library's fixed per-GPU scratch region."""
+    import time
+
+    p = _Prepared(**kw)
+    st = F.ear_stats()
+    t0 = time.perf_counter()
+    _check(ctx.lib, ctx.lib.ear_render_noalloc(ctx.h, scene.h, C.byref(p.desc), C.byref(p.hist), C.byref(st)),
+           ctx.h)
+    wall = time.perf_counter() - t0
+    return p.result({k: getattr(st, k) for k, _ in F.ear_stats._fields_}, wall)
+
+
 def render_batch(ctx: Context, items) -> tuple[list, dict, float]:
     """Several renders in one wavefront (ear_render_batch).  items: [(scene,
     kwargs of render)].  Returns (results, aggregated stats, wall seconds)."""

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -13,6 +14,38 @@
 #include "ear_internal.hpp"
 
 namespace ear {
+
+// The no-allocation render's device memory (ear_render_noalloc): one fixed
+// region per GPU, held from the library's first use of that GPU until the
+// process ends.  In host code `scratch` is only a stand-in for the symbol;
+// its device address on each GPU is looked up once, with that GPU current,
+// when the first context on it is created (hipGetSymbolAddress), and only
+// that address is ever passed on.
+static __device__ __align__(256) unsigned char scratch[EAR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct ScratchSlot {
+    std::once_flag once;
+    void* ptr = nullptr;          // device address of `scratch` on this GPU
+    hipError_t err = hipSuccess;  // the lookup's result
+    std::mutex busy;              // held for the whole of a no-allocation render
+};
+ScratchSlot g_scratch[kMaxDevices];
+
+// Called with `device` current.
+void lookup_scratch(int device) {
+    ScratchSlot& s = g_scratch[device];
+    std::call_once(s.once, [&] {
+        void* p = nullptr;
+        s.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (s.err == hipSuccess)
+            s.ptr = p;
+        else
+            (void)hipGetLastError();
+    });
+}
+}  // namespace
 
 ear_status hip_status(Ctx* c, hipError_t e, const char* what) {
     if (c) c->last_error = std::string(what) + ": " + hipGetErrorString(e);
@@ -86,6 +119,7 @@ ear_status ear_ctx_create(int device, ear_ctx** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    if (device < kMaxDevices) lookup_scratch(device);
     *out = c;
     return EAR_OK;
 }
@@ -200,6 +234,42 @@ ear_status ear_render(ear_ctx* c, ear_scene* s, const ear_render_desc* d, ear_hi
     if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
     try {
         return run_render(c, &s->ds, d, out, stats);
+    } catch (const std::bad_alloc&) {
+        c->last_error = "host allocation failed";
+        return EAR_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        c->last_error = "internal exception";
+        return EAR_ERR_INTERNAL;
+    }
+}
+
+ear_status ear_render_noalloc(ear_ctx* c, ear_scene* s, const ear_render_desc* d, ear_hist* out,
+                              ear_stats* stats) {
+    if (!c || !s || !d) return EAR_ERR_INVALID_ARGUMENT;
+    if (s->ctx != c) {
+        c->last_error = "scene belongs to another context";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    if (c->device >= kMaxDevices) {
+        c->last_error = "no scratch region for device ordinals >= 64";
+        return EAR_ERR_CAPACITY;
+    }
+    ScratchSlot& slot = g_scratch[c->device];
+    if (!slot.ptr) {
+        c->last_error = std::string("scratch region lookup (hipGetSymbolAddress): ") + hipGetErrorString(slot.err);
+        return EAR_ERR_HIP;
+    }
+    if (hipSetDevice(c->device) != hipSuccess) return EAR_ERR_NO_DEVICE;
+    try {
+        // one region per GPU: renders on it take turns
+        std::lock_guard<std::mutex> lock(slot.busy);
+        Arena arena;
+        arena.base = (char*)slot.ptr;
+        arena.size = EAR_NOALLOC_SCRATCH_BYTES;
+        DeviceScene* sc[1] = {&s->ds};
+        const ear_render_desc* dd[1] = {d};
+        ear_hist* oo[1] = {out};
+        return run_batch(c, 1, sc, dd, oo, stats, &arena);
     } catch (const std::bad_alloc&) {
         c->last_error = "host allocation failed";
         return EAR_ERR_OUT_OF_MEMORY;

@@ -132,6 +132,17 @@ struct Ctx {
     ear_tuning tuning{};
 };
 
+// A fixed device region that a render's buffers are carved from instead of
+// allocated (ear_render_noalloc): bump allocation in 256-byte steps, nothing
+// is freed.  `size` bytes at `base`, `used` of them taken so far.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+    size_t left() const { return size - used; }
+};
+constexpr size_t kArenaAlign = 256;
+
 // memory helpers honouring the context's cap
 ear_status dev_alloc(Ctx* c, void** p, size_t bytes);
 void dev_free(Ctx* c, void* p, size_t bytes);
@@ -153,8 +164,10 @@ ear_status run_backward(Ctx* c, DeviceScene* s, const ear_render_desc* d, const 
 ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes,
                               const ear_render_desc* const* descs, const double* const* adj_value,
                               const double* const* adj_abs, double* const* grad_refl, double* const* grad_spec);
+// arena != nullptr: every device buffer comes from the arena (no allocation);
+// the ray queues shrink to what it holds, EAR_ERR_CAPACITY if too little.
 ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const ear_render_desc* const* descs,
-                     ear_hist* const* outs, ear_stats* st);
+                     ear_hist* const* outs, ear_stats* st, Arena* arena = nullptr);
 ear_status run_query_closest(Ctx* c, DeviceScene* s, uint32_t n, const float* o, const float* d, float* t,
                              int32_t* tri);
 ear_status run_query_occluded(Ctx* c, DeviceScene* s, uint32_t n, const float* p, const float* q,
@@ -166,23 +179,38 @@ ear_status run_query_occluded(Ctx* c, DeviceScene* s, uint32_t n, const float* p
         if (_e != hipSuccess) return ::ear::hip_status(ctx, _e, #expr); \
     } while (0)
 
-// A device allocation owned by a scope (through dev_alloc / dev_free).
+// A device allocation owned by a scope (through dev_alloc / dev_free), or,
+// with an arena, a piece of the arena (never freed: the arena's owner reuses
+// the whole region on its next call).
 struct DevBuf {
     Ctx* c;
+    Arena* arena = nullptr;
     void* p = nullptr;
     size_t n = 0;
-    explicit DevBuf(Ctx* cc) : c(cc) {}
+    explicit DevBuf(Ctx* cc, Arena* a = nullptr) : c(cc), arena(a) {}
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) dev_free(c, p, n);
+        if (p && !arena) dev_free(c, p, n);
         p = nullptr;
         n = 0;
     }
     ear_status alloc(size_t bytes) {
         release();
         n = bytes ? bytes : 16;
+        if (arena) {
+            const size_t take = (n + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+            if (take > arena->left()) {
+                c->last_error = "render needs more device memory than the no-allocation scratch region holds (" +
+                                std::to_string(arena->size >> 20) + " MiB)";
+                n = 0;
+                return EAR_ERR_CAPACITY;
+            }
+            p = arena->base + arena->used;
+            arena->used += take;
+            return EAR_OK;
+        }
         return dev_alloc(c, &p, n);
     }
     template <class T>

@@ -37,6 +37,11 @@ constexpr int kBlock = 256;
 constexpr uint32_t kMaxBounces = 0xFFFFu;  // the bounce number's field in a queue entry (bits 8..23)
 constexpr int kLdsMin = 16;  // smallest LDS part of the traversal stack; the overflow area covers the rest
 constexpr int kShadeBlock = 256;  // k_shade: its block-wide appends take one counter atomic per block step
+// no-allocation render: traversal grids of at most this many waves per CU
+// (their overflow stacks come out of the fixed region), and the fewest rays
+// in flight it accepts
+constexpr uint32_t kNoallocWaves = 16;
+constexpr size_t kNoallocMinRays = 4096;
 
 enum StatIdx {
     ST_SEGMENTS = 0,
@@ -1227,6 +1232,11 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
     }
 }
 
+// p[0, n) = v, grid-stride (clears device memory without a memset node)
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
 __global__ void k_reset2(uint32_t* a, uint32_t* b) {
     if (threadIdx.x == 0) {
         *a = 0;
@@ -1692,8 +1702,11 @@ ear_status run_query_occluded(Ctx* c, DeviceScene* s, uint32_t n, const float* p
 // share the queues; per-instance scene data, parameters and histograms come
 // from the device instance table.  A plain render is a batch of one.
 ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const ear_render_desc* const* descs,
-                     ear_hist* const* outs, ear_stats* st_out) {
+                     ear_hist* const* outs, ear_stats* st_out, Arena* arena) {
     if (n_items == 0) return EAR_ERR_INVALID_ARGUMENT;
+    // no-allocation render: every buffer is carved from the arena, device
+    // memory is cleared by a kernel, no events, pinned memory or graphs
+    const bool noalloc = arena != nullptr;
     const bool batch = n_items > 1;
     const int width = (int)scenes[0]->width;
     std::vector<InstDev> inst(n_items);
@@ -1777,7 +1790,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const ear_tuning& tu = c->tuning;
     uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 23);
     uint64_t cap64 = std::min<uint64_t>((uint64_t)chunk * 3, std::max<uint64_t>(1, total_rays));
-    const uint32_t cap_rays = (uint32_t)cap64;
+    uint32_t cap_rays = (uint32_t)cap64;
     // walks vary in length; refilling idle lanes keeps waves busy (both
     // kernels by default)
     const bool fetch_e = tu.traversal != EAR_TRAVERSAL_STATIC;
@@ -1789,48 +1802,72 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     pick_kernels(cfg_e, &fe, &fs_unused);
     pick_kernels(cfg_s, &fe_unused, &fs);
     auto cap_for = [&](const void* fn, int block) -> uint32_t {
-        if (tu.waves_per_cu) return (uint32_t)c->num_cus * tu.waves_per_cu / std::max(1, block / 64);
-        int nb = resident_blocks(fn, block);
-        return (uint32_t)c->num_cus * (uint32_t)std::max(1, nb);
+        uint32_t cap;
+        if (tu.waves_per_cu) {
+            cap = (uint32_t)c->num_cus * tu.waves_per_cu / std::max(1, block / 64);
+        } else {
+            int nb = resident_blocks(fn, block);
+            cap = (uint32_t)c->num_cus * (uint32_t)std::max(1, nb);
+        }
+        // the no-allocation render keeps its overflow stacks small: at most
+        // kNoallocWaves waves per CU in the traversal grids
+        if (noalloc) cap = std::min<uint32_t>(cap, (uint32_t)c->num_cus * kNoallocWaves / std::max(1, block / 64));
+        return std::max<uint32_t>(1, cap);
     };
     const uint32_t cap_e = std::max<uint32_t>(1, cap_for((const void*)fe, cfg_e.block));
     const uint32_t cap_s = std::max<uint32_t>(1, cap_for((const void*)fs, cfg_s.block));
     const uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 32;  // non-traversal kernels
 
     hipStream_t st = c->stream;
-    DevBuf qo[2] = {DevBuf(c), DevBuf(c)}, qd[2] = {DevBuf(c), DevBuf(c)}, qm[2] = {DevBuf(c), DevBuf(c)};
-    DevBuf hits(c), so(c), sm(c), counters(c), stats(c), hist(c), gs(c), direct(c), value(c), logh(c), logn(c),
-        dinst(c), dsegs(c), dsegoff(c), drows(c), dvrows(c), msrc(c);
-    for (int k = 0; k < 2; ++k) {
-        TRY(qo[k].alloc((size_t)cap_rays * 16));
-        TRY(qd[k].alloc((size_t)cap_rays * 16));
-        TRY(qm[k].alloc((size_t)cap_rays * 16));
-    }
-    TRY(hits.alloc((size_t)cap_rays * 8));
-    // a queue's shading emits <= cap * recorders requests; mesh sources add
-    // their k = 0 connections of the rays generated in the same iteration
-    size_t cap_sh = (size_t)cap_rays * std::max<size_t>(1, max_recs) * (any_mesh ? 2 : 1);
-    TRY(so.alloc(cap_sh * 16));
-    TRY(sm.alloc(cap_sh * 16));
+    Arena* ar = arena;
+    DevBuf qo[2] = {DevBuf(c, ar), DevBuf(c, ar)}, qd[2] = {DevBuf(c, ar), DevBuf(c, ar)},
+           qm[2] = {DevBuf(c, ar), DevBuf(c, ar)};
+    DevBuf hits(c, ar), so(c, ar), sm(c, ar), counters(c, ar), stats(c, ar), hist(c, ar), gs(c, ar), direct(c, ar),
+        value(c, ar), logh(c, ar), logn(c, ar), dinst(c, ar), dsegs(c, ar), dsegoff(c, ar), drows(c, ar),
+        dvrows(c, ar), msrc(c, ar), segc(c, ar);
+    // device memory that must start at a value: hipMemsetAsync, or in the
+    // no-allocation render a fill kernel (no memset nodes if a caller
+    // captures the stream)
+    auto clear = [&](DevBuf& b, uint8_t byte) -> ear_status {
+        if (!noalloc) {
+            EAR_HIP_TRY(c, hipMemsetAsync(b.p, byte, b.n, st));
+            return EAR_OK;
+        }
+        const size_t words = b.n / 4;  // every cleared buffer is a whole number of words
+        const uint32_t w = 0x01010101u * byte;
+        k_fill_u32<<<persistent_grid(c, (uint32_t)std::min<size_t>(words, 0xFFFFFFFFu), 8), kBlock, 0, st>>>(
+            b.as<uint32_t>(), words, w);
+        EAR_HIP_TRY(c, hipGetLastError());
+        return EAR_OK;
+    };
     // [0] queue 0, [1] queue 1, [2] shadow queue, [3] generation cursor,
     // [6..7] plan {first, count}
     TRY(counters.alloc(16 * 4));
     // refill counters of the closest-hit and shadow kernels, one per line
-    DevBuf segc(c);
     TRY(segc.alloc(2 * kMaxSeg * kSegStride * 4));
-    EAR_HIP_TRY(c, hipMemsetAsync(segc.p, 0, segc.n, st));
+    TRY(clear(segc, 0));
     TRY(stats.alloc(ST_N * 8 * kStatCopies));
     TRY(hist.alloc(hist_cells * 16));
     uint32_t gthreads = std::max(cap_e * (uint32_t)cfg_e.block, cap_s * (uint32_t)cfg_s.block);
     gthreads = std::max<uint32_t>(gthreads, n_items * kBlock);
-    TRY(gs.alloc((size_t)(kStackDepth - kLdsMin) * gthreads * 4));
+    // overflow stack entries per persistent thread: the full depth, or in the
+    // no-allocation render the bound of the scenes' own trees (a 4-wide node
+    // visit pushes at most 3 entries, a 2-wide one at most 1, per level)
+    size_t gs_entries = kStackDepth - kLdsMin;
+    if (noalloc) {
+        uint32_t need = 1;
+        for (uint32_t i = 0; i < n_items; ++i)
+            need = std::max(need, scenes[i]->width == 4 ? 3 * scenes[i]->depth4 + 1 : scenes[i]->max_depth);
+        gs_entries = need > (uint32_t)kLdsMin ? need - kLdsMin : 1;
+    }
+    TRY(gs.alloc(gs_entries * gthreads * 4));
     TRY(direct.alloc(direct_cells * sizeof(DirectRamp)));
     if (any_value) TRY(value.alloc(value_cells * 8));
     if (any_log) {
         TRY(logh.alloc(log_hits * 4));
         TRY(logn.alloc(log_nseg * 4));
-        EAR_HIP_TRY(c, hipMemsetAsync(logh.p, 0xFE, logh.n, st));  // 0xFEFEFEFE read back as -2
-        EAR_HIP_TRY(c, hipMemsetAsync(logn.p, 0, logn.n, st));
+        TRY(clear(logh, 0xFE));  // 0xFEFEFEFE read back as -2
+        TRY(clear(logn, 0));
     }
     // mesh-source tables, one allocation
     size_t msrc_floats = 0;
@@ -1858,9 +1895,9 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     }
     TRY(dinst.alloc(sizeof(InstDev) * n_items));
     EAR_HIP_TRY(c, hipMemcpyAsync(dinst.p, inst.data(), sizeof(InstDev) * n_items, hipMemcpyHostToDevice, st));
-    EAR_HIP_TRY(c, hipMemsetAsync(hist.p, 0, hist.n, st));
-    EAR_HIP_TRY(c, hipMemsetAsync(stats.p, 0, stats.n, st));
-    EAR_HIP_TRY(c, hipMemsetAsync(counters.p, 0, counters.n, st));
+    TRY(clear(hist, 0));
+    TRY(clear(stats, 0));
+    TRY(clear(counters, 0));
 
     // every (instance, band) context as a run of rays, cut into chunks
     std::vector<GenSeg> all;
@@ -1869,20 +1906,67 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
             if ((descs[i]->band_mask >> b) & 1u && descs[i]->n_rays) all.push_back(GenSeg{i, (uint32_t)b, 0, descs[i]->n_rays});
     TRY(dsegs.alloc(sizeof(GenSeg) * std::max<size_t>(1, all.size())));
     TRY(dsegoff.alloc(4 * std::max<size_t>(1, all.size())));
+    // the histogram rows of the finishing kernels: one per (instance, band,
+    // channel, kind) for the prefix sums, one per (instance, band, channel)
+    // for the combination step
+    std::vector<uint4> rows, vrows;
+    uint32_t max_nb = 1;
+    for (uint32_t i = 0; i < n_items; ++i)
+        for (uint32_t b = 0; b < 3; ++b)
+            if ((descs[i]->band_mask >> b) & 1u)
+                for (uint32_t ch = 0; ch < inst[i].P.n_channels; ++ch)
+                    for (uint32_t kind = 0; kind < inst[i].P.hist_kinds; ++kind) rows.push_back(make_uint4(i, b, ch, kind));
+    for (const uint4& r : rows)
+        if (r.w == 0) {
+            vrows.push_back(r);
+            max_nb = std::max(max_nb, inst[r.x].P.n_bins);
+        }
+    TRY(drows.alloc(sizeof(uint4) * rows.size()));
+    TRY(dvrows.alloc(sizeof(uint4) * vrows.size()));
+
+    // the ray queues, their hit records and the shadow queue; a queue's
+    // shading emits <= cap * recorders requests, and mesh sources add their
+    // k = 0 connections of the rays generated in the same iteration
+    const size_t sh_per_ray = std::max<size_t>(1, max_recs) * (any_mesh ? 2 : 1);
+    if (noalloc) {
+        // as many rays in flight as the rest of the region holds
+        const size_t per_ray = 6 * 16 + 8 + sh_per_ray * 32;
+        const size_t slack = 9 * kArenaAlign;
+        const size_t fit = arena->left() > slack ? (arena->left() - slack) / per_ray : 0;
+        if (fit < kNoallocMinRays) {
+            c->last_error = "render needs more device memory than the no-allocation scratch region holds (" +
+                            std::to_string(arena->size >> 20) + " MiB): histograms, stacks and tables take " +
+                            std::to_string(arena->used >> 20) + " MiB";
+            return EAR_ERR_CAPACITY;
+        }
+        cap_rays = (uint32_t)std::min<size_t>(cap_rays, fit / 64 * 64);
+    }
+    for (int k = 0; k < 2; ++k) {
+        TRY(qo[k].alloc((size_t)cap_rays * 16));
+        TRY(qd[k].alloc((size_t)cap_rays * 16));
+        TRY(qm[k].alloc((size_t)cap_rays * 16));
+    }
+    TRY(hits.alloc((size_t)cap_rays * 8));
+    const size_t cap_sh = (size_t)cap_rays * sh_per_ray;
+    TRY(so.alloc(cap_sh * 16));
+    TRY(sm.alloc(cap_sh * 16));
 
     uint32_t* cnt = counters.as<uint32_t>();  // [0],[1] ray queues, [2] shadow queue
     unsigned long long* stp = stats.as<unsigned long long>();
     const InstDev* di = dinst.as<InstDev>();
     int* gsp = gs.as<int>();
 
-    uint32_t* pinned = nullptr;
-    EAR_HIP_TRY(c, hipHostMalloc((void**)&pinned, 16, hipHostMallocDefault));
+    // the queue counters read by the host: pinned memory, or in the
+    // no-allocation render a host array (a synchronous copy)
+    uint32_t host_cnt[4] = {0, 0, 0, 0};
+    uint32_t* pinned = host_cnt;
+    if (!noalloc) EAR_HIP_TRY(c, hipHostMalloc((void**)&pinned, 16, hipHostMallocDefault));
     struct PinGuard {
         uint32_t* p;
         ~PinGuard() {
             if (p) hipHostFree(p);
         }
-    } pg{pinned};
+    } pg{noalloc ? nullptr : pinned};
 
     std::vector<EventPair> ev_ext, ev_sha, ev_shw;
     auto mk = [&](std::vector<EventPair>& v) -> EventPair& {
@@ -1891,10 +1975,12 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         hipEventCreate(&v.back().b);
         return v.back();
     };
-    hipEvent_t t0, t1;
-    EAR_HIP_TRY(c, hipEventCreate(&t0));
-    EAR_HIP_TRY(c, hipEventCreate(&t1));
-    EAR_HIP_TRY(c, hipEventRecord(t0, st));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (!noalloc) {
+        EAR_HIP_TRY(c, hipEventCreate(&t0));
+        EAR_HIP_TRY(c, hipEventCreate(&t1));
+        EAR_HIP_TRY(c, hipEventRecord(t0, st));
+    }
 
     // The bounce loop is device driven: every iteration tops the current
     // queue up with new rays (k_plan + k_generate), extends and shades it
@@ -1925,7 +2011,10 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const uint32_t g_shade = persistent_grid(c, cap_rays, waves, kShadeBlock);
     const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
     const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
-    const bool timed = tu.graph_iters == 0;
+    // eager: the bounce loop launched from the host (always in the
+    // no-allocation render); timed: with per-kernel events
+    const bool eager = tu.graph_iters == 0 || noalloc;
+    const bool timed = eager && !noalloc;
     const uint32_t refill = tu.refill ? tu.refill : 24u;
     const uint32_t fchunk = tu.fetch_chunk;  // 0: one entry per idle lane
     const uint32_t nseg = tu.fetch_segments ? std::min<uint32_t>(tu.fetch_segments, kMaxSeg) : 64u;
@@ -2001,7 +2090,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const uint32_t max_iters = max_bounces + 2 + (uint32_t)((total_rays + cap_rays - 1) / cap_rays) * (max_bounces + 1);
     int cur = 0;
     bool done = false;
-    if (timed) {
+    if (eager) {
         const uint32_t every = 4;
         while (!done && status == EAR_OK) {
             for (uint32_t k = 0; k < every; ++k) {
@@ -2048,30 +2137,12 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         k_direct<4><<<n_items, kBlock, 0, st>>>(di, gsp, gthreads, stp);
     else
         k_direct<2><<<n_items, kBlock, 0, st>>>(di, gsp, gthreads, stp);
-    std::vector<uint4> rows;
-    for (uint32_t i = 0; i < n_items; ++i)
-        for (uint32_t b = 0; b < 3; ++b)
-            if ((descs[i]->band_mask >> b) & 1u)
-                for (uint32_t ch = 0; ch < inst[i].P.n_channels; ++ch)
-                    for (uint32_t kind = 0; kind < inst[i].P.hist_kinds; ++kind) rows.push_back(make_uint4(i, b, ch, kind));
-    TRY(drows.alloc(sizeof(uint4) * rows.size()));
     EAR_HIP_TRY(c, hipMemcpyAsync(drows.p, rows.data(), sizeof(uint4) * rows.size(), hipMemcpyHostToDevice, st));
     k_finalize<<<(uint32_t)rows.size(), kFinBlock, 0, st>>>(di, drows.as<uint4>());
-    {
-        // one row per (instance, band, channel) for the combination step
-        std::vector<uint4> vrows;
-        uint32_t max_nb = 1;
-        for (const uint4& r : rows)
-            if (r.w == 0) {
-                vrows.push_back(r);
-                max_nb = std::max(max_nb, inst[r.x].P.n_bins);
-            }
-        TRY(dvrows.alloc(sizeof(uint4) * vrows.size()));
-        EAR_HIP_TRY(c, hipMemcpyAsync(dvrows.p, vrows.data(), sizeof(uint4) * vrows.size(), hipMemcpyHostToDevice, st));
-        k_values<<<dim3((max_nb + 255) / 256, (uint32_t)vrows.size()), 256, 0, st>>>(di, dvrows.as<uint4>());
-    }
+    EAR_HIP_TRY(c, hipMemcpyAsync(dvrows.p, vrows.data(), sizeof(uint4) * vrows.size(), hipMemcpyHostToDevice, st));
+    k_values<<<dim3((max_nb + 255) / 256, (uint32_t)vrows.size()), 256, 0, st>>>(di, dvrows.as<uint4>());
     EAR_HIP_TRY(c, hipGetLastError());
-    EAR_HIP_TRY(c, hipEventRecord(t1, st));
+    if (t1) EAR_HIP_TRY(c, hipEventRecord(t1, st));
     EAR_HIP_TRY(c, hipStreamSynchronize(st));
 
     // copy out [band slot][channel][bin] rows of every rendered band
@@ -2143,7 +2214,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         st_out->bounces_launched = bounces;
         st_out->launches = launches;
         float ms = 0;
-        hipEventElapsedTime(&ms, t0, t1);
+        if (t0 && t1) hipEventElapsedTime(&ms, t0, t1);
         st_out->trace_ms = ms;
         auto sum = [](std::vector<EventPair>& v) {
             double a = 0;
@@ -2169,8 +2240,8 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
             hipEventDestroy(e.a);
             hipEventDestroy(e.b);
         }
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
+    if (t0) hipEventDestroy(t0);
+    if (t1) hipEventDestroy(t1);
     return EAR_OK;
 }
 

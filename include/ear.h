@@ -249,6 +249,31 @@ typedef struct {
 ear_status ear_render(ear_ctx* ctx, ear_scene* scene, const ear_render_desc* desc, ear_hist* out,
                       ear_stats* stats);
 
+/* ear_render without any device allocation.  No counterpart in the
+ * reference (its Scene::Render allocates freely): it is for callers that
+ * cannot allocate device memory, such as code under HIP graph capture, tight
+ * serving loops and harnesses that forbid hipMalloc.  Same arguments and
+ * same results as ear_render (bitwise identical histograms, paths and
+ * counts); the caller still passes the scene, the description and the host
+ * output buffers.  Every device buffer of the render (ray and shadow queues,
+ * hit records, traversal stacks, histogram accumulators, tables, counters)
+ * is carved from one fixed region of EAR_NOALLOC_SCRATCH_BYTES per GPU,
+ * held from the first ear_ctx_create on that GPU until the process ends; no
+ * hipMalloc, hipFree, pinned-memory, event or graph call is made, and device
+ * memory is cleared by a kernel, not a memset.  The ray queues shrink to
+ * what the region leaves after the histograms and stacks (at most as many
+ * rays in flight as ear_render keeps); a render whose histograms, stacks and
+ * tables leave room for fewer than 4096 rays in flight fails with
+ * EAR_ERR_CAPACITY before any launch (ear_ctx_last_error gives the sizes).
+ * The traversal grids are capped at 16 waves per CU.  All calls on one GPU
+ * share its region and its contents persist between calls; the library
+ * serialises them within a process (a per-GPU lock held for the whole
+ * call), so renders that must overlap use ear_render.  Timings in ear_stats
+ * (trace_ms, extend_ms, shade_ms, shadow_ms) are not measured and read 0. */
+#define EAR_NOALLOC_SCRATCH_BYTES (256u * 1024u * 1024u)
+ear_status ear_render_noalloc(ear_ctx* ctx, ear_scene* scene, const ear_render_desc* desc, ear_hist* out,
+                              ear_stats* stats);
+
 /* A batch of renders in one persistent wavefront (the reference runs its
  * SceneContexts on a thread pool, src/EAR.cpp:193-207): every item is a
  * (scene, description, output) triple; the rays of all items share the
