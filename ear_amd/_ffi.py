@@ -84,7 +84,8 @@ class ear_stats(C.Structure):
                 ("extend_ms", C.c_double), ("shade_ms", C.c_double), ("shadow_ms", C.c_double),
                 ("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32), ("extend_queries", C.c_uint64),
                 ("shadow_queries", C.c_uint64), ("dropped_ramps", C.c_uint64), ("bins_needed", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("reserved", C.c_uint32), ("shadow_contributions", C.c_uint64), ("shadow_ramps", C.c_uint64),
+                ("splat_ms", C.c_double)]
 
 
 class ear_batch_item(C.Structure):
@@ -100,10 +101,12 @@ class ear_tuning(C.Structure):
     _fields_ = [("block_extend", C.c_uint32), ("block_shade", C.c_uint32), ("block_shadow", C.c_uint32),
                 ("ray_chunk", C.c_uint32), ("waves_per_cu", C.c_uint32), ("stack_lds", C.c_uint32),
                 ("traversal", C.c_uint32), ("graph_iters", C.c_uint32), ("refill", C.c_uint32),
-                ("fetch_chunk", C.c_uint32), ("fetch_segments", C.c_uint32), ("tail_rays", C.c_uint32)]
+                ("fetch_chunk", C.c_uint32), ("fetch_segments", C.c_uint32), ("tail_rays", C.c_uint32),
+                ("splat", C.c_uint32)]
 
 
 EAR_TRAVERSAL_DEFAULT, EAR_TRAVERSAL_STATIC, EAR_TRAVERSAL_FETCH, EAR_TRAVERSAL_FETCH_ALL = 0, 1, 2, 3
+EAR_SPLAT_DEFAULT, EAR_SPLAT_ATOMIC, EAR_SPLAT_TILED = 0, 1, 2
 
 
 _lib = None

@@ -59,7 +59,7 @@ class Context:
         _check(self.lib, self.lib.ear_ctx_set_memory_cap(self.h, nbytes), self.h)
 
     def set_tuning(self, block_extend=0, block_shade=0, block_shadow=0, ray_chunk=0, waves_per_cu=0, stack_lds=0,
-                   traversal=0, graph_iters=0, refill=0, fetch_chunk=0, fetch_segments=0, tail_rays=0):
+                   traversal=0, graph_iters=0, refill=0, fetch_chunk=0, fetch_segments=0, tail_rays=0, splat=0):
         """Launch configuration (0 = library default); traversal: 1 static
         lanes, 2 closest-hit lanes refilled from the queue, 3 both kernels
         refilled (default); refill / fetch_chunk: idle lanes that trigger a
@@ -68,9 +68,11 @@ class Context:
         graph_iters > 0 replays the bounce loop as a captured HIP graph;
         tail_rays: the last <= tail_rays live rays go to one launch that
         traces each to its end (default: every ray when no scene of the call
-        has more than 64 triangles, else the last 2^20; 1 turns it off)."""
+        has more than 64 triangles, else the last 2^20; 1 turns it off);
+        splat: 1 direct 128-bit atomics per ramp point, 2 records binned by
+        time tile and accumulated in LDS (same histograms)."""
         t = F.ear_tuning(block_extend, block_shade, block_shadow, ray_chunk, waves_per_cu, stack_lds, traversal,
-                         graph_iters, refill, fetch_chunk, fetch_segments, tail_rays)
+                         graph_iters, refill, fetch_chunk, fetch_segments, tail_rays, splat)
         _check(self.lib, self.lib.ear_ctx_set_tuning(self.h, C.byref(t)), self.h)
 
     def get_tuning(self) -> dict:

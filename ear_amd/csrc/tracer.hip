@@ -56,7 +56,9 @@ enum StatIdx {
     ST_TAIL_SH = 9,   // wavefront kernels whose launch times are measured)
     ST_DROPPED = 10,  // ramps cut at the end of the histogram
     ST_BINS = 11,     // the largest ramp end among them (a maximum, not a sum)
-    ST_N = 12
+    ST_SH_CONTRIB = 12,  // connections recorded by the shadow kernels (not k_tail)
+    ST_SH_RAMPS = 13,    // and the ramps they made (one per channel)
+    ST_N = 14
 };
 
 struct Queue {
@@ -519,30 +521,84 @@ __device__ __forceinline__ void acc128(unsigned long long* cell, I128 q) {
 // carry (one atomic per point instead of two for negative values).  Returns
 // the ramp's end (one past its last bin) if bins at or past nb were cut, else
 // 0 (a start before bin 0 is the reference's own skip, src/StereoRecorder.cpp:92-95).
-__device__ uint32_t splat_ramp(unsigned long long* h, uint32_t nb, int s, int w, float A) {
-    if (w <= 0) return 0;
-    double S = (double)A / (double)w;
-    int ww = w;
+// The span of such a ramp: start s clipped to 0 (width ww = w + s then) and
+// *cut as splat_ramp returns it; false when no point lands in [0, nb).
+__device__ __forceinline__ bool ramp_span(uint32_t nb, int& s, int w, int& ww, uint32_t& cut) {
+    cut = 0;
+    if (w <= 0) return false;
+    ww = w;
     if (s < 0) {
         ww = w + s;
         s = 0;
-        if (ww <= 0) return 0;
+        if (ww <= 0) return false;
     }
     const uint32_t end = (uint32_t)s + (uint32_t)ww;
-    const uint32_t cut = end > nb ? end : 0u;
-    if ((uint32_t)s >= nb) return cut;
-    const bool neg = S < 0.0;
-    I128 sq = fixed_from_double(neg ? -S : S);
-    I128 aq = i128_mul_small(sq, (uint32_t)ww);
-    I128 p1 = i128_add(aq, sq);
-    unsigned long long* hp = neg ? h + 2 * (size_t)nb : h;  // gets +w*S and +S
-    unsigned long long* hm = neg ? h : h + 2 * (size_t)nb;  // gets (w+1)*S
-    uint32_t s0 = (uint32_t)s;
-    acc128(hp + 2 * (size_t)s0, aq);
-    if (s0 + 1 < nb) acc128(hm + 2 * (size_t)(s0 + 1), p1);
-    if (s0 + (uint32_t)ww + 1 < nb) acc128(hp + 2 * (size_t)(s0 + ww + 1), sq);
+    cut = end > nb ? end : 0u;
+    return (uint32_t)s < nb;
+}
+
+// The three points of a ramp of slope S = A / w (its original width) from
+// bin s0 (already clipped, ww bins), as magnitudes: part `hp` of the row gets
+// ww * Sq at s0 and Sq at s0 + ww + 1, part `hm` (ww + 1) * Sq at s0 + 1.
+struct RampPoints {
+    I128 sq, aq, p1;
+    bool neg;  // S < 0: the "+" and "-" parts swap
+};
+__device__ __forceinline__ RampPoints ramp_points(float A, int w, int ww) {
+    RampPoints r;
+    const double S = (double)A / (double)w;
+    r.neg = S < 0.0;
+    r.sq = fixed_from_double(r.neg ? -S : S);
+    r.aq = i128_mul_small(r.sq, (uint32_t)ww);
+    r.p1 = i128_add(r.aq, r.sq);
+    return r;
+}
+
+// The points of a clipped ramp (s0 < nb) added to row h with global atomics.
+__device__ __forceinline__ void splat_span(unsigned long long* h, uint32_t nb, uint32_t s0, int ww, int w, float A) {
+    const RampPoints q = ramp_points(A, w, ww);
+    unsigned long long* hp = q.neg ? h + 2 * (size_t)nb : h;  // gets +w*S and +S
+    unsigned long long* hm = q.neg ? h : h + 2 * (size_t)nb;  // gets (w+1)*S
+    acc128(hp + 2 * (size_t)s0, q.aq);
+    if (s0 + 1 < nb) acc128(hm + 2 * (size_t)(s0 + 1), q.p1);
+    if (s0 + (uint32_t)ww + 1 < nb) acc128(hp + 2 * (size_t)(s0 + ww + 1), q.sq);
+}
+
+__device__ uint32_t splat_ramp(unsigned long long* h, uint32_t nb, int s, int w, float A) {
+    int ww;
+    uint32_t cut;
+    if (!ramp_span(nb, s, w, ww, cut)) return cut;
+    splat_span(h, nb, (uint32_t)s, ww, w, A);
     return cut;
 }
+
+// ---------------------------------------------------------------------------
+// time-tiled splat (EAR_SPLAT_TILED)
+// ---------------------------------------------------------------------------
+// The shadow kernel does not add a visible connection's ramps to the
+// histogram itself: it writes one record per ramp, slot-indexed (the shadow
+// request's index times the records per request), keyed by (histogram row,
+// time tile of kTile bins).  Per bounce iteration a counting sort groups the
+// records by key (k_tile_count, k_tile_scan, k_tile_scatter), and
+// k_tile_splat gives each key one workgroup that adds the three points of
+// every ramp of its tile into a 128-bit accumulator tile in LDS (64-bit LDS
+// atomics with an explicit carry) and then adds that tile to the row in HBM,
+// one global 128-bit add per touched bin.  The sums are the same integers in
+// another order, so the histograms are bitwise those of the direct splat.
+constexpr uint32_t kTile = 1024;            // bins per tile
+constexpr uint32_t kHalo = 64;              // bins past its tile that a ramp starting in it may reach
+constexpr int kTileMaxW = (int)kHalo - 2;   // wider ramps are added directly
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;    // an empty record slot
+constexpr uint32_t kLdsKeys = 8192;         // keys counted in LDS (more: global atomics)
+constexpr uint32_t kTileDirect = 32;        // keys with at most this many ramps skip the LDS tile
+constexpr int kTileBlock = 256;
+
+// where the shadow kernels put their records (key == nullptr: direct splat)
+struct TileOut {
+    uint32_t* key;    // [request * stride + r]: tile key or kNoKey
+    uint2* pay;       // {A bits, offset in tile | ww << 16 | w << 24}
+    uint32_t stride;  // records per request: the most channels of one recorder
+};
 
 // Row layout of the accumulators.  Without the |c| histogram one signed row
 // per channel.  With it, contributions are split by sign: row 0 takes the
@@ -552,30 +608,75 @@ __device__ uint32_t splat_ramp(unsigned long long* h, uint32_t nb, int s, int w,
 struct Hist {
     unsigned long long* cells;  // [band slot][channel][kind][+/-][bin] x {lo, hi}
     uint32_t n_bins, n_ch, kinds;
+    uint32_t key_base, n_tiles;  // the time-tiled splat's keys of this render
+    __device__ uint32_t row_index(int band, int ch, int kind) const {
+        return ((uint32_t)band * n_ch + (uint32_t)ch) * kinds + (uint32_t)kind;
+    }
     // the row's "+" part; its "-" part follows at + 2 * n_bins
     __device__ unsigned long long* row(int band, int ch, int kind) const {
-        return cells + 4 * (size_t)n_bins * (((size_t)band * n_ch + ch) * kinds + kind);
+        return cells + 4 * (size_t)n_bins * row_index(band, ch, kind);
     }
 };
+
+// Records of one connection for the time-tiled splat: they go to
+// out.key / out.pay [base, base + out.stride); `used` so far.
+struct TileSink {
+    TileOut out;
+    uint32_t base, used;
+    // the slots this connection left empty
+    __device__ void close() {
+        for (uint32_t r = used; r < out.stride; ++r) out.key[base + r] = kNoKey;
+    }
+};
+
+// One ramp into row (band, ch, kind): splat directly, or as a record when a
+// sink is given (ramps wider than the tile halo are still splatted).
+__device__ __forceinline__ void put_ramp(const Hist& H, int band, int ch, int kind, int s, int w, float A, Drops& dr,
+                                         TileSink& sink, bool tiled) {
+    if (!tiled) {
+        dr.add(splat_ramp(H.row(band, ch, kind), H.n_bins, s, w, A));
+        return;
+    }
+    int ww;
+    uint32_t cut;
+    const bool any = ramp_span(H.n_bins, s, w, ww, cut);
+    dr.add(cut);
+    if (!any) return;
+    if (w > kTileMaxW) {
+        splat_span(H.row(band, ch, kind), H.n_bins, (uint32_t)s, ww, w, A);
+        return;
+    }
+    const uint32_t i = sink.base + sink.used++;
+    sink.out.key[i] = H.key_base + H.row_index(band, ch, kind) * H.n_tiles + (uint32_t)s / kTile;
+    sink.out.pay[i] = make_uint2(__float_as_uint(A), ((uint32_t)s % kTile) | ((uint32_t)ww << 16) | ((uint32_t)w << 24));
+}
 
 // Recorder::Record: MonoRecorder (src/MonoRecorder.cpp:83-97) or
 // StereoRecorder (src/StereoRecorder.cpp:97-130).
 // MonoRecorder::Record (src/MonoRecorder.cpp:83-97) into channel ch
-__device__ void record_channel(const Hist& H, int band, int ch, float a, float t, float dist, Drops& dr) {
+__device__ __forceinline__ void record_channel(const Hist& H, int band, int ch, float a, float t, float dist,
+                                               Drops& dr, TileSink& sink, bool tiled) {
     const bool split = H.kinds > 1;
     const uint32_t kind = split && a < 0.0f ? 1u : 0u;
     int s = (int)((double)t * 44100.0);
     float width = sqrtf(dist);
     float ampl = 2.0f * a / width;
     int w = (int)ceilf(width);
-    dr.add(splat_ramp(H.row(band, ch, kind), H.n_bins, s, w, split ? fabsf(ampl) : ampl));
+    put_ramp(H, band, ch, (int)kind, s, w, split ? fabsf(ampl) : ampl, dr, sink, tiled);
+}
+__device__ void record_channel(const Hist& H, int band, int ch, float a, float t, float dist, Drops& dr) {
+    TileSink none{};
+    record_channel(H, band, ch, a, t, dist, dr, none, false);
 }
 
-__device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, float a, float t, float dist, Drops& dr) {
+// Returns the number of ramps (channels) of the connection.
+__device__ __forceinline__ uint32_t record(const Hist& H, int band, const RecParams& R, V3 dir, float a, float t,
+                                           float dist, Drops& dr, TileSink& sink, bool tiled) {
     const bool split = H.kinds > 1;
     const uint32_t kind = split && a < 0.0f ? 1u : 0u;
     if (R.kind == EAR_REC_MONO) {
-        record_channel(H, band, R.channel, a, t, dist, dr);
+        record_channel(H, band, R.channel, a, t, dist, dr, sink, tiled);
+        return 1;
     } else {
         float dt = dot(dir, V3{R.rx, R.ry, R.rz});
         float td = R.head_size / 343.0f;
@@ -590,9 +691,15 @@ __device__ void record(const Hist& H, int band, const RecParams& R, V3 dir, floa
         else
             al *= factor * factor;
         int w = (int)ceilf(width);
-        dr.add(splat_ramp(H.row(band, R.channel, kind), H.n_bins, s_left, w, split ? fabsf(al) : al));
-        dr.add(splat_ramp(H.row(band, R.channel + 1, kind), H.n_bins, s_right, w, split ? fabsf(ar) : ar));
+        put_ramp(H, band, R.channel, (int)kind, s_left, w, split ? fabsf(al) : al, dr, sink, tiled);
+        put_ramp(H, band, R.channel + 1, (int)kind, s_right, w, split ? fabsf(ar) : ar, dr, sink, tiled);
+        return 2;
     }
+}
+__device__ uint32_t record(const Hist& H, int band, const RecParams& R, V3 dir, float a, float t, float dist,
+                           Drops& dr) {
+    TileSink none{};
+    return record(H, band, R, dir, a, t, dist, dr, none, false);
 }
 
 // Receiver spheres (EAR_EST_SPHERE): the segment o + s u, s in [0, seg),
@@ -655,6 +762,8 @@ struct InstDev {
     const float4* nm;
     const float4* mat_band;
     uint32_t n_tris;
+    uint32_t key_base;  // time-tiled splat: first tile key of this render
+    uint32_t n_tiles;   // and tiles per histogram row
     uint32_t pad0;
     MeshSrc M;
     unsigned long long* hist;  // [3][n_ch][kinds][bins] x 2
@@ -674,7 +783,7 @@ __device__ __forceinline__ Traversal make_trav(const InstDev& I, int* gstack, ui
     return Traversal{I.nodes, I.nodes4, I.tris, I.n_tris, gstack, gthreads};
 }
 __device__ __forceinline__ Hist make_hist(const InstDev& I) {
-    return Hist{I.hist, I.P.n_bins, I.P.n_channels, I.P.hist_kinds};
+    return Hist{I.hist, I.P.n_bins, I.P.n_channels, I.P.hist_kinds, I.key_base, I.n_tiles};
 }
 
 // ---------------------------------------------------------------------------
@@ -1057,14 +1166,15 @@ __global__ void __launch_bounds__(BLOCK) k_tail(const InstDev* __restrict__ inst
 
 // Scene::Connect + Mesh::LineIntersection (src/Scene.cpp:84-96,
 // src/Mesh.cpp:58-71), then Recorder::Record for visible connections.
-template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
+template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH, bool TILED>
 __global__ void __launch_bounds__(BLOCK) k_shadow(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
                                                    ShadowQ sq, const uint32_t* __restrict__ n_sq,
-                                                   unsigned long long* __restrict__ stats, Fetch /*fetch*/) {
+                                                   unsigned long long* __restrict__ stats, Fetch /*fetch*/,
+                                                   TileOut to) {
     __shared__ int lds[LDS * BLOCK];
     uint32_t n = *n_sq;
     uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0;
+    uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0, nramp = 0;
     Drops drops;
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         uint32_t i = base + threadIdx.x;
@@ -1084,17 +1194,21 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(const InstDev* __restrict__ in
             Traversal T = make_trav(In, gstack, gthreads);
             bool occ = traverse<LDS, BLOCK, WIDTH, true, INSTR>(T, p, dv, bt, bs, bo, lds + threadIdx.x, gtid, nvis,
                                                                 ntri);
+            TileSink sink{to, i * to.stride, 0};
             if (!occ) {
                 float l = length(dv);
                 V3 lsdir = normalized(dv);
                 float L = m.x;
-                record(make_hist(In), band, R, lsdir, pc.w, (L + l) / 343.0f, L + l, drops);
+                nramp += record(make_hist(In), band, R, lsdir, pc.w, (L + l) / 343.0f, L + l, drops, sink, TILED);
                 ++ncon;
             }
+            if (TILED) sink.close();
         }
     }
     stat_add(stats, ST_SHADOW, nsh);
     stat_add(stats, ST_CONTRIB, ncon);
+    stat_add(stats, ST_SH_CONTRIB, ncon);
+    stat_add(stats, ST_SH_RAMPS, nramp);
     stat_drops(stats, drops);
     if (INSTR) {
         stat_add(stats, ST_NV_SH, nvis);
@@ -1157,15 +1271,16 @@ __global__ void __launch_bounds__(BLOCK) k_extend_f(const InstDev* __restrict__ 
     }
 }
 
-template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH>
+template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH, bool TILED>
 __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
                                                      ShadowQ sq, const uint32_t* __restrict__ n_sq,
-                                                     unsigned long long* __restrict__ stats, Fetch fetch) {
+                                                     unsigned long long* __restrict__ stats, Fetch fetch,
+                                                     TileOut to) {
     __shared__ int lds[LDS * BLOCK];
     const uint32_t n = *n_sq;
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t lane = lane_id();
-    uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0;
+    uint32_t nvis = 0, ntri = 0, ncon = 0, nsh = 0, nramp = 0;
     Drops drops;
     Walker<LDS, BLOCK, WIDTH, true, INSTR> w(lds + threadIdx.x, gstack, gtid, gthreads);
     Traversal T = make_trav(inst[0], gstack, gthreads);
@@ -1189,7 +1304,10 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
                 const RecParams& R = In.P.rec[r];
                 const V3 dv = sub(V3{R.px, R.py, R.pz}, V3{pc.x, pc.y, pc.z});
                 const float l = length(dv);
-                record(make_hist(In), band, R, normalized(dv), pc.w, (m.x + l) / 343.0f, m.x + l, drops);
+                TileSink sink{to, slot * to.stride, 0};
+                nramp += record(make_hist(In), band, R, normalized(dv), pc.w, (m.x + l) / 343.0f, m.x + l, drops,
+                                sink, TILED);
+                if (TILED) sink.close();
                 ++ncon;
                 pending = false;
             }
@@ -1221,14 +1339,192 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
         if (active && w.step(T, nvis, ntri)) {
             active = false;
             pending = !w.found;
+            if (TILED && w.found) TileSink{to, slot * to.stride, 0}.close();  // occluded: no records
         }
     }
     stat_add(stats, ST_SHADOW, nsh);
     stat_add(stats, ST_CONTRIB, ncon);
+    stat_add(stats, ST_SH_CONTRIB, ncon);
+    stat_add(stats, ST_SH_RAMPS, nramp);
     stat_drops(stats, drops);
     if (INSTR) {
         stat_add(stats, ST_NV_SH, nvis);
         stat_add(stats, ST_NT_SH, ntri);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// time-tiled splat: counting sort of the records by key, then one workgroup
+// per key accumulating its tile in LDS
+// ---------------------------------------------------------------------------
+// cnt[key] += records of that key among the n_sq * stride slots.  Blocks
+// count their share in LDS when the keys fit, then add each non-zero count
+// to HBM once.  k_tile_scatter must run with the same grid (it recounts the
+// same records per block).
+__global__ void __launch_bounds__(kTileBlock) k_tile_count(const uint32_t* __restrict__ keys,
+                                                           const uint32_t* __restrict__ n_sq, uint32_t stride,
+                                                           uint32_t* __restrict__ cnt, uint32_t K) {
+    __shared__ uint32_t h[kLdsKeys];
+    const size_t n = (size_t)*n_sq * stride;
+    const bool in_lds = K <= kLdsKeys;
+    if (in_lds) {
+        for (uint32_t j = threadIdx.x; j < K; j += kTileBlock) h[j] = 0;
+        __syncthreads();
+    }
+    for (size_t i = (size_t)blockIdx.x * kTileBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kTileBlock) {
+        const uint32_t k = keys[i];
+        if (k == kNoKey) continue;
+        if (in_lds)
+            atomicAdd(&h[k], 1u);
+        else
+            atomicAdd(&cnt[k], 1u);
+    }
+    if (in_lds) {
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < K; j += kTileBlock)
+            if (h[j]) atomicAdd(&cnt[j], h[j]);
+    }
+}
+
+// off[k] = cur[k] = sum of cnt[0, k), off[K] = the total; cnt is cleared for
+// the next iteration.  One block of 1024 threads.
+__global__ void __launch_bounds__(1024) k_tile_scan(uint32_t* __restrict__ cnt, uint32_t* __restrict__ off,
+                                                    uint32_t* __restrict__ cur, uint32_t K) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < K; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < K ? cnt[i] : 0u;
+        if (i < K) cnt[i] = 0;
+        // inclusive scan within the wave
+        uint32_t x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if ((int)lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t t = lane < 16 ? wsum[lane] : 0u;
+            for (int o = 1; o < 16; o <<= 1) {
+                const uint32_t y = __shfl_up(t, o);
+                if ((int)lane >= o) t += y;
+            }
+            if (lane < 16) wsum[lane] = t;  // inclusive over waves
+        }
+        __syncthreads();
+        const uint32_t excl = carry + (wave ? wsum[wave - 1] : 0u) + x - v;
+        if (i < K) {
+            off[i] = excl;
+            cur[i] = excl;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry += wsum[15];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) off[K] = carry;
+}
+
+// Records to out[] grouped by key: a block recounts its share (the records
+// k_tile_count gave it), takes a range per key from cur with one atomic each,
+// then places its records there (LDS atomics give the ranks; the order
+// within a key does not matter to integer sums).
+__global__ void __launch_bounds__(kTileBlock) k_tile_scatter(const uint32_t* __restrict__ keys,
+                                                             const uint2* __restrict__ pay,
+                                                             const uint32_t* __restrict__ n_sq, uint32_t stride,
+                                                             uint32_t* __restrict__ cur, uint32_t K,
+                                                             uint2* __restrict__ out) {
+    __shared__ uint32_t h[kLdsKeys];
+    const size_t n = (size_t)*n_sq * stride;
+    const size_t step = (size_t)gridDim.x * kTileBlock;
+    if (K > kLdsKeys) {
+        for (size_t i = (size_t)blockIdx.x * kTileBlock + threadIdx.x; i < n; i += step) {
+            const uint32_t k = keys[i];
+            if (k != kNoKey) out[atomicAdd(&cur[k], 1u)] = pay[i];
+        }
+        return;
+    }
+    for (uint32_t j = threadIdx.x; j < K; j += kTileBlock) h[j] = 0;
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * kTileBlock + threadIdx.x; i < n; i += step) {
+        const uint32_t k = keys[i];
+        if (k != kNoKey) atomicAdd(&h[k], 1u);
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < K; j += kTileBlock)
+        if (h[j]) h[j] = atomicAdd(&cur[j], h[j]);
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * kTileBlock + threadIdx.x; i < n; i += step) {
+        const uint32_t k = keys[i];
+        if (k != kNoKey) out[atomicAdd(&h[k], 1u)] = pay[i];
+    }
+}
+
+// 128-bit add into the LDS tile (lo at 2 i, hi at 2 i + 1)
+__device__ __forceinline__ void lds_acc128(unsigned long long* tile, uint32_t i, I128 q) {
+    const unsigned long long lo = q.lo;
+    const unsigned long long old = atomicAdd(&tile[2 * i], lo);
+    const unsigned long long hi = (unsigned long long)q.hi + ((old + lo) < old ? 1ull : 0ull);
+    if (hi) atomicAdd(&tile[2 * i + 1], hi);
+}
+
+// One workgroup per key: the ramps of one (render, row, tile) are added into
+// an LDS copy of the tile's bins [t0, t0 + kTile + kHalo) of both parts, which
+// is then added to the row in HBM (bins < n_bins).  kbase: the first key of
+// each render of the batch, ascending, kbase[n_items] = K.
+__global__ void __launch_bounds__(kTileBlock) k_tile_splat(const InstDev* __restrict__ inst,
+                                                           const uint32_t* __restrict__ kbase, uint32_t n_items,
+                                                           const uint32_t* __restrict__ off,
+                                                           const uint2* __restrict__ pay) {
+    constexpr uint32_t kSpan = kTile + kHalo;
+    __shared__ unsigned long long acc[2 * kSpan * 2];  // [part][bin] x {lo, hi}
+    const uint32_t k = blockIdx.x;
+    const uint32_t b = off[k], e = off[k + 1];
+    if (b == e) return;
+    uint32_t lo = 0, hi = n_items;  // the render: the last item with kbase <= k
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (kbase[mid] <= k)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const InstDev& In = inst[lo];
+    const uint32_t rel = k - kbase[lo];
+    const uint32_t nb = In.P.n_bins;
+    unsigned long long* row = In.hist + 4 * (size_t)nb * (rel / In.n_tiles);
+    const uint32_t t0 = (rel % In.n_tiles) * kTile;
+    if (e - b <= kTileDirect) {
+        // a few ramps: straight to HBM
+        for (uint32_t i = b + threadIdx.x; i < e; i += kTileBlock) {
+            const uint2 p = pay[i];
+            splat_span(row, nb, t0 + (p.y & 0xFFFFu), (int)((p.y >> 16) & 0xFFu), (int)(p.y >> 24),
+                       __uint_as_float(p.x));
+        }
+        return;
+    }
+    for (uint32_t j = threadIdx.x; j < 4 * kSpan; j += kTileBlock) acc[j] = 0;
+    __syncthreads();
+    for (uint32_t i = b + threadIdx.x; i < e; i += kTileBlock) {
+        const uint2 p = pay[i];
+        const uint32_t o = p.y & 0xFFFFu;
+        const int ww = (int)((p.y >> 16) & 0xFFu), w = (int)(p.y >> 24);
+        const RampPoints q = ramp_points(__uint_as_float(p.x), w, ww);
+        unsigned long long* tp = acc + (q.neg ? 2 * kSpan : 0u);  // gets +w*S and +S
+        unsigned long long* tm = acc + (q.neg ? 0u : 2 * kSpan);  // gets (w+1)*S
+        lds_acc128(tp, o, q.aq);
+        if (t0 + o + 1 < nb) lds_acc128(tm, o + 1, q.p1);
+        if (t0 + o + (uint32_t)ww + 1 < nb) lds_acc128(tp, o + (uint32_t)ww + 1, q.sq);
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < 2 * kSpan; j += kTileBlock) {
+        const uint32_t part = j >= kSpan ? 1u : 0u, loc = j - part * kSpan;
+        const unsigned long long vlo = acc[2 * j], vhi = acc[2 * j + 1];
+        if ((vlo | vhi) && t0 + loc < nb)
+            acc128(row + 2 * (size_t)nb * part + 2 * (size_t)(t0 + loc), I128{vlo, (int64_t)vhi});
     }
 }
 
@@ -1449,19 +1745,28 @@ __global__ void __launch_bounds__(kBlock) k_query(Traversal T, uint32_t n, const
 struct TravCfg {
     int block, lds, width;
     bool instr, batch, fetch;
+    bool tiled;  // shadow kernels: write the time-tiled splat's records
 };
 
 using ExtendFn = void (*)(const InstDev*, int*, uint32_t, Queue, const uint32_t*, int2*, unsigned long long*, Fetch);
-using ShadowFn = void (*)(const InstDev*, int*, uint32_t, ShadowQ, const uint32_t*, unsigned long long*, Fetch);
+using ShadowFn = void (*)(const InstDev*, int*, uint32_t, ShadowQ, const uint32_t*, unsigned long long*, Fetch,
+                         TileOut);
 
 template <int B, int L, int W, bool BA>
-void pick_t(bool instr, bool fetch, ExtendFn* e, ShadowFn* s) {
+void pick_t(bool instr, bool fetch, bool tiled, ExtendFn* e, ShadowFn* s) {
     if (fetch) {
         *e = instr ? (ExtendFn)k_extend_f<B, L, W, true, BA> : (ExtendFn)k_extend_f<B, L, W, false, BA>;
-        *s = instr ? (ShadowFn)k_shadow_f<B, L, W, true, BA> : (ShadowFn)k_shadow_f<B, L, W, false, BA>;
+        if (tiled)
+            *s = instr ? (ShadowFn)k_shadow_f<B, L, W, true, BA, true> : (ShadowFn)k_shadow_f<B, L, W, false, BA, true>;
+        else
+            *s = instr ? (ShadowFn)k_shadow_f<B, L, W, true, BA, false>
+                       : (ShadowFn)k_shadow_f<B, L, W, false, BA, false>;
     } else {
         *e = instr ? (ExtendFn)k_extend<B, L, W, true, BA> : (ExtendFn)k_extend<B, L, W, false, BA>;
-        *s = instr ? (ShadowFn)k_shadow<B, L, W, true, BA> : (ShadowFn)k_shadow<B, L, W, false, BA>;
+        if (tiled)
+            *s = instr ? (ShadowFn)k_shadow<B, L, W, true, BA, true> : (ShadowFn)k_shadow<B, L, W, false, BA, true>;
+        else
+            *s = instr ? (ShadowFn)k_shadow<B, L, W, true, BA, false> : (ShadowFn)k_shadow<B, L, W, false, BA, false>;
     }
 }
 
@@ -1469,14 +1774,14 @@ template <bool BA>
 void pick_b(const TravCfg& c, ExtendFn* e, ShadowFn* s) {
     int key = (c.block >= 256 ? 4 : 0) | (c.lds >= 32 ? 2 : 0) | (c.width == 4 ? 1 : 0);
     switch (key) {
-        case 0: pick_t<128, 16, 2, BA>(c.instr, c.fetch, e, s); break;
-        case 1: pick_t<128, 16, 4, BA>(c.instr, c.fetch, e, s); break;
-        case 2: pick_t<128, 32, 2, BA>(c.instr, c.fetch, e, s); break;
-        case 3: pick_t<128, 32, 4, BA>(c.instr, c.fetch, e, s); break;
-        case 4: pick_t<256, 16, 2, BA>(c.instr, c.fetch, e, s); break;
-        case 5: pick_t<256, 16, 4, BA>(c.instr, c.fetch, e, s); break;
-        case 6: pick_t<256, 32, 2, BA>(c.instr, c.fetch, e, s); break;
-        default: pick_t<256, 32, 4, BA>(c.instr, c.fetch, e, s); break;
+        case 0: pick_t<128, 16, 2, BA>(c.instr, c.fetch, c.tiled, e, s); break;
+        case 1: pick_t<128, 16, 4, BA>(c.instr, c.fetch, c.tiled, e, s); break;
+        case 2: pick_t<128, 32, 2, BA>(c.instr, c.fetch, c.tiled, e, s); break;
+        case 3: pick_t<128, 32, 4, BA>(c.instr, c.fetch, c.tiled, e, s); break;
+        case 4: pick_t<256, 16, 2, BA>(c.instr, c.fetch, c.tiled, e, s); break;
+        case 5: pick_t<256, 16, 4, BA>(c.instr, c.fetch, c.tiled, e, s); break;
+        case 6: pick_t<256, 32, 2, BA>(c.instr, c.fetch, c.tiled, e, s); break;
+        default: pick_t<256, 32, 4, BA>(c.instr, c.fetch, c.tiled, e, s); break;
     }
 }
 
@@ -1720,6 +2025,11 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     bool any_value = false;
     size_t max_recs = 0;
     uint32_t any_mesh = 0;
+    // time-tiled splat: keys (render, row, tile) of every render, and the
+    // most channels of one recorder (records per shadow request)
+    uint64_t n_keys = 0;
+    uint32_t tile_stride = 1;
+    std::vector<uint32_t> kbase(n_items + 1, 0);
     for (uint32_t i = 0; i < n_items; ++i) {
         if (!scenes[i] || !descs[i]) return EAR_ERR_INVALID_ARGUMENT;
         if ((int)scenes[i]->width != width) {
@@ -1758,6 +2068,12 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
             I.P.src_n_tris = m;
             I.P.src_total_area = total;
         }
+        I.n_tiles = (I.P.n_bins + kTile - 1) / kTile;
+        I.key_base = (uint32_t)std::min<uint64_t>(n_keys, 0xFFFFFFFFull);
+        kbase[i] = I.key_base;
+        n_keys += (uint64_t)3 * I.P.n_channels * I.P.hist_kinds * I.n_tiles;
+        for (uint32_t r = 0; r < D->n_recorders; ++r)
+            if (D->recorders[r].kind == EAR_REC_STEREO) tile_stride = 2;
         int nb = 0;
         for (int b = 0; b < 3; ++b) nb += (D->band_mask >> b) & 1u;
         total_rays += (uint64_t)D->n_rays * nb;
@@ -1788,6 +2104,12 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         return EAR_ERR_CAPACITY;
     }
     const ear_tuning& tu = c->tuning;
+    // the shadow kernels' splat: records binned by time tile and accumulated
+    // in LDS, or direct global atomics (always in the no-allocation render:
+    // the records would not fit its region)
+    const bool tiled = !noalloc && tu.splat == EAR_SPLAT_TILED && n_keys < 0x7FFFFFFFull;  // one block per key
+    kbase[n_items] = (uint32_t)std::min<uint64_t>(n_keys, 0xFFFFFFFFull);
+    const uint32_t K = kbase[n_items];
     uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 23);
     uint64_t cap64 = std::min<uint64_t>((uint64_t)chunk * 3, std::max<uint64_t>(1, total_rays));
     uint32_t cap_rays = (uint32_t)cap64;
@@ -1795,8 +2117,10 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     // kernels by default)
     const bool fetch_e = tu.traversal != EAR_TRAVERSAL_STATIC;
     const bool fetch_s = tu.traversal == EAR_TRAVERSAL_FETCH_ALL || tu.traversal == EAR_TRAVERSAL_DEFAULT;
-    const TravCfg cfg_e{tu.block_extend == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch, fetch_e};
-    const TravCfg cfg_s{tu.block_shadow == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch, fetch_s};
+    const TravCfg cfg_e{tu.block_extend == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch, fetch_e,
+                        false};
+    const TravCfg cfg_s{tu.block_shadow == 128 ? 128 : 256, tu.stack_lds >= 32 ? 32 : 16, width, instr, batch, fetch_s,
+                        tiled};
     ExtendFn fe, fe_unused;
     ShadowFn fs, fs_unused;
     pick_kernels(cfg_e, &fe, &fs_unused);
@@ -1824,7 +2148,8 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
            qm[2] = {DevBuf(c, ar), DevBuf(c, ar)};
     DevBuf hits(c, ar), so(c, ar), sm(c, ar), counters(c, ar), stats(c, ar), hist(c, ar), gs(c, ar), direct(c, ar),
         value(c, ar), logh(c, ar), logn(c, ar), dinst(c, ar), dsegs(c, ar), dsegoff(c, ar), drows(c, ar),
-        dvrows(c, ar), msrc(c, ar), segc(c, ar);
+        dvrows(c, ar), msrc(c, ar), segc(c, ar), tkey(c, ar), tpay(c, ar), tsort(c, ar), tcnt(c, ar), toff(c, ar),
+        tcur(c, ar), tkb(c, ar);
     // device memory that must start at a value: hipMemsetAsync, or in the
     // no-allocation render a fill kernel (no memset nodes if a caller
     // captures the stream)
@@ -1950,6 +2275,21 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const size_t cap_sh = (size_t)cap_rays * sh_per_ray;
     TRY(so.alloc(cap_sh * 16));
     TRY(sm.alloc(cap_sh * 16));
+    TileOut tile_out{nullptr, nullptr, tile_stride};
+    if (tiled) {
+        const size_t n_rec = cap_sh * tile_stride;
+        TRY(tkey.alloc(n_rec * 4));
+        TRY(tpay.alloc(n_rec * 8));
+        TRY(tsort.alloc(n_rec * 8));
+        TRY(tcnt.alloc((size_t)K * 4));
+        TRY(toff.alloc(((size_t)K + 1) * 4));
+        TRY(tcur.alloc((size_t)K * 4));
+        TRY(tkb.alloc(((size_t)n_items + 1) * 4));
+        TRY(clear(tcnt, 0));
+        EAR_HIP_TRY(c, hipMemcpyAsync(tkb.p, kbase.data(), 4 * kbase.size(), hipMemcpyHostToDevice, st));
+        tile_out.key = tkey.as<uint32_t>();
+        tile_out.pay = tpay.as<uint2>();
+    }
 
     uint32_t* cnt = counters.as<uint32_t>();  // [0],[1] ray queues, [2] shadow queue
     unsigned long long* stp = stats.as<unsigned long long>();
@@ -1968,7 +2308,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         }
     } pg{noalloc ? nullptr : pinned};
 
-    std::vector<EventPair> ev_ext, ev_sha, ev_shw;
+    std::vector<EventPair> ev_ext, ev_sha, ev_shw, ev_spl;
     auto mk = [&](std::vector<EventPair>& v) -> EventPair& {
         v.emplace_back();
         hipEventCreate(&v.back().a);
@@ -2011,6 +2351,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const uint32_t g_shade = persistent_grid(c, cap_rays, waves, kShadeBlock);
     const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
     const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
+    const uint32_t g_tile = (uint32_t)c->num_cus * 4;  // count and scatter: the same grid
     // eager: the bounce loop launched from the host (always in the
     // no-allocation render); timed: with per-kernel events
     const bool eager = tu.graph_iters == 0 || noalloc;
@@ -2049,8 +2390,22 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         EventPair* e3 = timed ? &mk(ev_shw) : nullptr;
         if (e3) hipEventRecord(e3->a, st);
         hipLaunchKernelGGL(fs, dim3(g_shw), dim3(cfg_s.block), 0, st, di, gsp, gthreads, SQ, (const uint32_t*)(cnt + 2),
-                           stp, fq_s);
+                           stp, fq_s, tile_out);
         if (e3) hipEventRecord(e3->b, st);
+        if (tiled) {
+            // the records of this launch, grouped by key and splatted per tile
+            EventPair* e4 = timed ? &mk(ev_spl) : nullptr;
+            if (e4) hipEventRecord(e4->a, st);
+            k_tile_count<<<g_tile, kTileBlock, 0, st>>>(tkey.as<uint32_t>(), cnt + 2, tile_stride, tcnt.as<uint32_t>(),
+                                                        K);
+            k_tile_scan<<<1, 1024, 0, st>>>(tcnt.as<uint32_t>(), toff.as<uint32_t>(), tcur.as<uint32_t>(), K);
+            k_tile_scatter<<<g_tile, kTileBlock, 0, st>>>(tkey.as<uint32_t>(), tpay.as<uint2>(), cnt + 2, tile_stride,
+                                                          tcur.as<uint32_t>(), K, tsort.as<uint2>());
+            k_tile_splat<<<K, kTileBlock, 0, st>>>(di, tkb.as<uint32_t>(), n_items, toff.as<uint32_t>(),
+                                                   tsort.as<uint2>());
+            if (e4) hipEventRecord(e4->b, st);
+            launches += 4;
+        }
         // the consumed queue, the shadow queue and the fetch counters
         k_reset_iter<<<1, 64, 0, st>>>(cnt, (1u << cur) | 0x4u, segc.as<uint32_t>(), 2 * kMaxSeg);
         launches += 6;
@@ -2228,6 +2583,9 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         st_out->extend_ms = sum(ev_ext);
         st_out->shade_ms = sum(ev_sha);
         st_out->shadow_ms = sum(ev_shw);
+        st_out->splat_ms = sum(ev_spl);
+        st_out->shadow_contributions = hs[ST_SH_CONTRIB];
+        st_out->shadow_ramps = hs[ST_SH_RAMPS];
         st_out->extend_launches = (uint32_t)ev_ext.size();
         st_out->shadow_launches = (uint32_t)ev_shw.size();
         st_out->extend_queries = hs[ST_SEGMENTS] - hs[ST_TAIL_SEG];  // the timed k_extend launches' share
@@ -2235,7 +2593,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         st_out->dropped_ramps = hs[ST_DROPPED];
         st_out->bins_needed = (uint32_t)std::min<unsigned long long>(hs[ST_BINS], 0xFFFFFFFFull);
     }
-    for (auto* v : {&ev_ext, &ev_sha, &ev_shw})
+    for (auto* v : {&ev_ext, &ev_sha, &ev_shw, &ev_spl})
         for (auto& e : *v) {
             hipEventDestroy(e.a);
             hipEventDestroy(e.b);
@@ -3035,7 +3393,7 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     const bool fetch_e = tu.traversal == EAR_TRAVERSAL_FETCH || tu.traversal == EAR_TRAVERSAL_FETCH_ALL;
     ExtendFn fe;
     ShadowFn fs_unused;
-    const TravCfg cfg{256, 16, width, false, batch, fetch_e};
+    const TravCfg cfg{256, 16, width, false, batch, fetch_e, false};
     pick_kernels(cfg, &fe, &fs_unused);
     DevBuf segc(c);
     TRY(segc.alloc(kMaxSeg * kSegStride * 4));

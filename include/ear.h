@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EAR_ABI_VERSION 5
+#define EAR_ABI_VERSION 6
 
 typedef enum {
     EAR_OK = 0,
@@ -244,6 +244,12 @@ typedef struct {
     uint64_t dropped_ramps;
     uint32_t bins_needed;      /* the largest ramp end among them; 0 if none was cut */
     uint32_t reserved;
+    /* The shadow kernels' share (the connections traced by the tail launch
+     * excluded): connections recorded and the ramps they made (one per
+     * channel: mono 1, stereo 2). */
+    uint64_t shadow_contributions;
+    uint64_t shadow_ramps;
+    double splat_ms;           /* device time of the time-tiled splat's kernels (EAR_SPLAT_TILED) */
 } ear_stats;
 
 ear_status ear_render(ear_ctx* ctx, ear_scene* scene, const ear_render_desc* desc, ear_hist* out,
@@ -358,7 +364,16 @@ typedef struct {
                                 (same histograms); 0: default, 2^20, or every ray when no
                                 scene has more than 64 triangles; 1: never (the bounce loop
                                 to the end) */
+    uint32_t splat;          /* EAR_SPLAT_*: how the shadow kernels add ramps to the
+                                histograms (same histograms either way) */
 } ear_tuning;
+#define EAR_SPLAT_DEFAULT 0u /* = EAR_SPLAT_ATOMIC */
+#define EAR_SPLAT_ATOMIC 1u  /* each ramp point is a 128-bit add to the histogram in HBM */
+#define EAR_SPLAT_TILED 2u   /* visible connections write records keyed by (row, time tile of
+                                1024 bins); per bounce iteration a counting sort groups them and
+                                one workgroup per tile accumulates them in LDS (64-bit LDS
+                                atomics), then adds the tile to HBM once (ramps wider than 62
+                                bins are added directly) */
 #define EAR_TRAVERSAL_DEFAULT 0u   /* = EAR_TRAVERSAL_FETCH_ALL */
 #define EAR_TRAVERSAL_STATIC 1u    /* one queue entry per lane per grid-stride step */
 #define EAR_TRAVERSAL_FETCH 2u     /* closest hit: persistent lanes take the next entry when
