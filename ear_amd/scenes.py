@@ -218,6 +218,19 @@ def room(*, seed, target_tris=12, kind=None, alpha_range=(0.02, 0.6)):
                 alpha=alpha, src=src, recorders=[dict(kind=0, pos=rec)], air=EXPORTER_AIR)
 
 
+def batch_rooms(n_rooms=256, *, seed=3, tris=(12, 50_000), rays=(1e4, 2e6)):
+    """The batching workload: n_rooms box or L rooms (room(seed=1000 + i))
+    of log-uniform triangle counts in `tris` and log-uniform ray counts in
+    `rays`, drawn from one seeded stream.  Returns [(room, n_rays)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_rooms):
+        nt = int(np.exp(rng.uniform(np.log(tris[0]), np.log(tris[1]))))
+        nr = int(np.exp(rng.uniform(np.log(rays[0]), np.log(rays[1]))))
+        out.append((room(seed=1000 + i, target_tris=nt), nr))
+    return out
+
+
 def signed_volume(tris) -> float:
     """Mesh::Volume (src/Mesh.cpp:160-167) via Triangle::SignedVolume, in float64."""
     t = np.asarray(tris, np.float64)
