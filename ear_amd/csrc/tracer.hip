@@ -580,12 +580,17 @@ __device__ uint32_t splat_ramp(unsigned long long* h, uint32_t nb, int s, int w,
 // request's index times the records per request), keyed by (histogram row,
 // time tile of kTile bins).  Per bounce iteration a counting sort groups the
 // records by key (k_tile_count, k_tile_scan, k_tile_scatter), and
-// k_tile_splat gives each key one workgroup that adds the three points of
-// every ramp of its tile into a 128-bit accumulator tile in LDS (64-bit LDS
-// atomics with an explicit carry) and then adds that tile to the row in HBM,
-// one global 128-bit add per touched bin.  The sums are the same integers in
-// another order, so the histograms are bitwise those of the direct splat.
-constexpr uint32_t kTile = 1024;            // bins per tile
+// k_tile_splat cuts the sorted records into chunks of kTileChunk, one
+// workgroup each: for every key among its records it adds the three points of
+// each ramp into a 128-bit accumulator tile in LDS (64-bit LDS atomics with an
+// explicit carry), then adds that tile to the row in HBM, one global 128-bit
+// add per touched bin.  The rays of one bounce iteration have travelled
+// similar distances, so an iteration's records crowd into a few tiles; fixed
+// chunks spread a crowded tile over many workgroups.  The sums are the same
+// integers in another order, so the histograms are bitwise those of the
+// direct splat.
+constexpr uint32_t kTile = 256;             // bins per tile
+constexpr uint32_t kTileChunk = 8192;       // sorted records per k_tile_splat workgroup
 constexpr uint32_t kHalo = 64;              // bins past its tile that a ramp starting in it may reach
 constexpr int kTileMaxW = (int)kHalo - 2;   // wider ramps are added directly
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;    // an empty record slot
@@ -1357,6 +1362,12 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
 // time-tiled splat: counting sort of the records by key, then one workgroup
 // per key accumulating its tile in LDS
 // ---------------------------------------------------------------------------
+// Record slots are read kTileBatch per thread at a time (all loads issued
+// before the first is used): one wave step then waits for one memory round
+// trip instead of kTileBatch.  Slot i of step t of workgroup b is
+// (t gridDim + b) kTileBlock kTileBatch + j kTileBlock + threadIdx, j < kTileBatch.
+constexpr uint32_t kTileBatch = 16;
+
 // cnt[key] += records of that key among the n_sq * stride slots.  Blocks
 // count their share in LDS when the keys fit, then add each non-zero count
 // to HBM once.  k_tile_scatter must run with the same grid (it recounts the
@@ -1371,13 +1382,22 @@ __global__ void __launch_bounds__(kTileBlock) k_tile_count(const uint32_t* __res
         for (uint32_t j = threadIdx.x; j < K; j += kTileBlock) h[j] = 0;
         __syncthreads();
     }
-    for (size_t i = (size_t)blockIdx.x * kTileBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kTileBlock) {
-        const uint32_t k = keys[i];
-        if (k == kNoKey) continue;
-        if (in_lds)
-            atomicAdd(&h[k], 1u);
-        else
-            atomicAdd(&cnt[k], 1u);
+    constexpr size_t kStep = (size_t)kTileBlock * kTileBatch;
+    for (size_t base = (size_t)blockIdx.x * kStep; base < n; base += (size_t)gridDim.x * kStep) {
+        uint32_t k[kTileBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kTileBatch; ++j) {
+            const size_t i = base + j * kTileBlock + threadIdx.x;
+            k[j] = i < n ? keys[i] : kNoKey;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kTileBatch; ++j) {
+            if (k[j] == kNoKey) continue;
+            if (in_lds)
+                atomicAdd(&h[k[j]], 1u);
+            else
+                atomicAdd(&cnt[k[j]], 1u);
+        }
     }
     if (in_lds) {
         __syncthreads();
@@ -1386,10 +1406,17 @@ __global__ void __launch_bounds__(kTileBlock) k_tile_count(const uint32_t* __res
     }
 }
 
-// off[k] = cur[k] = sum of cnt[0, k), off[K] = the total; cnt is cleared for
-// the next iteration.  One block of 1024 threads.
+// Keys with more than kTileDirect records (crowded) get a range of the
+// sorted array: off[k] = cur[k] = the sum of the crowded counts before k,
+// off[K] = their total; a key with fewer has an empty range and cur[k] =
+// kSparse (k_tile_scatter adds its ramps to HBM directly).  cnt is cleared
+// for the next iteration; first[c] = the key holding sorted record
+// c kTileChunk (for the chunks that start inside the total).  One block of
+// 1024 threads.
+constexpr uint32_t kSparse = 0xFFFFFFFFu;
 __global__ void __launch_bounds__(1024) k_tile_scan(uint32_t* __restrict__ cnt, uint32_t* __restrict__ off,
-                                                    uint32_t* __restrict__ cur, uint32_t K) {
+                                                    uint32_t* __restrict__ cur, uint32_t K,
+                                                    uint32_t* __restrict__ first) {
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
@@ -1397,7 +1424,8 @@ __global__ void __launch_bounds__(1024) k_tile_scan(uint32_t* __restrict__ cnt, 
     __syncthreads();
     for (uint32_t base = 0; base < K; base += 1024) {
         const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < K ? cnt[i] : 0u;
+        const uint32_t c = i < K ? cnt[i] : 0u;
+        const uint32_t v = c > kTileDirect ? c : 0u;
         if (i < K) cnt[i] = 0;
         // inclusive scan within the wave
         uint32_t x = v;
@@ -1419,7 +1447,9 @@ __global__ void __launch_bounds__(1024) k_tile_scan(uint32_t* __restrict__ cnt, 
         const uint32_t excl = carry + (wave ? wsum[wave - 1] : 0u) + x - v;
         if (i < K) {
             off[i] = excl;
-            cur[i] = excl;
+            cur[i] = v ? excl : kSparse;
+            // the chunks whose first record is one of this key's [excl, excl + v)
+            for (uint32_t c = (excl + kTileChunk - 1) / kTileChunk; c * kTileChunk < excl + v; ++c) first[c] = i;
         }
         __syncthreads();
         if (threadIdx.x == 0) carry += wsum[15];
@@ -1428,38 +1458,94 @@ __global__ void __launch_bounds__(1024) k_tile_scan(uint32_t* __restrict__ cnt, 
     if (threadIdx.x == 0) off[K] = carry;
 }
 
-// Records to out[] grouped by key: a block recounts its share (the records
-// k_tile_count gave it), takes a range per key from cur with one atomic each,
-// then places its records there (LDS atomics give the ranks; the order
-// within a key does not matter to integer sums).
-__global__ void __launch_bounds__(kTileBlock) k_tile_scatter(const uint32_t* __restrict__ keys,
+// The row and first bin of tile key k: its render is the last item with
+// kbase <= k (kbase ascending, kbase[n_items] = K).
+struct TileRow {
+    unsigned long long* row;
+    uint32_t nb, t0;
+};
+__device__ __forceinline__ TileRow tile_row(const InstDev* __restrict__ inst, const uint32_t* __restrict__ kbase,
+                                            uint32_t n_items, uint32_t k) {
+    uint32_t lo = 0, hi = n_items;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (kbase[mid] <= k)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const InstDev& In = inst[lo];
+    const uint32_t rel = k - kbase[lo];
+    const uint32_t nb = In.P.n_bins;
+    return TileRow{In.hist + 4 * (size_t)nb * (rel / In.n_tiles), nb, (rel % In.n_tiles) * kTile};
+}
+
+// A record's ramp added to HBM directly.
+__device__ __forceinline__ void splat_record(const TileRow& r, uint2 p) {
+    splat_span(r.row, r.nb, r.t0 + (p.y & 0xFFFFu), (int)((p.y >> 16) & 0xFFu), (int)(p.y >> 24), __uint_as_float(p.x));
+}
+
+// Records of crowded keys to out[] grouped by key: a block recounts its share
+// (the records k_tile_count gave it), takes a range per key from cur with one
+// atomic each, then places its records there (LDS atomics give the ranks; the
+// order within a key does not matter to integer sums).  Records of sparse
+// keys (cur[k] = kSparse) are added to HBM here.
+__global__ void __launch_bounds__(kTileBlock) k_tile_scatter(const InstDev* __restrict__ inst,
+                                                             const uint32_t* __restrict__ kbase, uint32_t n_items,
+                                                             const uint32_t* __restrict__ keys,
                                                              const uint2* __restrict__ pay,
                                                              const uint32_t* __restrict__ n_sq, uint32_t stride,
                                                              uint32_t* __restrict__ cur, uint32_t K,
                                                              uint2* __restrict__ out) {
     __shared__ uint32_t h[kLdsKeys];
     const size_t n = (size_t)*n_sq * stride;
-    const size_t step = (size_t)gridDim.x * kTileBlock;
-    if (K > kLdsKeys) {
-        for (size_t i = (size_t)blockIdx.x * kTileBlock + threadIdx.x; i < n; i += step) {
-            const uint32_t k = keys[i];
-            if (k != kNoKey) out[atomicAdd(&cur[k], 1u)] = pay[i];
+    constexpr size_t kStep = (size_t)kTileBlock * kTileBatch;
+    const size_t step = (size_t)gridDim.x * kStep;
+    const bool in_lds = K <= kLdsKeys;
+    if (in_lds) {
+        for (uint32_t j = threadIdx.x; j < K; j += kTileBlock) h[j] = 0;
+        __syncthreads();
+        for (size_t base = (size_t)blockIdx.x * kStep; base < n; base += step) {
+            uint32_t k[kTileBatch];
+#pragma unroll
+            for (uint32_t j = 0; j < kTileBatch; ++j) {
+                const size_t i = base + j * kTileBlock + threadIdx.x;
+                k[j] = i < n ? keys[i] : kNoKey;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kTileBatch; ++j)
+                if (k[j] != kNoKey) atomicAdd(&h[k[j]], 1u);
         }
-        return;
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < K; j += kTileBlock) {
+            if (!h[j]) continue;
+            const uint32_t c = cur[j];
+            h[j] = c == kSparse ? kSparse : atomicAdd(&cur[j], h[j]);
+        }
+        __syncthreads();
     }
-    for (uint32_t j = threadIdx.x; j < K; j += kTileBlock) h[j] = 0;
-    __syncthreads();
-    for (size_t i = (size_t)blockIdx.x * kTileBlock + threadIdx.x; i < n; i += step) {
-        const uint32_t k = keys[i];
-        if (k != kNoKey) atomicAdd(&h[k], 1u);
-    }
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < K; j += kTileBlock)
-        if (h[j]) h[j] = atomicAdd(&cur[j], h[j]);
-    __syncthreads();
-    for (size_t i = (size_t)blockIdx.x * kTileBlock + threadIdx.x; i < n; i += step) {
-        const uint32_t k = keys[i];
-        if (k != kNoKey) out[atomicAdd(&h[k], 1u)] = pay[i];
+    for (size_t base = (size_t)blockIdx.x * kStep; base < n; base += step) {
+        uint32_t k[kTileBatch];
+        uint2 v[kTileBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kTileBatch; ++j) {
+            const size_t i = base + j * kTileBlock + threadIdx.x;
+            k[j] = i < n ? keys[i] : kNoKey;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kTileBatch; ++j) {
+            const size_t i = base + j * kTileBlock + threadIdx.x;
+            if (k[j] != kNoKey) v[j] = pay[i];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kTileBatch; ++j) {
+            if (k[j] == kNoKey) continue;
+            const uint32_t c = in_lds ? h[k[j]] : cur[k[j]];
+            if (c == kSparse)
+                splat_record(tile_row(inst, kbase, n_items, k[j]), v[j]);
+            else
+                out[in_lds ? atomicAdd(&h[k[j]], 1u) : atomicAdd(&cur[k[j]], 1u)] = v[j];
+        }
     }
 }
 
@@ -1471,60 +1557,84 @@ __device__ __forceinline__ void lds_acc128(unsigned long long* tile, uint32_t i,
     if (hi) atomicAdd(&tile[2 * i + 1], hi);
 }
 
-// One workgroup per key: the ramps of one (render, row, tile) are added into
-// an LDS copy of the tile's bins [t0, t0 + kTile + kHalo) of both parts, which
-// is then added to the row in HBM (bins < n_bins).  kbase: the first key of
-// each render of the batch, ascending, kbase[n_items] = K.
-__global__ void __launch_bounds__(kTileBlock) k_tile_splat(const InstDev* __restrict__ inst,
-                                                           const uint32_t* __restrict__ kbase, uint32_t n_items,
-                                                           const uint32_t* __restrict__ off,
-                                                           const uint2* __restrict__ pay) {
-    constexpr uint32_t kSpan = kTile + kHalo;
-    __shared__ unsigned long long acc[2 * kSpan * 2];  // [part][bin] x {lo, hi}
-    const uint32_t k = blockIdx.x;
-    const uint32_t b = off[k], e = off[k + 1];
-    if (b == e) return;
-    uint32_t lo = 0, hi = n_items;  // the render: the last item with kbase <= k
+// The key whose records include sorted position p (off[0] <= p < off[K]):
+// the last k with off[k] <= p.
+__device__ __forceinline__ uint32_t key_at(const uint32_t* __restrict__ off, uint32_t K, uint32_t p) {
+    uint32_t lo = 0, hi = K;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (kbase[mid] <= k)
+        if (off[mid] <= p)
             lo = mid;
         else
             hi = mid;
     }
-    const InstDev& In = inst[lo];
-    const uint32_t rel = k - kbase[lo];
-    const uint32_t nb = In.P.n_bins;
-    unsigned long long* row = In.hist + 4 * (size_t)nb * (rel / In.n_tiles);
-    const uint32_t t0 = (rel % In.n_tiles) * kTile;
-    if (e - b <= kTileDirect) {
-        // a few ramps: straight to HBM
-        for (uint32_t i = b + threadIdx.x; i < e; i += kTileBlock) {
-            const uint2 p = pay[i];
-            splat_span(row, nb, t0 + (p.y & 0xFFFFu), (int)((p.y >> 16) & 0xFFu), (int)(p.y >> 24),
-                       __uint_as_float(p.x));
+    return lo;
+}
+
+// Workgroup b takes the sorted records [b kTileChunk, (b + 1) kTileChunk) (all
+// of crowded keys) and walks the keys among them: the ramps of one (render,
+// row, tile) are added into an LDS copy of the tile's bins
+// [t0, t0 + kTile + kHalo) of both parts, which is then added to the row in
+// HBM (bins < n_bins); a key with at most kTileDirect records in the chunk
+// goes straight to HBM.  kbase: the first key of each render of the batch,
+// ascending, kbase[n_items] = K.
+__global__ void __launch_bounds__(kTileBlock) k_tile_splat(const InstDev* __restrict__ inst,
+                                                           const uint32_t* __restrict__ kbase, uint32_t n_items,
+                                                           const uint32_t* __restrict__ off, uint32_t K,
+                                                           const uint32_t* __restrict__ first,
+                                                           const uint2* __restrict__ pay) {
+    constexpr uint32_t kSpan = kTile + kHalo;
+    constexpr uint32_t kPer = kTileChunk / kTileBlock;  // records per thread and key, at most
+    constexpr uint32_t kSplatBatch = 8;
+    __shared__ unsigned long long acc[2 * kSpan * 2];  // [part][bin] x {lo, hi}
+    const uint32_t total = off[K];
+    const uint32_t c0 = blockIdx.x * kTileChunk;
+    if (c0 >= total) return;
+    const uint32_t c1 = min(c0 + kTileChunk, total);
+    for (uint32_t b = c0; b < c1;) {
+        const uint32_t k = b == c0 ? first[blockIdx.x] : key_at(off, K, b);
+        const uint32_t e = min(off[k + 1], c1);
+        const TileRow tr = tile_row(inst, kbase, n_items, k);
+        unsigned long long* row = tr.row;
+        const uint32_t nb = tr.nb, t0 = tr.t0;
+        if (e - b <= kTileDirect) {
+            // the chunk's share of the key is small: straight to HBM
+            for (uint32_t i = b + threadIdx.x; i < e; i += kTileBlock) splat_record(tr, pay[i]);
+        } else {
+            for (uint32_t j = threadIdx.x; j < 4 * kSpan; j += kTileBlock) acc[j] = 0;
+            __syncthreads();
+            // this thread's records of the key, kSplatBatch loads at a time
+            for (uint32_t jb = 0; jb < kPer && b + jb * kTileBlock < e; jb += kSplatBatch) {
+                uint2 pv[kSplatBatch];
+#pragma unroll
+                for (uint32_t j = 0; j < kSplatBatch; ++j) {
+                    const uint32_t i = b + (jb + j) * kTileBlock + threadIdx.x;
+                    if (i < e) pv[j] = pay[i];
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kSplatBatch; ++j) {
+                    if (b + (jb + j) * kTileBlock + threadIdx.x >= e) break;
+                    const uint2 p = pv[j];
+                    const uint32_t o = p.y & 0xFFFFu;
+                    const int ww = (int)((p.y >> 16) & 0xFFu), w = (int)(p.y >> 24);
+                    const RampPoints q = ramp_points(__uint_as_float(p.x), w, ww);
+                    unsigned long long* tp = acc + (q.neg ? 2 * kSpan : 0u);  // gets +w*S and +S
+                    unsigned long long* tm = acc + (q.neg ? 0u : 2 * kSpan);  // gets (w+1)*S
+                    lds_acc128(tp, o, q.aq);
+                    if (t0 + o + 1 < nb) lds_acc128(tm, o + 1, q.p1);
+                    if (t0 + o + (uint32_t)ww + 1 < nb) lds_acc128(tp, o + (uint32_t)ww + 1, q.sq);
+                }
+            }
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < 2 * kSpan; j += kTileBlock) {
+                const uint32_t part = j >= kSpan ? 1u : 0u, loc = j - part * kSpan;
+                const unsigned long long vlo = acc[2 * j], vhi = acc[2 * j + 1];
+                if ((vlo | vhi) && t0 + loc < nb)
+                    acc128(row + 2 * (size_t)nb * part + 2 * (size_t)(t0 + loc), I128{vlo, (int64_t)vhi});
+            }
+            __syncthreads();  // the tile is cleared again for the next key
         }
-        return;
-    }
-    for (uint32_t j = threadIdx.x; j < 4 * kSpan; j += kTileBlock) acc[j] = 0;
-    __syncthreads();
-    for (uint32_t i = b + threadIdx.x; i < e; i += kTileBlock) {
-        const uint2 p = pay[i];
-        const uint32_t o = p.y & 0xFFFFu;
-        const int ww = (int)((p.y >> 16) & 0xFFu), w = (int)(p.y >> 24);
-        const RampPoints q = ramp_points(__uint_as_float(p.x), w, ww);
-        unsigned long long* tp = acc + (q.neg ? 2 * kSpan : 0u);  // gets +w*S and +S
-        unsigned long long* tm = acc + (q.neg ? 0u : 2 * kSpan);  // gets (w+1)*S
-        lds_acc128(tp, o, q.aq);
-        if (t0 + o + 1 < nb) lds_acc128(tm, o + 1, q.p1);
-        if (t0 + o + (uint32_t)ww + 1 < nb) lds_acc128(tp, o + (uint32_t)ww + 1, q.sq);
-    }
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < 2 * kSpan; j += kTileBlock) {
-        const uint32_t part = j >= kSpan ? 1u : 0u, loc = j - part * kSpan;
-        const unsigned long long vlo = acc[2 * j], vhi = acc[2 * j + 1];
-        if ((vlo | vhi) && t0 + loc < nb)
-            acc128(row + 2 * (size_t)nb * part + 2 * (size_t)(t0 + loc), I128{vlo, (int64_t)vhi});
+        b = e;
     }
 }
 
@@ -2149,7 +2259,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     DevBuf hits(c, ar), so(c, ar), sm(c, ar), counters(c, ar), stats(c, ar), hist(c, ar), gs(c, ar), direct(c, ar),
         value(c, ar), logh(c, ar), logn(c, ar), dinst(c, ar), dsegs(c, ar), dsegoff(c, ar), drows(c, ar),
         dvrows(c, ar), msrc(c, ar), segc(c, ar), tkey(c, ar), tpay(c, ar), tsort(c, ar), tcnt(c, ar), toff(c, ar),
-        tcur(c, ar), tkb(c, ar);
+        tcur(c, ar), tkb(c, ar), tfirst(c, ar);
     // device memory that must start at a value: hipMemsetAsync, or in the
     // no-allocation render a fill kernel (no memset nodes if a caller
     // captures the stream)
@@ -2285,6 +2395,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         TRY(toff.alloc(((size_t)K + 1) * 4));
         TRY(tcur.alloc((size_t)K * 4));
         TRY(tkb.alloc(((size_t)n_items + 1) * 4));
+        TRY(tfirst.alloc((n_rec + kTileChunk - 1) / kTileChunk * 4));
         TRY(clear(tcnt, 0));
         EAR_HIP_TRY(c, hipMemcpyAsync(tkb.p, kbase.data(), 4 * kbase.size(), hipMemcpyHostToDevice, st));
         tile_out.key = tkey.as<uint32_t>();
@@ -2352,6 +2463,8 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
     const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
     const uint32_t g_tile = (uint32_t)c->num_cus * 4;  // count and scatter: the same grid
+    // splat: one workgroup per kTileChunk records of the most one launch can write
+    const uint32_t g_splat = (uint32_t)std::max<size_t>(1, (cap_sh * tile_stride + kTileChunk - 1) / kTileChunk);
     // eager: the bounce loop launched from the host (always in the
     // no-allocation render); timed: with per-kernel events
     const bool eager = tu.graph_iters == 0 || noalloc;
@@ -2398,11 +2511,13 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
             if (e4) hipEventRecord(e4->a, st);
             k_tile_count<<<g_tile, kTileBlock, 0, st>>>(tkey.as<uint32_t>(), cnt + 2, tile_stride, tcnt.as<uint32_t>(),
                                                         K);
-            k_tile_scan<<<1, 1024, 0, st>>>(tcnt.as<uint32_t>(), toff.as<uint32_t>(), tcur.as<uint32_t>(), K);
-            k_tile_scatter<<<g_tile, kTileBlock, 0, st>>>(tkey.as<uint32_t>(), tpay.as<uint2>(), cnt + 2, tile_stride,
-                                                          tcur.as<uint32_t>(), K, tsort.as<uint2>());
-            k_tile_splat<<<K, kTileBlock, 0, st>>>(di, tkb.as<uint32_t>(), n_items, toff.as<uint32_t>(),
-                                                   tsort.as<uint2>());
+            k_tile_scan<<<1, 1024, 0, st>>>(tcnt.as<uint32_t>(), toff.as<uint32_t>(), tcur.as<uint32_t>(), K,
+                                            tfirst.as<uint32_t>());
+            k_tile_scatter<<<g_tile, kTileBlock, 0, st>>>(di, tkb.as<uint32_t>(), n_items, tkey.as<uint32_t>(),
+                                                          tpay.as<uint2>(), cnt + 2, tile_stride, tcur.as<uint32_t>(),
+                                                          K, tsort.as<uint2>());
+            k_tile_splat<<<g_splat, kTileBlock, 0, st>>>(di, tkb.as<uint32_t>(), n_items, toff.as<uint32_t>(), K,
+                                                         tfirst.as<uint32_t>(), tsort.as<uint2>());
             if (e4) hipEventRecord(e4->b, st);
             launches += 4;
         }

@@ -370,10 +370,10 @@ typedef struct {
 #define EAR_SPLAT_DEFAULT 0u /* = EAR_SPLAT_ATOMIC */
 #define EAR_SPLAT_ATOMIC 1u  /* each ramp point is a 128-bit add to the histogram in HBM */
 #define EAR_SPLAT_TILED 2u   /* visible connections write records keyed by (row, time tile of
-                                1024 bins); per bounce iteration a counting sort groups them and
-                                one workgroup per tile accumulates them in LDS (64-bit LDS
-                                atomics), then adds the tile to HBM once (ramps wider than 62
-                                bins are added directly) */
+                                256 bins); per bounce iteration a counting sort groups them, and
+                                workgroups of 8192 sorted records accumulate each tile's ramps
+                                in LDS (64-bit LDS atomics), then add the tile to HBM once
+                                (ramps wider than 62 bins are added directly) */
 #define EAR_TRAVERSAL_DEFAULT 0u   /* = EAR_TRAVERSAL_FETCH_ALL */
 #define EAR_TRAVERSAL_STATIC 1u    /* one queue entry per lane per grid-stride step */
 #define EAR_TRAVERSAL_FETCH 2u     /* closest hit: persistent lanes take the next entry when

@@ -36,8 +36,8 @@ def _same(a, b):
 
 
 def test_tiled_splat_equals_direct_on_the_hall(gpu_ctx):
-    """20k-triangle hall, stereo, signed and |c| rows, 3 s: 3 x 12 rows x 130
-    tiles = 1560 keys, counted in LDS."""
+    """20k-triangle hall, stereo, signed and |c| rows, 3 s: 12 rows x 517
+    tiles of 256 bins = 6204 keys, counted in LDS."""
     s = scenes.hall(target_tris=20_000)
     kw = dict(recorders=s["recorders"], n_rays=300_000, src_pos=s["src"], air=s["air"], max_bounces=100,
               cutoff=1e-8, n_bins=132_300)
