@@ -10,16 +10,15 @@ for the three band contexts of one sound, fed by synthetic seeded input):
 The scene and its LBVH are resident in HBM before the timed region; each step
 emits, traces and splats every ray and finalizes the histograms.
 
-Prints one JSON line (rank 0, N=1).  `roofline` is for the closest-hit kernel
-(k_extend), priced at algorithmic bytes per query
-    24 (ray origin + direction) + 8 (hit record) + B * V + 36 * T
-with V / T the mean BVH node visits / triangle tests per query counted by an
-instrumented pass over the same rays, and B the bytes of one node that a visit
-reads (64 for the 4-wide BVH: origin and scales, four child boxes quantised
-to 8 bits per plane, four child references; 56 for the binary one: two fp32
-child boxes and two child references).  `cpu_baseline` is the CPU oracle (a
-brute-force restatement of the reference algorithm) timed on this host on the
-first K rays of the same Philox streams.
+Prints one JSON line (rank 0, N=1).  `rooflines` prices each bounce-loop
+kernel (k_extend_f, k_shade, k_shadow_f) at algorithmic bytes per unit of work
+over its HIP-event time (kernel_rooflines below), with node visits V and
+triangle tests T per query counted by an instrumented pass over the same rays;
+`roofline` is the dominant one (the largest time per render), and
+`render_roofline` the whole render at bytes per segment.  `traffic` is
+rocprofv3's FETCH_SIZE + WRITE_SIZE per launch where profiles/ holds it.
+`cpu_baseline` is the CPU oracle (a brute-force restatement of the reference
+algorithm) timed on this host on the first K rays of the same Philox streams.
 """
 from __future__ import annotations
 
@@ -53,9 +52,13 @@ def parse():
     ap.add_argument("--legs", default="main,batch,adjoint,animated,fit",
                     help="comma list of legs to run: main (the hall), batch, adjoint, animated, fit "
                          "(for profiling one leg)")
+    ap.add_argument("--splat", choices=["default", "atomic", "tiled"], default="default",
+                    help="the shadow kernels' splat (ear_tuning.splat) for the main leg")
+    ap.add_argument("--stack-lds", type=int, default=0, help="traversal stack entries in LDS (16 or 32; 0: default)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (PMC calibration) and exit")
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_extend_hall200k.json"))
+    ap.add_argument("--traffic-dir", default=os.path.join(ROOT, "profiles"),
+                    help="folder of traffic_<kernel>_hall200k.json (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch)")
     return ap.parse_args()
 
 
@@ -120,20 +123,23 @@ def main():
     log(f"LBVH: {info['n_nodes']} nodes, depth {info['max_depth']}, build {info['build_ms']:.2f} ms")
     kw = dict(recorders=s["recorders"], n_rays=a.rays, src_pos=s["src"], air=s["air"], max_bounces=a.max_bounces,
               cutoff=a.cutoff, n_bins=n_bins, abs_hist=True, want_raw=False, want_value=True)
+    ctx.set_tuning(splat={"default": 0, "atomic": 1, "tiled": 2}[a.splat], stack_lds=a.stack_lds)
 
     for i in range(a.warmup):
         r = api.render(ctx, sc, **kw)
         log(f"warmup {i}: {r.wall_s:.3f}s, {r.stats['segments']} segments")
-    walls, segs, ext_ms, ext_q, ext_launch, stats = [], 0, 0.0, 0, 0, None
+    walls, segs = [], 0
+    # per-kernel sums over the timed renders (HIP events around every launch)
+    acc = {k: 0 for k in ("extend_ms", "shade_ms", "shadow_ms", "splat_ms", "trace_ms", "extend_queries",
+                          "shadow_queries", "extend_launches", "shadow_launches", "shadow_contributions",
+                          "shadow_ramps", "segments", "rays")}
     t_start = time.perf_counter()
     for i in range(a.steps):
         r = api.render(ctx, sc, **kw)
         walls.append(r.wall_s)
         segs += r.stats["segments"]
-        ext_ms += r.stats["extend_ms"]
-        ext_q += r.stats["extend_queries"]
-        ext_launch += r.stats["extend_launches"]
-        stats = r.stats
+        for k in acc:
+            acc[k] += r.stats[k]
         log(f"step {i}: {r.wall_s:.3f}s, {r.stats['segments']} segments, trace {r.stats['trace_ms']:.1f} ms "
             f"(extend {r.stats['extend_ms']:.1f}, shade {r.stats['shade_ms']:.1f}, shadow {r.stats['shadow_ms']:.1f})")
     elapsed = time.perf_counter() - t_start
@@ -162,28 +168,16 @@ def main():
     except Exception as e:  # pragma: no cover - reporting only
         log(f"accuracy check failed: {e}")
 
-    roof = None
+    roof = rooflines = render_roof = None
     if not a.no_instrument:
         ri = api.render(ctx, sc, instrument=True, **kw)
-        st = ri.stats
-        V = st["node_visits_closest"] / max(1, st["extend_queries"])
-        T = st["tri_tests_closest"] / max(1, st["extend_queries"])
-        bpq = 24 + 8 + info["node_bytes"] * V + 36 * T
-        achieved = bpq * ext_q / (ext_ms * 1e-3) / 1e9
-        traffic = None
-        if os.path.exists(a.traffic_file):
-            with open(a.traffic_file) as f:
-                traffic = json.load(f).get("bytes_per_launch")
-        roof = {"bound": "hbm", "kernel": "k_extend", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_query": round(bpq, 1), "node_bytes": info["node_bytes"],
-                "node_visits_per_query": round(V, 3),
-                "tri_tests_per_query": round(T, 3),
-                "shadow_node_visits_per_query": round(st["node_visits_shadow"] / max(1, st["shadow_queries"]), 3),
-                "shadow_tri_tests_per_query": round(st["tri_tests_shadow"] / max(1, st["shadow_queries"]), 3),
-                "avg_launch_ms": round(ext_ms / max(1, ext_launch), 4), "launches": ext_launch,
-                "queries_per_launch": round(ext_q / max(1, ext_launch), 1)}
+        rooflines, render_roof = kernel_rooflines(acc, ri.stats, info, a.traffic_dir, a.steps)
+        # the dominant kernel (largest share of the timed renders' kernel time)
+        dom = max(rooflines, key=lambda k: rooflines[k]["ms_per_render"])
+        roof = dict(rooflines[dom], kernel=dom)
+        log(f"dominant kernel {dom}: {roof['achieved']:.0f} GB/s algorithmic, frac {roof['frac']:.3f}")
 
+    ctx.set_tuning()  # the other legs run with the library defaults
     extras = None
     if not a.no_extras and legs & {"batch", "adjoint", "animated", "fit"}:
         extras = {}
@@ -225,6 +219,8 @@ def main():
         "t60_mid_s": edc_t60,
         "accuracy": accuracy,
         "roofline": roof,
+        "rooflines": rooflines,
+        "render_roofline": render_roof,
         "cpu_baseline": cpu,
         "cpu_baseline_reference_threading": cpu_ref,
         "extras": extras,
@@ -232,21 +228,79 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def kernel_rooflines(acc, ist, info, traffic_dir, steps):
+    """Algorithmic-byte rooflines of the bounce loop's kernels over the timed
+    renders (acc: summed ear_stats), with node visits V and triangle tests T
+    per query from the instrumented render `ist`, and the whole render's
+    bytes per segment.  Bytes per unit:
+      k_extend_f per closest-hit query: 24 (o, d) + 8 (hit) + B V + 36 T
+      k_shade per shaded segment: 48 (queue entry) + 8 (hit) + 16 (normal and
+        material) + 16 (material row) + 48 x (segments that continue) + 32 x
+        (shadow requests written)
+      k_shadow_f per shadow query: 32 (request) + B V_sh + 36 T_sh, plus per
+        ramp 3 x 16 (128-bit points) with direct atomics, or with the
+        time-tiled splat 4 per record slot + 8 per record
+    B = 64 bytes per node visit.  `traffic` is rocprofv3's FETCH_SIZE +
+    WRITE_SIZE per launch of that kernel (scaled by a plain copy), where
+    profiles/ holds it for the kernel; measured_frac = traffic per average
+    launch time over the peak."""
+    B = info["node_bytes"]
+    qe, qs = max(1, acc["extend_queries"]), max(1, acc["shadow_queries"])
+    V = ist["node_visits_closest"] / max(1, ist["extend_queries"])
+    T = ist["tri_tests_closest"] / max(1, ist["extend_queries"])
+    Vs = ist["node_visits_shadow"] / max(1, ist["shadow_queries"])
+    Ts = ist["tri_tests_shadow"] / max(1, ist["shadow_queries"])
+    ramps_per_q = acc["shadow_ramps"] / qs
+    tiled = acc["splat_ms"] > 0
+    stride = 2 if acc["shadow_ramps"] > 1.5 * acc["shadow_contributions"] else 1
+    cont = 1.0 - acc["rays"] / max(1, acc["segments"])  # shaded segments whose ray goes on
+    per = {
+        "k_extend_f": (24 + 8 + B * V + 36 * T, qe, acc["extend_ms"], acc["extend_launches"], "closest-hit query"),
+        "k_shade": (48 + 8 + 16 + 16 + 48 * cont + 32 * qs / qe, qe, acc["shade_ms"], acc["extend_launches"],
+                    "shaded segment"),
+        "k_shadow_f": (32 + B * Vs + 36 * Ts + (4 * stride + 8 * ramps_per_q if tiled else 48 * ramps_per_q), qs,
+                       acc["shadow_ms"], acc["shadow_launches"], "shadow query"),
+    }
+    out = {}
+    for k, (bpu, units, ms, launches, unit) in per.items():
+        achieved = bpu * units / max(1e-9, ms * 1e-3) / 1e9
+        avg_ms = ms / max(1, launches)
+        traffic = None
+        tf = os.path.join(traffic_dir, f"traffic_{k}_hall200k.json")
+        if os.path.exists(tf):
+            with open(tf) as f:
+                traffic = json.load(f).get("bytes_per_launch")
+        out[k] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                  "measured_frac": (round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
+                  "bytes_per_unit": round(bpu, 1), "unit_of_work": unit,
+                  "units_per_launch": round(units / max(1, launches), 1), "avg_launch_ms": round(avg_ms, 4),
+                  "launches": launches, "ms_per_render": round(ms / steps, 2)}
+    out["k_extend_f"].update(node_visits_per_query=round(V, 3), tri_tests_per_query=round(T, 3))
+    out["k_shadow_f"].update(node_visits_per_query=round(Vs, 3), tri_tests_per_query=round(Ts, 3),
+                             ramps_per_query=round(ramps_per_q, 4), splat="tiled" if tiled else "atomic")
+    if tiled:
+        out["k_tile"] = {"ms_per_render": round(acc["splat_ms"] / steps, 2),
+                         "note": "k_tile_count + k_tile_scan + k_tile_scatter + k_tile_splat, per bounce iteration"}
+    # the whole render (all kernels, trace_ms): bytes per segment with every
+    # shadow query and ramp of the render
+    seg = max(1, acc["segments"])
+    b_seg = (96 + 20 + B * (V + Vs * qs / seg) + 36 * (T + Ts * qs / seg) + 48 * acc["shadow_ramps"] / seg)
+    ach = b_seg * seg / max(1e-9, acc["trace_ms"] * 1e-3) / 1e9
+    render_roof = {"bytes_per_segment": round(b_seg, 1), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                   "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                   "note": "96 ray state + 20 material + 64 (V + V_sh) + 36 (T + T_sh) + 48 per ramp, per segment"}
+    return out, render_roof
+
+
 def bench_batch(ctx, n_rooms=256, seed=3):
     """Batching: n_rooms box or L rooms of 12..50k triangles (log-uniform),
     10k..2M rays each (log-uniform), 3 bands, 1 s histograms, in one
     ear_render_batch, beside the same renders one call each."""
-    import numpy as np
-
     from ear_amd import api, scenes
 
-    rng = np.random.default_rng(seed)
     t0 = time.time()
-    specs = []
-    for i in range(n_rooms):
-        nt = int(np.exp(rng.uniform(np.log(12), np.log(50_000))))
-        rays = int(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
-        specs.append((scenes.room(seed=1000 + i, target_tris=nt), rays))
+    specs = scenes.batch_rooms(n_rooms, seed=seed)
     log(f"batch: {n_rooms} rooms generated in {time.time() - t0:.1f}s")
     t0 = time.perf_counter()
     scs = [api.Scene(ctx, rm["tris"], rm["mat"], rm["materials"]) for rm, _ in specs]

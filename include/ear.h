@@ -367,7 +367,9 @@ typedef struct {
     uint32_t splat;          /* EAR_SPLAT_*: how the shadow kernels add ramps to the
                                 histograms (same histograms either way) */
 } ear_tuning;
-#define EAR_SPLAT_DEFAULT 0u /* = EAR_SPLAT_ATOMIC */
+#define EAR_SPLAT_DEFAULT 0u /* EAR_SPLAT_TILED when the call's (row, tile) keys number at most
+                                8192 (3 bands x channels x kinds x tiles: a 3 s stereo render with
+                                the |c| histogram has 6204), else EAR_SPLAT_ATOMIC */
 #define EAR_SPLAT_ATOMIC 1u  /* each ramp point is a 128-bit add to the histogram in HBM */
 #define EAR_SPLAT_TILED 2u   /* visible connections write records keyed by (row, time tile of
                                 256 bins); per bounce iteration a counting sort groups them, and
