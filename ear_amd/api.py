@@ -71,7 +71,8 @@ class Context:
         has more than 64 triangles, else the last 2^20; 1 turns it off);
         splat: 1 direct 128-bit atomics per ramp point, 2 records binned by
         time tile and accumulated in LDS (same histograms); 0 the tiled one
-        when the call's (row, tile) keys fit in 8192, else the atomics."""
+        when the call's (row, tile) keys fit in 8192 and its queues hold at
+        least 2^22 rays, else the atomics."""
         t = F.ear_tuning(block_extend, block_shade, block_shadow, ray_chunk, waves_per_cu, stack_lds, traversal,
                          graph_iters, refill, fetch_chunk, fetch_segments, tail_rays, splat)
         _check(self.lib, self.lib.ear_ctx_set_tuning(self.h, C.byref(t)), self.h)

@@ -23,9 +23,10 @@ LAUNCH_SPACE = {
     "stack_lds": [16, 32],
     "ray_chunk": [1 << 21, 1 << 22, 1 << 23],
     "graph_iters": [0, 8],
+    "splat": [1, 2],
 }
 SCENE_SPACE = {
-    "builder": [1, 2],
+    "builder": [1, 2, 3, 4],
     "width": [2, 4],
     "leaf_size": [1, 2, 4],
 }
@@ -84,10 +85,12 @@ def tune(ctx, tris, mat, materials, render_kw: dict, *, sample_rays: int | None 
     if sample_rays:
         kw["n_rays"] = sample_rays
     ctx.set_tuning()
-    scene_opts = {"builder": 2, "width": 4, "leaf_size": 2}
+    scene_opts = {"builder": 4, "width": 4, "leaf_size": 2}
     scene_trials = []
     if tune_scene:
         def measure_scene(opts):
+            if opts["builder"] in (3, 4) and opts["width"] != 4:
+                return float("inf")  # the device PLOC and SAH builds make 4-wide trees only
             with api.Scene(ctx, tris, mat, materials, **opts) as sc:
                 t, _ = _timed_render(ctx, sc, kw, repeats)
             return t

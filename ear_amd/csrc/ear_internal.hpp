@@ -59,6 +59,7 @@ struct DeviceScene {
     uint32_t n_nodes = 0;
     uint32_t width = 2;
     uint32_t n_nodes4 = 0;
+    uint32_t n_nodes4_alloc = 0;  // nodes4 entries allocated (>= n_nodes4)
     uint32_t depth4 = 0;
     double collapse_ms = 0;
     Bvh4Node* nodes4 = nullptr;

@@ -2214,19 +2214,22 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
         return EAR_ERR_CAPACITY;
     }
     const ear_tuning& tu = c->tuning;
-    // the shadow kernels' splat: records binned by time tile and accumulated
-    // in LDS, or direct global atomics (always in the no-allocation render:
-    // the records would not fit its region).  By default the tiled splat when
-    // the call's keys fit the LDS counters (on the 200k hall, 463.9 against
-    // 468.5 ms per render); a batch of many renders keeps the atomics, whose
-    // cost does not grow with the number of keys.
-    const bool tiled = !noalloc && n_keys < 0x7FFFFFFFull &&
-                       (tu.splat == EAR_SPLAT_TILED || (tu.splat == EAR_SPLAT_DEFAULT && n_keys <= kLdsKeys));
     kbase[n_items] = (uint32_t)std::min<uint64_t>(n_keys, 0xFFFFFFFFull);
     const uint32_t K = kbase[n_items];
     uint32_t chunk = tu.ray_chunk ? tu.ray_chunk : (1u << 23);
     uint64_t cap64 = std::min<uint64_t>((uint64_t)chunk * 3, std::max<uint64_t>(1, total_rays));
     uint32_t cap_rays = (uint32_t)cap64;
+    // the shadow kernels' splat: records binned by time tile and accumulated
+    // in LDS, or direct global atomics (always in the no-allocation render:
+    // the records would not fit its region).  By default the tiled splat when
+    // the call's keys fit the LDS counters and the queues hold at least 2^22
+    // rays (on the 200k hall, 3 x 10M rays: 463.9 against 468.5 ms per
+    // render); otherwise the atomics: a batch of many renders has too many
+    // keys, and smaller renders do not repay the tile kernels' fixed cost per
+    // iteration (the 500k hall at 3 x 781k rays: 9% slower tiled).
+    const bool tiled = !noalloc && n_keys < 0x7FFFFFFFull &&
+                       (tu.splat == EAR_SPLAT_TILED ||
+                        (tu.splat == EAR_SPLAT_DEFAULT && n_keys <= kLdsKeys && cap_rays >= (1u << 22)));
     // walks vary in length; refilling idle lanes keeps waves busy (both
     // kernels by default)
     const bool fetch_e = tu.traversal != EAR_TRAVERSAL_STATIC;
@@ -2466,7 +2469,9 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const uint32_t g_shade = persistent_grid(c, cap_rays, waves, kShadeBlock);
     const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
     const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
-    const uint32_t g_tile = (uint32_t)c->num_cus * 4;  // count and scatter: the same grid
+    // count and scatter: the same grid, about 16k record slots per workgroup
+    const uint32_t g_tile = (uint32_t)std::max<size_t>(
+        1, std::min<size_t>((size_t)c->num_cus * 4, cap_sh * tile_stride / (kTileBlock * kTileBatch * 4)));
     // splat: one workgroup per kTileChunk records of the most one launch can write
     const uint32_t g_splat = (uint32_t)std::max<size_t>(1, (cap_sh * tile_stride + kTileChunk - 1) / kTileChunk);
     // eager: the bounce loop launched from the host (always in the

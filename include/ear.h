@@ -80,16 +80,23 @@ typedef struct {
 typedef struct {
     uint32_t leaf_size;  /* max triangles per BVH leaf, 1..16 (0 = library default, 2) */
     uint32_t width;      /* children per node: 2 or 4 (0 = library default, 4) */
-    uint32_t builder;    /* EAR_BVH_* (0 = library default, EAR_BVH_SAH) */
+    uint32_t builder;    /* EAR_BVH_* (0 = library default, EAR_BVH_SAH_GPU when 4-wide) */
     uint32_t reserved;
 } ear_bvh_opts;
-#define EAR_BVH_DEFAULT 0u
+#define EAR_BVH_DEFAULT 0u /* EAR_BVH_SAH_GPU for 4-wide trees, EAR_BVH_SAH for 2-wide ones */
 #define EAR_BVH_LBVH 1u  /* device linear BVH: Morton order, Karras hierarchy */
 #define EAR_BVH_SAH 2u   /* host binned surface-area-heuristic build, then uploaded */
+#define EAR_BVH_PLOC 3u  /* device build: locally-ordered clustering of the Morton-sorted
+                            triangles (Meister and Bittner 2018) and a device collapse into
+                            the quantised 4-wide nodes (4-wide only) */
+#define EAR_BVH_SAH_GPU 4u /* device build: EAR_BVH_SAH's binned-SAH splits made level by
+                              level on the device (one workgroup per node), then the same
+                              device collapse (4-wide only) */
 
 /* Uploads the merged triangle soup of every MESH block (the reference merges
  * all meshes into meshes[0], src/Scene.cpp:103-106, src/Mesh.cpp:116-123)
- * and builds its BVH (host SAH build by default, or the device LBVH).  Triangles are given as three vertex
+ * and builds its BVH (by default the device binned-SAH build; or the device
+ * PLOC, the host SAH or the device LBVH build).  Triangles are given as three vertex
  * arrays of n_tris xyz triples each, in the reference's triangle order; the
  * triangle index reported by queries is the position in that order.
  * n_tris may be 0 (the reference's Mesh::Empty(), src/EAR.cpp:160-163). */
@@ -369,7 +376,8 @@ typedef struct {
 } ear_tuning;
 #define EAR_SPLAT_DEFAULT 0u /* EAR_SPLAT_TILED when the call's (row, tile) keys number at most
                                 8192 (3 bands x channels x kinds x tiles: a 3 s stereo render with
-                                the |c| histogram has 6204), else EAR_SPLAT_ATOMIC */
+                                the |c| histogram has 6204) and its queues hold at least 2^22
+                                rays, else EAR_SPLAT_ATOMIC */
 #define EAR_SPLAT_ATOMIC 1u  /* each ramp point is a 128-bit add to the histogram in HBM */
 #define EAR_SPLAT_TILED 2u   /* visible connections write records keyed by (row, time tile of
                                 256 bins); per bounce iteration a counting sort groups them, and

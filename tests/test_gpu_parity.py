@@ -72,7 +72,8 @@ def test_shoebox_parity_bitexact_paths(gpu_ctx, oracle, spec):
         np.testing.assert_allclose(g.abs_value[b], expa, rtol=1e-9, atol=1e-12 * np.abs(expa).max())
 
 
-@pytest.mark.parametrize("width,builder,leaf", [(2, 1, 4), (4, 1, 4), (2, 2, 1), (4, 2, 3)])
+@pytest.mark.parametrize("width,builder,leaf", [(2, 1, 4), (4, 1, 4), (2, 2, 1), (4, 2, 3), (4, 3, 1), (4, 3, 2),
+                                               (4, 3, 5), (4, 4, 1), (4, 4, 2), (4, 4, 6)])
 def test_hall_queries_match_bruteforce(gpu_ctx, oracle, width, builder, leaf):
     s = scenes.hall(target_tris=20000)
     rng = np.random.default_rng(7)
@@ -99,7 +100,7 @@ def test_hall_queries_match_bruteforce(gpu_ctx, oracle, width, builder, leaf):
         assert occ[i] == osc.any_hit(o[i], q[i])
 
 
-@pytest.mark.parametrize("builder,leaf", [(1, 2), (2, 1)])
+@pytest.mark.parametrize("builder,leaf", [(1, 2), (2, 1), (3, 2), (4, 2)])
 @pytest.mark.parametrize("offset,scale", [((3.0e4, -1.5e4, 8.0e3), 1.0), ((0.0, 0.0, 0.0), 0.05)])
 def test_far_or_tiny_scene_queries_match_bruteforce(gpu_ctx, oracle, builder, leaf, offset, scale):
     """The hall 30 km from the origin (float spacing there: 2 mm, against
@@ -135,7 +136,8 @@ def test_far_or_tiny_scene_queries_match_bruteforce(gpu_ctx, oracle, builder, le
 
 
 @pytest.mark.parametrize("leaf,width,builder", [(1, 4, 1), (2, 2, 1), (8, 4, 1), (16, 2, 1), (1, 4, 2), (3, 2, 2),
-                                               (16, 4, 2)])
+                                               (16, 4, 2), (1, 4, 3), (2, 4, 3), (16, 4, 3), (1, 4, 4), (2, 4, 4),
+                                               (16, 4, 4)])
 def test_bvh_shape_does_not_change_results(gpu_ctx, shoebox, leaf, width, builder):
     s = scenes.shoebox(tess=(6, 6, 3))
     kw = dict(recorders=s["recorders"], n_rays=2000, src_pos=s["src"], air=s["air"], max_bounces=50, cutoff=1e-6,
@@ -146,6 +148,78 @@ def test_bvh_shape_does_not_change_results(gpu_ctx, shoebox, leaf, width, builde
         rb = api.render(gpu_ctx, b, **kw)
     np.testing.assert_array_equal(ra.raw, rb.raw)
     np.testing.assert_array_equal(ra.abs_raw, rb.abs_raw)
+
+
+@pytest.mark.parametrize("builder", [3, 4])
+@pytest.mark.parametrize("case", ["one", "two", "three", "coincident", "within_one_leaf", "random_soup"])
+def test_device_builds_small_and_degenerate_scenes_match_bruteforce(gpu_ctx, oracle, case, builder):
+    """The device builds (PLOC, device SAH) on the inputs their clustering,
+    splitting and collapse treat specially: 1, 2 and 3 triangles, many
+    triangles with one centroid (every Morton code equal; every SAH bin
+    empty but one, so ranges split by count), a scene of leaf_size triangles
+    (the root's range is already a leaf), and a soup of 3000 random
+    triangles: every closest and any hit equals the brute-force scan."""
+    rng = np.random.default_rng({"one": 1, "two": 2, "three": 3, "coincident": 4, "within_one_leaf": 5,
+                                 "random_soup": 6}[case])
+    n = {"one": 1, "two": 2, "three": 3, "coincident": 40, "within_one_leaf": 8, "random_soup": 3000}[case]
+    if case == "coincident":
+        c = np.array([1.0, 2.0, 3.0])
+        tris = c + rng.normal(size=(n, 3, 3)) * np.array([1.0, 1.0, 0.0])
+        tris -= tris.mean(axis=1, keepdims=True) - c  # every centroid exactly at c
+    else:
+        base = rng.uniform(-5, 5, size=(n, 1, 3))
+        tris = base + rng.normal(size=(n, 3, 3))
+    tris = tris.astype(np.float32)
+    mat = np.zeros(n, np.uint16)
+    mats = [([0.7] * 3, None, None)]
+    leaf = 8 if case == "within_one_leaf" else 2
+    with api.Scene(gpu_ctx, tris, mat, mats, leaf_size=leaf, width=4, builder=builder) as sc:
+        info = sc.info()
+        assert info["builder"] == builder and info["width"] == 4
+        m = 2000
+        o = rng.uniform(-8, 8, size=(m, 3)).astype(np.float32)
+        tgt = tris[rng.integers(0, n, m)].mean(axis=1)
+        d = tgt - o
+        d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        t, idx = sc.query_closest(o, d)
+        q = rng.uniform(-8, 8, size=(m, 3)).astype(np.float32)
+        occ = sc.query_occluded(o, q)
+    osc = oracle.OracleScene(tris, mat, mats)
+    hits = 0
+    for i in range(m):
+        ot, oi = osc.closest_hit(o[i], d[i])
+        assert idx[i] == oi, (i, idx[i], oi)
+        if oi >= 0:
+            hits += 1
+            assert np.float32(t[i]) == np.float32(ot)
+        assert occ[i] == osc.any_hit(o[i], q[i])
+    assert hits > m // 4
+
+
+def test_device_sah_tree_quality_equals_host_sah(gpu_ctx):
+    """The device SAH build of the 200k-triangle hall makes the host build's
+    splits, so its 4-wide tree needs the node visits and triangle tests per
+    query of the host tree (within 2%: the collapse's choice among children
+    of equal area may differ; an instrumented render of the same rays counts
+    them), and builds in under a quarter of the host build's time (without
+    the host's 4-wide collapse, which the device build includes).  The PLOC
+    tree's counts are printed beside them."""
+    s = scenes.hall()
+    kw = dict(recorders=s["recorders"], n_rays=200_000, src_pos=s["src"], air=s["air"], max_bounces=100,
+              cutoff=1e-8, n_bins=132_300, instrument=True)
+    out = {}
+    for b in (2, 3, 4):
+        with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"], builder=b) as sc:
+            st = api.render(gpu_ctx, sc, **kw).stats
+            out[b] = (st["node_visits_closest"] / st["extend_queries"], st["tri_tests_closest"] / st["extend_queries"],
+                      st["node_visits_shadow"] / st["shadow_queries"], st["tri_tests_shadow"] / st["shadow_queries"],
+                      sc.info()["build_ms"], sc.info()["n_nodes"])
+    for b, name in ((2, "host SAH"), (3, "PLOC"), (4, "device SAH")):
+        print("%-10s visits/tests closest %.3f %.3f shadow %.3f %.3f build %.1f ms nodes %d" % ((name,) + out[b]),
+              flush=True)
+    for k in range(4):
+        assert out[4][k] <= 1.02 * out[2][k], (k, out[4], out[2])
+    assert out[4][4] < 0.25 * out[2][4]
 
 
 def test_launch_shape_determinism(gpu_ctx, shoebox):
