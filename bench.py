@@ -7,7 +7,7 @@ for the three band contexts of one sound, fed by synthetic seeded input):
   alpha ~ U(0.02, 0.6), one point source, stereo recorder (head 0.17 m),
   10M rays per band, <= 100 segments per ray, intensity cut-off 1e-8,
   3 s histograms at 44.1 kHz (132,300 bins per channel and band).
-The scene and its LBVH are resident in HBM before the timed region; each step
+The scene and its BVH (built on the device) are resident in HBM before the timed region; each step
 emits, traces and splats every ray and finalizes the histograms.
 
 Prints one JSON line (rank 0, N=1).  `rooflines` prices each bounce-loop
@@ -120,7 +120,7 @@ def main():
     ctx = api.Context(0)
     sc = api.Scene(ctx, s["tris"], s["mat"], s["materials"])
     info = sc.info()
-    log(f"LBVH: {info['n_nodes']} nodes, depth {info['max_depth']}, build {info['build_ms']:.2f} ms")
+    log(f"BVH: {info['n_nodes']} nodes, depth {info['max_depth']}, build {info['build_ms']:.2f} ms")
     kw = dict(recorders=s["recorders"], n_rays=a.rays, src_pos=s["src"], air=s["air"], max_bounces=a.max_bounces,
               cutoff=a.cutoff, n_bins=n_bins, abs_hist=True, want_raw=False, want_value=True)
     ctx.set_tuning(splat={"default": 0, "atomic": 1, "tiled": 2}[a.splat], stack_lds=a.stack_lds)
