@@ -153,20 +153,33 @@ def main():
         edc_t60 = acoustics.t60_iso(r.abs_value[1, 0])
     except Exception as e:  # pragma: no cover - reporting only
         log(f"t60 failed: {e}")
-    # accuracy: the same render on an independent random stream; the EDC and
-    # T60 differences between the two bound each render's own error (about
-    # the difference over sqrt(2))
+    # accuracy: the EDC (every band and ear, to -60 dB) and the mid band's T60
+    # against a render of 4x the rays on an independent random stream (whose
+    # own error is about half this render's)
     accuracy = None
     try:
-        r2 = api.render(ctx, sc, **dict(kw, seed=0x5EED))
-        edc_err = max(acoustics.edc_error(r.abs_value[b, c], r2.abs_value[b, c]) for b in range(3)
+        r4 = api.render(ctx, sc, **dict(kw, n_rays=4 * a.rays, seed=0x5EED))
+        edc_err = max(acoustics.edc_error(r.abs_value[b, c], r4.abs_value[b, c]) for b in range(3)
                       for c in range(r.abs_value.shape[1]))
-        t60_b = acoustics.t60_iso(r2.abs_value[1, 0])
-        accuracy = {"edc_error_between_two_seeds": round(edc_err, 5),
-                    "t60_mid_s_other_seed": t60_b, "t60_rel_difference": round(abs(t60_b / edc_t60 - 1.0), 5)}
-        log(f"accuracy: EDC error between two seeds {edc_err:.4f}, T60 {edc_t60:.4f} / {t60_b:.4f} s")
+        t60_b = acoustics.t60_iso(r4.abs_value[1, 0])
+        accuracy = {"edc_error_vs_4x_rays": round(edc_err, 5), "t60_mid_s_4x_rays": t60_b,
+                    "t60_rel_error_vs_4x_rays": round(abs(edc_t60 / t60_b - 1.0), 5)}
+        log(f"accuracy: EDC error against 4x the rays {edc_err:.4f}, T60 {edc_t60:.4f} / {t60_b:.4f} s")
     except Exception as e:  # pragma: no cover - reporting only
         log(f"accuracy check failed: {e}")
+    # the bounce loop replayed as a captured HIP graph (same histograms)
+    graph = None
+    try:
+        ctx.set_tuning(splat={"default": 0, "atomic": 1, "tiled": 2}[a.splat], stack_lds=a.stack_lds, graph_iters=8)
+        api.render(ctx, sc, **kw)
+        t0 = time.perf_counter()
+        rg = api.render(ctx, sc, **kw)
+        graph = {"graph_iters": 8, "ms_per_render": round(1e3 * (time.perf_counter() - t0), 3),
+                 "segments": int(rg.stats["segments"])}
+        ctx.set_tuning(splat={"default": 0, "atomic": 1, "tiled": 2}[a.splat], stack_lds=a.stack_lds)
+        log(f"graph replay: {graph['ms_per_render']} ms per render")
+    except Exception as e:  # pragma: no cover - reporting only
+        log(f"graph replay failed: {e}")
 
     roof = rooflines = render_roof = None
     if not a.no_instrument:
@@ -218,6 +231,7 @@ def main():
         "bvh_build_ms": round(info["build_ms"], 3),
         "t60_mid_s": edc_t60,
         "accuracy": accuracy,
+        "graph_replay": graph,
         "roofline": roof,
         "rooflines": rooflines,
         "render_roofline": render_roof,
@@ -327,14 +341,15 @@ def bench_batch(ctx, n_rooms=256, seed=3):
             "single_calls_segments_per_s": round(segs1 / single, 1), "speedup": round(single / wall, 3)}
 
 
-def bench_adjoint(ctx, n_rooms=32):
+def bench_adjoint(ctx, n_rooms=32, rays_accurate=4_000_000):
     """Differentiation: the 12 x 9 x 5 m shoebox (6 walls x 3 bands
     absorption + 6 specularities), 1M rays, 4096-bin log-EDC loss against a
     target rendered at the true parameters, evaluated at parameters off by
     +-20%: forward, adjoint and their rate beside the forward alone; the EDC
     error of the 1M-ray forward against 8M rays; then n_rooms rooms of
-    1k..100k triangles (250k rays each) through the batched forward and
-    adjoint."""
+    1k..100k triangles through the batched forward and adjoint at 250k rays
+    each and at rays_accurate each (the accuracy point: the EDC error of the
+    rooms least accurate at 250k rays against 4x the rays)."""
     import numpy as np
 
     from ear_amd import acoustics, api, fit, scenes
@@ -410,22 +425,28 @@ def bench_adjoint(ctx, n_rooms=32):
     refs, _, _ = api.render_batch(ctx, [(sc, dict(k, n_rays=2_000_000, seed=0x5EED)) for sc, k in fwd_items])
     errs = [max(acoustics.edc_error(x.abs_value[b, 0], y.abs_value[b, 0]) for b in range(3))
             for x, y in zip(res, refs)]
-    # the same batch at 2M rays per room: forward + adjoint rate, and the EDC
-    # error between two independent 2M-ray renders (each render's own error
-    # is about that over sqrt(2))
-    kws2 = [dict(k, n_rays=2_000_000) for k in kws]
+    # the accuracy point: the same batch at R rays per room, forward + adjoint
+    # timed, and each room's EDC error (every band, to -60 dB) against an
+    # independent render of 4R rays, for the 8 rooms whose 250k-ray error
+    # was largest (the 4R render's own error is about half the R render's)
+    R = rays_accurate
+    kwsR = [dict(k, n_rays=R) for k in kws]
     t0b = time.perf_counter()
-    res2, st2, _ = api.render_batch(ctx, [(sc, k) for sc, k in zip(scs, kws2)])
-    adjs2 = [fit.log_edc_loss(x.abs_value * 1.1, t, fit.decay_mask(t))[1] for x, t in zip(res2, targets)]
+    resR, stR, _ = api.render_batch(ctx, [(sc, k) for sc, k in zip(scs, kwsR)])
+    adjsR = [fit.log_edc_loss(x.abs_value * 1.1, t, fit.decay_mask(t))[1] for x, t in zip(resR, targets)]
     t1b = time.perf_counter()
-    api.render_backward_batch(ctx, [(sc, dict(k, adj_abs=a, local=True)) for sc, k, a in zip(scs, kws2, adjs2)])
+    api.render_backward_batch(ctx, [(sc, dict(k, adj_abs=a, local=True)) for sc, k, a in zip(scs, kwsR, adjsR)])
     t2b = time.perf_counter()
-    errs2 = [max(acoustics.edc_error(x.abs_value[b, 0], y.abs_value[b, 0]) for b in range(3))
-             for x, y in zip(res2, refs)]
+    worst = [int(i) for i in np.argsort(errs)[::-1][:8]]
+    refs4, _, _ = api.render_batch(ctx, [(scs[i], dict(kws[i], n_rays=4 * R, seed=0x5EED)) for i in worst])
+    errsR = [max(acoustics.edc_error(resR[i].abs_value[b, 0], y.abs_value[b, 0]) for b in range(3))
+             for i, y in zip(worst, refs4)]
     for sc in scs:
         sc.close()
     bsegs = st["segments"]
     log(f"adjoint batch: {n_rooms} rooms, {bsegs} segments, forward {t1 - t0:.3f}s, backward {t2 - t1:.3f}s")
+    log(f"adjoint batch at {R} rays per room: forward {t1b - t0b:.3f}s, backward {t2b - t1b:.3f}s, EDC error "
+        f"max {max(errsR):.4f} over the 8 rooms checked")
     out["batch"] = {"workload": f"{n_rooms} box/L rooms, 1k..100k triangles, 250k rays each, 3 bands, 4096-bin "
                                 f"log-EDC loss", "segments": int(bsegs), "forward_s": round(t1 - t0, 4),
                     "backward_s": round(t2 - t1, 4),
@@ -433,12 +454,14 @@ def bench_adjoint(ctx, n_rooms=32):
                     "forward_segments_per_s": round(bsegs / (t1 - t0), 1),
                     "edc_error_vs_2M_rays_median": round(float(np.median(errs)), 5),
                     "edc_error_vs_2M_rays_max": round(float(np.max(errs)), 5),
-                    "at_2M_rays": {"segments": int(st2["segments"]), "forward_s": round(t1b - t0b, 4),
-                                   "backward_s": round(t2b - t1b, 4),
-                                   "forward_segments_per_s": round(st2["segments"] / (t1b - t0b), 1),
-                                   "forward_adjoint_segments_per_s": round(st2["segments"] / (t2b - t0b), 1),
-                                   "edc_error_between_two_seeds_median": round(float(np.median(errs2)), 5),
-                                   "edc_error_between_two_seeds_max": round(float(np.max(errs2)), 5)}}
+                    f"at_{R // 1_000_000}M_rays": {
+                        "segments": int(stR["segments"]), "forward_s": round(t1b - t0b, 4),
+                        "backward_s": round(t2b - t1b, 4),
+                        "forward_segments_per_s": round(stR["segments"] / (t1b - t0b), 1),
+                        "forward_adjoint_segments_per_s": round(stR["segments"] / (t2b - t0b), 1),
+                        "rooms_checked": worst,
+                        f"edc_error_vs_{4 * R // 1_000_000}M_rays_max": round(float(np.max(errsR)), 5),
+                        f"edc_error_vs_{4 * R // 1_000_000}M_rays_median": round(float(np.median(errsR)), 5)}}
     return out
 
 
@@ -548,6 +571,18 @@ def bench_fit(ctx, steps=200, n_rooms=32, room_steps=10):
     return out
 
 
+def cpu_model() -> str | None:
+    """The host CPU's model name (as lscpu reports it), from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(s, kw, target_s):
     """The CPU oracle (port of the reference's brute-force tracer) on the first
     K rays of each band, all host threads; K is calibrated to ~target_s."""
@@ -574,6 +609,7 @@ def cpu_baseline(s, kw, target_s):
         t, segs = run(k)
     log(f"cpu baseline: {3 * k} rays, {segs} segments in {t:.2f}s on {nthreads} threads")
     return {"value": round(segs / t, 2), "unit": "segments/s", "cores": nthreads, "kind": "port",
+            "cpu_model": cpu_model(), "cpus_online": os.cpu_count(),
             "sample": f"first {k} rays of each of the 3 band contexts of the same workload ({segs} segments, "
                       f"{t:.1f} s), brute-force triangle scans as the reference"}
 

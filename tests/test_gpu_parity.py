@@ -259,6 +259,36 @@ def test_stereo_and_mesh_source_parity(gpu_ctx, oracle):
                                o["hist_abs"], n_rays * 41)
 
 
+@pytest.mark.parametrize("case", ["stereo_shoebox", "mesh_source", "stereo_hall"])
+def test_tail_kernel_against_the_oracle(gpu_ctx, oracle, case):
+    """Renders that the tail kernel carries (no hit log, so the tail launch is
+    on): the 12-triangle stereo shoebox and its mesh source (the whole render
+    in k_tail), and the 20k-triangle hall with its stereo recorder (3000 rays
+    per band: the live rays are below 2^20 from the first check, so k_tail
+    takes them after four bounce iterations), against the oracle: segment
+    and contribution counts exactly, both ears' histograms within the gate."""
+    if case == "stereo_hall":
+        s = scenes.hall(target_tris=20_000)
+        n_rays, kw = 3000, dict(max_bounces=100, cutoff=1e-8, n_bins=132_300)
+    else:
+        s = scenes.shoebox(stereo=True)
+        n_rays, kw = 20_000, dict(max_bounces=200, cutoff=1e-8, n_bins=88_200)
+    src_tris = None
+    if case == "mesh_source":
+        src_tris = np.array([[[4.0, 3.0, 1.0], [5.0, 3.0, 1.0], [4.0, 4.0, 1.2]]], np.float32)
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
+        g = api.render(gpu_ctx, sc, recorders=s["recorders"], n_rays=n_rays, src_pos=s["src"], air=s["air"],
+                       src_tris=src_tris, **kw)
+    assert g.stats["bounces_launched"] <= 8  # the tail launch took the rest
+    ora = oracle_bands(oracle, s, (0, 1, 2), n_rays, src_tris=src_tris, **kw)
+    assert g.stats["segments"] == sum(int(o["segments"]) for o in ora.values())
+    assert g.stats["contributions"] == sum(int(o["contributions"]) for o in ora.values())
+    for b in (0, 1, 2):
+        o = ora[b]
+        assert_hist_parity(api.raw_to_float(g.raw[b]), api.raw_to_float(g.abs_raw[b]), o["hist_signed"],
+                           o["hist_abs"], n_rays * (kw["max_bounces"] + 1))
+
+
 @pytest.mark.parametrize("width", [2, 4])
 def test_fetch_traversal_is_bitwise_identical(gpu_ctx, width):
     """Persistent lanes that refill from the queue walk every ray exactly as
