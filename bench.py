@@ -371,10 +371,14 @@ def bench_adjoint(ctx, n_rooms=32, rays_accurate=4_000_000):
         r = api.render(ctx, sc, **kw)
         t_fwd = time.perf_counter() - t0
         loss, adj = fit.log_edc_loss(r.abs_value, target, mask)
+        api.render_backward(ctx, sc, adj_abs=adj, score=True, **kw)  # warm-up
+        t0 = time.perf_counter()
+        api.render_backward(ctx, sc, adj_abs=adj, score=True, **kw)
+        t_bwd = time.perf_counter() - t0
         api.render_backward(ctx, sc, adj_abs=adj, local=True, **kw)  # warm-up
         t0 = time.perf_counter()
         api.render_backward(ctx, sc, adj_abs=adj, local=True, **kw)
-        t_bwd = time.perf_counter() - t0
+        t_bwd_local = time.perf_counter() - t0
         ref = api.render(ctx, sc, **dict(kw, n_rays=8_000_000, seed=0x5EED))
         edc_err = max(acoustics.edc_error(r.abs_value[b, 0], ref.abs_value[b, 0]) for b in range(3))
         segs = r.stats["segments"]
@@ -385,16 +389,19 @@ def bench_adjoint(ctx, n_rooms=32, rays_accurate=4_000_000):
         t4f = time.perf_counter() - t0
         _, adj4 = fit.log_edc_loss(r4.abs_value, target, mask)
         t0 = time.perf_counter()
-        api.render_backward(ctx, sc, adj_abs=adj4, local=True, **kw4)
+        api.render_backward(ctx, sc, adj_abs=adj4, score=True, **kw4)
         t4b = time.perf_counter() - t0
         ref16 = api.render(ctx, sc, **dict(kw, n_rays=16_000_000, seed=0x5EED))
         edc4 = max(acoustics.edc_error(r4.abs_value[b, 0], ref16.abs_value[b, 0]) for b in range(3))
         segs4 = r4.stats["segments"]
-    log(f"adjoint: forward {t_fwd * 1e3:.1f} ms, backward {t_bwd * 1e3:.1f} ms, {segs} segments, "
+    log(f"adjoint: forward {t_fwd * 1e3:.1f} ms, backward {t_bwd * 1e3:.1f} ms (score form; local form "
+        f"{t_bwd_local * 1e3:.1f} ms), {segs} segments, "
         f"EDC error {edc_err:.4f}")
     out.update({"workload": "shoebox 12x9x5 m, 18 absorption + 6 specularity parameters, 1M rays, 4096-bin log-EDC "
                             "loss, parameters +-20% off the target's",
+                "adjoint_form": "score (EAR_FLAG_ADJ_SCORE)",
                 "segments": int(segs), "forward_ms": round(1e3 * t_fwd, 3), "backward_ms": round(1e3 * t_bwd, 3),
+                "backward_local_form_ms": round(1e3 * t_bwd_local, 3),
                 "forward_segments_per_s": round(segs / t_fwd, 1),
                 "forward_adjoint_segments_per_s": round(segs / (t_fwd + t_bwd), 1),
                 "forward_adjoint_vs_forward": round(t_fwd / (t_fwd + t_bwd), 4),
@@ -419,7 +426,7 @@ def bench_adjoint(ctx, n_rooms=32, rays_accurate=4_000_000):
     res, st, _ = api.render_batch(ctx, fwd_items)
     adjs = [fit.log_edc_loss(x.abs_value * 1.1, t, fit.decay_mask(t))[1] for x, t in zip(res, targets)]
     t1 = time.perf_counter()
-    api.render_backward_batch(ctx, [(sc, dict(k, adj_abs=a, local=True)) for sc, k, a in zip(scs, kws, adjs)])
+    api.render_backward_batch(ctx, [(sc, dict(k, adj_abs=a, score=True)) for sc, k, a in zip(scs, kws, adjs)])
     t2 = time.perf_counter()
     # accuracy of the 250k-ray rooms: EDC error against independent 2M-ray renders
     refs, _, _ = api.render_batch(ctx, [(sc, dict(k, n_rays=2_000_000, seed=0x5EED)) for sc, k in fwd_items])
@@ -435,7 +442,7 @@ def bench_adjoint(ctx, n_rooms=32, rays_accurate=4_000_000):
     resR, stR, _ = api.render_batch(ctx, [(sc, k) for sc, k in zip(scs, kwsR)])
     adjsR = [fit.log_edc_loss(x.abs_value * 1.1, t, fit.decay_mask(t))[1] for x, t in zip(resR, targets)]
     t1b = time.perf_counter()
-    api.render_backward_batch(ctx, [(sc, dict(k, adj_abs=a, local=True)) for sc, k, a in zip(scs, kwsR, adjsR)])
+    api.render_backward_batch(ctx, [(sc, dict(k, adj_abs=a, score=True)) for sc, k, a in zip(scs, kwsR, adjsR)])
     t2b = time.perf_counter()
     worst = [int(i) for i in np.argsort(errs)[::-1][:8]]
     refs4, _, _ = api.render_batch(ctx, [(scs[i], dict(kws[i], n_rays=4 * R, seed=0x5EED)) for i in worst])

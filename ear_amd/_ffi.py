@@ -20,7 +20,7 @@ STATUS_NAMES = {
 EAR_MAT_HAS_REFR, EAR_MAT_HAS_SPEC = 1, 2
 EAR_REC_MONO, EAR_REC_STEREO = 0, 1
 EAR_SRC_POINT, EAR_SRC_MESH = 0, 1
-EAR_FLAG_ABS_HIST, EAR_FLAG_INSTRUMENT, EAR_FLAG_TRACE_LOG, EAR_FLAG_ADJ_LOCAL = 1, 2, 4, 8
+EAR_FLAG_ABS_HIST, EAR_FLAG_INSTRUMENT, EAR_FLAG_TRACE_LOG, EAR_FLAG_ADJ_LOCAL, EAR_FLAG_ADJ_SCORE = 1, 2, 4, 8, 16
 EAR_FRAC_BITS = 80
 EAR_EST_NEE, EAR_EST_SPHERE = 0, 1
 

@@ -316,17 +316,20 @@ def render_batch(ctx: Context, items) -> tuple[list, dict, float]:
     return [p.result({}, wall) for p in preps], stats, wall
 
 
-def render_backward(ctx: Context, scene: Scene, *, adj_value=None, adj_abs=None, local=False, **kw):
+def render_backward(ctx: Context, scene: Scene, *, adj_value=None, adj_abs=None, local=False, score=False, **kw):
     """Gradient of sum(adj_value * value) + sum(adj_abs * abs_value) of the
     render `kw` describes with respect to each material's reflection
     coefficient and specularity (ear_render_backward).  The adjoints have the
     shape of RenderResult.value, (3, n_channels, n_bins).  Returns
     (d/drefl, d/dspec), each (n_mat, 3); bands outside `bands` are zero.
-    local: EAR_FLAG_ADJ_LOCAL (direction tangents end at their vertex)."""
+    local: EAR_FLAG_ADJ_LOCAL (direction tangents end at their vertex);
+    score: EAR_FLAG_ADJ_SCORE (the unbiased score form of d/dspec)."""
     kw = {k: v for k, v in kw.items() if k not in ("want_raw", "want_value")}  # no histograms come back
     p = _Prepared(want_raw=False, want_value=False, **kw)
     if local:
         p.desc.flags |= F.EAR_FLAG_ADJ_LOCAL
+    if score:
+        p.desc.flags |= F.EAR_FLAG_ADJ_SCORE
     nch = n_channels(kw["recorders"])
     shape = (3, nch, p.desc.hist_bins)
     a_v = None if adj_value is None else np.ascontiguousarray(np.broadcast_to(adj_value, shape), np.float64)
@@ -341,15 +344,18 @@ def render_backward(ctx: Context, scene: Scene, *, adj_value=None, adj_abs=None,
 def render_backward_batch(ctx: Context, items):
     """The adjoint of many renders at once (ear_render_backward_batch).
     items: sequence of (scene, kw) with kw as for render_backward (adj_value,
-    adj_abs, local and the render keywords).  Returns [(d/drefl, d/dspec)]."""
+    adj_abs, local, score and the render keywords).  Returns [(d/drefl, d/dspec)]."""
     keep, arr = [], (F.ear_backward_item * max(1, len(items)))()
     outs = []
     for i, (scene, kw) in enumerate(items):
         kw = {k: v for k, v in kw.items() if k not in ("want_raw", "want_value")}
         adj_value, adj_abs, local = kw.pop("adj_value", None), kw.pop("adj_abs", None), kw.pop("local", False)
+        score = kw.pop("score", False)
         p = _Prepared(want_raw=False, want_value=False, **kw)
         if local:
             p.desc.flags |= F.EAR_FLAG_ADJ_LOCAL
+        if score:
+            p.desc.flags |= F.EAR_FLAG_ADJ_SCORE
         shape = (3, n_channels(kw["recorders"]), p.desc.hist_bins)
         a_v = None if adj_value is None else np.ascontiguousarray(np.broadcast_to(adj_value, shape), np.float64)
         a_a = None if adj_abs is None else np.ascontiguousarray(np.broadcast_to(adj_abs, shape), np.float64)

@@ -293,6 +293,89 @@ EAR_HD V3 blend_spec(V3 v, V3 refl, float f) {
 EAR_HD V3 sample_hemi_spec(Rng& r, V3 n, V3 refl, float f) { return blend_spec(sample_hemi(r, n), refl, f); }
 
 // ---------------------------------------------------------------------------
+// The score form of the specularity gradient.  v = normalize((1-s) h + s r),
+// h uniform on the hemisphere of nu, r.nu > 0, s < 1/2, has the density
+// p(v) = t^2 / (2 pi (1-s) sqrt(D)) with c = v.r, D = (1-s)^2 - s^2 (1-c^2),
+// t = s c + sqrt(D) on the image of the hemisphere; probability leaves that
+// image through the image of the hemisphere's rim at the total rate
+// F = (r.nu) / ((1-s) sqrt((1-s)^2 - s^2 (1 - (r.nu)^2))), with density
+// w(phi) proportional to 1 / ((1-s) + s e(phi).r) along the rim e(phi).
+// ---------------------------------------------------------------------------
+EAR_HD double score_dlogp(double c, double s) {  // d log p(v) / ds at fixed v
+    double D = (1.0 - s) * (1.0 - s) - s * s * (1.0 - c * c);
+    double sq = sqrt(D);
+    double Ds = -2.0 * (1.0 - s) - 2.0 * s * (1.0 - c * c);
+    double t = s * c + sq;
+    double ts = c + Ds / (2.0 * sq);
+    return 2.0 * ts / t + 1.0 / (1.0 - s) - Ds / (2.0 * D);
+}
+EAR_HD double score_flux(double cr, double s) {  // F, cr = r.nu
+    double sin2 = 1.0 - cr * cr;
+    if (sin2 < 0.0) sin2 = 0.0;
+    return cr / ((1.0 - s) * sqrt((1.0 - s) * (1.0 - s) - s * s * sin2));
+}
+// The stream of vertex k's spawn decision and continuation: the ray's
+// counters under a key that depends on k.
+EAR_HD void score_rng_init(Rng& r, uint64_t seed, uint32_t ray, uint32_t ctx, uint32_t inst, uint32_t k,
+                           uint32_t draw) {
+    r.k0 = (uint32_t)seed ^ 0x85EBCA6Bu;
+    r.k1 = (uint32_t)(seed >> 32) ^ (0xC2B2AE35u * (k + 1u));
+    r.ray = ray;
+    r.ctx = ctx;
+    r.inst = inst;
+    r.draw = draw;
+    if (draw & 3u) r.refill();
+}
+// A rim direction normalize((1-s) e(phi) + s r) with phi of density w / F:
+// a uniform point z of the circle (rejection from the square) moved by the
+// disc automorphism (z + rho) / (1 + rho z) has a wrapped Cauchy angle
+// about 0; phi = pi + that angle, measured from r's component along the rim.
+EAR_HD V3 score_rim_dir(Rng& R, V3 nu, V3 r, float s) {
+    float zx, zy;
+    for (;;) {
+        float x = R.next() * 2.0f - 1.0f;
+        float y = R.next() * 2.0f - 1.0f;
+        float l = x * x + y * y;
+        if (l < 0.001f || l > 1.0f) continue;
+        float sl = sqrtf(l);
+        zx = x / sl;
+        zy = y / sl;
+        break;
+    }
+    float cr = dot(r, nu);
+    V3 rp{r.x - nu.x * cr, r.y - nu.y * cr, r.z - nu.z * cr};
+    float sr = length(rp);
+    V3 a;
+    if (sr > 1e-6f) {
+        a = V3{rp.x / sr, rp.y / sr, rp.z / sr};
+    } else {
+        // r along nu: any direction of the rim plane (the law is uniform)
+        float ax = fabsf(nu.x), ay = fabsf(nu.y), az = fabsf(nu.z);
+        V3 t{0.0f, 0.0f, 0.0f};
+        if (ax <= ay && ax <= az)
+            t.x = 1.0f;
+        else if (ay <= az)
+            t.y = 1.0f;
+        else
+            t.z = 1.0f;
+        float td = dot(t, nu);
+        a = normalized(V3{t.x - nu.x * td, t.y - nu.y * td, t.z - nu.z * td});
+        sr = 0.0f;
+    }
+    V3 b = cross(nu, a);
+    float A = 1.0f - s, B = s * sr;
+    float rho = B / (A + sqrtf(A * A - B * B));
+    float wx = (zx + rho) * (1.0f + rho * zx) + rho * zy * zy;
+    float wy = zy * (1.0f - rho * rho);
+    float wl = sqrtf(wx * wx + wy * wy);
+    wx = wx / wl;
+    wy = wy / wl;
+    V3 e{-wx * a.x - wy * b.x, -wx * a.y - wy * b.y, -wx * a.z - wy * b.z};
+    float g = 1.0f - s;
+    return normalized(V3{e.x * g + r.x * s, e.y * g + r.y * s, e.z * g + r.z * s});
+}
+
+// ---------------------------------------------------------------------------
 // 128-bit two's-complement fixed point (value * 2^EAR_FRAC_BITS) for the histograms.
 // ---------------------------------------------------------------------------
 struct I128 {

@@ -2761,6 +2761,7 @@ struct Tangents {
         return t[((size_t)q * 8 + field) * cap + slot];
     }
     __device__ float& refl(uint32_t q, uint32_t slot) const { return r[(size_t)q * cap + slot]; }
+    __device__ float& row(uint32_t j, uint32_t slot) const { return t[(size_t)j * cap + slot]; }  // score form
 };
 
 struct AdjArgs {
@@ -3128,52 +3129,77 @@ __global__ void __launch_bounds__(kBlock) k_shade_ad(const InstDev* __restrict__
 // weights the connection adds when it is visible, and which recorder,
 // band, material and instance it belongs to.
 struct AdjShadowQ {
-    float4* p;  // {px, py, pz, d c / d spec_mat * dL/dc}
+    float4* p;  // {px, py, pz, -}
     uint4* m;   // {rec | band << 8 | mat << 16, instance, -, -}
-    float4* w;  // [2 per request]: dL/dc * c * dlogI/drefl_q, q = 0..7
+    float4* w;  // [4 per request]: d/drefl_q (q = 0..7), then d/dspec_q (q = 0..7), each times dL/dc
 };
 
-// The adjoint with EAR_FLAG_ADJ_LOCAL for every ray: no direction or
-// position tangents travel between vertices, so a ray carries only
-// dlogI/drefl (n_mat floats).  At each connection of a hit on material m,
+// The score form (EAR_FLAG_ADJ_SCORE): a queue entry whose metadata carries
+// kContFlag is a continuation path.  Its tangent rows hold S (the weight of
+// its spawning material, zero elsewhere) and, in row n_mat, the vertex that
+// spawned it (its stream's key).
+constexpr uint32_t kContFlag = 1u << 31;
+constexpr float kScoreSpawn = 0.0625f;  // probability of a continuation per vertex
+
+struct ContOut {  // where a score-form vertex appends its continuation
+    Queue q;
+    uint32_t* n;
+    Tangents t;
+};
+
+// The adjoint with EAR_FLAG_ADJ_LOCAL (or EAR_FLAG_ADJ_SCORE) for every ray:
+// no direction or position tangents travel between vertices, so a ray
+// carries only dlogI/drefl (n_mat floats) and, in the score form, S (n_mat
+// floats).  At each connection of a hit on material m,
 // d c / d spec_m = sign I air^l 2 / (4 pi l^2) (lobe - diffuse) and
-// d c / d refl_q = c dlogI/drefl_q; times dL/dc they are queued as a request
-// whose visibility k_adj_vis decides, as k_shade queues the forward's.  Same
-// primal and draws as k_shade; arrays are indexed at compile time only.
-template <bool BATCH>
+// d c / d refl_q = c dlogI/drefl_q; the score form adds c S_q to d c / dspec_q.
+// Times dL/dc they are queued as a request whose visibility k_adj_vis
+// decides, as k_shade queues the forward's.  Same primal and draws as
+// k_shade; arrays are indexed at compile time only.
+//
+// SCORE: after its connections, a vertex whose path goes on adds to S_m
+// (m its material) d log p(v)/ds - F for the direction v it drew, and with
+// probability kScoreSpawn appends a continuation leaving the vertex along a
+// rim direction (score_rim_dir) with weight -F / kScoreSpawn on spec_m,
+// adding F / kScoreSpawn to S_m.  A continuation replays transport only:
+// no refl tangents, no explicit lobe term, no score, no spawns.
+template <bool BATCH, bool SCORE>
 __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict__ inst, Queue qin,
                                                        const uint32_t* __restrict__ n_in,
                                                        const int2* __restrict__ hits, Tangents tin, Queue qout,
                                                        uint32_t* __restrict__ n_out, Tangents tout, AdjShadowQ aq,
                                                        uint32_t* __restrict__ n_aq, uint32_t max_rec, AdjArgs A0,
-                                                       const AdjArgs* __restrict__ adj,
+                                                       const AdjArgs* __restrict__ adj, ContOut co,
                                                        unsigned long long* __restrict__ stats) {
-    __shared__ uint32_t sc[4][kBlock / 64 + 1];  // scratch of the block-wide appends, alternated
+    __shared__ uint32_t sc[6][kBlock / 64 + 1];  // scratch of the block-wide appends, alternated
     const uint32_t n = *n_in;
     uint32_t n_done = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         const uint32_t i = base + threadIdx.x;
-        bool cont = false, alive = false;
+        const uint32_t parity = (base / (gridDim.x * kBlock)) & 1u;
+        bool cont = false, alive = false, contn = false, spawn = false;
         float4 no{}, nd{};
         uint4 nmeta{};
-        float drefl[kAdMaxMat];
+        float drefl[kAdMaxMat], sacc[kAdMaxMat];
 #pragma unroll
-        for (int q = 0; q < kAdMaxMat; ++q) drefl[q] = 0.0f;
-        uint32_t nm = A0.n_mat, id = 0, k = 0, mat = 0, n_rec = 0;
+        for (int q = 0; q < kAdMaxMat; ++q) drefl[q] = sacc[q] = 0.0f;
+        uint32_t nm = A0.n_mat, id = 0, k = 0, mat = 0, n_rec = 0, kspawn = 0, sdraw = 0, ray = 0;
         int band = 0, bt = kReflect;
-        float I = 0.0f, L = 0.0f, spec = 0.0f;
-        V3 p{0, 0, 0}, nrm{0, 0, 0}, prev{0, 0, 0};
+        float I = 0.0f, L = 0.0f, spec = 0.0f, wcont = 0.0f;
+        V3 p{0, 0, 0}, nrm{0, 0, 0}, prev{0, 0, 0}, vdir{0, 0, 0}, rdir{0, 0, 0}, vb{0, 0, 0};
         if (i < n) {
             const float4 ro = qin.o[i];
             const float4 rd = qin.d[i];
             const uint4 rm = qin.m[i];
             id = BATCH ? rm.w : 0u;
+            ray = rm.x;
             const InstDev& In = inst[id];
             const RenderParams& P = In.P;
             nm = (BATCH ? adj[id] : A0).n_mat;
             n_rec = P.n_rec;
             band = (int)(rm.z & 0xFFu);
             k = ((rm.z >> 8) & 0xFFFFu) + 1;
+            contn = SCORE && (rm.z & kContFlag) != 0u;
             const int2 h = hits[i];
             if (h.y >= 0) {
                 const float t = __int_as_float(h.x);
@@ -3186,7 +3212,12 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
                 mat = __float_as_uint(nmv.w);
                 const float4 mb = In.mat_band[3 * mat + band];
                 Rng rng;
-                rng.init(P.seed, rm.x, P.ctx_word[band], P.instance, rm.y);
+                if (contn) {
+                    kspawn = __float_as_uint(tin.row(nm, i));
+                    score_rng_init(rng, P.seed, rm.x, P.ctx_word[band], P.instance, kspawn, rm.y);
+                } else {
+                    rng.init(P.seed, rm.x, P.ctx_word[band], P.instance, rm.y);
+                }
                 if (mb.x >= 0.0f) bt = (rng.next() <= mb.x) ? kReflect : kRefract;
                 spec = mb.y;
                 if (bt == kRefract) nrm = neg(nrm);
@@ -3197,17 +3228,21 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
                 I *= mb.z;
                 prev = normalized(d);
 #pragma unroll
-                for (int q = 0; q < kAdMaxMat; ++q)
-                    if ((uint32_t)q < nm) drefl[q] = tin.refl(q, i) + ((uint32_t)q == mat ? 1.0f / mb.z : 0.0f);
+                for (int q = 0; q < kAdMaxMat; ++q) {
+                    if ((uint32_t)q < nm) {
+                        drefl[q] = contn ? 0.0f : tin.refl(q, i) + ((uint32_t)q == mat ? 1.0f / mb.z : 0.0f);
+                        if (SCORE) sacc[q] = tin.row(q, i);
+                    }
+                }
                 alive = is_normal(I);
                 if (alive) {
                     cont = !((double)I < P.cutoff) && k < P.max_bounces;
                     if (cont) {
-                        const V3 rdir = bt == kRefract ? d : reflect(d, nrm);
-                        const V3 v = sample_hemi_spec(rng, nrm, rdir, spec);
+                        rdir = bt == kRefract ? d : reflect(d, nrm);
+                        vdir = sample_hemi_spec(rng, nrm, rdir, spec);
                         no = make_float4(p.x, p.y, p.z, I);
-                        nd = make_float4(v.x, v.y, v.z, L);
-                        nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), id);
+                        nd = make_float4(vdir.x, vdir.y, vdir.z, L);
+                        nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8) | (contn ? kContFlag : 0u), id);
                     }
                 }
             }
@@ -3249,7 +3284,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
                         if (a_c != 0.0) {
                             const double cnof =
                                 ((k & 1u) ? -1.0 : 1.0) * (double)I * (double)geo_air * (double)geo_inv;
-                            ds = (float)(a_c * cnof * ((double)kExpInt * (double)pw - (double)diff));
+                            ds = contn ? 0.0f : (float)(a_c * cnof * ((double)kExpInt * (double)pw - (double)diff));
                             ac = (float)(a_c * (double)c);
                             emit = true;
                         }
@@ -3258,21 +3293,72 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
             }
             const uint32_t slot = block_append<kBlock>(n_aq, emit, sc[r & 1u]);
             if (emit) {
-                aq.p[slot] = make_float4(p.x, p.y, p.z, ds);
+                float ws[kAdMaxMat];
+#pragma unroll
+                for (int q = 0; q < kAdMaxMat; ++q) ws[q] = ac * sacc[q] + ((uint32_t)q == mat ? ds : 0.0f);
+                aq.p[slot] = make_float4(p.x, p.y, p.z, 0.0f);
                 aq.m[slot] = make_uint4(r | ((uint32_t)band << 8) | (mat << 16), id, 0u, 0u);
-                aq.w[2 * (size_t)slot] = make_float4(ac * drefl[0], ac * drefl[1], ac * drefl[2], ac * drefl[3]);
-                aq.w[2 * (size_t)slot + 1] =
+                aq.w[4 * (size_t)slot] = make_float4(ac * drefl[0], ac * drefl[1], ac * drefl[2], ac * drefl[3]);
+                aq.w[4 * (size_t)slot + 1] =
                     make_float4(ac * drefl[4], ac * drefl[5], ac * drefl[6], ac * drefl[7]);
+                aq.w[4 * (size_t)slot + 2] = make_float4(ws[0], ws[1], ws[2], ws[3]);
+                aq.w[4 * (size_t)slot + 3] = make_float4(ws[4], ws[5], ws[6], ws[7]);
             }
         }
-        const uint32_t slot = block_append<kBlock>(n_out, cont, sc[2 + ((base / (gridDim.x * kBlock)) & 1u)]);
+        if (SCORE && cont && !contn) {
+            // the direction just drawn leaves this vertex: its score, the rim
+            // flux and perhaps a continuation (after this vertex's connections,
+            // which that direction does not reach)
+            const RenderParams& P = inst[id].P;
+            const double cdir = (double)vdir.x * (double)rdir.x + (double)vdir.y * (double)rdir.y +
+                                (double)vdir.z * (double)rdir.z;
+            const double cr = (double)rdir.x * (double)nrm.x + (double)rdir.y * (double)nrm.y +
+                              (double)rdir.z * (double)nrm.z;
+            const double F = score_flux(cr, (double)spec);
+            double coef = score_dlogp(cdir, (double)spec) - F;
+            Rng rs;
+            score_rng_init(rs, P.seed, ray, P.ctx_word[band], P.instance, k, 0u);
+            if (rs.next() < kScoreSpawn) {
+                vb = score_rim_dir(rs, nrm, rdir, spec);
+                const double wgt = F / (double)kScoreSpawn;
+                coef += wgt;
+                wcont = (float)(-wgt);
+                sdraw = rs.draw;
+                spawn = true;
+            }
+#pragma unroll
+            for (int q = 0; q < kAdMaxMat; ++q)
+                if ((uint32_t)q == mat) sacc[q] += (float)coef;
+        }
+        const uint32_t slot = block_append<kBlock>(n_out, cont, sc[2 + parity]);
         if (cont) {
             qout.o[slot] = no;
             qout.d[slot] = nd;
             qout.m[slot] = nmeta;
 #pragma unroll
-            for (int q = 0; q < kAdMaxMat; ++q)
-                if ((uint32_t)q < nm) tout.refl(q, slot) = drefl[q];
+            for (int q = 0; q < kAdMaxMat; ++q) {
+                if ((uint32_t)q < nm) {
+                    tout.refl(q, slot) = drefl[q];
+                    if (SCORE) tout.row(q, slot) = sacc[q];
+                }
+            }
+            if (SCORE && contn) tout.row(nm, slot) = __uint_as_float(kspawn);
+        }
+        if (SCORE) {
+            const uint32_t cs = block_append<kBlock>(co.n, spawn, sc[4 + parity]);
+            if (spawn) {
+                co.q.o[cs] = no;
+                co.q.d[cs] = make_float4(vb.x, vb.y, vb.z, L);
+                co.q.m[cs] = make_uint4(ray, sdraw, (uint32_t)band | (k << 8) | kContFlag, id);
+#pragma unroll
+                for (int q = 0; q < kAdMaxMat; ++q) {
+                    if ((uint32_t)q < nm) {
+                        co.t.refl(q, cs) = 0.0f;
+                        co.t.row(q, cs) = (uint32_t)q == mat ? wcont : 0.0f;
+                    }
+                }
+                co.t.row(nm, cs) = __uint_as_float(k);
+            }
         }
         n_done += i < n ? 1u : 0u;
     }
@@ -3309,11 +3395,9 @@ __global__ void __launch_bounds__(kBlock) k_adj_vis(const InstDev* __restrict__ 
         const uint32_t n_free = 64u - (uint32_t)__popcll(act);
         if (n_free >= fetch.refill || act == 0) {
             if (pending) {
-                const float4 pw = aq.p[slot];
                 const uint4 m = aq.m[slot];
                 const uint32_t id = BATCH ? m.y : 0u;
                 const int band = (int)((m.x >> 8) & 0xFFu);
-                const uint32_t mat = m.x >> 16;
                 if (BATCH && key != 0xFFFFFFFFu && (id != key || band != kband)) {
                     // a batch lane moving to another instance or band hands its partials on
                     const AdjArgs& A = adj[key];
@@ -3329,12 +3413,14 @@ __global__ void __launch_bounds__(kBlock) k_adj_vis(const InstDev* __restrict__ 
                 }
                 key = id;
                 kband = band;
-                const float4 w0 = aq.w[2 * (size_t)slot], w1 = aq.w[2 * (size_t)slot + 1];
+                const float4 w0 = aq.w[4 * (size_t)slot], w1 = aq.w[4 * (size_t)slot + 1];
+                const float4 w2 = aq.w[4 * (size_t)slot + 2], w3 = aq.w[4 * (size_t)slot + 3];
                 const float wq[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+                const float wsq[8] = {w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
 #pragma unroll
                 for (int q = 0; q < kAdMaxMat; ++q) {
                     g_refl[q] += (double)wq[q];
-                    g_spec[q] += (uint32_t)q == mat ? (double)pw.w : 0.0;
+                    g_spec[q] += (double)wsq[q];
                 }
                 pending = false;
             }
@@ -3442,6 +3528,19 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
             c->last_error = "no adjoint given";
             return EAR_ERR_INVALID_ARGUMENT;
         }
+        if ((D->flags & EAR_FLAG_ADJ_SCORE) && (D->flags & EAR_FLAG_ADJ_LOCAL)) {
+            c->last_error = "EAR_FLAG_ADJ_SCORE and EAR_FLAG_ADJ_LOCAL exclude each other";
+            return EAR_ERR_INVALID_ARGUMENT;
+        }
+        if (D->flags & EAR_FLAG_ADJ_SCORE) {
+            // the blend's direction map is one-to-one only below 1/2
+            for (uint32_t m = 0; m < s->n_mat; ++m)
+                for (uint32_t b = 0; b < 3; ++b)
+                    if (((D->band_mask >> b) & 1u) && !(s->mat_band_host[3 * m + b].y < 0.5f)) {
+                        c->last_error = "EAR_FLAG_ADJ_SCORE needs every specularity below 0.5";
+                        return EAR_ERR_INVALID_ARGUMENT;
+                    }
+        }
         I.nodes = s->nodes;
         I.nodes4 = s->nodes4;
         I.tris = s->tris;
@@ -3495,19 +3594,43 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
         TRY(tt[k].alloc((size_t)cap * nm_max * 8 * 4));
         TRY(tr[k].alloc((size_t)cap * nm_max * 4));
     }
-    TRY(hits.alloc((size_t)cap * 8));
     TRY(counters.alloc(32));
     TRY(stats.alloc(ST_N * 8 * kStatCopies));
     TRY(sqp.alloc(16));
     TRY(sqm.alloc(16));
     // connection requests of the local adjoint (one per ray and recorder)
-    bool all_local = true;
-    for (uint32_t i = 0; i < n_items; ++i) all_local = all_local && (inst[i].P.flags & EAR_FLAG_ADJ_LOCAL);
+    uint32_t n_local = 0, n_score = 0;
+    for (uint32_t i = 0; i < n_items; ++i) {
+        n_local += (inst[i].P.flags & EAR_FLAG_ADJ_LOCAL) ? 1u : 0u;
+        n_score += (inst[i].P.flags & EAR_FLAG_ADJ_SCORE) ? 1u : 0u;
+    }
+    if (n_score != 0 && n_score != n_items) {
+        c->last_error = "a batch mixes EAR_FLAG_ADJ_SCORE items with others";
+        return EAR_ERR_INVALID_ARGUMENT;
+    }
+    const bool all_score = n_score == n_items;
+    const bool all_local = all_score || n_local == n_items;  // the per-vertex kernels
     DevBuf aqp(c), aqm(c), aqw(c), segv(c);
-    const size_t cap_aq = all_local ? (size_t)cap * std::max<uint32_t>(1, max_rec) : 1;
+    // the score form's continuations wait in qc[ci] until a drain traces
+    // them; room for two chunks of spawns
+    const uint32_t cap_c = all_score ? (uint32_t)std::min<uint64_t>(2ull * cap, 0xFFFFFFFFull) : 1u;
+    const uint32_t cap_q = std::max(cap, cap_c);  // rays of one bounce: the chunk or a drain
+    TRY(hits.alloc((size_t)cap_q * 8));
+    const size_t cap_aq = all_local ? (size_t)cap_q * std::max<uint32_t>(1, max_rec) : 1;
     TRY(aqp.alloc(cap_aq * 16));
     TRY(aqm.alloc(cap_aq * 16));
-    TRY(aqw.alloc(cap_aq * 32));
+    TRY(aqw.alloc(cap_aq * 64));
+    DevBuf qco[2] = {DevBuf(c), DevBuf(c)}, qcd[2] = {DevBuf(c), DevBuf(c)}, qcm[2] = {DevBuf(c), DevBuf(c)};
+    DevBuf tct[2] = {DevBuf(c), DevBuf(c)}, tcr[2] = {DevBuf(c), DevBuf(c)}, cntc(c);
+    for (int k = 0; k < 2; ++k) {
+        TRY(qco[k].alloc((size_t)cap_c * 16));
+        TRY(qcd[k].alloc((size_t)cap_c * 16));
+        TRY(qcm[k].alloc((size_t)cap_c * 16));
+        TRY(tct[k].alloc((size_t)cap_c * (nm_max + 1) * 4));
+        TRY(tcr[k].alloc((size_t)cap_c * nm_max * 4));
+    }
+    TRY(cntc.alloc(8));
+    EAR_HIP_TRY(c, hipMemsetAsync(cntc.p, 0, cntc.n, st));
     TRY(segv.alloc(kMaxSeg * kSegStride * 4));
     EAR_HIP_TRY(c, hipMemsetAsync(segv.p, 0, segv.n, st));
     const AdjShadowQ AQ{aqp.as<float4>(), aqm.as<uint4>(), aqw.as<float4>()};
@@ -3563,6 +3686,14 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
                   Queue{qo[1].as<float4>(), qd[1].as<float4>(), qm[1].as<uint4>()}};
     Tangents T[2] = {Tangents{tt[0].as<float>(), tr[0].as<float>(), cap},
                      Tangents{tt[1].as<float>(), tr[1].as<float>(), cap}};
+    Queue qc[2] = {Queue{qco[0].as<float4>(), qcd[0].as<float4>(), qcm[0].as<uint4>()},
+                   Queue{qco[1].as<float4>(), qcd[1].as<float4>(), qcm[1].as<uint4>()}};
+    Tangents Tc[2] = {Tangents{tct[0].as<float>(), tcr[0].as<float>(), cap_c},
+                      Tangents{tct[1].as<float>(), tcr[1].as<float>(), cap_c}};
+    uint32_t* ncc = cntc.as<uint32_t>();
+    int ci = 0;           // the buffer continuations are appended to
+    uint32_t n_cont = 0;  // continuations waiting there
+    uint32_t pinned_c[2];
     const ShadowQ SQ{sqp.as<float4>(), sqm.as<float4>()};
     const AdjArgs A0 = args[0];
     const AdjArgs* dA = dargs.as<AdjArgs>();
@@ -3573,6 +3704,52 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
                                 : (batch ? k_adj_vis<2, true> : k_adj_vis<2, false>);
     const uint32_t cap_v = (uint32_t)c->num_cus * (uint32_t)std::max(1, resident_blocks((const void*)fv, kBlock));
     const Fetch fq_v{segv.as<uint32_t>(), tu.refill ? tu.refill : 24u, tu.fetch_chunk, nseg};
+    // one bounce of the per-vertex adjoint (local or score form): closest
+    // hits of queue qi, the shade (continuing rays into qo, continuations
+    // into co), the visibility of the connection requests
+    auto adl_bounce = [&](const Queue& qi, uint32_t* ni, const Tangents& ti, const Queue& qo, uint32_t* no,
+                          const Tangents& to, const ContOut& co, uint32_t n_now, uint32_t gsd, uint32_t ge) {
+        hipLaunchKernelGGL(fe, dim3(ge), dim3(256), 0, st, di, gs.as<int>(), gthreads, qi, (const uint32_t*)ni,
+                           hits.as<int2>(), stp, fq_e);
+        if (fetch_e) k_reset_iter<<<1, 64, 0, st>>>(cnt, 0u, segc.as<uint32_t>(), nseg);
+        if (batch) {
+            if (all_score)
+                k_shade_adl<true, true><<<gsd, kBlock, 0, st>>>(di, qi, ni, hits.as<int2>(), ti, qo, no, to, AQ,
+                                                                cnt + 2, max_rec, A0, dA, co, stp);
+            else
+                k_shade_adl<true, false><<<gsd, kBlock, 0, st>>>(di, qi, ni, hits.as<int2>(), ti, qo, no, to, AQ,
+                                                                 cnt + 2, max_rec, A0, dA, co, stp);
+        } else {
+            if (all_score)
+                k_shade_adl<false, true><<<gsd, kBlock, 0, st>>>(di, qi, ni, hits.as<int2>(), ti, qo, no, to, AQ,
+                                                                 cnt + 2, max_rec, A0, dA, co, stp);
+            else
+                k_shade_adl<false, false><<<gsd, kBlock, 0, st>>>(di, qi, ni, hits.as<int2>(), ti, qo, no, to, AQ,
+                                                                  cnt + 2, max_rec, A0, dA, co, stp);
+        }
+        const uint32_t n_req = (uint32_t)std::min<size_t>(cap_aq, (size_t)n_now * max_rec);
+        hipLaunchKernelGGL(fv, dim3(grid_for(n_req, cap_v, kBlock)), dim3(kBlock), 0, st, di, gs.as<int>(), gthreads,
+                           AQ, (const uint32_t*)(cnt + 2), A0, dA, stp, fq_v);
+        k_reset_iter<<<1, 64, 0, st>>>(cnt, 0u, segv.as<uint32_t>(), nseg);
+        k_reset2<<<1, 64, 0, st>>>(ni, cnt + 2);
+    };
+    // the waiting continuations traced to their ends (they spawn nothing,
+    // so a queue never grows within a drain)
+    auto drain = [&]() -> ear_status {
+        while (n_cont > 0) {
+            const uint32_t gsd = persistent_grid(c, n_cont, waves);
+            const uint32_t ge = grid_for(n_cont, cap_e, 256);
+            for (int gi = 0; gi < 4; ++gi) {
+                adl_bounce(qc[ci], ncc + ci, Tc[ci], qc[1 - ci], ncc + (1 - ci), Tc[1 - ci],
+                           ContOut{qc[ci], ncc + ci, Tc[ci]}, n_cont, gsd, ge);
+                ci = 1 - ci;
+            }
+            EAR_HIP_TRY(c, hipMemcpyAsync(pinned_c, ncc, 8, hipMemcpyDeviceToHost, st));
+            EAR_HIP_TRY(c, hipStreamSynchronize(st));
+            n_cont = pinned_c[ci];
+        }
+        return EAR_OK;
+    };
     for (uint32_t band = 0; band < 3; ++band) {
         // this band's contexts as runs of rays
         std::vector<GenSeg> segs;
@@ -3599,7 +3776,7 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
                 k_generate<false><<<persistent_grid(c, n_chunk, waves), kBlock, 0, st>>>(
                     di, dsegs.as<GenSeg>(), dsegoff.as<uint32_t>(), (uint32_t)segs.size(), n_chunk,
                     dplan.as<uint32_t>(), max_rec, 0u, q[0], cnt + 0, SQ, cnt + 2, stp);
-            if (!all_local) k_zero_f32<<<1024, 256, 0, st>>>(T[0].t, (size_t)cap * nm_max * 8);
+            if (!all_local || all_score) k_zero_f32<<<1024, 256, 0, st>>>(T[0].t, (size_t)cap * nm_max * 8);
             k_zero_f32<<<1024, 256, 0, st>>>(T[0].r, (size_t)cap * nm_max);
             EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
             EAR_HIP_TRY(c, hipStreamSynchronize(st));
@@ -3610,27 +3787,22 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
             // within a chunk, so grids sized at a group's start cover it
             uint32_t k = 1;
             while (k <= max_bounces && n_live > 0) {
-              const uint32_t group = std::min<uint32_t>(4u, max_bounces - k + 1);
+              uint32_t group = std::min<uint32_t>(4u, max_bounces - k + 1);
+              if (all_score) {
+                  // every bounce may append one continuation per live ray
+                  if (cap_c - n_cont < n_live) TRY(drain());
+                  group = std::min<uint32_t>(group, (cap_c - n_cont) / n_live);
+              }
               const uint32_t gsd = persistent_grid(c, n_live, waves);
               const uint32_t ge = grid_for(n_live, cap_e, 256);
               for (uint32_t gi = 0; gi < group; ++gi, ++k) {
-                hipLaunchKernelGGL(fe, dim3(ge), dim3(256), 0, st, di, gs.as<int>(), gthreads, q[cur],
-                                   (const uint32_t*)(cnt + cur), hits.as<int2>(), stp, fq_e);
-                if (fetch_e) k_reset_iter<<<1, 64, 0, st>>>(cnt, 0u, segc.as<uint32_t>(), nseg);
                 if (all_local) {
-                    if (batch)
-                        k_shade_adl<true><<<gsd, kBlock, 0, st>>>(di, q[cur], cnt + cur, hits.as<int2>(), T[cur],
-                                                                  q[1 - cur], cnt + (1 - cur), T[1 - cur], AQ,
-                                                                  cnt + 2, max_rec, A0, dA, stp);
-                    else
-                        k_shade_adl<false><<<gsd, kBlock, 0, st>>>(di, q[cur], cnt + cur, hits.as<int2>(), T[cur],
-                                                                   q[1 - cur], cnt + (1 - cur), T[1 - cur], AQ,
-                                                                   cnt + 2, max_rec, A0, dA, stp);
-                    const uint32_t n_req = (uint32_t)std::min<size_t>(cap_aq, (size_t)n_live * max_rec);
-                    hipLaunchKernelGGL(fv, dim3(grid_for(n_req, cap_v, kBlock)), dim3(kBlock), 0, st, di, gs.as<int>(),
-                                       gthreads, AQ, (const uint32_t*)(cnt + 2), A0, dA, stp, fq_v);
-                    k_reset_iter<<<1, 64, 0, st>>>(cnt, 0u, segv.as<uint32_t>(), nseg);
+                    adl_bounce(q[cur], cnt + cur, T[cur], q[1 - cur], cnt + (1 - cur), T[1 - cur],
+                               ContOut{qc[ci], ncc + ci, Tc[ci]}, n_live, gsd, ge);
                 } else {
+                    hipLaunchKernelGGL(fe, dim3(ge), dim3(256), 0, st, di, gs.as<int>(), gthreads, q[cur],
+                                       (const uint32_t*)(cnt + cur), hits.as<int2>(), stp, fq_e);
+                    if (fetch_e) k_reset_iter<<<1, 64, 0, st>>>(cnt, 0u, segc.as<uint32_t>(), nseg);
 #define EAR_AD_LAUNCH(W, B)                                                                                      \
     k_shade_ad<W, B><<<gsd, kBlock, 0, st>>>(di, gs.as<int>(), gthreads, q[cur], cnt + cur, hits.as<int2>(), \
                                              T[cur], q[1 - cur], cnt + (1 - cur), T[1 - cur], A0, dA, stp)
@@ -3646,14 +3818,17 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
                             EAR_AD_LAUNCH(2, false);
                     }
 #undef EAR_AD_LAUNCH
+                    k_reset2<<<1, 64, 0, st>>>(cnt + cur, cnt + 2);
                 }
-                k_reset2<<<1, 64, 0, st>>>(cnt + cur, cnt + 2);
                 cur = 1 - cur;
               }
               EAR_HIP_TRY(c, hipMemcpyAsync(pinned, cnt, 12, hipMemcpyDeviceToHost, st));
+              if (all_score) EAR_HIP_TRY(c, hipMemcpyAsync(pinned_c, ncc, 8, hipMemcpyDeviceToHost, st));
               EAR_HIP_TRY(c, hipStreamSynchronize(st));
               n_live = pinned[cur];
+              if (all_score) n_cont = pinned_c[ci];
             }
+            if (all_score) TRY(drain());
             k_reset2<<<1, 64, 0, st>>>(cnt + 0, cnt + 1);
             EAR_HIP_TRY(c, hipGetLastError());
         }

@@ -13,6 +13,11 @@ first 60 dB of the target's decay:
 Its adjoint with respect to abs_value is exact (log_edc_loss); the adjoint
 render turns it into d loss / d refl and d loss / d spec per material, and
 Adam steps alpha = 1 - refl and a per-wall specularity shared by the bands.
+The device fits use the score form of d / d spec (EAR_FLAG_ADJ_SCORE,
+unbiased at any bounce count), which needs specularities below 0.5: the
+specularity bounds default to [0, 0.45].  form="local" selects the local
+form (EAR_FLAG_ADJ_LOCAL), whose d / d spec keeps only each connection's
+explicit lobe term and is biased on long paths.
 """
 from __future__ import annotations
 
@@ -100,7 +105,7 @@ def materials(alpha: np.ndarray, spec: np.ndarray) -> list:
 
 
 def fit_loop(forward, backward, target_log_edc: np.ndarray, alpha0, spec0, *, steps: int, lr: float = 0.004,
-             decay: float = 5.0, momentum: float = 0.5, alpha_bounds=(0.01, 0.95), spec_bounds=(0.0, 0.95),
+             decay: float = 5.0, momentum: float = 0.5, alpha_bounds=(0.01, 0.95), spec_bounds=(0.0, 0.45),
              n_coarse: int = N_COARSE) -> FitState:
     """forward(mats) -> abs_value (3, n_ch, n_bins); backward(mats, adj_abs) ->
     (d/drefl, d/dspec), each (n_mat, 3).  Adam (first-moment decay
@@ -124,16 +129,25 @@ def fit_loop(forward, backward, target_log_edc: np.ndarray, alpha0, spec0, *, st
     return st
 
 
+def _form_flag(form: str) -> dict:
+    if form not in ("score", "local"):
+        raise ValueError(f"form must be 'score' or 'local', not {form!r}")
+    return {form: True}
+
+
 def fit_rooms(ctx, rooms: list, target_abs_values: list, alpha0s: list, spec0s: list, *, steps: int,
               lr: float = 0.004, decay: float = 5.0, momentum: float = 0.5, alpha_bounds=(0.01, 0.95),
-              spec_bounds=(0.0, 0.95), n_rays=1_000_000, max_bounces: int = 999, cutoff: float = 1e-8,
-              n_bins: int = 4096 * 11, seed: int = 0xEA2, n_coarse: int = N_COARSE) -> list:
+              spec_bounds=(0.0, 0.45), n_rays=1_000_000, max_bounces: int = 999, cutoff: float = 1e-8,
+              n_bins: int = 4096 * 11, seed: int = 0xEA2, n_coarse: int = N_COARSE, form: str = "score") -> list:
     """Many rooms fitted together: every step is one ear_render_batch of all
     rooms and one ear_render_backward_batch, then one Adam step per room (the
     same update as fit_loop).  The scenes are uploaded once; each step only
     replaces their material tables.  n_rays: one count or one per room.
-    Returns one FitState per room."""
+    form: "score" or "local", the form of d / d spec.  Returns one FitState
+    per room."""
     from . import api
+
+    flag = _form_flag(form)
 
     k = len(rooms)
     rays = list(n_rays) if np.ndim(n_rays) else [int(n_rays)] * k
@@ -154,7 +168,7 @@ def fit_rooms(ctx, rooms: list, target_abs_values: list, alpha0s: list, spec0s: 
             for sc, st in zip(scenes, states):
                 sc.set_materials(materials(st.alpha, st.spec))
             la = losses()
-            grads = api.render_backward_batch(ctx, [(sc, dict(kw, adj_abs=adj, local=True))
+            grads = api.render_backward_batch(ctx, [(sc, dict(kw, adj_abs=adj, **flag))
                                                     for sc, kw, (_, adj) in zip(scenes, kws, la)])
             for i, st in enumerate(states):
                 st.history.append((la[i][0], st.alpha.copy(), st.spec.copy()))
@@ -177,11 +191,14 @@ def fit_rooms(ctx, rooms: list, target_abs_values: list, alpha0s: list, spec0s: 
 def fit_room(ctx, room: dict, target_abs_value: np.ndarray, alpha0, spec0, *, steps: int, lr: float = 0.004,
              decay: float = 5.0,
              n_rays: int = 1_000_000, max_bounces: int = 999, cutoff: float = 1e-8, n_bins: int = 4096 * 11,
-             seed: int = 0xEA2) -> FitState:
+             seed: int = 0xEA2, form: str = "score") -> FitState:
     """fit_loop over the device: every step is one ear_render and one
     ear_render_backward of the room (tris, mat, recorders, src, air) with
-    the current materials."""
+    the current materials (form: "score" or "local", the form of
+    d / d spec)."""
     from . import api
+
+    flag = _form_flag(form)
 
     kw = dict(recorders=room["recorders"], src_pos=room["src"], air=room["air"], n_rays=n_rays,
               max_bounces=max_bounces, cutoff=cutoff, n_bins=n_bins, seed=seed, abs_hist=True)
@@ -202,7 +219,7 @@ def fit_room(ctx, room: dict, target_abs_value: np.ndarray, alpha0, spec0, *, st
         return api.render(ctx, scene(mats), **kw).abs_value
 
     def backward(mats, adj):
-        return api.render_backward(ctx, scene(mats), adj_abs=adj, local=True, **kw)
+        return api.render_backward(ctx, scene(mats), adj_abs=adj, **flag, **kw)
 
     try:
         return fit_loop(forward, backward, target, alpha0, spec0, steps=steps, lr=lr, decay=decay)

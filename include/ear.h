@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EAR_ABI_VERSION 6
+#define EAR_ABI_VERSION 7
 
 typedef enum {
     EAR_OK = 0,
@@ -166,6 +166,8 @@ typedef struct {
 #define EAR_FLAG_TRACE_LOG 4u     /* log per-ray hit triangle indices */
 #define EAR_FLAG_ADJ_LOCAL 8u     /* ear_render_backward: keep only each vertex's own
                                      dependence on spec (see there) */
+#define EAR_FLAG_ADJ_SCORE 16u    /* ear_render_backward: the score form of d/dspec,
+                                     unbiased at any path length (see there) */
 
 /* Receiver estimators.  EAR_EST_NEE is the reference's: a connection from
  * every hit point to every recorder, the shadow ray, the lobe-weighted
@@ -314,8 +316,17 @@ ear_status ear_render_batch(ear_ctx* ctx, const ear_batch_item* items, uint32_t 
  * bounces: d/dspec keeps only the explicit blend term (lobe - diffuse) of the
  * connection at each vertex.  That estimate is biased but of bounded variance;
  * the full path derivative's variance grows with the bounce count (the
- * 1000th-power lobe amplifies the accumulated tangents), so long renders
- * (fitting) use the local form. */
+ * 1000th-power lobe amplifies the accumulated tangents).
+ * With EAR_FLAG_ADJ_SCORE d/dspec is the score (likelihood-ratio) form: each
+ * connection is weighted by the derivative of the log density of the
+ * directions drawn before it, plus the rim term of the blend's moving
+ * support, estimated by continuation paths spawned with probability 1/16
+ * at each vertex.  Paths never move, so wall switches, visibility changes
+ * and arrival-time shifts are all accounted for: the estimate is unbiased at
+ * any bounce count.  It needs every specularity of the rendered bands below
+ * 0.5 (EAR_ERR_INVALID_ARGUMENT otherwise); d/drefl is the same as without
+ * the flag.  EAR_FLAG_ADJ_SCORE and EAR_FLAG_ADJ_LOCAL exclude each other, and
+ * a batch uses the score form for all its items or for none. */
 ear_status ear_render_backward(ear_ctx* ctx, ear_scene* scene, const ear_render_desc* desc,
                                const double* adj_value, const double* adj_abs, double* grad_refl,
                                double* grad_spec);

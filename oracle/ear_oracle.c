@@ -1001,7 +1001,8 @@ typedef struct {
     const double* adj_a;
     uint32_t begin, end;
     uint32_t n_mat;
-    int local;      /* direction tangents are not carried past their vertex */
+    int local;      /* 1: direction tangents are not carried past their vertex; 2: score form */
+    float q_spawn;  /* score form: probability of a continuation per vertex */
     double* g_refl; /* [n_mat] */
     double* g_spec;
 } gworker_t;
@@ -1256,17 +1257,372 @@ static void grad_ray(const gworker_t* G, uint32_t ray, tan_t* T, double* dlogI_r
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* The score form of the specularity gradient (local == 2): unbiased at any */
+/* path length.                                                             */
+/*                                                                          */
+/* Sample_Hemi (src/Distributions.h:71-75) draws h uniformly on the         */
+/* hemisphere of nu and returns v = normalize((1-s) h + s r), r.nu > 0.    */
+/* For s < 1/2 the map is one-to-one and v has the density                  */
+/*   p(v) = t^2 / (2 pi (1-s) sqrt(D)),  c = v.r,  D = (1-s)^2 - s^2 (1-c^2),*/
+/*   t = s c + sqrt(D)                                                      */
+/* on the image Omega(s) of the hemisphere.  With G(v) the expected         */
+/* downstream value of a path leaving a vertex in direction v,              */
+/*   d/ds int_Omega(s) G p = E[G(v) dlog p/ds] - int_0^2pi G(v_b(phi)) w(phi)*/
+/* where v_b(phi) = normalize((1-s) e(phi) + s r) is the image of the       */
+/* hemisphere's rim e(phi) and w(phi) = (r.nu) / (2 pi (1-s)((1-s) + s e.r))*/
+/* the rate at which probability leaves through it; its integral is         */
+/*   F = (r.nu) / ((1-s) sqrt((1-s)^2 - s^2 (1 - (r.nu)^2))) = E[dlog p/ds].*/
+/* The rim integral is estimated by a continuation path: with probability  */
+/* q at every vertex (a constant: the loss's adjoint weighs late, weak     */
+/* arrivals as much as early ones), a second path leaves the vertex in     */
+/* direction v_b(phi), phi drawn with density w / F (a wrapped             */
+/* Cauchy law, drawn as the Moebius image of a uniform point of the circle), */
+/* and adds -(F / q) times its connections to the gradient.  The main path  */
+/* adds G (dlog p/ds - F + [spawned] F / q): the last two terms have mean   */
+/* zero and cancel most of the continuation's weight.  Summed over the      */
+/* vertices of the material, each connection of the main path adds its     */
+/* dL/dc times S_m, the running sum of those coefficients, plus the         */
+/* explicit dependence of its lobe on s.  Nothing of a path moves, so       */
+/* wall switches, visibility and arrival times need no derivative.          */
+/* ------------------------------------------------------------------------ */
+static double score_dlogp(double c, double s) {
+    double D = (1.0 - s) * (1.0 - s) - s * s * (1.0 - c * c);
+    double sq = sqrt(D);
+    double Ds = -2.0 * (1.0 - s) - 2.0 * s * (1.0 - c * c);
+    double t = s * c + sq;
+    double ts = c + Ds / (2.0 * sq);
+    return 2.0 * ts / t + 1.0 / (1.0 - s) - Ds / (2.0 * D);
+}
+
+static double score_flux(double cr, double s) {
+    double sin2 = 1.0 - cr * cr;
+    if (sin2 < 0.0) sin2 = 0.0;
+    return cr / ((1.0 - s) * sqrt((1.0 - s) * (1.0 - s) - s * s * sin2));
+}
+
+/* the stream of the spawn decision and the continuation of vertex k */
+static void score_rng_init(or_rng* r, uint64_t seed, uint32_t ray, uint32_t ctx, uint32_t instance, uint32_t k) {
+    rng_init(r, seed, ray, ctx, instance);
+    r->key[0] ^= 0x85EBCA6Bu;
+    r->key[1] ^= 0xC2B2AE35u * (k + 1u);
+}
+
+/* the rim direction v_b (phi with density w / F) for hemisphere normal nu,
+ * blend target r and specularity s */
+static void score_rim_dir(or_rng* R, const float nu[3], const float r[3], float s, float out[3]) {
+    float zx, zy;
+    for (;;) {
+        float x = rng_next(R) * 2.0f - 1.0f;
+        float y = rng_next(R) * 2.0f - 1.0f;
+        float l = x * x + y * y;
+        if (l < 0.001f || l > 1.0f) continue;
+        float sl = sqrtf(l);
+        zx = x / sl;
+        zy = y / sl;
+        break;
+    }
+    float cr = dot3(r, nu);
+    float rp[3] = {r[0] - nu[0] * cr, r[1] - nu[1] * cr, r[2] - nu[2] * cr};
+    float sr = len3(rp);
+    float a[3];
+    if (sr > 1e-6f) {
+        a[0] = rp[0] / sr;
+        a[1] = rp[1] / sr;
+        a[2] = rp[2] / sr;
+    } else {
+        /* r along nu: any direction of the rim plane (the law is uniform) */
+        float ax = fabsf(nu[0]), ay = fabsf(nu[1]), az = fabsf(nu[2]);
+        float t[3] = {0.0f, 0.0f, 0.0f};
+        if (ax <= ay && ax <= az) t[0] = 1.0f;
+        else if (ay <= az) t[1] = 1.0f;
+        else t[2] = 1.0f;
+        float td = dot3(t, nu);
+        a[0] = t[0] - nu[0] * td;
+        a[1] = t[1] - nu[1] * td;
+        a[2] = t[2] - nu[2] * td;
+        normalize3(a);
+        sr = 0.0f;
+    }
+    float b[3];
+    cross3(nu, a, b);
+    float A = 1.0f - s, B = s * sr;
+    float rho = B / (A + sqrtf(A * A - B * B));
+    /* w = (z + rho) / (1 + rho z): arg w is wrapped Cauchy about 0 */
+    float wx = (zx + rho) * (1.0f + rho * zx) + rho * zy * zy;
+    float wy = zy * (1.0f - rho * rho);
+    float wl = sqrtf(wx * wx + wy * wy);
+    wx = wx / wl;
+    wy = wy / wl;
+    /* phi = pi + arg w: the rim point e = -wx a - wy b */
+    float e[3] = {-wx * a[0] - wy * b[0], -wx * a[1] - wy * b[1], -wx * a[2] - wy * b[2]};
+    float g = 1.0f - s;
+    out[0] = e[0] * g + r[0] * s;
+    out[1] = e[1] * g + r[1] * s;
+    out[2] = e[2] * g + r[2] * s;
+    normalize3(out);
+}
+
+double or_score_dlogp(double c, double s) { return score_dlogp(c, s); }
+double or_score_flux(double cr, double s) { return score_flux(cr, s); }
+/* n rim directions for hemisphere normal nu, blend target r, specularity s,
+ * from the stream of vertex 1 of rays 0..n-1 (after its spawn draw) */
+void or_score_rim_dirs(uint64_t seed, uint32_t n, const float nu[3], const float r[3], float s, float* out) {
+    for (uint32_t i = 0; i < n; i++) {
+        or_rng R;
+        score_rng_init(&R, seed, i, 0u, 0u, 1u);
+        (void)rng_next(&R);
+        score_rim_dir(&R, nu, r, s, out + 3 * (size_t)i);
+    }
+}
+
+/* the continuation of a rim direction: trace_ray's transport from vertex
+ * k0 (point o, direction d, energy I, length L) with the vertex's own
+ * stream; each connection adds weight * dL/dc * c to g_spec[m] */
+static void score_continue(const gworker_t* G, or_rng* R, uint32_t k0, const float o_in[3], const float d_in[3],
+                           float I, float L, double weight, int m) {
+    const rctx* C = G->C;
+    const or_desc* D = C->d;
+    const int band = D->band;
+    float o[3] = {o_in[0], o_in[1], o_in[2]}, dd[3] = {d_in[0], d_in[1], d_in[2]};
+    float prev[3] = {dd[0], dd[1], dd[2]}, n[3];
+    normalize3(prev);
+    for (uint32_t k = k0 + 1; k <= D->max_bounces; k++) {
+        float t;
+        int32_t idx;
+        if (!closest_hit_p(C->s, o, dd, &t, &idx)) break;
+        float p[3] = {o[0] + dd[0] * t, o[1] + dd[1] * t, o[2] + dd[2] * t};
+        const float* N = C->s->nrm + 3 * idx;
+        if (dot3(N, dd) > 0.0f) {
+            n[0] = N[0] * -1.0f;
+            n[1] = N[1] * -1.0f;
+            n[2] = N[2] * -1.0f;
+        } else {
+            n[0] = N[0];
+            n[1] = N[1];
+            n[2] = N[2];
+        }
+        int mat = C->s->mat[idx];
+        int bt = (int)material_bounce(C, R, mat, band);
+        float spec = (C->mats[mat].flags & OR_MAT_HAS_SPEC) ? C->mats[mat].spec[band] : 0.0f;
+        float v[3];
+        if (bt == OR_REFRACT) {
+            n[0] = -n[0];
+            n[1] = -n[1];
+            n[2] = -n[2];
+            sample_hemi_spec(R, n, dd, spec, v);
+        } else {
+            float rf[3];
+            reflect3(dd, n, rf);
+            sample_hemi_spec(R, n, rf, spec, v);
+        }
+        float dist[3];
+        sub3(p, o, dist);
+        float l = len3(dist);
+        I *= or_pow(D->air, l);
+        L += l;
+        o[0] = p[0];
+        o[1] = p[1];
+        o[2] = p[2];
+        dd[0] = v[0];
+        dd[1] = v[1];
+        dd[2] = v[2];
+        I *= C->ret[mat][band];
+        if (!is_normal_f(I)) break;
+        for (uint32_t r = 0; r < D->n_rec; r++) {
+            const float* RP = D->rec[r].pos;
+            float dv[3];
+            sub3(RP, o, dv);
+            if (any_hit_p(C->s, o, dv)) continue;
+            float lsdir[3] = {dv[0], dv[1], dv[2]};
+            normalize3(lsdir);
+            if (!(dot3(lsdir, n) > 0.0f)) continue;
+            float diff, sf;
+            if (bt == OR_REFLECT) {
+                float rv[3];
+                reflect3(prev, n, rv);
+                diff = -dot3(n, prev);
+                sf = fmaxf(0.0f, dot3(rv, lsdir));
+            } else {
+                diff = dot3(n, prev);
+                sf = fmaxf(0.0f, dot3(prev, lsdir));
+            }
+            float c = I;
+            c *= spec * OR_EXP_INT * or_pow(sf, OR_EXP) + (1.0f - spec) * diff;
+            float ls = len3(dv);
+            c *= or_pow(D->air, ls);
+            c *= 2.0f / (4.0f * OR_PI * ls * ls);
+            if (!is_normal_f(c)) continue;
+            if (k % 2) c *= -1.0f;
+            double a_c = contrib_adjoint(G, (int)r, lsdir, c, (L + ls) / 343.0f, L + ls);
+            G->g_spec[m] += weight * a_c * (double)c;
+        }
+        if ((double)I < D->cutoff) break;
+        prev[0] = dd[0];
+        prev[1] = dd[1];
+        prev[2] = dd[2];
+        normalize3(prev);
+    }
+}
+
+static void grad_ray_score(const gworker_t* G, uint32_t ray, double* S, double* dlogI_refl) {
+    const rctx* C = G->C;
+    const or_desc* D = C->d;
+    const int band = D->band;
+    const uint32_t nm = G->n_mat;
+    or_rng R;
+    rng_init(&R, D->seed, ray, D->ctx, D->instance);
+    float I = 1.0f, L = 0.0f;
+    float o[3], dd[3], prev[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+    int bt = OR_REFLECT, mat = -1;
+    for (uint32_t q = 0; q < nm; q++) S[q] = dlogI_refl[q] = 0.0;
+    for (uint32_t k = 0; k <= D->max_bounces; k++) {
+        float spec = 0.0f;
+        float h_nu[3] = {0, 0, 0}, h_r[3] = {0, 0, 0};
+        if (k == 0) {
+            o[0] = D->src_pos[0];
+            o[1] = D->src_pos[1];
+            o[2] = D->src_pos[2];
+            sample_sphere(&R, dd);
+        } else {
+            float t;
+            int32_t idx;
+            if (!closest_hit_p(C->s, o, dd, &t, &idx)) break;
+            float p[3] = {o[0] + dd[0] * t, o[1] + dd[1] * t, o[2] + dd[2] * t};
+            const float* N = C->s->nrm + 3 * idx;
+            if (dot3(N, dd) > 0.0f) {
+                n[0] = N[0] * -1.0f;
+                n[1] = N[1] * -1.0f;
+                n[2] = N[2] * -1.0f;
+            } else {
+                n[0] = N[0];
+                n[1] = N[1];
+                n[2] = N[2];
+            }
+            mat = C->s->mat[idx];
+            bt = (int)material_bounce(C, &R, mat, band);
+            spec = (C->mats[mat].flags & OR_MAT_HAS_SPEC) ? C->mats[mat].spec[band] : 0.0f;
+            float v[3];
+            if (bt == OR_REFRACT) {
+                n[0] = -n[0];
+                n[1] = -n[1];
+                n[2] = -n[2];
+                memcpy(h_r, dd, sizeof(h_r));
+            } else {
+                reflect3(dd, n, h_r);
+            }
+            memcpy(h_nu, n, sizeof(h_nu));
+            sample_hemi_spec(&R, n, h_r, spec, v);
+            float dist[3];
+            sub3(p, o, dist);
+            float l = len3(dist);
+            dlogI_refl[mat] += 1.0 / (double)C->ret[mat][band];
+            I *= or_pow(D->air, l);
+            L += l;
+            o[0] = p[0];
+            o[1] = p[1];
+            o[2] = p[2];
+            dd[0] = v[0];
+            dd[1] = v[1];
+            dd[2] = v[2];
+            I *= C->ret[mat][band];
+        }
+        if (!is_normal_f(I)) break;
+        if (k > 0) {
+            for (uint32_t r = 0; r < D->n_rec; r++) {
+                const float* RP = D->rec[r].pos;
+                float dv[3];
+                sub3(RP, o, dv);
+                if (any_hit_p(C->s, o, dv)) continue;
+                float lsdir[3] = {dv[0], dv[1], dv[2]};
+                normalize3(lsdir);
+                if (!(dot3(lsdir, n) > 0.0f)) continue;
+                float diff, sf;
+                if (bt == OR_REFLECT) {
+                    float rv[3];
+                    reflect3(prev, n, rv);
+                    diff = -dot3(n, prev);
+                    sf = fmaxf(0.0f, dot3(rv, lsdir));
+                } else {
+                    diff = dot3(n, prev);
+                    sf = fmaxf(0.0f, dot3(prev, lsdir));
+                }
+                float lobe = OR_EXP_INT * or_pow(sf, OR_EXP);
+                float c = I;
+                c *= spec * OR_EXP_INT * or_pow(sf, OR_EXP) + (1.0f - spec) * diff;
+                float ls = len3(dv);
+                float geo = or_pow(D->air, ls) * (2.0f / (4.0f * OR_PI * ls * ls));
+                c *= or_pow(D->air, ls);
+                c *= 2.0f / (4.0f * OR_PI * ls * ls);
+                if (!is_normal_f(c)) continue;
+                if (k % 2) c *= -1.0f;
+                double a_c = contrib_adjoint(G, (int)r, lsdir, c, (L + ls) / 343.0f, L + ls);
+                if (a_c == 0.0) continue;
+                double cnof = ((k % 2) ? -1.0 : 1.0) * (double)I * (double)geo;
+                for (uint32_t q = 0; q < nm; q++) {
+                    G->g_refl[q] += a_c * (double)c * dlogI_refl[q];
+                    G->g_spec[q] += a_c * (double)c * S[q];
+                }
+                G->g_spec[mat] += a_c * cnof * ((double)lobe - (double)diff);
+            }
+        }
+        if ((double)I < D->cutoff) break;
+        if (k > 0 && k < D->max_bounces) {
+            /* the direction just drawn at this vertex leaves it: its score,
+             * the rim flux and perhaps a continuation */
+            double cdir = (double)dd[0] * (double)h_r[0] + (double)dd[1] * (double)h_r[1] +
+                          (double)dd[2] * (double)h_r[2];
+            double cr = (double)h_r[0] * (double)h_nu[0] + (double)h_r[1] * (double)h_nu[1] +
+                        (double)h_r[2] * (double)h_nu[2];
+            double F = score_flux(cr, (double)spec);
+            double coef = score_dlogp(cdir, (double)spec) - F;
+            or_rng Rs;
+            score_rng_init(&Rs, D->seed, ray, D->ctx, D->instance, k);
+            const float qk = G->q_spawn;
+            if (rng_next(&Rs) < qk) {
+                float vb[3];
+                score_rim_dir(&Rs, h_nu, h_r, spec, vb);
+                double wgt = F / (double)qk;
+                coef += wgt;
+                score_continue(G, &Rs, k, o, vb, I, L, -wgt, mat);
+            }
+            S[mat] += coef;
+        }
+        prev[0] = dd[0];
+        prev[1] = dd[1];
+        prev[2] = dd[2];
+        normalize3(prev);
+    }
+}
+
 static void* gworker(void* arg) {
     gworker_t* G = (gworker_t*)arg;
     tan_t T[16];
     double dr[16];
-    for (uint32_t ray = G->begin; ray < G->end; ray++) grad_ray(G, ray, T, dr);
+    double S[16];
+    for (uint32_t ray = G->begin; ray < G->end; ray++) {
+        if (G->local == 2)
+            grad_ray_score(G, ray, S, dr);
+        else
+            grad_ray(G, ray, T, dr);
+    }
     return NULL;
 }
 
 int or_gradient(const or_scene* s, const or_desc* D, const double* adj_signed, const double* adj_abs, int local,
                 double* grad_refl, double* grad_spec) {
+    return or_gradient_k(s, D, adj_signed, adj_abs, local, OR_SCORE_SPAWN, grad_refl, grad_spec);
+}
+
+int or_gradient_k(const or_scene* s, const or_desc* D, const double* adj_signed, const double* adj_abs, int local,
+                  float q_spawn, double* grad_refl, double* grad_spec) {
     if (s->n_mat > 16 || D->band < 0 || D->band > 2) return -1;
+    if (local == 2 && !(q_spawn > 0.0f && q_spawn <= 1.0f)) return -1;
+    if (local == 2) {
+        if (D->src_kind != OR_SRC_POINT) return -1;
+        for (uint32_t m = 0; m < s->n_mat; m++)
+            if ((s->mats[m].flags & OR_MAT_HAS_SPEC) && !(s->mats[m].spec[D->band] < 0.5f)) return -3;
+    }
     prep_scene ps;
     if (!prep(s, &ps)) return -2;
     float (*ret)[3] = (float (*)[3])calloc(s->n_mat ? s->n_mat : 1, sizeof(float[3]));
@@ -1296,6 +1652,7 @@ int or_gradient(const or_scene* s, const or_desc* D, const double* adj_signed, c
         W[i].adj_a = adj_abs;
         W[i].n_mat = s->n_mat;
         W[i].local = local;
+        W[i].q_spawn = q_spawn;
         W[i].begin = D->ray_begin + (uint32_t)(((uint64_t)nr * i) / nt);
         W[i].end = D->ray_begin + (uint32_t)(((uint64_t)nr * (i + 1)) / nt);
         W[i].g_refl = (double*)calloc(16, sizeof(double));

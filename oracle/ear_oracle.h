@@ -150,10 +150,22 @@ int or_num_channels(const or_desc* d);
 /* FP64 reference gradient of sum(adj_signed * H_signed + adj_abs * H_abs)
  * (raw sums, before 1/N) with respect to the rendered band's per-material
  * reflection coefficient and specularity; adjoints are [n_ch][n_bins] or NULL.
- * local != 0: direction tangents are not carried past the vertex that
- * sampled them (only each connection's explicit dependence on spec). */
+ * local == 1: direction tangents are not carried past the vertex that
+ * sampled them (only each connection's explicit dependence on spec).
+ * local == 2: the score form of the specularity gradient (unbiased; point
+ * sources, specularities below 1/2, else -1 / -3), a continuation path
+ * spawned with probability q_spawn at every vertex (or_gradient:
+ * q_spawn = OR_SCORE_SPAWN). */
+#define OR_SCORE_SPAWN 0.0625f
 int or_gradient(const or_scene* s, const or_desc* d, const double* adj_signed, const double* adj_abs, int local,
                 double* grad_refl, double* grad_spec);
+int or_gradient_k(const or_scene* s, const or_desc* d, const double* adj_signed, const double* adj_abs, int local,
+                  float q_spawn, double* grad_refl, double* grad_spec);
+/* the score form's closed forms: d log p(v)/ds of the blend's direction
+ * density at c = v.r, the rim flux F at cr = r.nu, and rim directions */
+double or_score_dlogp(double c, double s);
+double or_score_flux(double cr, double s);
+void or_score_rim_dirs(uint64_t seed, uint32_t n, const float nu[3], const float r[3], float s, float* out);
 double or_now(void);
 
 /* ---- auralization: the three-band equalizer in single precision ---- */

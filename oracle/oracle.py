@@ -90,7 +90,14 @@ def lib():
         L.or_num_channels.argtypes = [C.POINTER(Desc)]
         L.or_gradient.argtypes = [C.POINTER(Scene), C.POINTER(Desc), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                   C.c_void_p]
+        L.or_gradient_k.argtypes = [C.POINTER(Scene), C.POINTER(Desc), C.c_void_p, C.c_void_p, C.c_int, C.c_float,
+                                    C.c_void_p, C.c_void_p]
         L.or_now.restype = C.c_double
+        L.or_score_dlogp.restype = C.c_double
+        L.or_score_dlogp.argtypes = [C.c_double, C.c_double]
+        L.or_score_flux.restype = C.c_double
+        L.or_score_flux.argtypes = [C.c_double, C.c_double]
+        L.or_score_rim_dirs.argtypes = [C.c_uint64, C.c_uint32, fp, fp, C.c_float, fp]
         _lib = L
     return _lib
 
@@ -279,10 +286,12 @@ def render(scene: OracleScene, *, band, air, recorders, n_rays, src_pos=(0, 0, 0
 
 def gradient(scene: OracleScene, *, band, air, recorders, n_rays, adj_signed=None, adj_abs=None,
              src_pos=(0, 0, 0), src_tris=None, max_bounces=50, cutoff=0.0, seed=0xEA2, ctx=None, instance=0,
-             n_bins=44100, n_threads=8, local=False):
+             n_bins=44100, n_threads=8, local=False, score=False, q_spawn=None):
     """FP64 reference gradient of sum(adj * H) (raw sums) with respect to the
     band's per-material reflection coefficient and specularity (or_gradient;
-    local: direction tangents end at their vertex)."""
+    local: direction tangents end at their vertex; score: the score form of
+    the specularity gradient, a continuation path spawned with probability
+    q_spawn at every vertex, None: the library's default)."""
     if ctx is None:
         ctx = band
     recs = make_recorders(recorders)
@@ -303,8 +312,14 @@ def gradient(scene: OracleScene, *, band, air, recorders, n_rays, adj_signed=Non
     a_a = None if adj_abs is None else np.ascontiguousarray(adj_abs, np.float64)
     gr = np.zeros(max(scene.n_mat, 1), np.float64)
     gs = np.zeros(max(scene.n_mat, 1), np.float64)
-    rc = lib().or_gradient(C.byref(scene.s), C.byref(d), None if a_s is None else a_s.ctypes.data,
-                           None if a_a is None else a_a.ctypes.data, int(local), gr.ctypes.data, gs.ctypes.data)
+    mode = 2 if score else int(bool(local))
+    if q_spawn is None:
+        rc = lib().or_gradient(C.byref(scene.s), C.byref(d), None if a_s is None else a_s.ctypes.data,
+                               None if a_a is None else a_a.ctypes.data, mode, gr.ctypes.data, gs.ctypes.data)
+    else:
+        rc = lib().or_gradient_k(C.byref(scene.s), C.byref(d), None if a_s is None else a_s.ctypes.data,
+                                 None if a_a is None else a_a.ctypes.data, mode, float(q_spawn), gr.ctypes.data,
+                                 gs.ctypes.data)
     if rc != 0:
         raise RuntimeError(f"or_gradient failed: {rc}")
     del src
@@ -325,3 +340,24 @@ def direct_ramp(direct, n_bins):
             if 0 <= b < n_bins:
                 h[c, b] += (w - i) * S
     return h
+
+
+def score_dlogp(c, s):
+    """d log p(v) / ds of the specular blend's direction density at c = v.r."""
+    return lib().or_score_dlogp(float(c), float(s))
+
+
+def score_flux(cr, s):
+    """The rate F at which the blend's direction probability leaves through
+    the image of the hemisphere's rim, cr = r.nu."""
+    return lib().or_score_flux(float(cr), float(s))
+
+
+def score_rim_dirs(n, nu, r, s, seed=0xEA2):
+    """n rim directions normalize((1-s) e(phi) + s r), phi drawn with the
+    flux density, as a continuation draws them."""
+    out = np.zeros((n, 3), np.float32)
+    nu = np.ascontiguousarray(nu, np.float32)
+    r = np.ascontiguousarray(r, np.float32)
+    lib().or_score_rim_dirs(seed, n, _fptr(nu), _fptr(r), float(s), _fptr(out))
+    return out

@@ -58,7 +58,7 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
 
 def test_status_strings_and_abi_version():
     lib = F.load()
-    assert lib.ear_abi_version() == 6
+    assert lib.ear_abi_version() == 7
     assert lib.ear_status_string(0) == b"ok"
     assert lib.ear_status_string(2) == b"Invalid material settings"
     assert lib.ear_status_string(99) == b"unknown status"

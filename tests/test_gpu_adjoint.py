@@ -105,6 +105,15 @@ def test_backward_argument_errors(gpu_ctx):
     s = _scene()
     a_v, _ = _adjoints(1)
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
+        with pytest.raises(api.EarError, match="exclude"):
+            api.render_backward(gpu_ctx, sc, adj_value=a_v, local=True, score=True, **_kw(s, n_rays=10, max_bounces=2))
+        sc.set_materials([(a, b, [0.3, 0.5, 0.3]) for a, b, _ in s["materials"]])
+        with pytest.raises(api.EarError, match="below 0.5") as e:
+            api.render_backward(gpu_ctx, sc, adj_value=a_v, score=True, **_kw(s, n_rays=10, max_bounces=2))
+        assert e.value.status_name == "EAR_ERR_INVALID_ARGUMENT"
+        # the band whose specularity is 0.5 is not rendered: accepted
+        api.render_backward(gpu_ctx, sc, adj_value=a_v, score=True, bands=(0, 2), **_kw(s, n_rays=10, max_bounces=2))
+        sc.set_materials(s["materials"])
         with pytest.raises(api.EarError, match="point sources"):
             api.render_backward(gpu_ctx, sc, adj_value=a_v, src_tris=np.eye(3, dtype=np.float32)[None],
                                 **_kw(s, n_rays=10, max_bounces=2))
@@ -248,3 +257,54 @@ def test_backward_is_linear_in_the_adjoint_full_size(gpu_ctx):
             for k in range(2):
                 want = g1[k] + 2 * g2[k]
                 np.testing.assert_allclose(g12[k], want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("stereo,max_bounces,cutoff", [(False, 12, 0.0), (True, 6, 0.0), (False, 999, 1e-8)])
+def test_score_form_matches_fp64_oracle(gpu_ctx, oracle, stereo, max_bounces, cutoff):
+    """EAR_FLAG_ADJ_SCORE against the oracle's score form on the same rays
+    and the same continuation paths (spawn decisions, rim directions and
+    their transport come from the vertex streams both draw alike)."""
+    s = _scene(stereo)
+    n_rays = 20_000
+    nch = 2 if stereo else 1
+    a_v, a_a = _adjoints(nch)
+    with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
+        g_r, g_s = api.render_backward(gpu_ctx, sc, adj_value=a_v, adj_abs=a_a, score=True,
+                                       **_kw(s, n_rays=n_rays, max_bounces=max_bounces, cutoff=cutoff))
+    osc = oracle.OracleScene(s["tris"], s["mat"], s["materials"])
+    for b in range(3):
+        o_r, o_s = oracle.gradient(osc, band=b, air=s["air"][b], recorders=s["recorders"], n_rays=n_rays,
+                                   adj_signed=a_v[b], adj_abs=a_a[b], src_pos=s["src"], max_bounces=max_bounces,
+                                   cutoff=cutoff, n_bins=NB, score=True)
+        o_r, o_s = o_r / n_rays, o_s / n_rays
+        np.testing.assert_allclose(g_r[:, b], o_r, rtol=0, atol=1e-4 * np.abs(o_r).max())
+        np.testing.assert_allclose(g_s[:, b], o_s, rtol=0, atol=1e-4 * np.abs(o_s).max())
+        assert np.abs(o_s).max() > 0
+
+
+def test_score_form_batch_equals_single(gpu_ctx):
+    """ear_render_backward_batch in the score form: each item's gradient
+    equals its own ear_render_backward (continuations included) up to the
+    summation order."""
+    rooms = [_scene(), scenes.room(seed=12, target_tris=400, kind="L"), scenes.room(seed=13, target_tris=2000)]
+    specs = [dict(n_rays=40_000, bands=(0, 1, 2)), dict(n_rays=12_345, bands=(1,)), dict(n_rays=25_000, bands=(0, 2))]
+    rng = np.random.default_rng(19)
+    handles, items, singles = [], [], []
+    try:
+        for r, sp in zip(rooms, specs):
+            sc = api.Scene(gpu_ctx, r["tris"], r["mat"], r["materials"])
+            handles.append(sc)
+            kw = dict(recorders=r["recorders"], src_pos=r["src"], air=r["air"], max_bounces=200, cutoff=1e-8,
+                      n_bins=4096, abs_hist=True, **sp)
+            a_a = rng.standard_normal((3, 1, 4096))
+            items.append((sc, dict(kw, adj_abs=a_a, score=True)))
+            singles.append(api.render_backward(gpu_ctx, sc, adj_abs=a_a, score=True, **kw))
+        got = api.render_backward_batch(gpu_ctx, items)
+        for (gr, gs), (wr, ws) in zip(got, singles):
+            for g, w in ((gr, wr), (gs, ws)):
+                np.testing.assert_allclose(g, w, rtol=1e-9, atol=1e-12 * np.abs(w).max())
+        with pytest.raises(api.EarError, match="mixes"):
+            api.render_backward_batch(gpu_ctx, [items[0], (items[1][0], dict(items[1][1], score=False))])
+    finally:
+        for sc in handles:
+            sc.close()

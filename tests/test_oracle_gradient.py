@@ -73,6 +73,100 @@ def test_spec_gradient_equals_central_difference(oracle, room, max_bounces, rtol
         assert abs(fd - g_s[m]) <= rtol * abs(fd), (m, fd, g_s[m])
 
 
+@pytest.mark.parametrize("theta,s", [(0, 0.3), (40, 0.3), (75, 0.45), (20, 0.05), (60, 0.0)])
+def test_score_terms_match_a_pathwise_derivative(oracle, theta, s):
+    """The score form's closed forms against the mathematics they restate: for
+    a smooth G the derivative of E[G(v)], v = normalize((1-s) h + s r) with h
+    uniform on the hemisphere, is the common-random-number difference of the
+    sample means (no discontinuity, so exact up to O(eps^2)); the score form
+    writes it as E[G dlog p/ds] - F E[G(v_b)] with v_b drawn by the rim
+    sampler.  E[dlog p/ds] = F (G = 1) pins the density and the flux
+    together; a dropped rim term or a wrong flux is off by ~F E[G]."""
+    rng = np.random.default_rng(11 + theta)
+    nu = np.array([0.0, 0.0, 1.0])
+    r = np.array([np.sin(np.radians(theta)), 0.0, np.cos(np.radians(theta))])
+    n = 1_000_000
+    h = rng.standard_normal((n, 3))
+    h /= np.linalg.norm(h, axis=1, keepdims=True)
+    h[h @ nu < 0] *= -1
+
+    def blend(t):
+        x = (1 - t) * h + t * r
+        return x / np.linalg.norm(x, axis=1, keepdims=True)
+
+    a = np.array([0.7, -1.1, 0.4])
+
+    def G(w):
+        return np.exp(w @ a) * (1 + w[:, 0] ** 2)
+
+    eps = 1e-4
+    truth = (G(blend(s + eps)).mean() - G(blend(s - eps)).mean()) / (2 * eps)
+    v = blend(s)
+    dl = np.array([oracle.score_dlogp(c, s) for c in v @ r])
+    F = oracle.score_flux(float(r @ nu), s)
+    vb = oracle.score_rim_dirs(400_000, nu, r, s).astype(np.float64)
+    assert abs(dl.mean() - F) < 4 * dl.std() / np.sqrt(n) + 1e-9
+    np.testing.assert_allclose(np.linalg.norm(vb, axis=1), 1.0, atol=1e-6)
+    # every rim direction lies on the image of the rim: h = ((t v - s r)/(1-s)) . nu = 0
+    c = vb @ r
+    t = s * c + np.sqrt((1 - s) ** 2 - s * s * (1 - c * c))
+    np.testing.assert_allclose(((t[:, None] * vb - s * r) / (1 - s)) @ nu, 0.0, atol=2e-6)
+    gv = G(v) * dl
+    gb = G(vb)
+    est = gv.mean() - F * gb.mean()
+    se = np.hypot(gv.std() / np.sqrt(n), F * gb.std() / np.sqrt(len(gb)))
+    assert abs(est - truth) < 4 * se, (est, truth, se)
+
+
+def test_score_gradient_equals_central_differences_over_seeds(oracle, room):
+    """The score form of d/dspec (continuations included) against central
+    differences of the oracle render with common random numbers at h = 0.05,
+    six bounces: both are unbiased estimates of the same derivative of the
+    expected loss (the differences up to O(h^2)), so over 8 seeds they agree
+    within their standard errors (at most 4 of them per wall, chi^2 over the
+    six walls below 20)."""
+    _, adj_a = _adj()
+    band = 2
+    kw = dict(band=band, air=room["air"][band], recorders=room["recorders"], src_pos=room["src"], cutoff=0.0,
+              n_bins=NB, max_bounces=6)
+    n, seeds, h = 200_000, 8, 0.05
+    sc = oracle.OracleScene(room["tris"], room["mat"], room["materials"])
+
+    def pert(m, d):
+        out = [(list(a), list(b) if b is not None else None, list(c) if c is not None else None)
+               for a, b, c in room["materials"]]
+        out[m][2][band] += d
+        return out
+
+    def loss(mats, seed):
+        r = oracle.render(oracle.OracleScene(room["tris"], room["mat"], mats), n_rays=n, seed=seed, n_threads=8, **kw)
+        return float((adj_a * r["hist_abs"]).sum())
+
+    g = np.array([oracle.gradient(sc, adj_abs=adj_a, n_rays=n, score=True, seed=500 + k, **kw)[1]
+                  for k in range(seeds)])
+    fd = np.array([[(loss(pert(m, h), 900 + k) - loss(pert(m, -h), 900 + k)) / (2 * h) for m in range(6)]
+                   for k in range(seeds)])
+    se = np.hypot(g.std(0, ddof=1), fd.std(0, ddof=1)) / np.sqrt(seeds)
+    z = (g.mean(0) - fd.mean(0)) / se
+    print("score", g.mean(0), "fd", fd.mean(0), "z", z)
+    assert np.all(np.abs(z) < 4) and float((z ** 2).sum()) < 20, z
+
+
+def test_score_form_keeps_refl_and_rejects_what_it_cannot_do(oracle, room):
+    """d/drefl does not depend on the form; specularities of 0.5 and above
+    (where the blend's direction map folds) and mesh sources are refused."""
+    adj_s, adj_a = _adj()
+    sc = oracle.OracleScene(room["tris"], room["mat"], room["materials"])
+    args = dict(band=1, air=room["air"][1], recorders=room["recorders"], adj_signed=adj_s, adj_abs=adj_a,
+                src_pos=room["src"], cutoff=0.0, n_bins=NB, n_rays=500, max_bounces=12)
+    full = oracle.gradient(sc, **args)
+    sco = oracle.gradient(sc, score=True, **args)
+    np.testing.assert_allclose(sco[0], full[0], rtol=1e-12)
+    mats = [(a, b, [0.5, 0.5, 0.5]) for a, b, _ in room["materials"]]
+    with pytest.raises(RuntimeError, match="-3"):
+        oracle.gradient(oracle.OracleScene(room["tris"], room["mat"], mats), score=True, **args)
+
+
 def test_local_mode_drops_only_carried_tangents(oracle, room):
     """At one bounce no tangent is carried, so the local and full gradients
     agree exactly; at several bounces refl is untouched and spec differs."""
