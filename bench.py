@@ -345,7 +345,8 @@ def bench_adjoint(ctx, n_rooms=32, rays_accurate=4_000_000):
     """Differentiation: the 12 x 9 x 5 m shoebox (6 walls x 3 bands
     absorption + 6 specularities), 1M rays, 4096-bin log-EDC loss against a
     target rendered at the true parameters, evaluated at parameters off by
-    +-20%: forward, adjoint and their rate beside the forward alone; the EDC
+    +-20%: forward, adjoint (the score form; the local form timed beside it)
+    and their rate beside the forward alone; the EDC
     error of the 1M-ray forward against 8M rays; then n_rooms rooms of
     1k..100k triangles through the batched forward and adjoint at 250k rays
     each and at rays_accurate each (the accuracy point: the EDC error of the
@@ -545,9 +546,12 @@ def bench_fit(ctx, steps=200, n_rooms=32, room_steps=10):
            "steps": steps, "wall_s": round(wall, 3), "ms_per_step": round(1e3 * wall / steps, 2),
            "loss_first": round(st.history[0][0], 6), "loss_last": round(st.history[-1][0], 6),
            "area_weighted_alpha_rel_error_first": round(err(st.history[0][1]), 5),
-           "area_weighted_alpha_rel_error_last": round(err(st.history[-1][1]), 5)}
+           "area_weighted_alpha_rel_error_last": round(err(st.history[-1][1]), 5),
+           "spec_mean_abs_error_first": round(float(np.abs(st.history[0][2] - spec_t).mean()), 5),
+           "spec_mean_abs_error_last": round(float(np.abs(st.history[-1][2] - spec_t).mean()), 5)}
     log(f"fit: {steps} steps in {wall:.2f}s, loss {out['loss_first']} -> {out['loss_last']}, area-weighted alpha "
-        f"error {out['area_weighted_alpha_rel_error_first']} -> {out['area_weighted_alpha_rel_error_last']}")
+        f"error {out['area_weighted_alpha_rel_error_first']} -> {out['area_weighted_alpha_rel_error_last']}, "
+        f"spec error {out['spec_mean_abs_error_first']} -> {out['spec_mean_abs_error_last']}")
 
     rng = np.random.default_rng(17)
     rooms = [scenes.room(seed=5000 + i, target_tris=int(np.exp(rng.uniform(np.log(1000), np.log(100_000)))))

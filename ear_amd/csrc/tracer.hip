@@ -2764,9 +2764,12 @@ struct Tangents {
     __device__ float& row(uint32_t j, uint32_t slot) const { return t[(size_t)j * cap + slot]; }  // score form
 };
 
+// The adjoint histograms as prefix sums: row (band, channel) holds bins + 1
+// pairs {sum_{b<j} a[b], sum_{b<j} b a[b]}, so that a ramp's weighted sum
+// sum_b (w + s - b) a[b] over its bins is four loads at any width.
 struct AdjArgs {
-    const double* adj_s;  // [3][n_ch][bins] dL / d raw (signed), or null
-    const double* adj_a;  // [3][n_ch][bins] dL / d raw (|.|), or null
+    const double2* adj_s;  // [3][n_ch][bins + 1] prefix sums of dL / d raw (signed), or null
+    const double2* adj_a;  // [3][n_ch][bins + 1] prefix sums of dL / d raw (|.|), or null
     double* grad;         // [3][2][n_mat]: band, {refl, spec}, material (copy 0)
     uint32_t n_mat;
     uint32_t copy_stride;  // doubles from one gradient copy to the next
@@ -2819,14 +2822,14 @@ __global__ void k_sum_grad_copies(double* g, uint32_t n, uint32_t copy_stride) {
     }
 }
 
-__device__ double ramp_adjoint(const double* adj, uint32_t nb, int s, int w, double per_c) {
-    double acc = 0.0;
-    for (int i = 0; i < w; ++i) {
-        int b = s + i;
-        if (b < 0 || (uint32_t)b >= nb) continue;
-        acc += adj[b] * ((double)(w - i) / (double)w) * per_c;
-    }
-    return acc;
+// sum over the ramp's bins b = s .. s + w - 1 inside [0, nb) of
+// a[b] (w - (b - s)) / w * per_c, from the row's prefix sums
+__device__ double ramp_adjoint(const double2* pre, uint32_t nb, int s, int w, double per_c) {
+    const int lo = s < 0 ? 0 : s;
+    const int hi = s + w > (int)nb ? (int)nb : s + w;
+    if (lo >= hi) return 0.0;
+    const double2 a = pre[lo], b = pre[hi];
+    return ((double)(w + s) * (b.x - a.x) - (b.y - a.y)) / (double)w * per_c;
 }
 
 // dL/dc of a contribution c at time t, distance dist, direction dir.
@@ -2838,7 +2841,7 @@ __device__ double contribution_adjoint(const RenderParams& P, const AdjArgs& A, 
     double per_c = 2.0 / (double)width;
     double sgn = c < 0.0f ? -1.0 : 1.0;
     double a = 0.0;
-    auto row = [&](const double* base, int ch) { return base + ((size_t)band * nch + ch) * nb; };
+    auto row = [&](const double2* base, int ch) { return base + ((size_t)band * nch + ch) * (nb + 1); };
     if (R.kind == EAR_REC_MONO) {
         int s = (int)((double)t * 44100.0);
         if (A.adj_s) a += ramp_adjoint(row(A.adj_s, R.channel), nb, s, w, per_c);
@@ -3553,7 +3556,8 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
         adj_off[i] = adj_cells;
         adj_cells += (size_t)3 * I.P.n_channels * I.P.n_bins;
     }
-    // value = raw * (1/N) * gain^2 (k_finalize), so dL/draw = dL/dvalue * (1/N) * gain^2
+    // value = raw * (1/N) * gain^2 (k_finalize), so dL/draw = dL/dvalue * (1/N) * gain^2;
+    // each row is then turned into prefix sums (ramp_adjoint)
     std::vector<double> as(adj_cells, 0.0), aa(adj_cells, 0.0);
     bool any_s = false, any_a = false;
     for (uint32_t i = 0; i < n_items; ++i) {
@@ -3660,18 +3664,49 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     const uint32_t grad_n = 3 * 2 * nm_max * n_items;  // one copy
     TRY(dgrad.alloc(sizeof(double) * (size_t)grad_n * kGradCopies));
     TRY(dargs.alloc(sizeof(AdjArgs) * n_items));
+    // prefix sums of every row (bins + 1 pairs), accumulated in long double
+    std::vector<size_t> pre_off(n_items);
+    size_t pre_pairs = 0;
+    for (uint32_t i = 0; i < n_items; ++i) {
+        pre_off[i] = pre_pairs;
+        pre_pairs += (size_t)3 * inst[i].P.n_channels * (inst[i].P.n_bins + 1);
+    }
+    auto prefix = [&](const std::vector<double>& a) {
+        std::vector<double> pre(2 * pre_pairs, 0.0);
+        for (uint32_t i = 0; i < n_items; ++i) {
+            const uint32_t nb = inst[i].P.n_bins, rows = 3 * inst[i].P.n_channels;
+            for (uint32_t r = 0; r < rows; ++r) {
+                const double* src = a.data() + adj_off[i] + (size_t)r * nb;
+                double* dst = pre.data() + 2 * (pre_off[i] + (size_t)r * (nb + 1));
+                long double s1 = 0.0L, s2 = 0.0L;
+                for (uint32_t b = 0; b <= nb; ++b) {
+                    dst[2 * b] = (double)s1;
+                    dst[2 * b + 1] = (double)s2;
+                    if (b < nb) {
+                        s1 += (long double)src[b];
+                        s2 += (long double)b * (long double)src[b];
+                    }
+                }
+            }
+        }
+        return pre;
+    };
     if (any_s) {
-        TRY(dadj_s.alloc(adj_cells * 8));
-        EAR_HIP_TRY(c, hipMemcpyAsync(dadj_s.p, as.data(), adj_cells * 8, hipMemcpyHostToDevice, st));
+        const std::vector<double> pre = prefix(as);
+        TRY(dadj_s.alloc(pre.size() * 8));
+        EAR_HIP_TRY(c, hipMemcpyAsync(dadj_s.p, pre.data(), pre.size() * 8, hipMemcpyHostToDevice, st));
+        EAR_HIP_TRY(c, hipStreamSynchronize(st));  // pre leaves scope
     }
     if (any_a) {
-        TRY(dadj_a.alloc(adj_cells * 8));
-        EAR_HIP_TRY(c, hipMemcpyAsync(dadj_a.p, aa.data(), adj_cells * 8, hipMemcpyHostToDevice, st));
+        const std::vector<double> pre = prefix(aa);
+        TRY(dadj_a.alloc(pre.size() * 8));
+        EAR_HIP_TRY(c, hipMemcpyAsync(dadj_a.p, pre.data(), pre.size() * 8, hipMemcpyHostToDevice, st));
+        EAR_HIP_TRY(c, hipStreamSynchronize(st));
     }
     std::vector<AdjArgs> args(n_items);
     for (uint32_t i = 0; i < n_items; ++i)
-        args[i] = AdjArgs{adj_value[i] ? dadj_s.as<double>() + adj_off[i] : nullptr,
-                          adj_abs[i] ? dadj_a.as<double>() + adj_off[i] : nullptr,
+        args[i] = AdjArgs{adj_value[i] ? dadj_s.as<double2>() + pre_off[i] : nullptr,
+                          adj_abs[i] ? dadj_a.as<double2>() + pre_off[i] : nullptr,
                           dgrad.as<double>() + (size_t)i * 3 * 2 * nm_max, scenes[i]->n_mat, grad_n};
     EAR_HIP_TRY(c, hipMemcpyAsync(dargs.p, args.data(), sizeof(AdjArgs) * n_items, hipMemcpyHostToDevice, st));
     EAR_HIP_TRY(c, hipMemcpyAsync(dinst.p, inst.data(), sizeof(InstDev) * n_items, hipMemcpyHostToDevice, st));
