@@ -285,7 +285,7 @@ unsigned Recorder::getLength(float tresh) const {
 void Recorder::Truncate(int len) {
     for (RecorderTrack* t : tracks) t->Truncate((unsigned)len);
 }
-void Recorder::AddHistogram(int channel, const double* values, size_t n) {
+void Recorder::AddHistogram(int channel, const double* values, size_t n, long last_written) {
     RecorderTrack* t = tracks.at((size_t)channel);
     t->data.assign(n, 0.0f);
     size_t first = n, last = 0;
@@ -298,7 +298,11 @@ void Recorder::AddHistogram(int channel, const double* values, size_t n) {
     }
     t->first_sample = first == n ? 0 : (unsigned)first;
     t->real_length = first == n ? 0 : (unsigned)last + 1;
-    has_samples = has_samples || first != n;
+    if (last_written >= 0) {
+        t->first_sample = first == n ? (unsigned)last_written : (unsigned)std::min<size_t>(first, (size_t)last_written);
+        t->real_length = (unsigned)last_written;
+    }
+    has_samples = has_samples || first != n || last_written >= 0;
 }
 
 static void splat(RecorderTrack* tr, int s, float ampl, int w, bool skip_negative) {
@@ -545,8 +549,10 @@ struct PreparedRender {
     std::vector<float> mesh_v;
     ear_render_desc d{};
     std::vector<double> value;
+    std::vector<int64_t> raw, abs_raw;  // RenderOptions::multiply_quirk
     ear_hist h{};
     int nch = 0;
+    bool quirk = false;
 };
 
 void prepare_render(const Scene& scene, PreparedRender& p, int sound, const float absorbtion_factor[3],
@@ -592,16 +598,59 @@ void prepare_render(const Scene& scene, PreparedRender& p, int sound, const floa
     for (const ear_recorder& r : p.recs) p.nch += r.kind == EAR_REC_STEREO ? 2 : 1;
     p.value.assign((size_t)3 * p.nch * d.hist_bins, 0.0);
     p.h.value = p.value.data();
+    p.quirk = scene.options.multiply_quirk;
+    if (p.quirk) {
+        // the written bins are the ones whose |ramp| sum is nonzero
+        d.flags |= EAR_FLAG_ABS_HIST;
+        p.raw.assign((size_t)2 * 3 * p.nch * d.hist_bins, 0);
+        p.abs_raw.assign(p.raw.size(), 0);
+        p.h.raw = p.raw.data();
+        p.h.abs_raw = p.abs_raw.data();
+    }
+}
+
+// The reference's track of one (band, channel) after Scene::Render
+// (RenderOptions::multiply_quirk): bins written by a path ramp are the ones
+// whose exact |ramp| sum is nonzero (every ramp weight (w - i) / w is
+// positive).  The highest of them keeps its paths' sum P without 1/N, plus
+// the direct sound D, without gain^2 (a path is never shorter than the
+// direct sound, so its ramps reach at least as far).  With no path in the
+// track, the direct sound's highest bin keeps D without gain^2.  Returns that
+// bin (the track's real_length), -1 if nothing was written.
+long multiply_quirk(PreparedRender& p, int band, int ch) {
+    const size_t nb = p.d.hist_bins, off = ((size_t)band * p.nch + ch) * nb;
+    const int64_t* ab = p.abs_raw.data() + 2 * off;
+    const int64_t* rw = p.raw.data() + 2 * off;
+    double* v = p.value.data() + off;
+    const double g2 = (double)(p.d.source.gain * p.d.source.gain);
+    for (size_t j = nb; j-- > 0;) {
+        if (ab[2 * j] == 0 && ab[2 * j + 1] == 0) continue;
+        const long double P = ((long double)rw[2 * j + 1] * 18446744073709551616.0L +
+                               (long double)(uint64_t)rw[2 * j]) /
+                              (long double)std::ldexp(1.0, EAR_FRAC_BITS);
+        const long double N = (long double)std::max<uint32_t>(1, p.d.n_rays);
+        const long double D = g2 != 0.0 ? (long double)v[j] / g2 - P / N : 0.0L;
+        v[j] = (double)(P + D);
+        return (long)j;
+    }
+    for (size_t j = nb; j-- > 0;) {
+        if (v[j] == 0.0) continue;
+        if (g2 != 0.0) v[j] /= g2;
+        return (long)j;
+    }
+    return -1;
 }
 
 // Recorder::AddHistogram of every rendered band's tracks.
-void deliver_render(const PreparedRender& p, const std::vector<Recorder*> rec[3], unsigned band_mask) {
+void deliver_render(PreparedRender& p, const std::vector<Recorder*> rec[3], unsigned band_mask) {
     for (int b = 0; b < 3; ++b) {
         if (!((band_mask >> b) & 1u)) continue;
         int ch = 0;
         for (Recorder* r : rec[b]) {
-            for (int k = 0; k < r->trackCount(); ++k, ++ch)
-                r->AddHistogram(k, p.value.data() + ((size_t)b * p.nch + ch) * p.d.hist_bins, p.d.hist_bins);
+            for (int k = 0; k < r->trackCount(); ++k, ++ch) {
+                const long last = p.quirk ? multiply_quirk(p, b, ch) : -1;
+                r->AddHistogram(k, p.value.data() + ((size_t)b * p.nch + ch) * p.d.hist_bins, p.d.hist_bins, last);
+            }
         }
     }
 }

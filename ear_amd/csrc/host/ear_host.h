@@ -155,7 +155,10 @@ class Recorder {
     unsigned getLength(float tresh = -1.0f) const;
     void Truncate(int len);
     // Replaces track `channel` with the finalized histogram of a render.
-    void AddHistogram(int channel, const double* values, size_t n);
+    // last_written >= 0: real_length = last_written, the reference's highest
+    // index written (RenderOptions::multiply_quirk); otherwise one past the
+    // last nonzero value.
+    void AddHistogram(int channel, const double* values, size_t n, long last_written = -1);
     // src/Recorder.cpp:342-364: each track convolved with the section of sf
     // from `offset` seconds (to its end, or `length` seconds with the
     // cross-fade into r's tracks), appended to processed_tracks.
@@ -293,6 +296,13 @@ struct RenderOptions {
     unsigned leaf_size = 0, bvh_width = 0, bvh_builder = 0;  // 0: library defaults
     unsigned estimator = EAR_EST_NEE;  // EAR_EST_SPHERE: receiver spheres of receiver_radius
     float receiver_radius = 0.0f;
+    // FloatBuffer::Multiply (src/Recorder.cpp:85-89) stops one bin short of
+    // real_length, which is the highest index written (src/Recorder.cpp:52-57),
+    // so after Scene::Render (src/Scene.cpp:286-316) that bin of each track
+    // holds its paths' sum without 1/N and without gain^2, and every later
+    // Multiply or Power skips it.  false: every bin is scaled (real_length one
+    // past the last nonzero bin); true: the reference's tracks, bin for bin.
+    bool multiply_quirk = false;
 };
 
 // Scene (src/Scene.h:60-79).

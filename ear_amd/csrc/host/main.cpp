@@ -21,7 +21,8 @@
 // ramps run past it is repeated with room for them, as the reference's
 // buffers grow, unless growir (int4) is 0), device (int4), radius (flt4:
 // receiver spheres of this radius instead of the reference's point
-// connections).
+// connections), multiplyquirk (int4: 1 keeps each track's highest written
+// bin unscaled, as the reference's FloatBuffer::Multiply does).
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -88,6 +89,7 @@ Scene* build_scene(EarFile& F, bool load_audio) {
     if (F.has("cutoff")) o.energy_cutoff = F.get_float("cutoff");
     if (F.has("irlength")) o.hist_bins = (unsigned)std::lround(F.get_float("irlength") * 44100.0f);
     if (F.has("growir")) o.grow_histogram = F.get_int("growir") != 0;
+    if (F.has("multiplyquirk")) o.multiply_quirk = F.get_int("multiplyquirk") != 0;
     if (F.has("device")) o.device = F.get_int("device");
     if (F.has("radius")) {
         o.estimator = EAR_EST_SPHERE;

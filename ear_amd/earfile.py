@@ -110,10 +110,10 @@ class EarWriter:
 
 
 def scene_to_ear(s: dict, path: str, *, samples=10 ** 5, absorption=(0.01, 0.02, 0.05), wav="dry.wav",
-                 src_mesh=None, **extra):
+                 src_mesh=None, src_gain=None, **extra):
     """Writes a scenes.* dict as a .EAR file: one MAT + MESH block per
     material; src_mesh=(material index, triangles) makes the source an
-    emitting mesh."""
+    emitting mesh; src_gain: the source's gain field."""
     import numpy as np
 
     w = EarWriter()
@@ -126,9 +126,9 @@ def scene_to_ear(s: dict, path: str, *, samples=10 ** 5, absorption=(0.01, 0.02,
         if len(sel):
             w.mesh(f"m{i}", sel.tolist())
     if src_mesh is not None:
-        w.source(wav, mesh=(f"m{src_mesh[0]}", src_mesh[1]))
+        w.source(wav, mesh=(f"m{src_mesh[0]}", src_mesh[1]), gain=src_gain)
     else:
-        w.source(wav, location=s["src"])
+        w.source(wav, location=s["src"], gain=src_gain)
     for r in s["recorders"]:
         if r.get("kind", 0) == 1:
             raw = tuple(1.0 - h ** 0.25 for h in r["head_abs"])  # undo the ctor's max(0,(1-h)^4)

@@ -191,3 +191,51 @@ def test_render_two_sources(tmp_path):
     assert not np.array_equal(response(debug, 0, 0, 1), response(debug, 0, 1, 1))
     got = read_wave(tmp_path / "out.wav")[:, 0]
     check_close(got, OA.listener_output(tracks))
+
+
+def test_multiply_quirk_keeps_the_highest_written_bin_unscaled(tmp_path):
+    """SET multiplyquirk 1: the reference's FloatBuffer::Multiply stops one
+    bin short of real_length, the highest index written (src/Recorder.cpp:
+    52-57, 85-89), so after Scene::Render (src/Scene.cpp:286-316) that bin of
+    each track holds its paths' sum without 1/N and without gain^2, and Power
+    (the dumps are raised to 0.335) skips it as well.  With one bounce and a
+    quiet source (gain 0.001) that bin stays above the truncation threshold
+    (max / 256), so the dumps show it: the quirk's response runs to it (the
+    dump holds real_length + 1 values), its earlier bins equal the default
+    render's, and it holds the paths' raw sum.  The bin is the last one whose
+    |ramp| sum is nonzero in the same render through the C ABI (the CLI
+    traces samples / 10 rays, src/EAR.cpp:78-82)."""
+    from ear_amd import api
+
+    s = scenes.shoebox(spec=0.2)
+    samples, seed, gain = 2000, 77, 0.001
+    n_rays = samples // 10
+    dumps = {}
+    for quirk in (0, 1):
+        d = tmp_path / f"q{quirk}"
+        d.mkdir()
+        (d / "debug").mkdir()
+        write_dry(d / "dry.wav", 4000)
+        earfile.scene_to_ear(s, d / "box.EAR", samples=samples, seed=seed, maxbounces=1, wav=str(d / "dry.wav"),
+                             src_gain=gain, debugdir=str(d / "debug"), noprocessing=1, multiplyquirk=quirk)
+        run(d, d / "box.EAR")
+        dumps[quirk] = [np.fromfile(d / "debug" / f"response-0.sound-0.band-{b}{NAMES[b]}.bin", "<f4")
+                        for b in range(3)]
+    air = tuple(1.0 - a for a in (0.01, 0.02, 0.05))
+    with api.Context(0) as ctx, api.Scene(ctx, s["tris"], s["mat"], s["materials"]) as sc:
+        r = api.render(ctx, sc, recorders=s["recorders"], src_pos=s["src"], air=air, n_rays=n_rays, seed=seed,
+                       max_bounces=1, cutoff=1e-8, n_bins=3 * 44100, abs_hist=True, gain=gain)
+    for b in range(3):
+        written = np.flatnonzero(np.any(r.abs_raw[b, 0] != 0, axis=-1))
+        L = int(written[-1])
+        P = float(api.raw_to_float(r.raw[b, 0, L]))
+        n, q = dumps[0][b], dumps[1][b]
+        print("band", b, "last written bin", L, "dump lengths", len(n), len(q), "P", P, flush=True)
+        assert len(q) == L + 2 and len(n) <= L + 2
+        np.testing.assert_array_equal(q[:L], n[:L])
+        # the paths' sum, neither divided by N nor multiplied by gain^2 nor powered
+        np.testing.assert_allclose(q[L], np.float32(P), rtol=1e-6)
+        if len(n) > L:
+            # the default scales and powers it like every other bin
+            want = np.sign(P) * (abs(P) / n_rays * gain * gain) ** 0.335
+            np.testing.assert_allclose(n[L], want, rtol=1e-5)
