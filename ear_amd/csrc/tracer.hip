@@ -1241,32 +1241,46 @@ __global__ void __launch_bounds__(BLOCK) k_extend_f(const InstDev* __restrict__ 
     uint32_t nvis = 0, ntri = 0;
     Walker<LDS, BLOCK, WIDTH, false, INSTR> w(lds + threadIdx.x, gstack, gtid, gthreads);
     Traversal T = make_trav(inst[0], gstack, gthreads);
-    bool active = false, exhausted = n == 0;
+    // a lane is walking (active), holds a finished hit not yet stored
+    // (pending) or is free; pending hits are stored together when the wave
+    // refills, so the 8-byte records of a claimed run land in its cache lines
+    // in a few bursts instead of one lane at a time
+    bool active = false, pending = false, exhausted = n == 0;
     uint32_t slot = 0;
+    int2 hit = make_int2(0, -1);
     uint32_t c_next = 0, c_end = 0;  // the wave's claimed entries [c_next, c_end)
     uint32_t seg = ((blockIdx.x * BLOCK + threadIdx.x) >> 6) % fetch.n_seg;  // the wave's first segment
     for (;;) {
         uint64_t act = __ballot(active);
         const uint32_t n_idle = 64u - (uint32_t)__popcll(act);
-        if (!exhausted && (n_idle >= fetch.refill || act == 0)) {
-            const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted, seg);
-            if (mine != 0xFFFFFFFFu) {
-                slot = mine;
-                const float4 o = q.o[mine], d = q.d[mine];
-                if (BATCH) T = make_trav(inst[q.m[mine].w], gstack, gthreads);
-                if (w.start(T, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 1000000.0f))
-                    active = true;
-                else
-                    hits[mine] = make_int2(__float_as_int(1000000.0f), -1);
+        if (n_idle >= fetch.refill || act == 0) {
+            if (pending) {
+                hits[slot] = hit;
+                pending = false;
+            }
+            if (!exhausted) {
+                const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted, seg);
+                if (mine != 0xFFFFFFFFu) {
+                    slot = mine;
+                    const float4 o = q.o[mine], d = q.d[mine];
+                    if (BATCH) T = make_trav(inst[q.m[mine].w], gstack, gthreads);
+                    if (w.start(T, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 1000000.0f))
+                        active = true;
+                    else
+                        hits[mine] = make_int2(__float_as_int(1000000.0f), -1);
+                }
             }
             act = __ballot(active);
         }
+        // the loop is left by the whole wave at once, with no hit pending
+        // (the refill step above has just stored them)
         if (act == 0) {
             if (exhausted) break;
             continue;
         }
         if (active && w.step(T, nvis, ntri)) {
-            hits[slot] = make_int2(__float_as_int(w.best_t), w.best_s);
+            hit = make_int2(__float_as_int(w.best_t), w.best_s);
+            pending = true;
             active = false;
         }
     }
@@ -3142,7 +3156,10 @@ struct AdjShadowQ {
 // its spawning material, zero elsewhere) and, in row n_mat, the vertex that
 // spawned it (its stream's key).
 constexpr uint32_t kContFlag = 1u << 31;
-constexpr float kScoreSpawn = 0.0625f;  // probability of a continuation per vertex
+// a vertex spawns a continuation with probability min(1, kScoreRate F), F
+// its rim flux (the rim term's weight: the least variance for a given
+// number of continuations)
+constexpr float kScoreRate = 0.05f;
 
 struct ContOut {  // where a score-form vertex appends its continuation
     Queue q;
@@ -3162,11 +3179,13 @@ struct ContOut {  // where a score-form vertex appends its continuation
 //
 // SCORE: after its connections, a vertex whose path goes on adds to S_m
 // (m its material) d log p(v)/ds - F for the direction v it drew, and with
-// probability kScoreSpawn appends a continuation leaving the vertex along a
-// rim direction (score_rim_dir) with weight -F / kScoreSpawn on spec_m,
-// adding F / kScoreSpawn to S_m.  A continuation replays transport only:
+// probability q = min(1, kScoreRate F) appends a continuation leaving the
+// vertex along a rim direction (score_rim_dir) with weight -F / q on spec_m,
+// adding F / q to S_m.  A continuation replays transport only:
 // no refl tangents, no explicit lobe term, no score, no spawns.
-template <bool BATCH, bool SCORE>
+// CONT: every entry of the queue is a continuation (the drains), so the
+// main paths' steps compile away.
+template <bool BATCH, bool SCORE, bool CONT = false>
 __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict__ inst, Queue qin,
                                                        const uint32_t* __restrict__ n_in,
                                                        const int2* __restrict__ hits, Tangents tin, Queue qout,
@@ -3202,7 +3221,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
             n_rec = P.n_rec;
             band = (int)(rm.z & 0xFFu);
             k = ((rm.z >> 8) & 0xFFFFu) + 1;
-            contn = SCORE && (rm.z & kContFlag) != 0u;
+            contn = CONT || (SCORE && (rm.z & kContFlag) != 0u);
             const int2 h = hits[i];
             if (h.y >= 0) {
                 const float t = __int_as_float(h.x);
@@ -3308,7 +3327,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
                 aq.w[4 * (size_t)slot + 3] = make_float4(ws[4], ws[5], ws[6], ws[7]);
             }
         }
-        if (SCORE && cont && !contn) {
+        if (SCORE && !CONT && cont && !contn) {
             // the direction just drawn leaves this vertex: its score, the rim
             // flux and perhaps a continuation (after this vertex's connections,
             // which that direction does not reach)
@@ -3321,9 +3340,10 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
             double coef = score_dlogp(cdir, (double)spec) - F;
             Rng rs;
             score_rng_init(rs, P.seed, ray, P.ctx_word[band], P.instance, k, 0u);
-            if (rs.next() < kScoreSpawn) {
+            const float qk = fminf(1.0f, (float)(F * (double)kScoreRate));
+            if (rs.next() < qk) {
                 vb = score_rim_dir(rs, nrm, rdir, spec);
-                const double wgt = F / (double)kScoreSpawn;
+                const double wgt = F / (double)qk;
                 coef += wgt;
                 wcont = (float)(-wgt);
                 sdraw = rs.draw;
@@ -3347,7 +3367,7 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
             }
             if (SCORE && contn) tout.row(nm, slot) = __uint_as_float(kspawn);
         }
-        if (SCORE) {
+        if (SCORE && !CONT) {
             const uint32_t cs = block_append<kBlock>(co.n, spawn, sc[4 + parity]);
             if (spawn) {
                 co.q.o[cs] = no;
@@ -3743,11 +3763,19 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     // hits of queue qi, the shade (continuing rays into qo, continuations
     // into co), the visibility of the connection requests
     auto adl_bounce = [&](const Queue& qi, uint32_t* ni, const Tangents& ti, const Queue& qo, uint32_t* no,
-                          const Tangents& to, const ContOut& co, uint32_t n_now, uint32_t gsd, uint32_t ge) {
+                          const Tangents& to, const ContOut& co, uint32_t n_now, uint32_t gsd, uint32_t ge,
+                          bool conts) {
         hipLaunchKernelGGL(fe, dim3(ge), dim3(256), 0, st, di, gs.as<int>(), gthreads, qi, (const uint32_t*)ni,
                            hits.as<int2>(), stp, fq_e);
         if (fetch_e) k_reset_iter<<<1, 64, 0, st>>>(cnt, 0u, segc.as<uint32_t>(), nseg);
-        if (batch) {
+        if (conts) {
+            if (batch)
+                k_shade_adl<true, true, true><<<gsd, kBlock, 0, st>>>(di, qi, ni, hits.as<int2>(), ti, qo, no, to,
+                                                                      AQ, cnt + 2, max_rec, A0, dA, co, stp);
+            else
+                k_shade_adl<false, true, true><<<gsd, kBlock, 0, st>>>(di, qi, ni, hits.as<int2>(), ti, qo, no, to,
+                                                                       AQ, cnt + 2, max_rec, A0, dA, co, stp);
+        } else if (batch) {
             if (all_score)
                 k_shade_adl<true, true><<<gsd, kBlock, 0, st>>>(di, qi, ni, hits.as<int2>(), ti, qo, no, to, AQ,
                                                                 cnt + 2, max_rec, A0, dA, co, stp);
@@ -3776,7 +3804,7 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
             const uint32_t ge = grid_for(n_cont, cap_e, 256);
             for (int gi = 0; gi < 4; ++gi) {
                 adl_bounce(qc[ci], ncc + ci, Tc[ci], qc[1 - ci], ncc + (1 - ci), Tc[1 - ci],
-                           ContOut{qc[ci], ncc + ci, Tc[ci]}, n_cont, gsd, ge);
+                           ContOut{qc[ci], ncc + ci, Tc[ci]}, n_cont, gsd, ge, true);
                 ci = 1 - ci;
             }
             EAR_HIP_TRY(c, hipMemcpyAsync(pinned_c, ncc, 8, hipMemcpyDeviceToHost, st));
@@ -3833,7 +3861,7 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
               for (uint32_t gi = 0; gi < group; ++gi, ++k) {
                 if (all_local) {
                     adl_bounce(q[cur], cnt + cur, T[cur], q[1 - cur], cnt + (1 - cur), T[1 - cur],
-                               ContOut{qc[ci], ncc + ci, Tc[ci]}, n_live, gsd, ge);
+                               ContOut{qc[ci], ncc + ci, Tc[ci]}, n_live, gsd, ge, false);
                 } else {
                     hipLaunchKernelGGL(fe, dim3(ge), dim3(256), 0, st, di, gs.as<int>(), gthreads, q[cur],
                                        (const uint32_t*)(cnt + cur), hits.as<int2>(), stp, fq_e);

@@ -320,8 +320,9 @@ ear_status ear_render_batch(ear_ctx* ctx, const ear_batch_item* items, uint32_t 
  * With EAR_FLAG_ADJ_SCORE d/dspec is the score (likelihood-ratio) form: each
  * connection is weighted by the derivative of the log density of the
  * directions drawn before it, plus the rim term of the blend's moving
- * support, estimated by continuation paths spawned with probability 1/16
- * at each vertex.  Paths never move, so wall switches, visibility changes
+ * support, estimated by continuation paths spawned at a vertex with
+ * probability F / 20, F the vertex's rim flux (at most 0.3 below s = 0.5).
+ * Paths never move, so wall switches, visibility changes
  * and arrival-time shifts are all accounted for: the estimate is unbiased at
  * any bounce count.  It needs every specularity of the rendered bands below
  * 0.5 (EAR_ERR_INVALID_ARGUMENT otherwise); d/drefl is the same as without

@@ -1002,7 +1002,7 @@ typedef struct {
     uint32_t begin, end;
     uint32_t n_mat;
     int local;      /* 1: direction tangents are not carried past their vertex; 2: score form */
-    float q_spawn;  /* score form: probability of a continuation per vertex */
+    float q_spawn;  /* score form: continuations per vertex per unit rim flux F */
     double* g_refl; /* [n_mat] */
     double* g_spec;
 } gworker_t;
@@ -1274,9 +1274,10 @@ static void grad_ray(const gworker_t* G, uint32_t ray, tan_t* T, double* dlogI_r
 /* the rate at which probability leaves through it; its integral is         */
 /*   F = (r.nu) / ((1-s) sqrt((1-s)^2 - s^2 (1 - (r.nu)^2))) = E[dlog p/ds].*/
 /* The rim integral is estimated by a continuation path: with probability  */
-/* q at every vertex (a constant: the loss's adjoint weighs late, weak     */
-/* arrivals as much as early ones), a second path leaves the vertex in     */
-/* direction v_b(phi), phi drawn with density w / F (a wrapped             */
+/* q = min(1, rate F) at a vertex (proportional to the rim term's weight,  */
+/* which minimises the variance for a given number of continuations), a    */
+/* second path leaves the vertex in direction v_b(phi), phi drawn with     */
+/* density w / F (a wrapped                                                 */
 /* Cauchy law, drawn as the Moebius image of a uniform point of the circle), */
 /* and adds -(F / q) times its connections to the gradient.  The main path  */
 /* adds G (dlog p/ds - F + [spawned] F / q): the last two terms have mean   */
@@ -1578,7 +1579,7 @@ static void grad_ray_score(const gworker_t* G, uint32_t ray, double* S, double* 
             double coef = score_dlogp(cdir, (double)spec) - F;
             or_rng Rs;
             score_rng_init(&Rs, D->seed, ray, D->ctx, D->instance, k);
-            const float qk = G->q_spawn;
+            const float qk = fminf(1.0f, (float)(F * (double)G->q_spawn));
             if (rng_next(&Rs) < qk) {
                 float vb[3];
                 score_rim_dir(&Rs, h_nu, h_r, spec, vb);
@@ -1617,9 +1618,8 @@ int or_gradient(const or_scene* s, const or_desc* D, const double* adj_signed, c
 int or_gradient_k(const or_scene* s, const or_desc* D, const double* adj_signed, const double* adj_abs, int local,
                   float q_spawn, double* grad_refl, double* grad_spec) {
     if (s->n_mat > 16 || D->band < 0 || D->band > 2) return -1;
-    if (local == 2 && !(q_spawn > 0.0f && q_spawn <= 1.0f)) return -1;
     if (local == 2) {
-        if (D->src_kind != OR_SRC_POINT) return -1;
+        if (D->src_kind != OR_SRC_POINT || !(q_spawn > 0.0f)) return -1;
         for (uint32_t m = 0; m < s->n_mat; m++)
             if ((s->mats[m].flags & OR_MAT_HAS_SPEC) && !(s->mats[m].spec[D->band] < 0.5f)) return -3;
     }

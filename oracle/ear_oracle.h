@@ -154,9 +154,9 @@ int or_num_channels(const or_desc* d);
  * sampled them (only each connection's explicit dependence on spec).
  * local == 2: the score form of the specularity gradient (unbiased; point
  * sources, specularities below 1/2, else -1 / -3), a continuation path
- * spawned with probability q_spawn at every vertex (or_gradient:
- * q_spawn = OR_SCORE_SPAWN). */
-#define OR_SCORE_SPAWN 0.0625f
+ * spawned at a vertex with probability min(1, q_spawn F), F its rim flux
+ * (or_gradient: q_spawn = OR_SCORE_SPAWN). */
+#define OR_SCORE_SPAWN 0.05f
 int or_gradient(const or_scene* s, const or_desc* d, const double* adj_signed, const double* adj_abs, int local,
                 double* grad_refl, double* grad_spec);
 int or_gradient_k(const or_scene* s, const or_desc* d, const double* adj_signed, const double* adj_abs, int local,

@@ -290,8 +290,9 @@ def gradient(scene: OracleScene, *, band, air, recorders, n_rays, adj_signed=Non
     """FP64 reference gradient of sum(adj * H) (raw sums) with respect to the
     band's per-material reflection coefficient and specularity (or_gradient;
     local: direction tangents end at their vertex; score: the score form of
-    the specularity gradient, a continuation path spawned with probability
-    q_spawn at every vertex, None: the library's default)."""
+    the specularity gradient, a continuation path spawned at a vertex with
+    probability min(1, q_spawn F), F its rim flux; None: the library's
+    default)."""
     if ctx is None:
         ctx = band
     recs = make_recorders(recorders)
