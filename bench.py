@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-instrument", action="store_true")
+    ap.add_argument("--no-checks", action="store_true",
+                    help="skip the 4x-ray accuracy render and the graph replay (counter runs: every render of the "
+                         "main leg is then a default one)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batch (256 rooms), forward + adjoint, animated-hall and fitting legs")
     ap.add_argument("--legs", default="main,batch,adjoint,animated,fit",
@@ -158,6 +161,8 @@ def main():
     # own error is about half this render's)
     accuracy = None
     try:
+        if a.no_checks:
+            raise RuntimeError("skipped (--no-checks)")
         r4 = api.render(ctx, sc, **dict(kw, n_rays=4 * a.rays, seed=0x5EED))
         edc_err = max(acoustics.edc_error(r.abs_value[b, c], r4.abs_value[b, c]) for b in range(3)
                       for c in range(r.abs_value.shape[1]))
@@ -170,6 +175,8 @@ def main():
     # the bounce loop replayed as a captured HIP graph (same histograms)
     graph = None
     try:
+        if a.no_checks:
+            raise RuntimeError("skipped (--no-checks)")
         ctx.set_tuning(splat={"default": 0, "atomic": 1, "tiled": 2}[a.splat], stack_lds=a.stack_lds, graph_iters=8)
         api.render(ctx, sc, **kw)
         t0 = time.perf_counter()
