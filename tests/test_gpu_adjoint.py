@@ -259,12 +259,18 @@ def test_backward_is_linear_in_the_adjoint_full_size(gpu_ctx):
                 np.testing.assert_allclose(g12[k], want, rtol=1e-7, atol=1e-9 * np.abs(want).max())
 
 
-@pytest.mark.parametrize("stereo,max_bounces,cutoff", [(False, 12, 0.0), (True, 6, 0.0), (False, 999, 1e-8)])
-def test_score_form_matches_fp64_oracle(gpu_ctx, oracle, stereo, max_bounces, cutoff):
+@pytest.mark.parametrize("stereo,max_bounces,cutoff,refract", [(False, 12, 0.0, False), (True, 6, 0.0, False),
+                                                               (False, 999, 1e-8, False), (False, 30, 1e-8, True)])
+def test_score_form_matches_fp64_oracle(gpu_ctx, oracle, stereo, max_bounces, cutoff, refract):
     """EAR_FLAG_ADJ_SCORE against the oracle's score form on the same rays
     and the same continuation paths (spawn decisions, rim directions and
-    their transport come from the vertex streams both draw alike)."""
+    their transport come from the vertex streams both draw alike).  refract:
+    two walls transmit 0.2 of each band (the blend then targets the incoming
+    direction about the negated normal, src/Scene.cpp:65-70)."""
     s = _scene(stereo)
+    if refract:
+        s["materials"] = [([0.75 * x for x in a], [0.2, 0.2, 0.2] if m in (0, 3) else b, c)
+                          for m, (a, b, c) in enumerate(s["materials"])]
     n_rays = 20_000
     nch = 2 if stereo else 1
     a_v, a_a = _adjoints(nch)
