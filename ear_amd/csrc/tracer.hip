@@ -3635,9 +3635,11 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     const bool all_score = n_score == n_items;
     const bool all_local = all_score || n_local == n_items;  // the per-vertex kernels
     DevBuf aqp(c), aqm(c), aqw(c), segv(c);
-    // the score form's continuations wait in qc[ci] until a drain traces
-    // them; room for two chunks of spawns
-    const uint32_t cap_c = all_score ? (uint32_t)std::min<uint64_t>(2ull * cap, 0xFFFFFFFFull) : 1u;
+    // the score form's continuations: traced one bounce per main bounce in
+    // their own queues (qc), then to their ends after the chunk's main rays;
+    // room for four chunks (about the most that are alive at once: spawns
+    // build up as ~0.05 F per live ray and bounce), a drain when full
+    const uint32_t cap_c = all_score ? (uint32_t)std::min<uint64_t>(4ull * cap, 1ull << 25) : 1u;
     const uint32_t cap_q = std::max(cap, cap_c);  // rays of one bounce: the chunk or a drain
     TRY(hits.alloc((size_t)cap_q * 8));
     const size_t cap_aq = all_local ? (size_t)cap_q * std::max<uint32_t>(1, max_rec) : 1;
@@ -3746,8 +3748,9 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     Tangents Tc[2] = {Tangents{tct[0].as<float>(), tcr[0].as<float>(), cap_c},
                       Tangents{tct[1].as<float>(), tcr[1].as<float>(), cap_c}};
     uint32_t* ncc = cntc.as<uint32_t>();
-    int ci = 0;           // the buffer continuations are appended to
-    uint32_t n_cont = 0;  // continuations waiting there
+    int ci = 0;              // the continuations' input buffer
+    uint32_t n_cont = 0;     // continuations waiting there
+    bool cont_live = false;  // any may be waiting (a main bounce has run since the last drain)
     uint32_t pinned_c[2];
     const ShadowQ SQ{sqp.as<float4>(), sqm.as<float4>()};
     const AdjArgs A0 = args[0];
@@ -3799,6 +3802,7 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     // the waiting continuations traced to their ends (they spawn nothing,
     // so a queue never grows within a drain)
     auto drain = [&]() -> ear_status {
+        cont_live = false;
         while (n_cont > 0) {
             const uint32_t gsd = persistent_grid(c, n_cont, waves);
             const uint32_t ge = grid_for(n_cont, cap_e, 256);
@@ -3852,16 +3856,29 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
             while (k <= max_bounces && n_live > 0) {
               uint32_t group = std::min<uint32_t>(4u, max_bounces - k + 1);
               if (all_score) {
-                  // every bounce may append one continuation per live ray
+                  // every bounce may append one continuation per live ray, and
+                  // the continuations' own bounce never adds any
                   if (cap_c - n_cont < n_live) TRY(drain());
                   group = std::min<uint32_t>(group, (cap_c - n_cont) / n_live);
               }
               const uint32_t gsd = persistent_grid(c, n_live, waves);
               const uint32_t ge = grid_for(n_live, cap_e, 256);
+              const uint32_t n_cmax = all_score ? n_cont + group * n_live : 0u;
+              const uint32_t gsd_c = persistent_grid(c, std::max(1u, n_cmax), waves);
+              const uint32_t ge_c = grid_for(std::max(1u, n_cmax), cap_e, 256);
               for (uint32_t gi = 0; gi < group; ++gi, ++k) {
                 if (all_local) {
+                    // the main bounce: its spawns join the continuations' next input
                     adl_bounce(q[cur], cnt + cur, T[cur], q[1 - cur], cnt + (1 - cur), T[1 - cur],
-                               ContOut{qc[ci], ncc + ci, Tc[ci]}, n_live, gsd, ge, false);
+                               ContOut{qc[1 - ci], ncc + (1 - ci), Tc[1 - ci]}, n_live, gsd, ge, false);
+                    if (all_score) {
+                        // one bounce of the waiting continuations, beside the main rays
+                        if (cont_live)
+                            adl_bounce(qc[ci], ncc + ci, Tc[ci], qc[1 - ci], ncc + (1 - ci), Tc[1 - ci],
+                                       ContOut{qc[ci], ncc + ci, Tc[ci]}, n_cmax, gsd_c, ge_c, true);
+                        cont_live = true;
+                        ci = 1 - ci;
+                    }
                 } else {
                     hipLaunchKernelGGL(fe, dim3(ge), dim3(256), 0, st, di, gs.as<int>(), gthreads, q[cur],
                                        (const uint32_t*)(cnt + cur), hits.as<int2>(), stp, fq_e);
