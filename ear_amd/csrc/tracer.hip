@@ -3183,8 +3183,8 @@ struct ContOut {  // where a score-form vertex appends its continuation
 // vertex along a rim direction (score_rim_dir) with weight -F / q on spec_m,
 // adding F / q to S_m.  A continuation replays transport only:
 // no refl tangents, no explicit lobe term, no score, no spawns.
-// CONT: every entry of the queue is a continuation (the drains), so the
-// main paths' steps compile away.
+// CONT: every entry of the queue is a continuation (the continuations' own
+// queues), so the main paths' steps compile away.
 template <bool BATCH, bool SCORE, bool CONT = false>
 __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict__ inst, Queue qin,
                                                        const uint32_t* __restrict__ n_in,
