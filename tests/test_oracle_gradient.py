@@ -162,6 +162,11 @@ def test_score_form_keeps_refl_and_rejects_what_it_cannot_do(oracle, room):
     full = oracle.gradient(sc, **args)
     sco = oracle.gradient(sc, score=True, **args)
     np.testing.assert_allclose(sco[0], full[0], rtol=1e-12)
+    # one bounce: no direction drawn at a vertex reaches a connection, so the
+    # score form is the local one (the explicit lobe term only)
+    one = dict(args, max_bounces=1)
+    np.testing.assert_allclose(oracle.gradient(sc, score=True, **one)[1], oracle.gradient(sc, local=True, **one)[1],
+                               rtol=1e-12, atol=0)
     mats = [(a, b, [0.5, 0.5, 0.5]) for a, b, _ in room["materials"]]
     with pytest.raises(RuntimeError, match="-3"):
         oracle.gradient(oracle.OracleScene(room["tris"], room["mat"], mats), score=True, **args)
