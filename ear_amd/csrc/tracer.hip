@@ -138,6 +138,92 @@ __device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred, u
     return sc[kWaves] + sc[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Orders one wave's LDS accesses (lanes read what other lanes of the same
+// wave wrote; the LDS executes a wave's instructions in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sample_Hemi's rejection loops (src/Distributions.h:48-70) for the lanes of
+// one wave together.  A lane's candidates are its consecutive triples of
+// uniforms, so which one it accepts and how many draws it uses do not depend
+// on which lane computes its Philox blocks.  In each round the wave's lanes
+// compute blocks for the A lanes still sampling (64 / A blocks for each),
+// and every sampling lane scans its own words in order with sample_hemi's
+// tests and arithmetic: same vector, same draw count.  A wave then runs
+// until its sampling lanes' candidates are used up together, instead of one
+// block per candidate until its slowest lane accepts.  All lanes of the wave
+// call it; `need` marks those that sample: around normal n if `hemi`
+// (sample_hemi), else the whole sphere (sample_sphere).
+struct HemiLds {
+    uint4 own[64];   // the rank-th sampling lane's ray, ctx, inst and first block to compute
+    uint2 key[64];   // its Philox key
+    uint4 blk[128];  // per sampling lane (m + 1 each): its current block, then its m new blocks
+};
+__device__ __forceinline__ V3 sample_hemi_wave(bool need, bool hemi, Rng& rng, V3 n, HemiLds& S) {
+    const uint32_t lane = lane_id();
+    V3 v{0.0f, 0.0f, 0.0f};
+    uint64_t act = __ballot(need);
+    while (act) {
+        const uint32_t A = (uint32_t)__popcll(act);
+        const uint32_t m = 64u / A;  // A * (m + 1) <= 64 + A block slots
+        const bool mine = (act >> lane) & 1ull;
+        const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+        uint32_t ph = 0;
+        if (mine) {
+            ph = rng.draw & 3u;
+            S.own[rank] = make_uint4(rng.ray, rng.ctx, rng.inst, (rng.draw >> 2) + (ph ? 1u : 0u));
+            S.key[rank] = make_uint2(rng.k0, rng.k1);
+            S.blk[rank * (m + 1)] = make_uint4(rng.buf[0], rng.buf[1], rng.buf[2], rng.buf[3]);
+        }
+        wave_sync();
+        const uint32_t a = lane / m, o = lane - a * m;
+        if (a < A) {
+            const uint4 w = S.own[a];
+            const uint2 k = S.key[a];
+            uint32_t c[4] = {w.w + o, w.x, w.y, w.z};
+            philox4x32_10(c, k.x, k.y);
+            S.blk[a * (m + 1) + 1 + o] = make_uint4(c[0], c[1], c[2], c[3]);
+        }
+        wave_sync();
+        bool left = false;
+        if (mine) {
+            // word `off` of this lane's run; the first unused one is its draw
+            const uint32_t* words = (const uint32_t*)&S.blk[rank * (m + 1)];
+            const uint32_t off0 = ph ? ph : 4u, end = 4u * (m + 1u);
+            uint32_t off = off0;
+            left = true;
+            while (off + 3u <= end) {
+                const float f1 = Rng::to_unit(words[off]) * 2.0f - 1.0f;
+                const float f2 = Rng::to_unit(words[off + 1u]) * 2.0f - 1.0f;
+                const float f3 = Rng::to_unit(words[off + 2u]) * 2.0f - 1.0f;
+                off += 3u;
+                const float l = f1 * f1 + f2 * f2 + f3 * f3;
+                if (l < 0.001f || l > 1.0f) continue;
+                const float s = sqrtf(l);
+                const V3 c{f1 / s, f2 / s, f3 / s};
+                if (hemi && dot(n, c) < 0.0f) continue;
+                v = c;
+                left = false;
+                break;
+            }
+            rng.draw += off - off0;
+            if (off & 3u) {
+                const uint4 b = S.blk[rank * (m + 1) + (off >> 2)];
+                rng.buf[0] = b.x;
+                rng.buf[1] = b.y;
+                rng.buf[2] = b.z;
+                rng.buf[3] = b.w;
+            }
+        }
+        wave_sync();  // this round's reads before the next round's writes
+        act = __ballot(left);
+    }
+    return v;
+}
+
 // Statistics counters: [kStatCopies][ST_N], a block adding into copy
 // blockIdx % kStatCopies (the host sums the copies), so that the one atomic
 // per wave at the end of every launch does not pile up on one address.
@@ -811,6 +897,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__
         first = plan[0];
         total = plan[1];
     }
+    __shared__ HemiLds hl[kBlock / 64];
     uint32_t n_done = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < total; base += gridDim.x * kBlock) {
         uint32_t gid = first + base + threadIdx.x;
@@ -822,7 +909,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__
         int band = 0;
         uint32_t id = 0;
         Rng rng;
-        V3 o{0, 0, 0};
+        V3 o{0, 0, 0}, sn{0, 0, 0};
         bool mesh_src = false;
         uint32_t n_rec = 0;
         if (valid) {
@@ -841,7 +928,6 @@ __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__
             band = (int)g.band;
             uint32_t ray = g.ray0 + (gid - seg_off[lo]);
             rng.init(P.seed, ray, P.ctx_word[band], P.instance, 0);
-            V3 d;
             mesh_src = P.src_kind == EAR_SRC_MESH;
             n_rec = P.n_rec;
             if (mesh_src) {
@@ -861,15 +947,18 @@ __global__ void __launch_bounds__(kBlock) k_generate(const InstDev* __restrict__
                 float ca = 1.0f - sr1, cb = sr1 * (1.0f - r2), cc = sr1 * r2;
                 o = V3{ca * v[0] + cb * v[3] + cc * v[6], ca * v[1] + cb * v[4] + cc * v[7],
                        ca * v[2] + cb * v[5] + cc * v[8]};
-                d = sample_hemi(rng, V3{I.M.nrm[3 * pick], I.M.nrm[3 * pick + 1], I.M.nrm[3 * pick + 2]});
+                sn = V3{I.M.nrm[3 * pick], I.M.nrm[3 * pick + 1], I.M.nrm[3 * pick + 2]};
             } else {
                 o = V3{P.src_pos[0], P.src_pos[1], P.src_pos[2]};
-                d = sample_sphere(rng);
             }
+            live = !((double)1.0f < P.cutoff) && P.max_bounces > 0;
+        }
+        // a mesh source's hemisphere, a point source's sphere
+        const V3 d = sample_hemi_wave(valid, mesh_src, rng, sn, hl[threadIdx.x >> 6]);
+        if (valid) {
             ro = make_float4(o.x, o.y, o.z, 1.0f);
             rd = make_float4(d.x, d.y, d.z, 0.0f);
-            rm = make_uint4(ray, rng.draw, (uint32_t)band, id);
-            live = !((double)1.0f < P.cutoff) && P.max_bounces > 0;
+            rm = make_uint4(rng.ray, rng.draw, (uint32_t)band, id);
         }
         // mesh sources connect to every recorder at k = 0 (src/Scene.cpp:185)
         if (any_mesh) {
@@ -1007,6 +1096,7 @@ __global__ void __launch_bounds__(kShadeBlock, 6) k_shade(const InstDev* __restr
                                                    uint32_t* __restrict__ n_sq, uint32_t max_rec,
                                                    unsigned long long* __restrict__ stats) {
     __shared__ uint32_t sc[4][kShadeBlock / 64 + 1];  // scratch of the block-wide appends, alternated
+    __shared__ HemiLds hl[kShadeBlock / 64];
     uint32_t n = *n_in;
     uint32_t n_done = 0, n_sph = 0;
     Drops drops;
@@ -1066,11 +1156,12 @@ __global__ void __launch_bounds__(kShadeBlock, 6) k_shade(const InstDev* __restr
         if (alive) cont = !((double)I < inst[id].P.cutoff) && k < inst[id].P.max_bounces;
         float4 no{}, nd{};
         uint4 nmeta{};
+        const V3 hv = sample_hemi_wave(cont, true, rng, nrm, hl[threadIdx.x >> 6]);
         if (cont) {
             // REFRACT: the negated normal and the incoming direction; REFLECT:
             // the mirror direction about the ray-facing normal (src/Scene.cpp:65-74)
             V3 rdir = bt == kRefract ? d : reflect(d, nrm);
-            V3 v = sample_hemi_spec(rng, nrm, rdir, spec);
+            V3 v = blend_spec(hv, rdir, spec);
             no = make_float4(p.x, p.y, p.z, I);
             nd = make_float4(v.x, v.y, v.z, L);
             nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8), id);
@@ -3194,12 +3285,14 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
                                                        const AdjArgs* __restrict__ adj, ContOut co,
                                                        unsigned long long* __restrict__ stats) {
     __shared__ uint32_t sc[6][kBlock / 64 + 1];  // scratch of the block-wide appends, alternated
+    __shared__ HemiLds hl[kBlock / 64];
     const uint32_t n = *n_in;
     uint32_t n_done = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         const uint32_t i = base + threadIdx.x;
         const uint32_t parity = (base / (gridDim.x * kBlock)) & 1u;
         bool cont = false, alive = false, contn = false, spawn = false;
+        Rng rng;
         float4 no{}, nd{};
         uint4 nmeta{};
         float drefl[kAdMaxMat], sacc[kAdMaxMat];
@@ -3233,7 +3326,6 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
                 nrm = dot(N, d) > 0.0f ? V3{N.x * -1.0f, N.y * -1.0f, N.z * -1.0f} : N;
                 mat = __float_as_uint(nmv.w);
                 const float4 mb = In.mat_band[3 * mat + band];
-                Rng rng;
                 if (contn) {
                     kspawn = __float_as_uint(tin.row(nm, i));
                     score_rng_init(rng, P.seed, rm.x, P.ctx_word[band], P.instance, kspawn, rm.y);
@@ -3259,14 +3351,17 @@ __global__ void __launch_bounds__(kBlock) k_shade_adl(const InstDev* __restrict_
                 alive = is_normal(I);
                 if (alive) {
                     cont = !((double)I < P.cutoff) && k < P.max_bounces;
-                    if (cont) {
-                        rdir = bt == kRefract ? d : reflect(d, nrm);
-                        vdir = sample_hemi_spec(rng, nrm, rdir, spec);
-                        no = make_float4(p.x, p.y, p.z, I);
-                        nd = make_float4(vdir.x, vdir.y, vdir.z, L);
-                        nmeta = make_uint4(rm.x, rng.draw, (uint32_t)band | (k << 8) | (contn ? kContFlag : 0u), id);
-                    }
+                    if (cont) rdir = bt == kRefract ? d : reflect(d, nrm);
                 }
+            }
+        }
+        {
+            const V3 hv = sample_hemi_wave(cont, true, rng, nrm, hl[threadIdx.x >> 6]);
+            if (cont) {
+                vdir = blend_spec(hv, rdir, spec);
+                no = make_float4(p.x, p.y, p.z, I);
+                nd = make_float4(vdir.x, vdir.y, vdir.z, L);
+                nmeta = make_uint4(ray, rng.draw, (uint32_t)band | (k << 8) | (contn ? kContFlag : 0u), id);
             }
         }
         // connections to every recorder (src/Scene.cpp:185-268), queued with
