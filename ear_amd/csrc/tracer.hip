@@ -2574,9 +2574,13 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     const uint32_t g_shade = persistent_grid(c, cap_rays, waves, kShadeBlock);
     const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
     const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
-    // count and scatter: the same grid, about 16k record slots per workgroup
+    // count and scatter: the same grid, two workgroups per CU (at least 16k
+    // record slots each).  Fewer, longer-running workgroups give each key
+    // longer runs of the sorted array per workgroup, so fewer cache lines are
+    // written in part from several XCDs (on the hall: 36.0 -> 30.3 ms of
+    // tile kernels per render against four per CU, 31.5 ms at one)
     const uint32_t g_tile = (uint32_t)std::max<size_t>(
-        1, std::min<size_t>((size_t)c->num_cus * 4, cap_sh * tile_stride / (kTileBlock * kTileBatch * 4)));
+        1, std::min<size_t>((size_t)c->num_cus * 2, cap_sh * tile_stride / (kTileBlock * kTileBatch * 4)));
     // splat: one workgroup per kTileChunk records of the most one launch can write
     const uint32_t g_splat = (uint32_t)std::max<size_t>(1, (cap_sh * tile_stride + kTileChunk - 1) / kTileChunk);
     // eager: the bounce loop launched from the host (always in the
