@@ -2712,6 +2712,7 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
             launches += 1;
             if (le != hipSuccess) status = hip_status(c, le, "bounce-loop graph launch");
             if (status == EAR_OK) status = finished(0, &done);
+            if (status == EAR_OK) status = tail(0, &done);  // an even replay ends on queue 0
             if (!done && bounces > max_iters + per) {
                 c->last_error = "bounce loop did not drain";
                 status = EAR_ERR_INTERNAL;

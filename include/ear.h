@@ -371,7 +371,9 @@ typedef struct {
     uint32_t stack_lds;      /* traversal stack entries kept in LDS (16 or 32) */
     uint32_t traversal;      /* EAR_TRAVERSAL_*: how BVH walks are assigned to lanes */
     uint32_t graph_iters;    /* > 0: bounce-loop iterations captured in one HIP graph and
-                                replayed (per-kernel times are then not measured) */
+                                replayed (per-kernel times are then not measured); after a
+                                replay the last live rays go to the tail launch as in the
+                                eager loop */
     uint32_t refill;         /* refilling traversal: idle lanes (of 64) that trigger a refill
                                 (default 24) */
     uint32_t fetch_chunk;    /* refilling traversal: queue entries a wave claims per atomic

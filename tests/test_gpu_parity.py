@@ -346,7 +346,9 @@ def test_graph_bounce_loop_is_bitwise_identical(gpu_ctx):
     replayed until the queues drain, with a host synchronisation between
     replays, gives the eager loop's histograms on every render; a small
     queue (ray_chunk 3000) makes each render replay it many times and
-    regenerate rays in most iterations."""
+    regenerate rays in most iterations.  With 2 iterations the replays run
+    to the end (tail_rays = 1); with 6 the last live rays go to the tail
+    launch after a replay, as in the eager loop."""
     s = scenes.hall(target_tris=20_000)
     src_tris = np.array([[[4.0, 13.0, 1.0], [5.0, 13.0, 1.0], [4.0, 14.0, 1.2]]], np.float32)
     kw = dict(recorders=s["recorders"], n_rays=20_000, src_pos=s["src"], air=s["air"], max_bounces=100,
@@ -354,8 +356,8 @@ def test_graph_bounce_loop_is_bitwise_identical(gpu_ctx):
     with api.Scene(gpu_ctx, s["tris"], s["mat"], s["materials"]) as sc:
         gpu_ctx.set_tuning()
         ref = [api.render(gpu_ctx, sc, **kw), api.render(gpu_ctx, sc, src_tris=src_tris, **kw)]
-        for g, trav in ((2, 1), (6, 3)):
-            gpu_ctx.set_tuning(graph_iters=g, ray_chunk=3000, traversal=trav)
+        for g, trav, tail in ((2, 1, 1), (6, 3, 0)):
+            gpu_ctx.set_tuning(graph_iters=g, ray_chunk=3000, traversal=trav, tail_rays=tail)
             for rep in range(2):
                 for j, extra in enumerate(({}, {"src_tris": src_tris})):
                     r = api.render(gpu_ctx, sc, **kw, **extra)
