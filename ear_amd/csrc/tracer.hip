@@ -2045,6 +2045,16 @@ uint32_t grid_for(uint32_t work, uint32_t cap_blocks, uint32_t block) {
     return std::max<uint32_t>(1, std::min(blocks_needed, std::max<uint32_t>(1, cap_blocks)));
 }
 
+// Block cap of a kernel that loops over its queue with the grid's stride (so
+// every block gets the same share): three rounds of the blocks one CU holds
+// at once.  A grid of whole rounds leaves no CU a last partial round to run
+// alone (32 waves per CU of a kernel that fits 24 ran its last quarter of
+// blocks at a third of the occupancy).  waves_per_cu, when set, overrides.
+uint32_t rounds_cap(Ctx* c, const void* fn, uint32_t waves_per_cu, int block) {
+    if (waves_per_cu) return (uint32_t)c->num_cus * waves_per_cu / (uint32_t)std::max(1, block / 64);
+    return (uint32_t)c->num_cus * (uint32_t)std::max(1, resident_blocks(fn, block)) * 3u;
+}
+
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
 };
@@ -2362,7 +2372,6 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     };
     const uint32_t cap_e = std::max<uint32_t>(1, cap_for((const void*)fe, cfg_e.block));
     const uint32_t cap_s = std::max<uint32_t>(1, cap_for((const void*)fs, cfg_s.block));
-    const uint32_t waves = tu.waves_per_cu ? tu.waves_per_cu : 32;  // non-traversal kernels
 
     hipStream_t st = c->stream;
     Arena* ar = arena;
@@ -2570,8 +2579,13 @@ ear_status run_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scenes, const
     }
     const uint32_t n_segs = (uint32_t)std::max<size_t>(1, all.size());
     const uint32_t total32 = (uint32_t)total_rays;
-    const uint32_t g_gen = persistent_grid(c, cap_rays, waves);
-    const uint32_t g_shade = persistent_grid(c, cap_rays, waves, kShadeBlock);
+    const uint32_t g_gen = grid_for(
+        cap_rays, rounds_cap(c, batch ? (const void*)k_generate<true> : (const void*)k_generate<false>, tu.waves_per_cu, kBlock),
+        kBlock);
+    const uint32_t g_shade = grid_for(
+        cap_rays,
+        rounds_cap(c, batch ? (const void*)k_shade<true> : (const void*)k_shade<false>, tu.waves_per_cu, kShadeBlock),
+        kShadeBlock);
     const uint32_t g_ext = grid_for(cap_rays, cap_e, cfg_e.block);
     const uint32_t g_shw = grid_for((uint32_t)std::min<size_t>(cap_sh, 0xFFFFFFFFu), cap_s, cfg_s.block);
     // count and scatter: the same grid, two workgroups per CU (at least 16k
@@ -3861,6 +3875,16 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     const VisFn fv = width == 4 ? (batch ? k_adj_vis<4, true> : k_adj_vis<4, false>)
                                 : (batch ? k_adj_vis<2, true> : k_adj_vis<2, false>);
     const uint32_t cap_v = (uint32_t)c->num_cus * (uint32_t)std::max(1, resident_blocks((const void*)fv, kBlock));
+    // grids of the per-vertex shading kernels (no walk inside them, so no
+    // overflow stacks to cover) and of the continuations' shading
+    const uint32_t cap_sl =
+        rounds_cap(c,
+                   batch ? (all_score ? (const void*)k_shade_adl<true, true> : (const void*)k_shade_adl<true, false>)
+                         : (all_score ? (const void*)k_shade_adl<false, true> : (const void*)k_shade_adl<false, false>),
+                   tu.waves_per_cu, kBlock);
+    const uint32_t cap_sc = rounds_cap(
+        c, batch ? (const void*)k_shade_adl<true, true, true> : (const void*)k_shade_adl<false, true, true>,
+        tu.waves_per_cu, kBlock);
     const Fetch fq_v{segv.as<uint32_t>(), tu.refill ? tu.refill : 24u, tu.fetch_chunk, nseg};
     // one bounce of the per-vertex adjoint (local or score form): closest
     // hits of queue qi, the shade (continuing rays into qo, continuations
@@ -3904,7 +3928,7 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
     auto drain = [&]() -> ear_status {
         cont_live = false;
         while (n_cont > 0) {
-            const uint32_t gsd = persistent_grid(c, n_cont, waves);
+            const uint32_t gsd = grid_for(n_cont, cap_sc, kBlock);
             const uint32_t ge = grid_for(n_cont, cap_e, 256);
             for (int gi = 0; gi < 4; ++gi) {
                 adl_bounce(qc[ci], ncc + ci, Tc[ci], qc[1 - ci], ncc + (1 - ci), Tc[1 - ci],
@@ -3961,10 +3985,11 @@ ear_status run_backward_batch(Ctx* c, uint32_t n_items, DeviceScene* const* scen
                   if (cap_c - n_cont < n_live) TRY(drain());
                   group = std::min<uint32_t>(group, (cap_c - n_cont) / n_live);
               }
-              const uint32_t gsd = persistent_grid(c, n_live, waves);
+              // k_shade_ad walks inside: its grid stays within the overflow stacks' gmax
+              const uint32_t gsd = all_local ? grid_for(n_live, cap_sl, kBlock) : persistent_grid(c, n_live, waves);
               const uint32_t ge = grid_for(n_live, cap_e, 256);
               const uint32_t n_cmax = all_score ? n_cont + group * n_live : 0u;
-              const uint32_t gsd_c = persistent_grid(c, std::max(1u, n_cmax), waves);
+              const uint32_t gsd_c = grid_for(std::max(1u, n_cmax), cap_sc, kBlock);
               const uint32_t ge_c = grid_for(std::max(1u, n_cmax), cap_e, 256);
               for (uint32_t gi = 0; gi < group; ++gi, ++k) {
                 if (all_local) {
