@@ -676,7 +676,7 @@ __device__ uint32_t splat_ramp(unsigned long long* h, uint32_t nb, int s, int w,
 // integers in another order, so the histograms are bitwise those of the
 // direct splat.
 constexpr uint32_t kTile = 256;             // bins per tile
-constexpr uint32_t kTileChunk = 8192;       // sorted records per k_tile_splat workgroup
+constexpr uint32_t kTileChunk = 4096;       // sorted records per k_tile_splat workgroup
 constexpr uint32_t kHalo = 64;              // bins past its tile that a ramp starting in it may reach
 constexpr int kTileMaxW = (int)kHalo - 2;   // wider ramps are added directly
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;    // an empty record slot
