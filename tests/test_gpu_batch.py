@@ -57,6 +57,31 @@ def test_batch_with_small_chunks_is_identical(gpu_ctx):
             o.close()
 
 
+def test_batch_under_graph_replay_with_the_tail(gpu_ctx):
+    """A batch of rooms of 600 to 9000 triangles with the bounce loop replayed
+    as a captured graph (4 iterations, small queues so that it replays many
+    times), the last live rays handed to the tail launch after a replay,
+    equals the eager batch, on two calls in a row."""
+    rooms = [scenes.room(seed=20 + i, target_tris=[600, 3000, 9000][i % 3]) for i in range(5)]
+    objs = [api.Scene(gpu_ctx, r["tris"], r["mat"], r["materials"]) for r in rooms]
+    try:
+        items = [(sc, _kw(r, n_rays=2000 + 700 * i, instance=i)) for i, (r, sc) in enumerate(zip(rooms, objs))]
+        ref, st_ref, _ = api.render_batch(gpu_ctx, items)
+        gpu_ctx.set_tuning(graph_iters=4, ray_chunk=3000, tail_rays=4000)
+        try:
+            for _ in range(2):
+                res, st, _ = api.render_batch(gpu_ctx, items)
+                assert st["segments"] == st_ref["segments"]
+                for x, y in zip(ref, res):
+                    np.testing.assert_array_equal(x.raw, y.raw)
+                    np.testing.assert_array_equal(x.abs_raw, y.abs_raw)
+        finally:
+            gpu_ctx.set_tuning()
+    finally:
+        for o in objs:
+            o.close()
+
+
 def test_config1_full_size_against_fp64_oracle(gpu_ctx, oracle):
     """Shoebox 10x8x4, 12 triangles, 100k rays, <= 50 bounces, cut-off 1e-6,
     88,200 bins per band: histograms within 1e-5 of the FP64 oracle."""
