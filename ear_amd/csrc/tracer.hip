@@ -1387,6 +1387,8 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
                                                      unsigned long long* __restrict__ stats, Fetch fetch,
                                                      TileOut to) {
     __shared__ int lds[LDS * BLOCK];
+    // per wave: request slots of visible connections awaiting their records
+    __shared__ uint32_t visq[BLOCK / 64][192];
     const uint32_t n = *n_sq;
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t lane = lane_id();
@@ -1394,35 +1396,52 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
     Drops drops;
     Walker<LDS, BLOCK, WIDTH, true, INSTR> w(lds + threadIdx.x, gstack, gtid, gthreads);
     Traversal T = make_trav(inst[0], gstack, gthreads);
-    // a lane is walking (active), holds a visible connection not yet
-    // recorded (pending) or is free; pending lanes are recorded together when
-    // the wave refills, so the splat runs with many lanes at once
-    bool active = false, pending = false, exhausted = n == 0;
+    // A lane is walking (active) or free.  A walk that ends visible leaves
+    // its request slot in the wave's list and the lane takes new work at the
+    // next refill; once 64 slots wait, every lane of the wave records one
+    // (the splat then runs with the whole wave instead of the few lanes that
+    // happened to finish together).  The rest are recorded at the end.
+    uint32_t* vq = visq[threadIdx.x >> 6];
+    uint32_t nq = 0;  // slots in vq, the same in every lane
+    auto push = [&](bool v, uint32_t s) {  // the whole wave
+        const uint64_t b = __ballot(v);
+        if (v) vq[nq + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = s;
+        nq += (uint32_t)__popcll(b);
+    };
+    auto drain = [&](uint32_t take) {  // the whole wave: records the last `take` (<= 64) slots
+        wave_sync();
+        const bool mine = lane < take;
+        const uint32_t rs = mine ? vq[nq - take + lane] : 0u;
+        wave_sync();
+        nq -= take;
+        if (mine) {
+            const float4 pc = sq.p[rs], m = sq.m[rs];
+            const uint32_t packed = __float_as_uint(m.y);
+            const int r = (int)(packed & 0xFFu), band = (int)((packed >> 8) & 0xFFu);
+            const uint32_t id = BATCH ? __float_as_uint(m.z) : 0;
+            const InstDev& In = inst[id];
+            const RecParams& R = In.P.rec[r];
+            const V3 dv = sub(V3{R.px, R.py, R.pz}, V3{pc.x, pc.y, pc.z});
+            const float l = length(dv);
+            TileSink sink{to, rs * to.stride, 0};
+            nramp += record(make_hist(In), band, R, normalized(dv), pc.w, (m.x + l) / 343.0f, m.x + l, drops, sink,
+                            TILED);
+            if (TILED) sink.close();
+            ++ncon;
+        }
+    };
+    bool active = false, exhausted = n == 0;
     uint32_t slot = 0;
     uint32_t c_next = 0, c_end = 0;  // the wave's claimed entries [c_next, c_end)
     uint32_t seg = ((blockIdx.x * BLOCK + threadIdx.x) >> 6) % fetch.n_seg;  // the wave's first segment
     for (;;) {
+        if (nq >= 64) drain(64);
         uint64_t act = __ballot(active);
         const uint32_t n_free = 64u - (uint32_t)__popcll(act);
         if (n_free >= fetch.refill || act == 0) {
-            if (pending) {
-                const float4 pc = sq.p[slot], m = sq.m[slot];
-                const uint32_t packed = __float_as_uint(m.y);
-                const int r = (int)(packed & 0xFFu), band = (int)((packed >> 8) & 0xFFu);
-                const uint32_t id = BATCH ? __float_as_uint(m.z) : 0;
-                const InstDev& In = inst[id];
-                const RecParams& R = In.P.rec[r];
-                const V3 dv = sub(V3{R.px, R.py, R.pz}, V3{pc.x, pc.y, pc.z});
-                const float l = length(dv);
-                TileSink sink{to, slot * to.stride, 0};
-                nramp += record(make_hist(In), band, R, normalized(dv), pc.w, (m.x + l) / 343.0f, m.x + l, drops,
-                                sink, TILED);
-                if (TILED) sink.close();
-                ++ncon;
-                pending = false;
-            }
             if (!exhausted) {
                 const uint32_t mine = take_entries(fetch, n, act, lane, c_next, c_end, exhausted, seg);
+                bool vis = false;
                 if (mine != 0xFFFFFFFFu) {
                     slot = mine;
                     ++nsh;
@@ -1435,22 +1454,28 @@ __global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ 
                     if (w.start(T, p, sub(V3{R.px, R.py, R.pz}, p), 0.0f))
                         active = true;
                     else
-                        pending = true;  // empty scene: visible
+                        vis = true;  // empty scene: visible
                 }
+                push(vis, slot);
             }
             act = __ballot(active);
         }
         // the loop is left by the whole wave at once (take_entries is a
-        // wave-wide step)
+        // wave-wide step), after the waiting records
         if (act == 0) {
-            if (exhausted && __ballot(pending) == 0) break;
+            if (exhausted) {
+                while (nq > 0) drain(nq < 64u ? nq : 64u);
+                break;
+            }
             continue;
         }
+        bool vis = false;
         if (active && w.step(T, nvis, ntri)) {
             active = false;
-            pending = !w.found;
+            vis = !w.found;
             if (TILED && w.found) TileSink{to, slot * to.stride, 0}.close();  // occluded: no records
         }
+        push(vis, slot);
     }
     stat_add(stats, ST_SHADOW, nsh);
     stat_add(stats, ST_CONTRIB, ncon);
