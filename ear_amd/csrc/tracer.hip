@@ -1382,7 +1382,7 @@ __global__ void __launch_bounds__(BLOCK) k_extend_f(const InstDev* __restrict__ 
 }
 
 template <int BLOCK, int LDS, int WIDTH, bool INSTR, bool BATCH, bool TILED>
-__global__ void __launch_bounds__(BLOCK) k_shadow_f(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
+__global__ void __launch_bounds__(BLOCK, 6) k_shadow_f(const InstDev* __restrict__ inst, int* gstack, uint32_t gthreads,
                                                      ShadowQ sq, const uint32_t* __restrict__ n_sq,
                                                      unsigned long long* __restrict__ stats, Fetch fetch,
                                                      TileOut to) {
