@@ -616,8 +616,8 @@ static void trace_ray(const rctx* C, acc_t* acc, uint32_t ray, int32_t* log_hits
                 sample_sphere(&R, dd);
             }
         } else {
-            float t;
-            int32_t idx;
+            float t = 0.0f;  /* set by closest_hit_p when it hits */
+            int32_t idx = -1;
             acc->segments++;
             int hit = closest_hit_p(C->s, o, dd, &t, &idx);
             if (log_hits && nseg < D->trace_segs) log_hits[nseg] = hit ? idx : -1;
