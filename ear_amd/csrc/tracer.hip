@@ -1388,6 +1388,8 @@ __global__ void __launch_bounds__(BLOCK, 6) k_shadow_f(const InstDev* __restrict
                                                      TileOut to) {
     __shared__ int lds[LDS * BLOCK];
     // per wave: request slots of visible connections awaiting their records
+    // (a lane adds at most one per pass of the loop below and each pass
+    // starts by recording 64 once 64 wait, so at most 127 ever wait)
     __shared__ uint32_t visq[BLOCK / 64][192];
     const uint32_t n = *n_sq;
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
